@@ -1,0 +1,152 @@
+"""ctypes binding of the C ABI in ``include/diffsims_amd.h``.
+
+The native library is the only compute path: when it is missing, or no GPU is
+visible, every device entry point raises instead of computing anything on the
+CPU.  torch is imported before the library is loaded so that the library binds
+to the HIP runtime torch already loaded (same soname), keeping one runtime per
+process.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+LIB_PATH = os.environ.get(
+    "DIFFSIMS_AMD_LIB",
+    os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdiffsims_amd.so"))
+
+DSX_OK = 0
+DSX_EINVAL = -1
+DSX_ELAUNCH = -2
+DSX_ECAPACITY = -3
+
+SF_LINEAR = 0
+SF_LORENTZIAN = 1
+SF_SINC = 2
+SF_SIN2C = 3
+SF_ATANC = 4
+SF_BINARY = 5
+SF_LORENTZIAN_PRECESSION = 6
+
+SCATTER_NONE = 0
+SCATTER_LOBATO = 1
+SCATTER_XTABLES = 2
+
+MAX_TAP_RADIUS = 64
+
+# Every symbol the header declares (checked by the CPU test suite).
+EXPORTED = (
+    "dsx_version",
+    "dsx_last_error",
+    "dsx_hkl_workspace_ints",
+    "dsx_hkl_enumerate",
+    "dsx_structure_factor",
+    "dsx_simulate",
+    "dsx_render_max_spots",
+    "dsx_render_fast",
+    "dsx_render_subpixel",
+)
+
+
+class SimParams(ctypes.Structure):
+    """dsx_sim_params."""
+    _fields_ = [
+        ("k", ctypes.c_double),
+        ("max_excitation_error", ctypes.c_double),
+        ("shape_width", ctypes.c_double),
+        ("shape_model", ctypes.c_int),
+        ("minima_number", ctypes.c_double),
+        ("precession_angle", ctypes.c_double),
+        ("minimum_intensity", ctypes.c_double),
+        ("g_max", ctypes.c_double),
+        ("rec", ctypes.c_double * 9),
+    ]
+
+
+class RenderParams(ctypes.Structure):
+    """dsx_render_params."""
+    _fields_ = [
+        ("height", ctypes.c_int),
+        ("width", ctypes.c_int),
+        ("calibration", ctypes.c_float),
+        ("center_x", ctypes.c_float),
+        ("center_y", ctypes.c_float),
+        ("cos_angle", ctypes.c_float),
+        ("sin_angle", ctypes.c_float),
+        ("mirrored", ctypes.c_int),
+        ("normalize", ctypes.c_int),
+        ("sigma", ctypes.c_float),
+        ("clip_threshold", ctypes.c_float),
+        ("radius", ctypes.c_int),
+        ("taps", ctypes.c_float * (MAX_TAP_RADIUS + 1)),
+    ]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_SIGS = {
+    "dsx_version": (_I, []),
+    "dsx_last_error": (ctypes.c_char_p, []),
+    "dsx_hkl_workspace_ints": (ctypes.c_size_t, [ctypes.POINTER(_I)]),
+    "dsx_hkl_enumerate": (_I, [ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double),
+                               ctypes.c_double, _P, _I, _P, _P, _P]),
+    "dsx_structure_factor": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _P, _I,
+                                  _P, _I, _I, _P, _P, _P, _P]),
+    "dsx_simulate": (_I, [_P, _P, _P, _I, _P, _I, ctypes.POINTER(SimParams), _I, _P, _P, _P,
+                          _P, _P, _P]),
+    "dsx_render_max_spots": (_I, []),
+    "dsx_render_fast": (_I, [_P, _P, _I, _I, ctypes.POINTER(RenderParams), _P, _P, _P]),
+    "dsx_render_subpixel": (_I, [_P, _P, _I, _I, ctypes.POINTER(RenderParams), _P, _P, _P]),
+}
+
+_lib = None
+
+
+def load(require_gpu: bool = False):
+    """Load the native library (once).  Raises RuntimeError when it is absent."""
+    global _lib
+    if _lib is None:
+        import torch  # noqa: F401  (binds the library to torch's HIP runtime)
+
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"diffsims_amd native library not found at {LIB_PATH}; build it with "
+                "`python -m diffsims_amd.build`")
+        lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    if require_gpu:
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("diffsims_amd needs a ROCm GPU; none is visible")
+    return _lib
+
+
+def last_error() -> str:
+    return load().dsx_last_error().decode(errors="replace")
+
+
+def check(status: int, what: str) -> None:
+    """Map a DSX status to the Python exception the reference would raise."""
+    if status == DSX_OK:
+        return
+    msg = f"{what}: {last_error()}"
+    if status == DSX_EINVAL:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def ptr(t) -> int:
+    """Device pointer of a torch tensor (0 for None)."""
+    return 0 if t is None else t.data_ptr()
+
+
+def stream_handle(device=None) -> int:
+    import torch
+
+    return torch.cuda.current_stream(device).cuda_stream

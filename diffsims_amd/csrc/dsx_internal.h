@@ -1,0 +1,84 @@
+// Shared helpers for the diffsims_amd HIP kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "diffsims_amd.h"
+
+namespace dsx {
+
+constexpr int kWave = 64;
+
+// Thread-local error string behind dsx_last_error().
+void set_error(const char *fmt, ...);
+void clear_error();
+
+// Map a HIP status to DSX_ELAUNCH with a message naming the call site.
+int check_launch(const char *what);
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Number of compute units of the current device (cached per device).
+int cu_count();
+
+// Non-temporal 16-byte store (streamed output that is not re-read soon).
+typedef float native_float4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_nt(float4 v, float4 *p) {
+  native_float4 t = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(t, reinterpret_cast<native_float4 *>(p));
+}
+
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also waits
+// for every outstanding global store of the calling wave (s_waitcnt vmcnt(0)),
+// which stalls kernels that stream their output between barriers.
+__device__ __forceinline__ void lds_sync() {
+#ifdef DSX_FULL_BARRIER
+  __syncthreads();
+#else
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+#endif
+}
+
+// ---- wave-level helpers -------------------------------------------------
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Number of set bits of `mask` below this lane.
+__device__ __forceinline__ int mask_rank(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0));
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Inclusive prefix sum across the wave.
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    int t = __shfl_up(v, off, kWave);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+
+}  // namespace dsx

@@ -1,0 +1,299 @@
+// (b) Persistent per-orientation kernel: rotate g, excitation error against the
+// Ewald sphere, shape factor, intensity, relative cut, ordered compaction.
+//
+// Follows calculate_diffraction2d's orientation loop and
+// get_intersecting_reflections (diffsims/generators/simulation_generator.py:
+// 211-241, :319-412) and the models of diffsims/utils/shape_factor_models.py.
+//
+// Mapping: one wave64 owns a tile of kOT orientations; its lanes sweep the
+// reflection table 64 rows at a time.  A cheap rotation-invariant FP32 pretest
+// (|g|^2 - 2 k z, one dot product) rejects almost all rows; rows that pass are
+// re-evaluated in FP64 from their integer hkl (g = rec hkl, g_lab = R g) with
+// the reference's arithmetic made cancellation-free, and survivors are
+// appended in reflection order with a wave ballot + mbcnt prefix.  The relative intensity cut needs the maximum over the whole
+// intersecting set, so survivors are written first and, only if some fall
+// below the cut, re-compacted in place by the same wave.
+#include <algorithm>
+
+#include "dsx_internal.h"
+
+namespace dsx {
+
+constexpr int kOT = 4;            // orientations per wave
+constexpr int kSimThreads = 256;  // 4 waves per block
+
+struct SimArgs {
+  const int4 *hkl;
+  const float4 *g4;  // {gx, gy, gz, |g|^2}
+  const float *F2;
+  int n_refl;
+  const double *Rt;  // [n_orient][9], g_lab = Rt g
+  int n_orient;
+  double rec[9];
+  double k, k2;
+  double me;         // max_excitation_error
+  double width;      // shape width
+  double sigma;      // pi / width
+  double fac;        // pi * minima_number / width (|width| for atanc)
+  int model;
+  int precession;
+  double phi, Pz, Pt;  // precession angle (rad), k cos(phi), k sin(phi)
+  float min_I;
+  float k_f;
+  float pre_thr;
+  int cap;
+  int *idx;
+  float4 *spots;
+  int *count;
+  int *ncand;
+  float *maxI;
+};
+
+// shape_factor_models.py in FP64 (evaluated for candidates only).
+__device__ __forceinline__ double shape_factor(double s, double r, const SimArgs &a) {
+  switch (a.model) {
+    case DSX_SF_LINEAR:
+      return fmax(1.0 - fabs(s) / a.width, 0.0);
+    case DSX_SF_LORENTZIAN: {
+      const double t = a.sigma * s;
+      return 1.0 / (1.0 + t * t);
+    }
+    case DSX_SF_SINC:
+    case DSX_SF_SIN2C: {
+      const double x = a.fac * s;
+      if (x == 0.0) return 0.0;  // the reference's where=denom != 0 leaves 0
+      const double v = fabs(sin(x) / x);
+      return a.model == DSX_SF_SINC ? v : v * v;
+    }
+    case DSX_SF_ATANC: {
+      const double x = a.fac * s;
+      return x == 0.0 ? 1.0 : atan(x) / x;  // nan_to_num(0/0) = 1
+    }
+    case DSX_SF_BINARY:
+      return 1.0;
+    case DSX_SF_LORENTZIAN_PRECESSION: {
+      // (sigma/pi) sqrt(2 (u + z) / z^2), u = sigma^2 (r^2 phi^2 - s^2) + 1,
+      // z = sqrt(u^2 + 4 sigma^2 s^2); u + z rewritten for u < 0.
+      const double A = r * a.phi;
+      const double sg2 = a.sigma * a.sigma;
+      const double u = sg2 * ((A - s) * (A + s)) + 1.0;
+      const double ss = 4.0 * sg2 * s * s;
+      const double z = sqrt(u * u + ss);
+      const double upz = u >= 0.0 ? u + z : ss / (z - u);
+      return (a.sigma * 0.31830988618379067) * sqrt(2.0 * upz) / z;
+    }
+    default:
+      return 0.0;
+  }
+}
+
+// Membership test of simulation_generator.py:355-375 in FP64 with the
+// cancellations removed:
+//   k - sqrt(k^2 - r^2)            = r^2 / (k + sqrt(k^2 - r^2))
+//   Pz - sqrt(k^2 - (r +- Pt)^2)   = r (r +- 2 Pt) / (Pz + sqrt(k^2 - (r +- Pt)^2))
+__device__ __forceinline__ bool intersects(double x, double y, double z, const SimArgs &a,
+                                           double &s, double &r) {
+  const double r2 = x * x + y * y;
+  r = sqrt(r2);
+  if (!(r2 < a.k2)) return false;
+  s = r2 / (a.k + sqrt(a.k2 - r2)) - z;
+  if (!a.precession) return fabs(s) < a.me;
+  const double rp = r + a.Pt, rm = r - a.Pt;
+  const double qp = a.k2 - rp * rp, qm = a.k2 - rm * rm;
+  if (!(qp >= 0.0) || !(qm >= 0.0)) return false;
+  const double up = r * (r + 2.0 * a.Pt) / (a.Pz + sqrt(qp));
+  const double dn = r * (r - 2.0 * a.Pt) / (a.Pz + sqrt(qm));
+  return (z - a.me <= up) && (z + a.me >= dn);
+}
+
+__global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
+  const int lane = lane_id();
+  const int wave = (blockIdx.x * kSimThreads + threadIdx.x) / kWave;
+  const int n_waves = gridDim.x * (kSimThreads / kWave);
+  const int n_tiles = (a.n_orient + kOT - 1) / kOT;
+  const float two_k = 2.0f * a.k_f;
+
+  for (int tile = wave; tile < n_tiles; tile += n_waves) {
+    const int o0 = tile * kOT;
+    float R[kOT][9];
+    int cnt[kOT];
+    float lmax[kOT], lmin[kOT];
+#pragma unroll
+    for (int o = 0; o < kOT; ++o) {
+      const bool ok = o0 + o < a.n_orient;
+#pragma unroll
+      for (int j = 0; j < 9; ++j) R[o][j] = ok ? static_cast<float>(a.Rt[(size_t)(o0 + o) * 9 + j]) : 0.0f;
+      cnt[o] = 0;
+      lmax[o] = 0.0f;
+      lmin[o] = __builtin_huge_valf();
+    }
+
+    for (int base = 0; base < a.n_refl; base += kWave) {
+      const int i = base + lane;
+      const bool v = i < a.n_refl;
+      const float4 g = v ? a.g4[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+      for (int o = 0; o < kOT; ++o) {
+        if (o0 + o >= a.n_orient) break;  // uniform
+        const float z = fmaf(R[o][6], g.x, fmaf(R[o][7], g.y, R[o][8] * g.z));
+        const bool pre = v && fabsf(fmaf(-two_k, z, g.w)) < a.pre_thr;
+        if (__ballot(pre) == 0) continue;
+        bool in = false;
+        double x = 0.0, y = 0.0, zd = 0.0;
+        float I = 0.0f;
+        if (pre) {
+          const int4 h = a.hkl[i];
+          const double hx = h.x, hy = h.y, hz = h.z;
+          const double gx = a.rec[0] * hx + a.rec[1] * hy + a.rec[2] * hz;
+          const double gy = a.rec[3] * hx + a.rec[4] * hy + a.rec[5] * hz;
+          const double gz = a.rec[6] * hx + a.rec[7] * hy + a.rec[8] * hz;
+          const double *Rd = a.Rt + (size_t)(o0 + o) * 9;  // uniform: scalar loads
+          x = Rd[0] * gx + Rd[1] * gy + Rd[2] * gz;
+          y = Rd[3] * gx + Rd[4] * gy + Rd[5] * gz;
+          zd = Rd[6] * gx + Rd[7] * gy + Rd[8] * gz;
+          double s, r;
+          in = intersects(x, y, zd, a, s, r);
+          if (in) I = static_cast<float>(shape_factor(s, r, a) * static_cast<double>(a.F2[i]));
+        }
+        const uint64_t m = __ballot(in);
+        if (m == 0) continue;
+        if (in) {
+          const int pos = cnt[o] + mask_rank(m);
+          lmax[o] = fmaxf(lmax[o], I);
+          lmin[o] = fminf(lmin[o], I);
+          if (pos < a.cap) {
+            const size_t off = (size_t)(o0 + o) * a.cap + pos;
+            a.idx[off] = i;
+            a.spots[off] = make_float4(static_cast<float>(x), static_cast<float>(y),
+                                       static_cast<float>(zd), I);
+          }
+        }
+        cnt[o] += __popcll(m);
+      }
+    }
+
+#pragma unroll
+    for (int o = 0; o < kOT; ++o) {
+      const int orient = o0 + o;
+      if (orient >= a.n_orient) break;
+      const float mx = wave_max(lmax[o]);
+      const float mn = wave_min(lmin[o]);
+      const int n = cnt[o];
+      const int nw = min(n, a.cap);
+      int final_count = nw;
+      const float thr = mx * a.min_I;
+      // mx == 0 with n > 0: every intersecting reflection is kinematically
+      // forbidden (|F| exactly 0); the reference keeps them all (its FP64
+      // round-off passes the relative cut), so they are kept with I = 0.
+      if (n > 0 && mx > 0.0f && !(mn > thr)) {
+        __threadfence_block();
+        const size_t off = (size_t)orient * a.cap;
+        int w = 0;
+        for (int b = 0; b < nw; b += kWave) {
+          const int j = b + lane;
+          float4 sp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          int id = 0;
+          bool keep = false;
+          if (j < nw) {
+            sp = a.spots[off + j];
+            id = a.idx[off + j];
+            keep = sp.w > thr;
+          }
+          const uint64_t km = __ballot(keep);
+          if (keep) {
+            const int pos = w + mask_rank(km);
+            a.spots[off + pos] = sp;
+            a.idx[off + pos] = id;
+          }
+          w += __popcll(km);
+        }
+        final_count = w;
+      }
+      if (lane == 0) {
+        a.count[orient] = final_count;
+        a.ncand[orient] = n;
+        a.maxI[orient] = mx;
+      }
+    }
+  }
+}
+
+}  // namespace dsx
+
+using namespace dsx;
+
+extern "C" int dsx_simulate(const int *hkl_d, const float *g4_d, const float *F2_d, int n_refl,
+                            const double *Rt_d, int n_orient, const dsx_sim_params *p, int cap,
+                            int *idx_d, float *spot_d, int *count_d, int *ncand_d, float *maxI_d,
+                            void *stream) {
+  clear_error();
+  if (!p || n_refl < 0 || n_orient < 0 || cap < 1) {
+    set_error("dsx_simulate: bad arguments (n_refl=%d n_orient=%d cap=%d)", n_refl, n_orient, cap);
+    return DSX_EINVAL;
+  }
+  if (n_orient == 0) return DSX_OK;
+  if (!Rt_d || !idx_d || !spot_d || !count_d || !ncand_d || !maxI_d ||
+      (n_refl > 0 && (!hkl_d || !g4_d || !F2_d))) {
+    set_error("dsx_simulate: null device pointer");
+    return DSX_EINVAL;
+  }
+  if (p->shape_model < DSX_SF_LINEAR || p->shape_model > DSX_SF_LORENTZIAN_PRECESSION) {
+    set_error("dsx_simulate: unknown shape model %d", p->shape_model);
+    return DSX_EINVAL;
+  }
+  if (!(p->k > 0.0) || !(p->max_excitation_error > 0.0) || p->shape_width == 0.0 ||
+      !(p->precession_angle >= 0.0) || !(p->g_max >= 0.0)) {
+    set_error("dsx_simulate: bad physical parameters");
+    return DSX_EINVAL;
+  }
+  if (p->shape_model == DSX_SF_LORENTZIAN_PRECESSION && !(p->precession_angle > 0.0)) {
+    set_error("dsx_simulate: the precession shape factor needs a positive precession angle");
+    return DSX_EINVAL;
+  }
+  if ((size_t)n_orient * (size_t)cap > (size_t)1 << 40) {
+    set_error("dsx_simulate: n_orient*cap too large");
+    return DSX_EINVAL;
+  }
+  SimArgs a;
+  a.hkl = reinterpret_cast<const int4 *>(hkl_d);
+  a.g4 = reinterpret_cast<const float4 *>(g4_d);
+  a.F2 = F2_d;
+  a.n_refl = n_refl;
+  a.Rt = Rt_d;
+  a.n_orient = n_orient;
+  for (int i = 0; i < 9; ++i) a.rec[i] = p->rec[i];
+  const double k = p->k;
+  a.k = k;
+  a.k2 = k * k;
+  a.k_f = static_cast<float>(k);
+  a.me = p->max_excitation_error;
+  const double w = p->shape_width;
+  a.width = w;
+  a.sigma = M_PI / w;
+  a.fac = M_PI * p->minima_number / (p->shape_model == DSX_SF_ATANC ? fabs(w) : w);
+  a.model = p->shape_model;
+  const double phi = p->precession_angle;
+  a.precession = phi > 0.0;
+  a.phi = phi;
+  a.Pz = k * cos(phi);
+  a.Pt = k * sin(phi);
+  a.min_I = static_cast<float>(p->minimum_intensity);
+  // Pretest bound: e = |g|^2 - 2 k z = s (2k - z_sphere - z), and |s| <= me
+  // (+ r sin(phi) with precession); the slack covers FP32 rounding of e.
+  const double gmax = p->g_max;
+  const double me_eff = p->max_excitation_error + (phi > 0.0 ? 1.1 * gmax * sin(phi) : 0.0);
+  a.pre_thr = static_cast<float>(1.05 * me_eff * (2.0 * k + gmax) + 1e-3 +
+                                 1e-6 * (2.0 * k * gmax + gmax * gmax));
+  a.cap = cap;
+  a.idx = idx_d;
+  a.spots = reinterpret_cast<float4 *>(spot_d);
+  a.count = count_d;
+  a.ncand = ncand_d;
+  a.maxI = maxI_d;
+  const int n_tiles = (n_orient + kOT - 1) / kOT;
+  const int waves_per_block = kSimThreads / kWave;
+  int blocks = (n_tiles + waves_per_block - 1) / waves_per_block;
+  blocks = std::min(blocks, cu_count() * 8);
+  hipLaunchKernelGGL(simulate_kernel, dim3(blocks), dim3(kSimThreads), 0, as_stream(stream), a);
+  return check_launch("dsx_simulate");
+}

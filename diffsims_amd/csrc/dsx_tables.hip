@@ -1,0 +1,335 @@
+// Per-phase tables: (a) integer hkl enumeration with the reference's box rule,
+// and the kinematical structure factor |F(hkl)|^2 with g-vectors.
+//
+// hkl enumeration follows ReciprocalLatticeVector.from_min_dspacing
+// (diffsims/crystallography/reciprocal_lattice_vector.py:1077-1142): a box of
+// integer indices, a d-spacing filter in FP64, (000) dropped, and the order
+// orix's Vector3d.unique leaves (descending lexicographic; see hkl_index_to_hkl).
+// The structure factor follows _get_kinematical_structure_factor
+// (diffsims/utils/sim_utils.py:256-304), computed once per reflection instead of
+// once per orientation.
+#include "dsx_internal.h"
+
+namespace dsx {
+
+struct Metric {
+  double g00, g11, g22, g01, g02, g12;
+};
+
+struct Box {
+  int H, K, L;
+  int nk, nl;      // 2K+1, 2L+1
+  long long total; // (2H+1)(2K+1)(2L+1)
+};
+
+constexpr int kEnumThreads = 256;
+constexpr int kEnumItems = 8;  // consecutive box entries per thread
+constexpr int kEnumPerBlock = kEnumThreads * kEnumItems;
+
+// Box entry i (row-major over descending h, k, l).
+__device__ __forceinline__ void hkl_index_to_hkl(long long i, const Box &b, int &h, int &k,
+                                                 int &l) {
+  const long long plane = static_cast<long long>(b.nk) * b.nl;
+  const int ih = static_cast<int>(i / plane);
+  const int rem = static_cast<int>(i - static_cast<long long>(ih) * plane);
+  const int ik = rem / b.nl;
+  const int il = rem - ik * b.nl;
+  h = b.H - ih;
+  k = b.K - ik;
+  l = b.L - il;
+}
+
+// d-spacing filter in IEEE FP64 with no contraction, in the same operation
+// order as the host-side reference metric evaluation, so both agree bit for bit.
+__device__ __forceinline__ bool keep_hkl(int h, int k, int l, const Metric &m, double dmin) {
+#pragma clang fp contract(off)
+  if ((h | k | l) == 0) return false;
+  const double H = h, K = k, L = l;
+  double q = H * H * m.g00;
+  q = q + K * K * m.g11;
+  q = q + L * L * m.g22;
+  double cross = H * K * m.g01;
+  cross = cross + H * L * m.g02;
+  cross = cross + K * L * m.g12;
+  q = q + 2.0 * cross;
+  const double d = 1.0 / sqrt(q);
+  return d >= dmin;
+}
+
+// Pass 0 counts survivors per block; pass 1 writes them, order preserved.
+template <int kPass>
+__global__ __launch_bounds__(kEnumThreads) void hkl_enum_kernel(Box box, Metric m, double dmin,
+                                                                int *__restrict__ work,
+                                                                int *__restrict__ hkl_out,
+                                                                int capacity) {
+  __shared__ int s_scan[kEnumThreads];
+  const int tid = threadIdx.x;
+  const long long first = static_cast<long long>(blockIdx.x) * kEnumPerBlock + tid * kEnumItems;
+  unsigned keep_bits = 0;
+  int n = 0;
+#pragma unroll
+  for (int j = 0; j < kEnumItems; ++j) {
+    const long long i = first + j;
+    if (i < box.total) {
+      int h, k, l;
+      hkl_index_to_hkl(i, box, h, k, l);
+      if (keep_hkl(h, k, l, m, dmin)) {
+        keep_bits |= 1u << j;
+        ++n;
+      }
+    }
+  }
+  if (kPass == 0) {
+    // block total
+    s_scan[tid] = n;
+    __syncthreads();
+    for (int off = kEnumThreads / 2; off > 0; off >>= 1) {
+      if (tid < off) s_scan[tid] += s_scan[tid + off];
+      __syncthreads();
+    }
+    if (tid == 0) work[blockIdx.x] = s_scan[0];
+    return;
+  }
+  // exclusive scan of n across the block (Hillis-Steele in LDS)
+  s_scan[tid] = n;
+  __syncthreads();
+  for (int off = 1; off < kEnumThreads; off <<= 1) {
+    int t = tid >= off ? s_scan[tid - off] : 0;
+    __syncthreads();
+    s_scan[tid] += t;
+    __syncthreads();
+  }
+  int pos = work[gridDim.x + blockIdx.x] + s_scan[tid] - n;
+#pragma unroll
+  for (int j = 0; j < kEnumItems; ++j) {
+    if (keep_bits & (1u << j)) {
+      if (pos < capacity) {
+        int h, k, l;
+        hkl_index_to_hkl(first + j, box, h, k, l);
+        reinterpret_cast<int4 *>(hkl_out)[pos] = make_int4(h, k, l, 0);
+      }
+      ++pos;
+    }
+  }
+}
+
+// Exclusive scan of per-block counts (one block), total to count_out.
+__global__ __launch_bounds__(1024) void hkl_scan_kernel(int *__restrict__ work, int nblocks,
+                                                        int *__restrict__ count_out) {
+  __shared__ int s[1024];
+  __shared__ int carry;
+  const int tid = threadIdx.x;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nblocks; base += 1024) {
+    const int i = base + tid;
+    const int v = i < nblocks ? work[i] : 0;
+    s[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int t = tid >= off ? s[tid - off] : 0;
+      __syncthreads();
+      s[tid] += t;
+      __syncthreads();
+    }
+    if (i < nblocks) work[nblocks + i] = carry + s[tid] - v;
+    __syncthreads();
+    if (tid == 1023) carry += s[1023];
+    __syncthreads();
+  }
+  if (tid == 0) *count_out = carry;
+}
+
+// ---------------------------------------------------------------------------
+// Structure factor
+// ---------------------------------------------------------------------------
+
+constexpr int kSfThreads = 256;
+
+struct SfArgs {
+  const int4 *hkl;
+  int n;
+  double rec[9];
+  const uint4 *xyz_fx;
+  const int *elem;
+  const float *occ;
+  const float *B;
+  int n_atoms;
+  const double *coeff;  // [n_elem][10]
+  int n_elem;
+  int scatter;
+  float4 *g4;
+  float *F2;
+  float2 *F;
+};
+
+// f(g^2) in FP64: the Lobato parameterisation sums terms of opposite sign that
+// are up to ~100x larger than the result (e.g. He: a2 = -62.0, a3 = +64.0), so
+// FP32 evaluation loses 2+ digits.  Once per element per reflection; cheap.
+__device__ __forceinline__ double atomic_f(const double *c, double g2, int scatter) {
+  if (scatter == DSX_SCATTER_NONE) return 1.0;
+  double f = 0.0;
+  if (scatter == DSX_SCATTER_LOBATO) {
+#pragma unroll
+    for (int t = 0; t < 5; ++t) {
+      const double bg = c[2 * t + 1] * g2;
+      const double d = 1.0 + bg;
+      f += c[2 * t] * (2.0 + bg) / (d * d);
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < 5; ++t) f += c[2 * t] * exp(-0.25 * c[2 * t + 1] * g2);
+  }
+  return f;
+}
+
+// One thread per reflection; atoms are staged through LDS in chunks.  The
+// atomic scattering factor is recomputed only when the element changes along
+// the atom list, so a host-side sort by element makes it once per element.
+__global__ __launch_bounds__(kSfThreads) void structure_factor_kernel(SfArgs a) {
+  __shared__ uint4 s_xyz[kSfThreads];
+  __shared__ float s_occ[kSfThreads];
+  __shared__ float s_B[kSfThreads];
+  __shared__ int s_el[kSfThreads];
+
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * kSfThreads + tid;
+  const bool valid = i < a.n;
+  int4 hkl = valid ? a.hkl[i] : make_int4(0, 0, 0, 0);
+
+  double gx = a.rec[0] * hkl.x + a.rec[1] * hkl.y + a.rec[2] * hkl.z;
+  double gy = a.rec[3] * hkl.x + a.rec[4] * hkl.y + a.rec[5] * hkl.z;
+  double gz = a.rec[6] * hkl.x + a.rec[7] * hkl.y + a.rec[8] * hkl.z;
+  const double g2d = gx * gx + gy * gy + gz * gz;
+  const float g2 = static_cast<float>(g2d);
+
+  const uint32_t uh = static_cast<uint32_t>(hkl.x), uk = static_cast<uint32_t>(hkl.y),
+                 ul = static_cast<uint32_t>(hkl.z);
+  const float mq = -0.25f * g2;
+  double Fr = 0.0, Fi = 0.0;  // FP64 accumulation: full rate on CDNA4
+  int cur_el = -1;
+  double f = 0.0;
+  for (int base = 0; base < a.n_atoms; base += kSfThreads) {
+    __syncthreads();
+    const int j = base + tid;
+    if (j < a.n_atoms) {
+      s_xyz[tid] = a.xyz_fx[j];
+      s_occ[tid] = a.occ[j];
+      s_B[tid] = a.B[j];
+      s_el[tid] = a.elem[j];
+    }
+    __syncthreads();
+    const int m = min(kSfThreads, a.n_atoms - base);
+    for (int jj = 0; jj < m; ++jj) {
+      const int el = min(max(s_el[jj], 0), a.n_elem - 1);
+      if (el != cur_el) {  // uniform across the block
+        f = atomic_f(a.coeff + 10 * el, g2d, a.scatter);
+        cur_el = el;
+      }
+      const uint4 x = s_xyz[jj];
+      // exact phase in turns mod 1: 32-bit wrap-around of h x + k y + l z
+      const uint32_t t = uh * x.x + uk * x.y + ul * x.z;
+      const float turns = static_cast<float>(static_cast<int32_t>(t)) * 2.3283064365386963e-10f;
+      float sn, cs;
+      sincospif(2.0f * turns, &sn, &cs);
+      const double amp = f * static_cast<double>(s_occ[jj] * expf(mq * s_B[jj]));
+      Fr = fma(amp, static_cast<double>(cs), Fr);
+      Fi = fma(amp, static_cast<double>(sn), Fi);
+    }
+  }
+  if (valid) {
+    a.g4[i] = make_float4(static_cast<float>(gx), static_cast<float>(gy), static_cast<float>(gz), g2);
+    a.F2[i] = static_cast<float>(Fr * Fr + Fi * Fi);
+    if (a.F) a.F[i] = make_float2(static_cast<float>(Fr), static_cast<float>(Fi));
+  }
+}
+
+}  // namespace dsx
+
+using namespace dsx;
+
+extern "C" size_t dsx_hkl_workspace_ints(const int highest[3]) {
+  const long long total = static_cast<long long>(2 * highest[0] + 1) * (2 * highest[1] + 1) *
+                          (2 * highest[2] + 1);
+  const long long nb = (total + kEnumPerBlock - 1) / kEnumPerBlock;
+  return static_cast<size_t>(2 * nb);
+}
+
+extern "C" int dsx_hkl_enumerate(const int highest[3], const double metric[6], double d_min,
+                                 int *hkl_d, int capacity, int *count_d, int *work_d,
+                                 void *stream) {
+  clear_error();
+  if (!highest || !metric || !count_d || !work_d || (capacity > 0 && !hkl_d)) {
+    set_error("dsx_hkl_enumerate: null argument");
+    return DSX_EINVAL;
+  }
+  if (highest[0] < 0 || highest[1] < 0 || highest[2] < 0 || !(d_min > 0.0)) {
+    set_error("dsx_hkl_enumerate: bad box {%d,%d,%d} or d_min %g", highest[0], highest[1],
+              highest[2], d_min);
+    return DSX_EINVAL;
+  }
+  Box b;
+  b.H = highest[0];
+  b.K = highest[1];
+  b.L = highest[2];
+  b.nk = 2 * b.K + 1;
+  b.nl = 2 * b.L + 1;
+  b.total = static_cast<long long>(2 * b.H + 1) * b.nk * b.nl;
+  if (b.total > (1ll << 31)) {
+    set_error("dsx_hkl_enumerate: box of %lld entries exceeds 2^31", b.total);
+    return DSX_EINVAL;
+  }
+  Metric m{metric[0], metric[1], metric[2], metric[3], metric[4], metric[5]};
+  const int nb = static_cast<int>((b.total + kEnumPerBlock - 1) / kEnumPerBlock);
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(hkl_enum_kernel<0>, dim3(nb), dim3(kEnumThreads), 0, s, b, m, d_min, work_d,
+                     hkl_d, capacity);
+  hipLaunchKernelGGL(hkl_scan_kernel, dim3(1), dim3(1024), 0, s, work_d, nb, count_d);
+  hipLaunchKernelGGL(hkl_enum_kernel<1>, dim3(nb), dim3(kEnumThreads), 0, s, b, m, d_min, work_d,
+                     hkl_d, capacity);
+  return check_launch("dsx_hkl_enumerate");
+}
+
+extern "C" int dsx_structure_factor(const int *hkl_d, int n, const double rec[9],
+                                    const uint32_t *xyz_fx_d, const int *elem_d,
+                                    const float *occ_d, const float *B_d, int n_atoms,
+                                    const double *coeff_d, int n_elem, int scatter, float *g4_d,
+                                    float *F2_d, float *F_d, void *stream) {
+  clear_error();
+  if (n < 0 || n_atoms < 0 || n_elem < 0 || !rec) {
+    set_error("dsx_structure_factor: bad sizes n=%d n_atoms=%d n_elem=%d", n, n_atoms, n_elem);
+    return DSX_EINVAL;
+  }
+  if (scatter != DSX_SCATTER_NONE && scatter != DSX_SCATTER_LOBATO && scatter != DSX_SCATTER_XTABLES) {
+    set_error("dsx_structure_factor: unknown scattering parameterisation %d", scatter);
+    return DSX_EINVAL;
+  }
+  if (n == 0) return DSX_OK;
+  if (n_atoms > 0 && n_elem == 0) {
+    set_error("dsx_structure_factor: atoms given but no element coefficients");
+    return DSX_EINVAL;
+  }
+  if (!hkl_d || !g4_d || !F2_d || (n_atoms > 0 && (!xyz_fx_d || !elem_d || !occ_d || !B_d)) ||
+      (n_elem > 0 && !coeff_d)) {
+    set_error("dsx_structure_factor: null device pointer");
+    return DSX_EINVAL;
+  }
+  SfArgs a;
+  a.hkl = reinterpret_cast<const int4 *>(hkl_d);
+  a.n = n;
+  for (int i = 0; i < 9; ++i) a.rec[i] = rec[i];
+  a.xyz_fx = reinterpret_cast<const uint4 *>(xyz_fx_d);
+  a.elem = elem_d;
+  a.occ = occ_d;
+  a.B = B_d;
+  a.n_atoms = n_atoms;
+  a.coeff = coeff_d;
+  a.n_elem = n_elem;
+  a.scatter = scatter;
+  a.g4 = reinterpret_cast<float4 *>(g4_d);
+  a.F2 = F2_d;
+  a.F = reinterpret_cast<float2 *>(F_d);
+  hipLaunchKernelGGL(structure_factor_kernel, dim3((n + kSfThreads - 1) / kSfThreads),
+                     dim3(kSfThreads), 0, as_stream(stream), a);
+  return check_launch("dsx_structure_factor");
+}

@@ -1,0 +1,328 @@
+"""Device pipeline: per-phase reflection tables, per-orientation spot lists and
+rendered patterns, all resident in HBM as torch tensors.
+
+This is the array-level layer under the drop-in ``SimulationGenerator`` /
+``Simulation2D`` classes; every computation runs in the HIP kernels of
+``libdiffsims_amd.so`` (no CPU path).
+
+Layout in HBM:
+  PhaseTable   hkl  int32 [N, 4]   {h, k, l, 0}, reference order, then the
+                                    direct-beam rows when with_direct_beam
+               g4   float32 [N, 4] {gx, gy, gz, |g|^2} (crystal frame, 1/A)
+               F2   float32 [N]    |F(hkl)|^2
+  SpotBatch    idx  int32 [n, cap]     row into the table
+               spots float32 [n, cap, 4] {x, y, z, I} lab frame
+               count/ncand int32 [n], maxI float32 [n]
+  patterns     float32 [n, H, W]
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import warnings
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from .crystallography import (PhaseArrays, element_of, highest_hkl, lattice_setup,
+                              reciprocal_metric)
+from .scattering import electron_wavelength, table
+
+SHAPE_MODEL_IDS = {
+    "linear": _lib.SF_LINEAR,
+    "lorentzian": _lib.SF_LORENTZIAN,
+    "sinc": _lib.SF_SINC,
+    "sin2c": _lib.SF_SIN2C,
+    "atanc": _lib.SF_ATANC,
+    "binary": _lib.SF_BINARY,
+}
+
+SCATTER_IDS = {None: _lib.SCATTER_NONE, "lobato": _lib.SCATTER_LOBATO,
+               "xtables": _lib.SCATTER_XTABLES}
+
+
+def _device(device=None) -> torch.device:
+    _lib.load(require_gpu=True)
+    if device is None:
+        return torch.device("cuda", torch.cuda.current_device())
+    d = torch.device(device)
+    if d.type != "cuda":
+        raise ValueError(f"diffsims_amd runs on ROCm GPUs only, got device {d}")
+    return d
+
+
+@dataclass
+class PhaseTable:
+    """Device-resident reflection table of one phase."""
+    hkl: torch.Tensor
+    g4: torch.Tensor
+    F2: torch.Tensor
+    F: torch.Tensor
+    n_box: int              # rows from the hkl box (before direct-beam rows)
+    with_direct_beam: bool
+    g_max: float
+    rec: np.ndarray         # FP64 reciprocal basis (columns a*, b*, c*)
+    base: np.ndarray
+    name: str = ""
+
+    @property
+    def n(self) -> int:
+        return int(self.hkl.shape[0])
+
+    def hkl_host(self) -> np.ndarray:
+        return self.hkl[:, :3].cpu().numpy()
+
+
+def _fixed_point(frac: np.ndarray) -> np.ndarray:
+    """Fractional coordinates as uint32 turns: round(frac * 2^32) mod 2^32."""
+    f = np.asarray(frac, dtype=np.float64)
+    f = f - np.floor(f)
+    return (np.round(f * 4294967296.0).astype(np.uint64) & 0xFFFFFFFF).astype(np.uint32)
+
+
+def build_phase_table(pa: PhaseArrays, reciprocal_radius: float, with_direct_beam: bool,
+                      scattering_params: Optional[str] = "lobato", device=None,
+                      warn_missing: bool = True) -> PhaseTable:
+    """hkl set + g vectors + |F|^2 of a phase on the GPU.
+
+    Follows DiffractingVector.from_min_dspacing(phase, 1/reciprocal_radius,
+    include_zero_vector=with_direct_beam) and the structure-factor part of
+    get_kinematical_intensities (reciprocal_lattice_vector.py:1077-1142,
+    sim_utils.py:256-354); the extra (000) row the reference stacks on in
+    get_intersecting_reflections (simulation_generator.py:351-353) is included.
+    """
+    dev = _device(device)
+    L = _lib.load()
+    if not reciprocal_radius > 0:
+        raise ValueError("reciprocal_radius must be positive")
+    d_min = 1.0 / reciprocal_radius
+    base, rec = lattice_setup(pa.abc, pa.angles)
+    metric = reciprocal_metric(rec)
+    H = highest_hkl(rec, d_min)
+    h3 = (ctypes.c_int * 3)(*[int(v) for v in H])
+    m6 = (ctypes.c_double * 6)(*metric.tolist())
+    stream = _lib.stream_handle(dev)
+    work = torch.empty(int(L.dsx_hkl_workspace_ints(h3)), dtype=torch.int32, device=dev)
+    count = torch.zeros(1, dtype=torch.int32, device=dev)
+    _lib.check(L.dsx_hkl_enumerate(h3, m6, d_min, None, 0, _lib.ptr(count), _lib.ptr(work), stream),
+               "dsx_hkl_enumerate")
+    n_box = int(count.item())
+    n_extra = 2 if with_direct_beam else 0
+    hkl = torch.zeros((n_box + n_extra, 4), dtype=torch.int32, device=dev)
+    _lib.check(L.dsx_hkl_enumerate(h3, m6, d_min, _lib.ptr(hkl), n_box, _lib.ptr(count),
+                                   _lib.ptr(work), stream), "dsx_hkl_enumerate")
+
+    # atoms sorted by element so the kernel evaluates f once per element
+    n_atoms = len(pa.symbols)
+    elements = [element_of(s) for s in pa.symbols]
+    if scattering_params is not None:
+        tab = table(scattering_params)
+    else:
+        tab = {}
+    uniq = sorted(set(elements))
+    coeff = np.zeros((max(len(uniq), 1), 10), dtype=np.float64)
+    for i, el in enumerate(uniq):
+        if scattering_params is None:
+            continue
+        if el in tab:
+            coeff[i] = tab[el]
+        elif warn_missing:
+            sym = pa.symbols[elements.index(el)]
+            warnings.warn(f"Element {el} from atom type symbol {sym} not found in scattering "
+                          "parameter library.", UserWarning)
+    el_idx = np.array([uniq.index(e) for e in elements], dtype=np.int32)
+    order = np.argsort(el_idx, kind="stable")
+    xyz_fx = np.zeros((max(n_atoms, 1), 4), dtype=np.uint32)
+    if n_atoms:
+        xyz_fx[:n_atoms, :3] = _fixed_point(pa.xyz[order])
+    occ = np.asarray(pa.occ, dtype=np.float32)[order] if n_atoms else np.zeros(1, np.float32)
+    B = np.asarray(pa.B, dtype=np.float32)[order] if n_atoms else np.zeros(1, np.float32)
+    el_sorted = el_idx[order] if n_atoms else np.zeros(1, np.int32)
+
+    t_xyz = torch.from_numpy(xyz_fx.view(np.int32)).to(dev)
+    t_el = torch.from_numpy(np.ascontiguousarray(el_sorted)).to(dev)
+    t_occ = torch.from_numpy(np.ascontiguousarray(occ)).to(dev)
+    t_B = torch.from_numpy(np.ascontiguousarray(B)).to(dev)
+    t_coeff = torch.from_numpy(coeff).to(dev)
+    N = n_box + n_extra
+    g4 = torch.empty((N, 4), dtype=torch.float32, device=dev)
+    F2 = torch.empty(N, dtype=torch.float32, device=dev)
+    F = torch.empty((N, 2), dtype=torch.float32, device=dev)
+    rec9 = (ctypes.c_double * 9)(*rec.reshape(-1).tolist())
+    _lib.check(L.dsx_structure_factor(_lib.ptr(hkl), N, rec9, _lib.ptr(t_xyz), _lib.ptr(t_el),
+                                      _lib.ptr(t_occ), _lib.ptr(t_B), n_atoms, _lib.ptr(t_coeff),
+                                      len(uniq), SCATTER_IDS[scattering_params], _lib.ptr(g4),
+                                      _lib.ptr(F2), _lib.ptr(F), stream), "dsx_structure_factor")
+    g_max = float(torch.sqrt(g4[:, 3].max()).item()) if N else 0.0
+    return PhaseTable(hkl, g4, F2, F, n_box, with_direct_beam, g_max, rec, base, pa.name)
+
+
+@dataclass
+class SimSettings:
+    """Physical settings of SimulationGenerator + calculate_diffraction2d."""
+    accelerating_voltage: float = 200.0
+    max_excitation_error: float = 1e-2
+    shape_factor_width: Optional[float] = None
+    shape_model: str = "lorentzian"
+    minima_number: float = 5.0
+    precession_angle: float = 0.0        # degrees
+    approximate_precession: bool = True
+    minimum_intensity: float = 1e-20
+
+    def params(self, g_max: float, rec: np.ndarray) -> _lib.SimParams:
+        lam = electron_wavelength(self.accelerating_voltage)
+        if lam == 0:
+            raise ValueError("infinite accelerating voltage is not supported by the simulation")
+        width = self.shape_factor_width if self.shape_factor_width is not None else self.max_excitation_error
+        phi = abs(self.precession_angle)
+        if phi > 0:
+            if not self.approximate_precession:
+                raise NotImplementedError("full precession integration is not implemented on the GPU")
+            model = _lib.SF_LORENTZIAN_PRECESSION
+        else:
+            if self.shape_model not in SHAPE_MODEL_IDS:
+                raise NotImplementedError(f"shape factor model {self.shape_model!r} has no GPU kernel")
+            model = SHAPE_MODEL_IDS[self.shape_model]
+        p = _lib.SimParams()
+        p.k = 1.0 / lam
+        p.max_excitation_error = self.max_excitation_error
+        p.shape_width = width
+        p.shape_model = model
+        p.minima_number = self.minima_number
+        p.precession_angle = math.radians(phi)
+        p.minimum_intensity = self.minimum_intensity
+        p.g_max = g_max
+        for i, v in enumerate(np.asarray(rec, dtype=np.float64).reshape(-1)):
+            p.rec[i] = float(v)
+        return p
+
+
+@dataclass
+class SpotBatch:
+    """Per-orientation spot lists on the device (see module docstring)."""
+    table: PhaseTable
+    idx: torch.Tensor
+    spots: torch.Tensor
+    count: torch.Tensor
+    ncand: torch.Tensor
+    maxI: torch.Tensor
+    cap: int
+
+    @property
+    def n(self) -> int:
+        return int(self.count.shape[0])
+
+
+def rotation_matrices_to_device(matrices: np.ndarray, device=None) -> torch.Tensor:
+    """(n, 3, 3) orix to_matrix() -> float64 [n, 9] rows of M^T on the device."""
+    M = np.asarray(matrices, dtype=np.float64).reshape(-1, 3, 3)
+    Rt = np.ascontiguousarray(np.transpose(M, (0, 2, 1)).reshape(-1, 9), dtype=np.float64)
+    return torch.from_numpy(Rt).to(_device(device))
+
+
+def expected_spots(tab: PhaseTable, settings: SimSettings, reciprocal_radius: float) -> int:
+    """Mean number of intersecting reflections for a random orientation: the
+    table fraction inside a shell of half-thickness max_excitation_error (plus
+    r sin(phi) with precession) around the Ewald sphere."""
+    me = settings.max_excitation_error + tab.g_max * math.sin(math.radians(abs(settings.precession_angle)))
+    rr = max(reciprocal_radius, 1e-6)
+    return int(math.ceil(tab.n * min(1.0, 1.5 * me / rr)))
+
+
+def simulate(tab: PhaseTable, Rt: torch.Tensor, settings: SimSettings, reciprocal_radius: float,
+             cap: Optional[int] = None, stream=None) -> SpotBatch:
+    """Spot lists of every orientation (rows of Rt) against one phase table."""
+    L = _lib.load(require_gpu=True)
+    dev = Rt.device
+    n = int(Rt.shape[0])
+    if Rt.dtype != torch.float64 or Rt.dim() != 2 or Rt.shape[1] != 9 or not Rt.is_contiguous():
+        raise ValueError("Rt must be a contiguous float64 [n, 9] tensor")
+    params = settings.params(tab.g_max, tab.rec)
+    if cap is None:
+        cap = min(max(tab.n, 1), 2 * expected_spots(tab, settings, reciprocal_radius) + 128)
+    stream = _lib.stream_handle(dev) if stream is None else stream
+    for attempt in range(2):
+        idx = torch.empty((n, cap), dtype=torch.int32, device=dev)
+        spots = torch.empty((n, cap, 4), dtype=torch.float32, device=dev)
+        count = torch.empty(n, dtype=torch.int32, device=dev)
+        ncand = torch.empty(n, dtype=torch.int32, device=dev)
+        maxI = torch.empty(n, dtype=torch.float32, device=dev)
+        _lib.check(L.dsx_simulate(_lib.ptr(tab.hkl), _lib.ptr(tab.g4), _lib.ptr(tab.F2), tab.n,
+                                  _lib.ptr(Rt), n,
+                                  ctypes.byref(params), cap, _lib.ptr(idx), _lib.ptr(spots),
+                                  _lib.ptr(count), _lib.ptr(ncand), _lib.ptr(maxI), stream),
+                   "dsx_simulate")
+        need = int(ncand.max().item()) if n else 0
+        if need <= cap:
+            break
+        cap = need
+    return SpotBatch(tab, idx, spots, count, ncand, maxI, cap)
+
+
+def gaussian_taps(sigma: float):
+    """Half of scipy.ndimage's normalised Gaussian kernel (truncate 4): taps[0..R]."""
+    R = int(4.0 * float(sigma) + 0.5)
+    x = np.arange(-R, R + 1)
+    phi = np.exp(-0.5 / (float(sigma) ** 2) * x ** 2)
+    phi = phi / phi.sum()
+    return R, phi[R:]
+
+
+def render_params(shape, sigma, calibration, direct_beam_position=None, in_plane_angle=0.0,
+                  mirrored=False, normalize=True, fast=True, clip_threshold=1.0) -> _lib.RenderParams:
+    H, W = int(shape[0]), int(shape[1])
+    if direct_beam_position is None:
+        direct_beam_position = (W // 2, H // 2) if fast else ((W - 1) / 2, (H - 1) / 2)
+    p = _lib.RenderParams()
+    p.height, p.width = H, W
+    p.calibration = calibration
+    p.center_x, p.center_y = float(direct_beam_position[0]), float(direct_beam_position[1])
+    a = math.radians(in_plane_angle)
+    p.cos_angle, p.sin_angle = (1.0, 0.0) if in_plane_angle == 0 else (math.cos(a), math.sin(a))
+    p.mirrored = int(bool(mirrored))
+    p.normalize = int(bool(normalize))
+    p.sigma = float(sigma)
+    p.clip_threshold = float(clip_threshold)
+    if fast:
+        R, taps = gaussian_taps(sigma)
+        if R > _lib.MAX_TAP_RADIUS:
+            raise NotImplementedError(f"Gaussian radius {R} (sigma={sigma}) exceeds {_lib.MAX_TAP_RADIUS}")
+        p.radius = R
+        for i, t in enumerate(taps):
+            p.taps[i] = float(t)
+    return p
+
+
+def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigma, calibration,
+                 direct_beam_position=None, in_plane_angle=0.0, mirrored=False, normalize=True,
+                 fast=True, clip_threshold=1.0, out: Optional[torch.Tensor] = None,
+                 stream=None) -> torch.Tensor:
+    """Render n spot lists (float4 {x, y, z, I}, stride cap) to [n, H, W] float32."""
+    L = _lib.load(require_gpu=True)
+    dev = spots.device
+    n = int(count.shape[0])
+    H, W = int(shape[0]), int(shape[1])
+    if out is None:
+        out = torch.empty((n, H, W), dtype=torch.float32, device=dev)
+    elif out.shape != (n, H, W) or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError("out must be a contiguous float32 [n, H, W] tensor")
+    status = torch.zeros(n, dtype=torch.int32, device=dev)
+    p = render_params(shape, sigma, calibration, direct_beam_position, in_plane_angle, mirrored,
+                      normalize, fast, clip_threshold)
+    stream = _lib.stream_handle(dev) if stream is None else stream
+    fn = L.dsx_render_fast if fast else L.dsx_render_subpixel
+    _lib.check(fn(_lib.ptr(spots), _lib.ptr(count), cap, n, ctypes.byref(p), _lib.ptr(out),
+                  _lib.ptr(status), stream), "dsx_render")
+    if n and int(status.max().item()) != 0:
+        bad = int(torch.nonzero(status)[0].item())
+        raise RuntimeError(f"pattern {bad} has more than {L.dsx_render_max_spots()} distinct "
+                           "in-frame spots; the renderer cannot hold it")
+    return out
+
+
+def render(batch: SpotBatch, shape, sigma, calibration, **kw) -> torch.Tensor:
+    return render_spots(batch.spots, batch.count, batch.cap, shape, sigma, calibration, **kw)

@@ -1,0 +1,192 @@
+/*
+ * diffsims_amd — C ABI of the MI355X (gfx950) kinematical diffraction path.
+ *
+ * Every pointer argument named *_d is a device pointer (HIP global memory,
+ * typically a torch tensor's data_ptr()); every other pointer is host memory.
+ * `stream` is a hipStream_t passed as void* (NULL = the default stream).  All
+ * launches are asynchronous on that stream; the library never allocates, frees
+ * or synchronises unless an entry point says so.  Functions return 0 on
+ * success or a negative DSX_E* code; dsx_last_error() then describes the
+ * failure (thread-local string).
+ *
+ * The reference (diffsims 0.7.0) is pure Python; each entry point names the
+ * reference function whose work it replaces.  A Python caller binds them with
+ * ctypes (see diffsims_amd/_lib.py).
+ */
+#ifndef DIFFSIMS_AMD_H
+#define DIFFSIMS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSX_OK 0
+#define DSX_EINVAL (-1)    /* bad argument (shape, size, enum)            */
+#define DSX_ELAUNCH (-2)   /* HIP launch or runtime error                 */
+#define DSX_ECAPACITY (-3) /* caller buffer too small (see the entry)     */
+
+/* Shape-factor models (diffsims/utils/shape_factor_models.py). */
+#define DSX_SF_LINEAR 0     /* :52-73                                      */
+#define DSX_SF_LORENTZIAN 1 /* :154-180                                    */
+#define DSX_SF_SINC 2       /* :76-101                                     */
+#define DSX_SF_SIN2C 3      /* :104-123                                    */
+#define DSX_SF_ATANC 4      /* :126-151                                    */
+#define DSX_SF_BINARY 5     /* :33-49                                      */
+/* precession forms (simulation_generator.py:396-411) */
+#define DSX_SF_LORENTZIAN_PRECESSION 6 /* shape_factor_models.py:183-219   */
+
+/* Scattering-factor parameterisations (sim_utils.py:227-253, :281-286). */
+#define DSX_SCATTER_NONE 0    /* f = 1                                     */
+#define DSX_SCATTER_LOBATO 1  /* sum a (2 + b g^2) / (1 + b g^2)^2          */
+#define DSX_SCATTER_XTABLES 2 /* sum a exp(-b g^2 / 4)                      */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int dsx_version(void);
+
+/* Description of the last failure on the calling thread ("" if none). */
+const char *dsx_last_error(void);
+
+/*
+ * hkl enumeration — replaces ReciprocalLatticeVector.from_min_dspacing
+ * (diffsims/crystallography/reciprocal_lattice_vector.py:1077-1142) minus the
+ * zero-vector handling, which the caller appends.
+ *
+ * Enumerates the box [-H,H]x[-K,K]x[-L,L] (highest[] = {H,K,L}) in descending
+ * lexicographic (h,k,l) order, keeps 1/sqrt(q) >= d_min with
+ * q = h^2 G00 + k^2 G11 + l^2 G22 + 2 (hk G01 + hl G02 + kl G12) evaluated in
+ * IEEE FP64 in exactly that order, drops (000), and writes the survivors,
+ * order preserved, as int4 {h,k,l,0} to hkl_d.
+ *   metric[6] = {G00, G11, G22, G01, G02, G12} (host, FP64 reciprocal metric)
+ *   hkl_d     : device int32 [capacity][4]
+ *   count_d   : device int32 [1], receives the number of survivors
+ *   work_d    : device int32 scratch of dsx_hkl_workspace_ints(highest) ints
+ * If the number of survivors exceeds capacity only `capacity` rows are
+ * written; *count_d still receives the full count.
+ */
+size_t dsx_hkl_workspace_ints(const int highest[3]);
+int dsx_hkl_enumerate(const int highest[3], const double metric[6], double d_min,
+                      int *hkl_d, int capacity, int *count_d, int *work_d,
+                      void *stream);
+
+/*
+ * Per-reflection tables — replaces the per-orientation structure-factor work of
+ * get_kinematical_intensities / _get_kinematical_structure_factor
+ * (diffsims/utils/sim_utils.py:256-354), computed once per phase.
+ *
+ * For each of n rows of hkl_d (int4 {h,k,l,_}):
+ *   g (crystal Cartesian, 1/A) = rec * hkl in FP64 (rec row-major 3x3 host),
+ *   g4_d[i] = {gx, gy, gz, |g|^2} as FP32,
+ *   F = sum_j f_e(j)(|g|^2) occ_j exp(-B_j |g|^2 / 4) exp(2 pi i hkl.x_j) in
+ *       FP32 with FP32 accumulation; the phase hkl.x_j is reduced exactly with
+ *       32-bit fixed-point fractional positions (xyz_fx_d, uint32 [n_atoms][4],
+ *       value = round(frac * 2^32) mod 2^32),
+ *   F2_d[i] = |F|^2,  F_d[i] = {Re F, Im F} (F_d may be NULL).
+ * f_e is evaluated and F accumulated in FP64 (the Lobato sums cancel by up to
+ * two digits); the per-atom sincos and Debye-Waller exp are FP32.
+ * Atoms: elem_d int32 [n_atoms] (row into coeff_d), occ_d, B_d float
+ * [n_atoms]; coeff_d double [n_elem][10] = {a1,b1,...,a5,b5};
+ * scatter = DSX_SCATTER_*.
+ */
+int dsx_structure_factor(const int *hkl_d, int n, const double rec[9],
+                         const uint32_t *xyz_fx_d, const int *elem_d,
+                         const float *occ_d, const float *B_d, int n_atoms,
+                         const double *coeff_d, int n_elem, int scatter,
+                         float *g4_d, float *F2_d, float *F_d, void *stream);
+
+/*
+ * Simulation parameters (SimulationGenerator.__init__ / calculate_diffraction2d
+ * arguments, diffsims/generators/simulation_generator.py:74-146).
+ */
+typedef struct dsx_sim_params {
+  double k;                 /* 1 / wavelength (1/A)                          */
+  double max_excitation_error;
+  double shape_width;       /* shape_factor_width or max_excitation_error     */
+  int shape_model;          /* DSX_SF_*                                       */
+  double minima_number;     /* sinc / sin2c / atanc parameter (default 5)     */
+  double precession_angle;  /* radians; 0 = no precession                     */
+  double minimum_intensity; /* relative cut (default 1e-20)                   */
+  double g_max;             /* max |g| over the table (pretest bound)         */
+  double rec[9];            /* FP64 reciprocal basis, row-major, g = rec hkl  */
+} dsx_sim_params;
+
+/*
+ * Per-orientation excitation error, shape factor, relative cut and ordered
+ * compaction — replaces the orientation loop of calculate_diffraction2d with
+ * get_intersecting_reflections (simulation_generator.py:211-241, :319-412).
+ *
+ *   hkl_d, g4_d, F2_d : tables (n_refl rows; int4 hkl as enumerated, with
+ *                the direct-beam rows appended by the caller when
+ *                with_direct_beam; g4/F2 from dsx_structure_factor)
+ *   Rt_d       : double [n_orient][9] row-major R = M^T (g_lab = R g), M the
+ *                orix to_matrix() of each orientation
+ * The FP32 g4 table only drives a conservative pretest; every candidate is
+ * re-evaluated in FP64 from its integer hkl, rec and Rt, so the excitation
+ * error, the |s| < max_excitation_error test and the shape factor are FP64.
+ * Outputs, per orientation o, in the reference's reflection order:
+ *   idx_d[o*cap + j]   int32 row index into the tables
+ *   spot_d[o*cap + j]  float4 {x, y, z, I} lab frame (1/A) and intensity
+ *   count_d[o]         survivors after the relative cut (<= cap written)
+ *   ncand_d[o]         intersecting reflections before the cut (may exceed cap)
+ *   maxI_d[o]          max intensity over the intersecting set
+ * An orientation whose intersecting set exceeds cap writes cap rows and sets
+ * ncand_d[o] > cap; the caller re-runs it with a larger cap.
+ */
+int dsx_simulate(const int *hkl_d, const float *g4_d, const float *F2_d, int n_refl,
+                 const double *Rt_d, int n_orient, const dsx_sim_params *params,
+                 int cap, int *idx_d, float *spot_d, int *count_d, int *ncand_d,
+                 float *maxI_d, void *stream);
+
+/*
+ * Render parameters (Simulation2D.get_diffraction_pattern,
+ * diffsims/simulations/simulation2d.py:357-442).
+ */
+typedef struct dsx_render_params {
+  int height, width;
+  float calibration;       /* 1/A per pixel                                   */
+  float center_x, center_y;/* direct-beam pixel position                      */
+  float cos_angle, sin_angle; /* in-plane rotation                            */
+  int mirrored;
+  int normalize;
+  float sigma;             /* Gaussian sigma in pixels                        */
+  float clip_threshold;    /* sub-pixel mode only                             */
+  int radius;              /* fast mode: int(4 sigma + 0.5)                   */
+  float taps[65];          /* fast mode: taps[0..R] = w(0..R) of scipy's
+                              normalised Gaussian kernel, R <= 64        */
+} dsx_render_params;
+
+/*
+ * fast=True rendering of n patterns — replaces get_diffraction_pattern with
+ * get_pattern_from_pixel_coordinates_and_intensities' integer path
+ * (simulation2d.py:405-442, detector_functions.py:293-300): truncate to integer
+ * pixels, last write wins per pixel, separable Gaussian with reflect
+ * boundaries and truncate 4 (scipy.ndimage.gaussian_filter), divide by max.
+ *   spot_d/count_d : as written by dsx_simulate (stride cap per pattern)
+ *   out_d          : float [n][height][width]
+ *   status_d       : device int32 [n] (may be NULL); 0 = rendered, 1 = the
+ *                    pattern has more than dsx_render_max_spots() distinct
+ *                    in-frame spots and was not rendered completely.
+ */
+int dsx_render_max_spots(void);
+int dsx_render_fast(const float *spot_d, const int *count_d, int cap, int n,
+                    const dsx_render_params *params, float *out_d, int *status_d,
+                    void *stream);
+
+/*
+ * fast=False rendering — replaces _subpixel_gaussian
+ * (detector_functions.py:314-359): per spot, the clipped box of
+ * I/(2 pi s^2) exp(-d^2 / (2 s^2)) (skipped when I/(2 pi s^2) < clip_threshold),
+ * centre ((W-1)/2, (H-1)/2) unless given, optional normalisation.  Arguments
+ * as for dsx_render_fast.
+ */
+int dsx_render_subpixel(const float *spot_d, const int *count_d, int cap, int n,
+                        const dsx_render_params *params, float *out_d,
+                        int *status_d, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIFFSIMS_AMD_H */

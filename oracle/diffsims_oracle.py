@@ -1,0 +1,719 @@
+"""FP64 CPU oracle for the kinematical 2-D diffraction-pattern path.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: it may be imported by
+``tests/``, by ``__graft_entry__.smoke()`` and by the ``cpu_baseline`` leg of
+``bench.py`` and by nothing else.  The product (``diffsims_amd``) never imports,
+calls or falls back to it.
+
+It is a plain NumPy/SciPy restatement of the algorithm of diffsims 0.7.0
+(pure Python, GPL-3.0; its code is not copied here).  Every function cites the
+reference ``file:line`` it follows.  The reference itself cannot be imported in
+this environment (orix, diffpy.structure, numba and transforms3d are absent), so
+the restatement is pinned by the reference's own golden data and known-answer
+tests, by closed forms and by brute force; the tests in ``tests/test_oracle*.py``
+hold those checks.
+
+Parity pins (what fixes each function):
+  electron_wavelength        λ KATs (tests/utils/test_sim_utils.py:49-60)
+  lattice_basis / rnorm      ferrite g-spacings (test_reciprocal_lattice_vector.py:157-187),
+                             golden old_simulation.npy (hexagonal frame)
+  highest_hkl / hkl_box      ferrite 18/92/750, SiC box, Al 798/256 + order KATs
+  bunge_matrix / quat*       golden old_simulation.npy; closed-form orthogonality;
+                             quaternion<->matrix round trip
+  structure factor / f(g)    Ni 43.0979 KAT, unknown-element KAT, brute-force F,
+                             Friedel |F(g)| = |F(-g)|, FCC/diamond extinctions
+  intersecting / shape       Si 70-spot and precession 250-spot KATs, linear KAT,
+                             closed forms, scipy.integrate.quad for the full
+                             precession integral
+  render_fast                golden old_simulation.npy (uses scipy.ndimage, as the
+                             reference does), total-intensity KAT
+  render_subpixel            detector KATs (test_detector_functions.py:160-306),
+                             fast-vs-slow equality KAT (test_simulations2d.py:133-180)
+Parity unpinned (no reference test and no closed form fixes it):
+  * the order of the Cartesian sort for oblique cells when rounding could tie
+    (exact arithmetic gives descending lexicographic (h,k,l); see hkl_box);
+  * orix's Euler/quaternion branch handling beyond the golden case;
+  * the d == d_min equality case of the hkl box;
+  * the custom-callable shape-model count (test_custom_shape_func expects 52).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+# --------------------------------------------------------------------------
+# a1: wavelength  (diffsims/utils/sim_utils.py:57-79)
+# --------------------------------------------------------------------------
+
+
+def electron_wavelength(kv) -> float:
+    """Relativistic electron wavelength in Angstrom for a voltage in kV.
+
+    lambda = h / sqrt(2 m_e e E (1 + e E / (2 m_e c^2))) * 1e10, E in volts,
+    CODATA constants from scipy.constants (sim_utils.py:57-79).
+    """
+    from scipy.constants import c, e, h, m_e
+
+    if kv in (np.inf, "inf"):
+        return 0.0
+    E = kv * 1e3
+    return h / math.sqrt(2 * m_e * e * E * (1 + (e / (2 * m_e * c * c)) * E)) * 1e10
+
+
+# --------------------------------------------------------------------------
+# Lattice in the orix frame (x || a, y in the ab-plane, z || c*)
+# --------------------------------------------------------------------------
+
+
+def cosd(x: float) -> float:
+    """cos of an angle in degrees, exact where the cosine is 0, +-1/2 or +-1."""
+    r = math.fmod(float(x), 360.0)
+    if r < 0:
+        r += 360.0
+    exact = {0.0: 1.0, 60.0: 0.5, 90.0: 0.0, 120.0: -0.5, 180.0: -1.0,
+             240.0: -0.5, 270.0: 0.0, 300.0: 0.5}
+    if r in exact:
+        return exact[r]
+    return math.cos(math.radians(x))
+
+
+def sind(x: float) -> float:
+    """sin of an angle in degrees, exact where the sine is 0, +-1/2 or +-1."""
+    return cosd(90.0 - float(x))
+
+
+def lattice_basis(a, b, c, alpha, beta, gamma):
+    """Direct basis rows (a, b, c) and reciprocal basis in the orix frame.
+
+    The reference aligns every structure with x || a and z || c* (stated at
+    diffsims/tests/generators/test_simulation_generator.py:311; undone for atom
+    positions at diffsims/utils/sim_utils.py:290-291).  The basis is lower
+    triangular in that frame; its inverse is written in closed form so that the
+    structurally-zero entries are exact zeros.
+
+    Returns (base, rec) with ``base`` rows = direct vectors in Cartesian
+    Angstrom and ``rec`` = inv(base), whose *columns* are a*, b*, c*; the
+    Cartesian reciprocal vector of integer indices hkl is ``hkl @ rec.T``.
+    """
+    ca, cb, cg = cosd(alpha), cosd(beta), cosd(gamma)
+    sg = sind(gamma)
+    a21, a22 = b * cg, b * sg
+    a31 = c * cb
+    a32 = c * (ca - cb * cg) / sg
+    a33 = c * math.sqrt(1.0 - cb * cb - ((ca - cb * cg) / sg) ** 2)
+    base = np.array([[a, 0.0, 0.0], [a21, a22, 0.0], [a31, a32, a33]])
+    rec = np.zeros((3, 3))
+    rec[0, 0] = 1.0 / a
+    rec[1, 1] = 1.0 / a22
+    rec[2, 2] = 1.0 / a33
+    rec[1, 0] = -a21 / (a * a22)
+    rec[2, 1] = -a32 / (a22 * a33)
+    rec[2, 0] = (a21 * a32 - a22 * a31) / (a * a22 * a33)
+    return base, rec
+
+
+def reciprocal_metric(rec: np.ndarray) -> np.ndarray:
+    """G* = (a*, b*, c*) Gram matrix; 1/d^2 = hkl G* hkl."""
+    return rec.T @ rec
+
+
+def rnorm(hkl: np.ndarray, rec: np.ndarray) -> np.ndarray:
+    """|g| of integer/float indices (diffpy Lattice.rnorm, used at
+    reciprocal_lattice_vector.py:440 and :1135).
+
+    Evaluated as the explicit quadratic form
+    q = h^2 G00 + k^2 G11 + l^2 G22 + 2 (h k G01 + h l G02 + k l G12),
+    in this summation order, so the GPU enumeration kernel can reproduce it
+    bit for bit.
+    """
+    G = reciprocal_metric(rec)
+    h = np.asarray(hkl, dtype=np.float64)
+    h0, h1, h2 = h[..., 0], h[..., 1], h[..., 2]
+    q = h0 * h0 * G[0, 0]
+    q = q + h1 * h1 * G[1, 1]
+    q = q + h2 * h2 * G[2, 2]
+    cross = h0 * h1 * G[0, 1]
+    cross = cross + h0 * h2 * G[0, 2]
+    cross = cross + h1 * h2 * G[1, 2]
+    q = q + 2.0 * cross
+    return np.sqrt(q)
+
+
+# --------------------------------------------------------------------------
+# a2: hkl box rule  (reciprocal_lattice_vector.py:1077-1142, get_hkl.py:49-57)
+# --------------------------------------------------------------------------
+
+
+def highest_hkl(rec: np.ndarray, d_min: float) -> np.ndarray:
+    """Per-axis box bound: the largest n with d(n e_i) > d_min.
+
+    Follows the bound loop of diffsims/crystallography/get_hkl.py:49-57 (which
+    returns one more than this; orix's variant used at
+    reciprocal_lattice_vector.py:1131 subtracts it again, as
+    diffsims/tests/crystallography/test_get_hkl.py:52-64 shows).
+    """
+    out = np.zeros(3, dtype=np.int64)
+    for i in range(3):
+        e = np.zeros(3)
+        d = d_min + 1.0
+        while d > d_min:
+            e[i] += 1
+            d = 1.0 / rnorm(e, rec)
+        out[i] = int(e[i]) - 1
+    return out
+
+
+def hkl_box(rec: np.ndarray, d_min: float, include_zero: bool = False) -> np.ndarray:
+    """Integer hkl of ``from_min_dspacing`` in the reference's order.
+
+    Box [-H,H]x[-K,K]x[-L,L] (H from :func:`highest_hkl`), keep
+    1/rnorm >= d_min, drop (000), order as orix ``Vector3d.unique`` leaves it
+    (descending lexicographic on Cartesian x, y, z;
+    reciprocal_lattice_vector.py:1131-1141, :1509-1563).  In the orix frame
+    g_x = h/a, g_y depends on (h, k) increasing in k, g_z increasing in l, so
+    that order is descending lexicographic (h, k, l) in exact arithmetic; this
+    function uses the integer order.  (000) is appended last when
+    ``include_zero`` (:1139-1141).
+    """
+    H = highest_hkl(rec, d_min)
+    hs = np.arange(H[0], -H[0] - 1, -1)
+    ks = np.arange(H[1], -H[1] - 1, -1)
+    ls = np.arange(H[2], -H[2] - 1, -1)
+    hkl = np.stack(np.meshgrid(hs, ks, ls, indexing="ij"), axis=-1).reshape(-1, 3)
+    with np.errstate(divide="ignore"):
+        d = 1.0 / rnorm(hkl, rec)
+    keep = (d >= d_min) & np.any(hkl != 0, axis=1)
+    hkl = hkl[keep]
+    if include_zero:
+        hkl = np.vstack([hkl, np.zeros((1, 3), dtype=hkl.dtype)])
+    return hkl
+
+
+def g_cartesian(hkl: np.ndarray, rec: np.ndarray) -> np.ndarray:
+    """Cartesian reciprocal vectors (1/Angstrom) of indices hkl."""
+    return np.asarray(hkl, dtype=np.float64) @ rec.T
+
+
+# --------------------------------------------------------------------------
+# a3: rotations  (_diffracting_vector.py:127-161; orix Rotation conventions)
+# --------------------------------------------------------------------------
+
+
+def bunge_matrix(euler: np.ndarray) -> np.ndarray:
+    """Passive Bunge (ZXZ) matrices M(phi1, Phi, phi2), shape (..., 3, 3).
+
+    The reference rotates vectors with ``~rotation * miller``
+    (_diffracting_vector.py:160), i.e. g_lab = M^T g_crystal with M =
+    ``Rotation.to_matrix()``; pinned by the golden pattern
+    (test_simulation_generator.py:283-343).
+    """
+    e = np.asarray(euler, dtype=np.float64)
+    p1, P, p2 = e[..., 0], e[..., 1], e[..., 2]
+    c1, s1 = np.cos(p1), np.sin(p1)
+    c, s = np.cos(P), np.sin(P)
+    c2, s2 = np.cos(p2), np.sin(p2)
+    M = np.empty(e.shape[:-1] + (3, 3))
+    M[..., 0, 0] = c1 * c2 - s1 * s2 * c
+    M[..., 0, 1] = s1 * c2 + c1 * s2 * c
+    M[..., 0, 2] = s2 * s
+    M[..., 1, 0] = -c1 * s2 - s1 * c2 * c
+    M[..., 1, 1] = -s1 * s2 + c1 * c2 * c
+    M[..., 1, 2] = c2 * s
+    M[..., 2, 0] = s1 * s
+    M[..., 2, 1] = -c1 * s
+    M[..., 2, 2] = c
+    return M
+
+
+def quat_from_bunge(euler: np.ndarray) -> np.ndarray:
+    """Unit quaternions (w, x, y, z) of Bunge angles, Rowenhorst et al. (2015)
+    convention with P = +1 and a non-negative scalar part (orix
+    ``Rotation.from_euler``)."""
+    e = np.asarray(euler, dtype=np.float64)
+    p1, P, p2 = e[..., 0], e[..., 1], e[..., 2]
+    sigma = 0.5 * (p1 + p2)
+    delta = 0.5 * (p1 - p2)
+    c, s = np.cos(P / 2), np.sin(P / 2)
+    q = np.stack([c * np.cos(sigma), -s * np.cos(delta), -s * np.sin(delta),
+                  -c * np.sin(sigma)], axis=-1)
+    q = np.where(q[..., :1] < 0, -q, q)
+    return q
+
+
+def matrix_from_quat(q: np.ndarray) -> np.ndarray:
+    """Rotation matrix of a unit quaternion, R v = q v q^-1 (what orix
+    ``Rotation.to_matrix`` returns); equals :func:`bunge_matrix` for quaternions
+    from :func:`quat_from_bunge`."""
+    q = np.asarray(q, dtype=np.float64)
+    w, x, y, z = q[..., 0], q[..., 1], q[..., 2], q[..., 3]
+    qq = w * w - (x * x + y * y + z * z)
+    M = np.empty(q.shape[:-1] + (3, 3))
+    M[..., 0, 0] = qq + 2 * x * x
+    M[..., 0, 1] = 2 * (x * y - w * z)
+    M[..., 0, 2] = 2 * (x * z + w * y)
+    M[..., 1, 0] = 2 * (y * x + w * z)
+    M[..., 1, 1] = qq + 2 * y * y
+    M[..., 1, 2] = 2 * (y * z - w * x)
+    M[..., 2, 0] = 2 * (z * x - w * y)
+    M[..., 2, 1] = 2 * (z * y + w * x)
+    M[..., 2, 2] = qq + 2 * z * z
+    return M
+
+
+def random_euler(n: int, seed: int) -> np.ndarray:
+    """Seeded Haar-uniform Bunge angles: phi1, phi2 ~ U[0, 2pi), Phi = arccos U[-1, 1]."""
+    rng = np.random.default_rng(seed)
+    p1 = rng.uniform(0, 2 * np.pi, n)
+    P = np.arccos(rng.uniform(-1, 1, n))
+    p2 = rng.uniform(0, 2 * np.pi, n)
+    return np.stack([p1, P, p2], axis=1)
+
+
+# --------------------------------------------------------------------------
+# a5: shape-factor models  (diffsims/utils/shape_factor_models.py)
+# --------------------------------------------------------------------------
+
+
+def sf_binary(s, w):
+    """shape_factor_models.py:33-49: unit factor."""
+    return np.ones_like(np.asarray(s, dtype=np.float64))
+
+
+def sf_linear(s, w):
+    """shape_factor_models.py:52-73: max(1 - |s|/w, 0)."""
+    sf = 1 - np.abs(s) / w
+    return np.maximum(sf, 0.0)
+
+
+def sf_sinc(s, w, minima_number=5):
+    """shape_factor_models.py:76-101: |sin(x)/x|, x = pi*n*s/w, and 0 at s = 0."""
+    s = np.asarray(s, dtype=np.float64)
+    fac = np.pi * minima_number / w
+    num = np.sin(fac * s)
+    den = fac * s
+    out = np.zeros_like(num)
+    np.divide(num, den, out=out, where=den != 0)
+    return np.abs(out)
+
+
+def sf_sin2c(s, w, minima_number=5):
+    """shape_factor_models.py:104-123: sinc^2 (0 at s = 0)."""
+    return sf_sinc(s, w, minima_number) ** 2
+
+
+def sf_atanc(s, w, minima_number=5):
+    """shape_factor_models.py:126-151: atan(x)/x with x = pi*n*s/|w|, 1 at s = 0."""
+    s = np.asarray(s, dtype=np.float64)
+    fac = np.pi * minima_number / np.abs(w)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        r = np.arctan(fac * s) / (fac * s)
+    return np.nan_to_num(r, nan=1.0)
+
+
+def sf_lorentzian(s, w):
+    """shape_factor_models.py:154-180: sigma/(pi (sigma^2 s^2 + 1)) * w, sigma = pi/w."""
+    sigma = np.pi / w
+    return sigma / (np.pi * (sigma ** 2 * np.asarray(s) ** 2 + 1)) * w
+
+
+def sf_lorentzian_precession(s, w, r, phi):
+    """shape_factor_models.py:183-219 (Palatinus et al. 2019, eq. 10)."""
+    sigma = np.pi / w
+    u = sigma ** 2 * (r ** 2 * phi ** 2 - s ** 2) + 1
+    z = np.sqrt(u ** 2 + 4 * sigma ** 2 * s ** 2)
+    return (sigma / np.pi) * np.sqrt(2 * (u + z) / z ** 2)
+
+
+def sf_precession_integral(s, r, phi, model: Callable, w, **kw):
+    """shape_factor_models.py:222-268: (1/2pi) * integral_0^2pi model(s + r phi cos t) dt,
+    with scipy.integrate.quad per spot as in the reference."""
+    from scipy.integrate import quad
+
+    s = np.asarray(s, dtype=np.float64)
+    out = np.zeros(s.shape)
+    for i, (si, ri) in enumerate(zip(s, np.asarray(r, dtype=np.float64))):
+        def integrand(t, si=si, ri=ri):
+            return model(si + ri * phi * np.cos(t), w, **kw)
+        out[i] = (1 / (2 * np.pi)) * quad(integrand, 0, 2 * np.pi)[0]
+    return out
+
+
+SHAPE_MODELS = {
+    "linear": sf_linear,
+    "atanc": sf_atanc,
+    "sinc": sf_sinc,
+    "sin2c": sf_sin2c,
+    "lorentzian": sf_lorentzian,
+}
+
+
+# --------------------------------------------------------------------------
+# a6-a9: scattering factors and structure factor  (sim_utils.py:165-354)
+# --------------------------------------------------------------------------
+
+
+def load_table(name: str) -> dict:
+    """Element -> (5, 2) coefficient array from the package's JSON data file."""
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(__file__), "..", "diffsims_amd", "data",
+                        f"scattering_{name}.json")
+    d = json.load(open(path))
+    return {el: np.asarray(c, dtype=np.float64).reshape(5, 2)
+            for el, c in zip(d["elements"], d["coeffs"])}
+
+
+def element_of(symbol: str) -> str:
+    """Alphabetic head of a CIF atom-type symbol
+    (structure_factor/atomic_scattering_parameters.py:145-164)."""
+    i = 0
+    while i < len(symbol) and symbol[i].isalpha() and symbol[i].isascii():
+        i += 1
+    return symbol[:i]
+
+
+def atom_coeffs(symbols: Sequence[str], table: Optional[str]):
+    """(n, 5, 2) coefficients, zeros (and a miss flag) for unknown elements
+    (sim_utils.py:165-224)."""
+    coeffs = np.zeros((len(symbols), 5, 2))
+    missing = []
+    if table is None:
+        return coeffs, missing
+    t = load_table(table)
+    for i, s in enumerate(symbols):
+        el = element_of(s)
+        if el in t:
+            coeffs[i] = t[el]
+        else:
+            missing.append(el)
+    return coeffs, missing
+
+
+def atomic_f(g2: np.ndarray, coeffs: np.ndarray, table: Optional[str]) -> np.ndarray:
+    """f_j(g^2), shape (M, n) (sim_utils.py:227-253, :281-286).
+
+    lobato: sum_i a_i (2 + b_i g^2) / (1 + b_i g^2)^2; xtables: sum_i a_i
+    exp(-b_i g^2 / 4); None: 1.
+    """
+    g2 = np.asarray(g2, dtype=np.float64)
+    if table is None:
+        return np.ones((g2.shape[0], coeffs.shape[0]))
+    bg = g2[:, None, None] * coeffs[None, :, :, 1]
+    if table == "lobato":
+        f = (2 + bg) * (1 / np.square(1 + bg))
+    elif table == "xtables":
+        f = np.exp(-0.25 * bg)
+    else:
+        raise NotImplementedError(table)
+    return np.sum(coeffs[None, :, :, 0] * f, axis=-1)
+
+
+def _exact_phase_expi(t: np.ndarray) -> np.ndarray:
+    """exp(2 pi i t) with t reduced to its fractional part first; exact
+    for t in (1/4) Z (the kinematic-extinction cases)."""
+    fr = t - np.round(t)                     # in [-1/2, 1/2]
+    c = np.cos(2 * np.pi * fr)
+    s = np.sin(2 * np.pi * fr)
+    q = fr * 4
+    on = q == np.round(q)
+    qi = np.round(q).astype(np.int64) % 4
+    c = np.where(on, np.choose(qi, [1.0, 0.0, -1.0, 0.0]), c)
+    s = np.where(on, np.choose(qi, [0.0, 1.0, 0.0, -1.0]), s)
+    return c + 1j * s
+
+
+def structure_factor(hkl, g, xyz, occ, B, coeffs, table, phase_mode="reference"):
+    """Complex F(hkl) = sum_j f_j occ_j exp(2 pi i hkl.x_j - g^2 B_j / 4)
+    (sim_utils.py:256-304).
+
+    ``g`` is |g| in 1/Angstrom; ``B`` is per atom (the reference looks B up per
+    element from ``debye_waller_factors`` with default 0, sim_utils.py:220, which
+    is the special case of a per-atom array that is constant per element).
+
+    phase_mode "reference" evaluates exp(2 pi i x) as the reference does (so
+    kinematically forbidden reflections carry ~1e-16 round-off); "exact"
+    reduces hkl.x to its fractional part first, so forbidden reflections are
+    exactly zero (the GPU kernel's behaviour).
+    """
+    hkl = np.asarray(hkl, dtype=np.float64)
+    g2 = np.asarray(g, dtype=np.float64) ** 2
+    f = atomic_f(g2, coeffs, table)                      # (M, n)
+    t = hkl @ np.asarray(xyz, dtype=np.float64).T        # (M, n) turns
+    dw = np.exp(-0.25 * np.outer(g2, np.asarray(B, dtype=np.float64)))
+    if phase_mode == "reference":
+        ph = np.exp(2j * np.pi * t)
+    else:
+        ph = _exact_phase_expi(t)
+    return np.sum(f * np.asarray(occ, dtype=np.float64) * dw * ph, axis=-1)
+
+
+# --------------------------------------------------------------------------
+# a4, a10: Ewald intersection and the relative cut
+#   (simulation_generator.py:198-241, :319-412)
+# --------------------------------------------------------------------------
+
+
+@dataclass
+class PhaseSpec:
+    """Arrays describing a phase: lattice parameters and per-atom data."""
+    abc: Sequence[float]
+    angles: Sequence[float]
+    symbols: Sequence[str]
+    xyz: np.ndarray
+    occ: np.ndarray
+    B: np.ndarray
+
+    def basis(self):
+        return lattice_basis(*self.abc, *self.angles)
+
+
+@dataclass
+class SimParams:
+    kv: float = 200.0
+    reciprocal_radius: float = 1.0
+    max_excitation_error: float = 1e-2
+    shape_factor_width: Optional[float] = None
+    shape_model: object = "lorentzian"       # name or callable
+    shape_kwargs: dict = field(default_factory=dict)
+    precession_angle: float = 0.0             # degrees
+    approximate_precession: bool = True
+    minimum_intensity: float = 1e-20
+    with_direct_beam: bool = True
+    scattering_params: Optional[str] = "lobato"
+
+
+@dataclass
+class PhaseTables:
+    hkl: np.ndarray          # (N, 3) int, including the duplicated (000) rows
+    g: np.ndarray            # (N, 3) Cartesian crystal frame
+    gnorm: np.ndarray        # (N,)
+    F2: np.ndarray           # (N,) |F|^2 (exact-phase mode)
+    n_box: int               # rows before the appended direct-beam copies
+
+
+def phase_tables(ph: PhaseSpec, p: SimParams, phase_mode="exact") -> PhaseTables:
+    """Per-phase hkl set, g vectors and |F|^2 (what the GPU computes once)."""
+    base, rec = ph.basis()
+    hkl = hkl_box(rec, 1.0 / p.reciprocal_radius, include_zero=p.with_direct_beam)
+    n_box = hkl.shape[0]
+    if p.with_direct_beam:
+        # second (000) row: simulation_generator.py:351-353
+        hkl = np.vstack([hkl, np.zeros((1, 3), dtype=hkl.dtype)])
+    g = g_cartesian(hkl, rec)
+    gn = rnorm(hkl, rec)
+    coeffs, _ = atom_coeffs(ph.symbols, p.scattering_params)
+    F = structure_factor(hkl, gn, ph.xyz, ph.occ, ph.B, coeffs, p.scattering_params,
+                         phase_mode)
+    return PhaseTables(hkl, g, gn, (F * F.conjugate()).real, n_box)
+
+
+def shape_factor(s, r, p: SimParams):
+    """Shape factor dispatch (simulation_generator.py:387-411)."""
+    w = p.shape_factor_width if p.shape_factor_width is not None else p.max_excitation_error
+    if p.precession_angle == 0:
+        model = p.shape_model if callable(p.shape_model) else SHAPE_MODELS[p.shape_model]
+        return model(s, w, **p.shape_kwargs)
+    phi = np.deg2rad(abs(p.precession_angle))
+    if p.approximate_precession:
+        return sf_lorentzian_precession(s, w, r, phi)
+    model = p.shape_model if callable(p.shape_model) else SHAPE_MODELS[p.shape_model]
+    return sf_precession_integral(s, r, phi, model, w, **p.shape_kwargs)
+
+
+def intersect(g_lab: np.ndarray, k: float, p: SimParams):
+    """Excitation error, in-plane radius and the intersection mask
+    (simulation_generator.py:355-375)."""
+    r = np.sqrt(np.sum(np.square(g_lab[:, :2]), axis=1))
+    z = g_lab[:, 2]
+    z_sphere = -np.sqrt(k ** 2 - r ** 2) + k
+    s = z_sphere - z
+    me = p.max_excitation_error
+    if p.precession_angle == 0:
+        mask = np.abs(s) < me
+    else:
+        phi = np.deg2rad(abs(p.precession_angle))
+        Pz, Pt = k * np.cos(phi), k * np.sin(phi)
+        with np.errstate(invalid="ignore"):
+            up = Pz - np.sqrt(k ** 2 - (r + Pt) ** 2)
+            do = Pz - np.sqrt(k ** 2 - (r - Pt) ** 2)
+        mask = (z - me <= up) & (z + me >= do)
+    return s, r, mask
+
+
+@dataclass
+class Spots:
+    idx: np.ndarray          # indices into the phase hkl table
+    xyz: np.ndarray          # (S, 3) lab-frame 1/Angstrom
+    intensity: np.ndarray    # (S,)
+    n_intersect: int         # spots before the relative cut
+    max_intensity: float
+
+
+def simulate_one(tab: PhaseTables, M: np.ndarray, k: float, p: SimParams) -> Spots:
+    """One orientation of the reference's hot loop with |F|^2 from the table
+    (simulation_generator.py:211-241).  An empty intersection returns no spots
+    (the reference raises ValueError from np.max at :237)."""
+    g_lab = tab.g @ M                          # (M^T g)^T = g^T M
+    s, r, mask = intersect(g_lab, k, p)
+    idx = np.nonzero(mask)[0]
+    if idx.size == 0:
+        return Spots(idx, np.zeros((0, 3)), np.zeros(0), 0, 0.0)
+    sf = shape_factor(s[idx], r[idx], p)
+    I = sf * tab.F2[idx]
+    mx = float(np.max(I))
+    keep = I > mx * p.minimum_intensity
+    if mx == 0.0:
+        # Every intersecting reflection is kinematically forbidden: the
+        # reference keeps them all (their FP64 round-off |F|^2 passes its
+        # relative cut); this build keeps them all with intensity exactly 0.
+        keep = np.ones_like(keep)
+    return Spots(idx[keep], g_lab[idx[keep]], I[keep], int(idx.size), mx)
+
+
+def simulate_one_reference(ph: PhaseSpec, hkl_box_rows: np.ndarray, rec: np.ndarray,
+                           M: np.ndarray, k: float, p: SimParams) -> Spots:
+    """The reference's per-orientation loop body with F recomputed for the
+    intersecting reflections (sim_utils.py:307-354 called at
+    simulation_generator.py:227-234), reference-mode phases.  Used for the KAT
+    counts and as the CPU baseline."""
+    hkl = hkl_box_rows
+    if p.with_direct_beam:
+        hkl = np.vstack([hkl, np.zeros((1, 3), dtype=hkl.dtype)])
+    g_lab = g_cartesian(hkl, rec) @ M
+    s, r, mask = intersect(g_lab, k, p)
+    idx = np.nonzero(mask)[0]
+    if idx.size == 0:
+        return Spots(idx, np.zeros((0, 3)), np.zeros(0), 0, 0.0)
+    sf = shape_factor(s[idx], r[idx], p)
+    gn = np.sqrt(np.sum(g_lab[idx] ** 2, axis=1))
+    coeffs, _ = atom_coeffs(ph.symbols, p.scattering_params)
+    F = structure_factor(hkl[idx], gn, ph.xyz, ph.occ, ph.B, coeffs,
+                         p.scattering_params, "reference")
+    I = sf * (F * F.conjugate()).real
+    mx = float(np.max(I))
+    keep = I > mx * p.minimum_intensity
+    return Spots(idx[keep], g_lab[idx[keep]], I[keep], int(idx.size), mx)
+
+
+# --------------------------------------------------------------------------
+# a12, a13: rendering  (simulation2d.py:261-285, :357-442;
+#                       detector_functions.py:251-359)
+# --------------------------------------------------------------------------
+
+
+def pixel_coords(xy: np.ndarray, shape, calibration, center, in_plane_angle=0.0,
+                 mirrored=False):
+    """Polar round trip of Simulation2D._get_transformed_coordinates
+    (simulation2d.py:261-285) in pixel units."""
+    x = np.asarray(xy[:, 0], dtype=np.float64) / calibration
+    y = np.asarray(xy[:, 1], dtype=np.float64) / calibration
+    mf = -1 if mirrored else 1
+    theta = mf * np.arctan2(y, x) + np.deg2rad(in_plane_angle)
+    rd = np.sqrt(x ** 2 + y ** 2)
+    return rd * np.cos(theta) + center[0], rd * np.sin(theta) + center[1]
+
+
+def render_fast(xy, intensity, shape, sigma, calibration, in_plane_angle=0.0,
+                mirrored=False, normalize=True, center=None):
+    """fast=True rendering (simulation2d.py:405-442, detector_functions.py:293-300):
+    truncate to int pixels, last write wins, scipy.ndimage.gaussian_filter
+    (reflect, truncate 4), divide by the maximum."""
+    from scipy import ndimage as ndi
+
+    H, W = shape
+    if center is None:
+        center = (W // 2, H // 2)
+    X, Y = pixel_coords(xy, shape, calibration, center, in_plane_angle, mirrored)
+    inf = (X >= 0) & (X < W) & (Y >= 0) & (Y < H)
+    I = np.asarray(intensity, dtype=np.float64)[inf]
+    if I.size == 0:
+        return np.zeros(shape)
+    xi = X[inf].astype(int)
+    yi = Y[inf].astype(int)
+    out = np.zeros(shape)
+    out[yi, xi] = I
+    out = ndi.gaussian_filter(out, sigma)
+    if normalize:
+        mx = np.max(out)
+        if mx == 0:
+            return out
+        out = out / mx
+    return out
+
+
+def subpixel_gaussian(coords_xy, intensity, shape, sigma, clip_threshold=1.0):
+    """detector_functions.py:314-359: per-spot clipped box of
+    I/(2 pi s^2) exp(-d^2 / 2 s^2); a spot is skipped when the box radius is NaN."""
+    H, W = shape
+    out = np.zeros(shape)
+    pref = 1 / (2 * np.pi * sigma ** 2)
+    ep = -1 / (2 * sigma ** 2)
+    yy, xx = np.indices(shape)
+    for (cx, cy), I in zip(np.asarray(coords_xy, dtype=np.float64)[:, :2],
+                           np.asarray(intensity, dtype=np.float64)):
+        with np.errstate(invalid="ignore", divide="ignore"):
+            radius = np.sqrt(np.log(clip_threshold / (pref * I)) / ep)
+        if np.isnan(radius):
+            continue
+        y0 = max(0, int(np.ceil(cy - radius)))
+        y1 = min(H, int(np.floor(cy + radius + 1)))
+        x0 = max(0, int(np.ceil(cx - radius)))
+        x1 = min(W, int(np.floor(cx + radius + 1)))
+        if y1 <= y0 or x1 <= x0:
+            continue
+        d2 = (yy[y0:y1, x0:x1] - cy) ** 2 + (xx[y0:y1, x0:x1] - cx) ** 2
+        out[y0:y1, x0:x1] += I * pref * np.exp(ep * d2)
+    return out
+
+
+def render_subpixel(xy, intensity, shape, sigma, calibration, in_plane_angle=0.0,
+                    mirrored=False, normalize=True, clip_threshold=1.0, center=None):
+    """fast=False rendering: centre ((W-1)/2, (H-1)/2), float coordinates,
+    :func:`subpixel_gaussian`, optional normalisation (simulation2d.py:405-442)."""
+    H, W = shape
+    if center is None:
+        center = ((W - 1) / 2, (H - 1) / 2)
+    X, Y = pixel_coords(xy, shape, calibration, center, in_plane_angle, mirrored)
+    inf = (X >= 0) & (X < W) & (Y >= 0) & (Y < H)
+    I = np.asarray(intensity, dtype=np.float64)[inf]
+    if I.size == 0:
+        return np.zeros(shape)
+    out = subpixel_gaussian(np.stack([X[inf], Y[inf]], 1), I, shape, sigma, clip_threshold)
+    if normalize:
+        mx = np.max(out)
+        if mx == 0:
+            return out
+        out = out / mx
+    return out
+
+
+# --------------------------------------------------------------------------
+# Reference-shaped library loop (the CPU baseline)
+# --------------------------------------------------------------------------
+
+
+def simulate_library(ph: PhaseSpec, rotations_M: np.ndarray, p: SimParams,
+                     render: Optional[dict] = None):
+    """calculate_diffraction2d for one phase and many orientations, followed by
+    get_diffraction_pattern per orientation when ``render`` is given
+    (simulation_generator.py:198-241 + simulation2d.py:357-442).  Returns the
+    list of :class:`Spots` and, if rendering, an (n, H, W) float64 array."""
+    base, rec = ph.basis()
+    rows = hkl_box(rec, 1.0 / p.reciprocal_radius, include_zero=p.with_direct_beam)
+    k = 1.0 / electron_wavelength(p.kv)
+    spots = []
+    pats = None
+    if render is not None:
+        pats = np.zeros((len(rotations_M),) + tuple(render["shape"]))
+    for i, M in enumerate(rotations_M):
+        sp = simulate_one_reference(ph, rows, rec, M, k, p)
+        spots.append(sp)
+        if render is not None:
+            pats[i] = render_fast(sp.xyz[:, :2], sp.intensity, render["shape"],
+                                  render["sigma"], render["calibration"])
+    return spots, pats
