@@ -1,0 +1,181 @@
+"""Throughput of the kinematical pattern-simulation path on one MI355X.
+
+One step = simulate + render the whole orthorhombic 40-atom template library
+(200,000 orientations, 256x256 FP32 patterns) with every input resident in HBM:
+the per-orientation kernel (excitation error, shape factor, cut, compaction)
+followed by the renderer, writing all patterns to one [n, 256, 256] tensor.
+
+Prints one JSON line (see --help).  The CPU baseline is the FP64 NumPy
+restatement of the reference's loop timed on a bounded sample on this host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "patterns/s: batched fwd & fwd+bwd vs plain fwd, 1 MI355X, rel err ≤1e-5 vs FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--orientations", type=int, default=200_000)
+    ap.add_argument("--cpu-sample", type=int, default=2000,
+                    help="orientations in the CPU-baseline sample (0 disables it)")
+    return ap.parse_args()
+
+
+def cpu_baseline(wl, n_sample, batch, pats):
+    """Reference-shaped FP64 loop (the oracle) on the first n_sample
+    orientations, single thread; its patterns are then compared with the GPU's
+    for the same orientations: max |P - P64| / max(P64), with the carve-outs
+    of tests/parity.py (index sets equal; spots within 1e-4 px of a pixel
+    edge compared with the flipped pixel)."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import diffsims_oracle as O
+    from tests import parity as P
+
+    pa = wl.phases[0]
+    ph = O.PhaseSpec(pa.abc, pa.angles, pa.symbols, pa.xyz, pa.occ, pa.B)
+    st = wl.settings
+    p = O.SimParams(kv=st.accelerating_voltage, reciprocal_radius=wl.reciprocal_radius[0],
+                    max_excitation_error=st.max_excitation_error, shape_model=st.shape_model,
+                    minimum_intensity=st.minimum_intensity, with_direct_beam=wl.with_direct_beam)
+    M = wl.rotations[0][:n_sample]
+    render = {"shape": wl.shape, "sigma": wl.sigma, "calibration": wl.calibration}
+    with threadpool_limits(limits=1):
+        O.simulate_library(ph, M[:8], p, render)          # warm caches
+        t0 = time.perf_counter()
+        spots, _ = O.simulate_library(ph, M, p, render)
+        dt = time.perf_counter() - t0
+    # accuracy of the GPU patterns on the same sample (exact-phase |F|^2 oracle)
+    otab = O.phase_tables(ph, p)
+    k = 1 / O.electron_wavelength(st.accelerating_voltage)
+    worst, compared = 0.0, 0
+    for o in range(0, n_sample, max(1, n_sample // 64)):
+        ref = O.simulate_one(otab, M[o], k, p)
+        c = int(batch.count[o])
+        if not np.array_equal(batch.idx[o, :c].cpu().numpy(), ref.idx):
+            continue
+        rp = P.flipped_reference_pattern(ref, batch.spots[o, :c, :2].cpu().numpy(), wl.shape,
+                                         wl.sigma, wl.calibration)
+        worst = max(worst, float(np.abs(pats[o].cpu().numpy() - rp).max() / max(rp.max(), 1e-300)))
+        compared += 1
+    return n_sample / dt, dt, worst, compared
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one GPU; run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from diffsims_amd import engine, workloads
+
+    dev = torch.device("cuda", 0)
+    wl = workloads.template_library(args.orientations)
+    pa = wl.phases[0]
+    rr = wl.reciprocal_radius[0]
+    tab = engine.build_phase_table(pa, rr, wl.with_direct_beam, "lobato", device=dev)
+    Rt = engine.rotation_matrices_to_device(wl.rotations[0], dev)
+    n = Rt.shape[0]
+    # capacity from one calibration pass (ncand is exact even on overflow)
+    probe = engine.simulate(tab, Rt[:4096], wl.settings, rr)
+    cap = int(max(probe.cap, int(probe.ncand.max()) + 64))
+    batch = engine.empty_batch(tab, n, cap, dev)
+    H, W = wl.shape
+    out = torch.empty((n, H, W), dtype=torch.float32, device=dev)
+    status = torch.zeros(n, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        engine.simulate_into(tab, Rt, wl.settings, batch)
+        engine.render_spots(batch.spots, batch.count, batch.cap, wl.shape, wl.sigma,
+                            wl.calibration, out=out, status=status)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    if int(batch.ncand.max()) > cap:             # re-size once if the grid needs more
+        cap = int(batch.ncand.max()) + 64
+        batch = engine.empty_batch(tab, n, cap, dev)
+        step()
+        torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1, e2 in ev:
+        e0.record(stream)
+        engine.simulate_into(tab, Rt, wl.settings, batch)
+        e1.record(stream)
+        engine.render_spots(batch.spots, batch.count, batch.cap, wl.shape, wl.sigma,
+                            wl.calibration, out=out, status=status)
+        e2.record(stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    engine.check_render_status(status)
+    assert int(batch.ncand.max()) <= cap
+    sim_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
+    ren_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
+    counts = batch.count.float()
+    mean_spots = float(counts.mean())
+    # algorithmic bytes of one render launch: every pattern written once, its
+    # spot list and count read once
+    render_bytes = n * (H * W * 4) + float(counts.sum()) * 16 + n * 4
+    achieved = render_bytes / (ren_ms * 1e-3) / 1e9
+    value = args.steps * n / dt
+    res = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "patterns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "orthorhombic 40-atom template library (a,b,c~U[4,12] A, Z in {6,8,14,26}, "
+                        "B~U[0.3,1]): r=2.0 1/A, max_exc 0.02, lorentzian, 200 kV, no direct beam, "
+                        f"{n} orientations (super-Fibonacci SO(3) grid), {H}x{W} f32 patterns at "
+                        f"{wl.calibration} 1/A/px, sigma {wl.sigma} px, fast render, normalised",
+            "n_orientations": n, "n_hkl": tab.n, "mean_spots": round(mean_spots, 2),
+            "mode": "batched forward (simulate + render)",
+            "kernel_ms": {"simulate": round(sim_ms, 3), "render": round(ren_ms, 3)},
+        },
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+    }
+    if args.cpu_sample > 0:
+        v, t, err, compared = cpu_baseline(wl, args.cpu_sample, batch, out)
+        res["cpu_baseline"] = {"value": round(v, 2), "unit": "patterns/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"first {args.cpu_sample} of {n} orientations, FP64 NumPy "
+                                         f"restatement of the reference loop incl. render "
+                                         f"({t:.1f} s, 1 thread)"}
+        res["config"]["max_rel_err_vs_fp64"] = err
+        res["config"]["accuracy_patterns"] = compared
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -161,6 +161,32 @@ def build_phase_table(pa: PhaseArrays, reciprocal_radius: float, with_direct_bea
     return PhaseTable(hkl, g4, F2, F, n_box, with_direct_beam, g_max, rec, base, pa.name)
 
 
+def table_from_hkl(hkl: np.ndarray, abc, angles, F2: Optional[np.ndarray] = None,
+                   device=None) -> PhaseTable:
+    """Reflection table for given integer indices (g vectors on the GPU);
+    |F|^2 = 1 unless given.  Used for a caller's own reflection set
+    (get_intersecting_reflections)."""
+    dev = _device(device)
+    L = _lib.load()
+    base, rec = lattice_setup(abc, angles)
+    h = np.zeros((len(hkl), 4), dtype=np.int32)
+    h[:, :3] = np.asarray(hkl, dtype=np.int32).reshape(-1, 3)
+    t_hkl = torch.from_numpy(h).to(dev)
+    N = h.shape[0]
+    g4 = torch.empty((N, 4), dtype=torch.float32, device=dev)
+    F2t = torch.empty(N, dtype=torch.float32, device=dev)
+    rec9 = (ctypes.c_double * 9)(*rec.reshape(-1).tolist())
+    _lib.check(L.dsx_structure_factor(_lib.ptr(t_hkl), N, rec9, None, None, None, None, 0, None, 0,
+                                      _lib.SCATTER_NONE, _lib.ptr(g4), _lib.ptr(F2t), None,
+                                      _lib.stream_handle(dev)), "dsx_structure_factor")
+    if F2 is None:
+        F2t.fill_(1.0)
+    else:
+        F2t.copy_(torch.as_tensor(np.asarray(F2, dtype=np.float32)))
+    g_max = float(torch.sqrt(g4[:, 3].max()).item()) if N else 0.0
+    return PhaseTable(t_hkl, g4, F2t, None, N, False, g_max, rec, base)
+
+
 @dataclass
 class SimSettings:
     """Physical settings of SimulationGenerator + calculate_diffraction2d."""
@@ -179,7 +205,9 @@ class SimSettings:
             raise ValueError("infinite accelerating voltage is not supported by the simulation")
         width = self.shape_factor_width if self.shape_factor_width is not None else self.max_excitation_error
         phi = abs(self.precession_angle)
-        if phi > 0:
+        if phi > 0 and self.shape_model != "binary":
+            # the precession band test applies to every model; with precession
+            # the reference's shape factor is the precession-averaged Lorentzian
             if not self.approximate_precession:
                 raise NotImplementedError("full precession integration is not implemented on the GPU")
             model = _lib.SF_LORENTZIAN_PRECESSION
@@ -300,8 +328,14 @@ def render_params(shape, sigma, calibration, direct_beam_position=None, in_plane
 def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigma, calibration,
                  direct_beam_position=None, in_plane_angle=0.0, mirrored=False, normalize=True,
                  fast=True, clip_threshold=1.0, out: Optional[torch.Tensor] = None,
-                 stream=None) -> torch.Tensor:
-    """Render n spot lists (float4 {x, y, z, I}, stride cap) to [n, H, W] float32."""
+                 stream=None, status: Optional[torch.Tensor] = None) -> torch.Tensor:
+    """Render n spot lists (float4 {x, y, z, I}, stride cap) to [n, H, W] float32.
+
+    Without ``status`` the call synchronises and raises if a pattern had more
+    distinct in-frame spots than the renderer holds; with a caller-provided
+    int32 [n] ``status`` tensor it only launches (check it with
+    :func:`check_render_status`).
+    """
     L = _lib.load(require_gpu=True)
     dev = spots.device
     n = int(count.shape[0])
@@ -310,18 +344,48 @@ def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigm
         out = torch.empty((n, H, W), dtype=torch.float32, device=dev)
     elif out.shape != (n, H, W) or out.dtype != torch.float32 or not out.is_contiguous():
         raise ValueError("out must be a contiguous float32 [n, H, W] tensor")
-    status = torch.zeros(n, dtype=torch.int32, device=dev)
+    sync_check = status is None
+    if status is None:
+        status = torch.zeros(n, dtype=torch.int32, device=dev)
     p = render_params(shape, sigma, calibration, direct_beam_position, in_plane_angle, mirrored,
                       normalize, fast, clip_threshold)
     stream = _lib.stream_handle(dev) if stream is None else stream
     fn = L.dsx_render_fast if fast else L.dsx_render_subpixel
     _lib.check(fn(_lib.ptr(spots), _lib.ptr(count), cap, n, ctypes.byref(p), _lib.ptr(out),
                   _lib.ptr(status), stream), "dsx_render")
-    if n and int(status.max().item()) != 0:
-        bad = int(torch.nonzero(status)[0].item())
-        raise RuntimeError(f"pattern {bad} has more than {L.dsx_render_max_spots()} distinct "
-                           "in-frame spots; the renderer cannot hold it")
+    if sync_check:
+        check_render_status(status)
     return out
+
+
+def check_render_status(status: torch.Tensor) -> None:
+    """Raise if any rendered pattern overflowed the renderer's spot capacity."""
+    if status.numel() and int(status.max().item()) != 0:
+        bad = int(torch.nonzero(status)[0].item())
+        raise RuntimeError(f"pattern {bad} has more than {_lib.load().dsx_render_max_spots()} "
+                           "distinct in-frame spots; the renderer cannot hold it")
+
+
+def simulate_into(tab: PhaseTable, Rt: torch.Tensor, settings: SimSettings, out: SpotBatch,
+                  stream=None) -> None:
+    """Launch-only variant of :func:`simulate` into preallocated buffers (no
+    synchronisation; the caller checks ``out.ncand.max() <= out.cap``)."""
+    L = _lib.load(require_gpu=True)
+    params = settings.params(tab.g_max, tab.rec)
+    stream = _lib.stream_handle(Rt.device) if stream is None else stream
+    _lib.check(L.dsx_simulate(_lib.ptr(tab.hkl), _lib.ptr(tab.g4), _lib.ptr(tab.F2), tab.n,
+                              _lib.ptr(Rt), int(Rt.shape[0]), ctypes.byref(params), out.cap,
+                              _lib.ptr(out.idx), _lib.ptr(out.spots), _lib.ptr(out.count),
+                              _lib.ptr(out.ncand), _lib.ptr(out.maxI), stream), "dsx_simulate")
+
+
+def empty_batch(tab: PhaseTable, n: int, cap: int, device) -> SpotBatch:
+    dev = torch.device(device)
+    return SpotBatch(tab, torch.empty((n, cap), dtype=torch.int32, device=dev),
+                     torch.empty((n, cap, 4), dtype=torch.float32, device=dev),
+                     torch.empty(n, dtype=torch.int32, device=dev),
+                     torch.empty(n, dtype=torch.int32, device=dev),
+                     torch.empty(n, dtype=torch.float32, device=dev), cap)
 
 
 def render(batch: SpotBatch, shape, sigma, calibration, **kw) -> torch.Tensor:

@@ -216,7 +216,7 @@ class SimulationGenerator:
                                       "GPU kernel")
         settings = self._settings(max_excitation_error, shape_factor_width, minimum_intensity=-1.0)
         settings.accelerating_voltage = _voltage_of(wavelength)
-        batch = engine.simulate(tab, Rt, settings, float(np.sqrt(tab.g4[:, 3].max().item())) or 1.0)
+        batch = engine.simulate(tab, Rt, settings, max(tab.g_max, 1e-6), cap=max(tab.n, 1))
         idx, xyz, sf, _ = _flat_spots(batch, M)
         vecs = DiffractingVector(recip.phase, xyz=xyz, rotation_matrix=M[0])
         vecs._hkl = hkl[idx].astype(np.float64)

@@ -1,0 +1,382 @@
+"""The FP64 oracle against the reference's golden data and known-answer tests,
+closed forms and brute force (CPU only).
+
+Each test names the reference test or source it pins.  Some tests plant a
+mistake in a copy of an oracle function and check that the pinning data
+catches it, so that the pins are shown to be sensitive.
+"""
+
+import os
+
+import numpy as np
+import pytest
+from scipy.integrate import quad
+
+from oracle import diffsims_oracle as O
+from tests.parity import FCC_POS, si_positions
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "old_simulation.npy")
+
+
+# -- a1: wavelength (diffsims/tests/utils/test_sim_utils.py:49-60) ------------
+
+@pytest.mark.parametrize("kv, lam", [(100, 0.0370143659), (200, 0.0250793403),
+                                     (300, 0.0196874888), ("inf", 0)])
+def test_wavelength_kat(kv, lam):
+    np.testing.assert_almost_equal(O.electron_wavelength(kv), lam)
+
+
+def test_wavelength_closed_form_limits():
+    # non-relativistic limit h / sqrt(2 m e E) at low voltage within 1e-5
+    from scipy.constants import e, h, m_e
+    lam = O.electron_wavelength(0.01)
+    assert abs(lam - h / np.sqrt(2 * m_e * e * 10.0) * 1e10) / lam < 1e-5
+
+
+# -- a2: hkl box (reciprocal_lattice_vector.py:1077-1142 and its tests) --------
+
+def _rec(a, b, c, al=90, be=90, ga=90):
+    return O.lattice_basis(a, b, c, al, be, ga)[1]
+
+
+@pytest.mark.parametrize("include_zero", [True, False])
+@pytest.mark.parametrize("d, size", [(2, 18), (1, 92), (0.5, 750)])
+def test_ferrite_sizes(d, size, include_zero):
+    # test_reciprocal_lattice_vector.py:76-87
+    hkl = O.hkl_box(_rec(2.8665, 2.8665, 2.8665), d, include_zero)
+    assert hkl.shape[0] == size + include_zero
+    if include_zero:
+        assert np.array_equal(hkl[-1], [0, 0, 0])
+
+
+def test_ferrite_first_row():
+    # test_reciprocal_lattice_vector.py:118-124: repr starts with [1, 1, 0]
+    assert np.array_equal(O.hkl_box(_rec(2.8665, 2.8665, 2.8665), 2)[0], [1, 1, 0])
+
+
+def test_al_docstring_counts_and_order():
+    # reciprocal_lattice_vector.py:1107-1127 (Al a = 4.04)
+    rec = _rec(4.04, 4.04, 4.04)
+    h = O.hkl_box(rec, 0.7)
+    assert h.shape[0] == 798
+    assert np.array_equal(h[0], [5, 2, 2]) and np.array_equal(h[1], [5, 2, 1])
+    assert np.array_equal(h[2], [5, 2, 0]) and np.array_equal(h[-1], [-5, -2, -2])
+    assert O.hkl_box(rec, 1.0).shape[0] == 256
+    d = 1 / O.rnorm(h, rec)
+    assert abs(d.min() - 0.7032737300610338) < 1e-12
+
+
+@pytest.mark.parametrize("d, box", [(0.5, [6, 6, 21]), (1, [3, 3, 11]), (1.5, [2, 2, 7])])
+def test_sic_box(d, box):
+    # test_get_hkl.py:52-64: max hkl + 1 equals the deprecated bound
+    h = O.hkl_box(_rec(3.073, 3.073, 10.053, 90, 90, 120), d)
+    assert np.array_equal(h.max(axis=0) + 1, box)
+
+
+def test_sic_d3_order():
+    # test_reciprocal_lattice_vector.py:147-155
+    h = O.hkl_box(_rec(3.073, 3.073, 10.053, 90, 90, 120), 3)
+    assert np.array_equal(h[:, 2], [3, 2, 1, -1, -2, -3]) and not h[:, :2].any()
+
+
+def test_box_drops_in_sphere_reflections_for_oblique_cells():
+    # the box rule, not the sphere: SiC d_min = 0.5 keeps 2688 of 2764 in-sphere
+    rec = _rec(3.073, 3.073, 10.053, 90, 90, 120)
+    h = O.hkl_box(rec, 0.5)
+    assert h.shape[0] == 2688
+    big = np.stack(np.meshgrid(*[np.arange(-30, 31)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    big = big[np.any(big != 0, axis=1)]
+    with np.errstate(divide="ignore"):
+        assert int(np.sum(1 / O.rnorm(big, rec) >= 0.5)) == 2764
+
+
+def test_cartesian_order_equals_hkl_order():
+    # orix sorts by Cartesian (x, y, z) descending; in the orix frame that is
+    # the descending (h, k, l) order for any cell
+    for abc, ang in [((3.0, 4.0, 5.0), (80, 95, 105)), ((3.073, 3.073, 10.053), (90, 90, 120))]:
+        _, rec = O.lattice_basis(*abc, *ang)
+        h = O.hkl_box(rec, 0.8)
+        g = O.g_cartesian(h, rec)
+        order = np.lexsort((-g[:, 2], -g[:, 1], -g[:, 0]))
+        assert np.array_equal(order, np.arange(h.shape[0]))
+
+
+@pytest.mark.parametrize("hkl, g", [((1, 0, 0), 0.34885749), ((2, 0, 0), 0.69771498),
+                                    ((1, 1, 0), 0.493359), ((2, 1, 0), 0.78006907),
+                                    ((2, 2, 0), 0.98671799), ((1, 1, 1), 0.604238),
+                                    ((2, 1, 1), 0.85452285)])
+def test_ferrite_gspacing(hkl, g):
+    # test_reciprocal_lattice_vector.py:157-187
+    # np.allclose tolerances, as the reference test uses
+    assert np.allclose(O.rnorm(np.array(hkl), _rec(2.8665, 2.8665, 2.8665)), g)
+
+
+def test_orix_frame_exact_zeros():
+    base, rec = O.lattice_basis(3.0, 4.0, 5.0, 90, 90, 90)
+    assert np.count_nonzero(base - np.diag(np.diag(base))) == 0
+    base, rec = O.lattice_basis(2.464, 2.464, 6.711, 90, 90, 120)
+    assert base[0, 1] == base[0, 2] == base[1, 2] == 0.0 and rec[0, 1] == rec[0, 2] == 0.0
+    assert base[1, 0] == -0.5 * 2.464  # degree-exact cos(120)
+
+
+# -- a3: rotations -------------------------------------------------------------
+
+def test_quaternion_matrix_matches_bunge():
+    e = O.random_euler(200, 3)
+    M = O.bunge_matrix(e)
+    assert np.abs(O.matrix_from_quat(O.quat_from_bunge(e)) - M).max() < 1e-14
+    assert np.abs(np.einsum("nij,nkj->nik", M, M) - np.eye(3)).max() < 1e-14
+    assert np.abs(np.linalg.det(M) - 1).max() < 1e-14
+
+
+def test_rotation_recovers_hkl():
+    # test_diffracting_vector.py:47-54: rotating vectors and basis recovers hkl
+    _, rec = O.lattice_basis(3.0, 4.0, 5.0, 80, 95, 105)
+    hkl = np.array([[1, 2, 3], [-2, 0, 1]])
+    M = O.bunge_matrix(np.array([0.3, 1.1, -0.4]))
+    g_lab = O.g_cartesian(hkl, rec) @ M
+    back = (g_lab @ M.T) @ np.linalg.inv(rec).T
+    assert np.abs(back - hkl).max() < 1e-12
+
+
+# -- a5: shape factors (shape_factor_models.py) ----------------------------------
+
+def test_linear_kat():
+    # test_simulation_generator.py:106-110
+    s = np.array([-2, -1, -0.5, 0, 0.5, 1, 2])
+    np.testing.assert_allclose(O.sf_linear(s, 1), [0, 0, 0.5, 1, 0.5, 0, 0])
+
+
+def test_model_special_values():
+    assert O.sf_sinc(np.array([0.0]), 0.1)[0] == 0.0        # :95-101
+    assert O.sf_sin2c(np.array([0.0]), 0.1)[0] == 0.0
+    assert O.sf_atanc(np.array([0.0]), 0.1)[0] == 1.0       # :147-151
+    s = np.linspace(-0.05, 0.05, 11)
+    np.testing.assert_allclose(O.sf_lorentzian(s, 0.02), 1 / (1 + (np.pi * s / 0.02) ** 2))
+
+
+def test_lorentzian_precession_zero_angle():
+    s = np.linspace(-0.05, 0.05, 11)
+    np.testing.assert_allclose(O.sf_lorentzian_precession(s, 0.02, 1.0, 0.0),
+                               (2 / 0.02) * O.sf_lorentzian(s, 0.02))
+
+
+def test_lorentzian_precession_is_scaled_exact_average():
+    # eq. 10 of Palatinus et al. equals (2 sigma / pi) times the exact
+    # precession average of the Lorentzian; check against quad
+    w, phi = 0.02, np.deg2rad(1.0)
+    for s, r in [(0.003, 0.5), (-0.01, 1.7), (0.0, 2.5)]:
+        avg = quad(lambda t: O.sf_lorentzian(s + r * phi * np.cos(t), w), 0, 2 * np.pi,
+                   limit=200)[0] / (2 * np.pi)
+        sigma = np.pi / w
+        assert abs(O.sf_lorentzian_precession(s, w, r, phi) - 2 * sigma / np.pi * avg) < 1e-8 * (
+            2 * sigma / np.pi)
+
+
+# -- a6-a9: scattering and structure factors ------------------------------------
+
+def test_ni_hex_kat():
+    # test_sim_utils.py:330-353: Ni hex (3, 3, 5, 120), r = 0.2 -> only (000),
+    # lobato I = 43.0979, None -> 1
+    base, rec = O.lattice_basis(3, 3, 5, 90, 90, 120)
+    coeffs, _ = O.atom_coeffs(["Ni"], "lobato")
+    F = O.structure_factor(np.zeros((1, 3)), np.zeros(1), np.zeros((1, 3)), np.ones(1),
+                           np.zeros(1), coeffs, "lobato")
+    assert abs(abs(F[0]) ** 2 - 43.0979) < 1e-4
+    F = O.structure_factor(np.zeros((1, 3)), np.zeros(1), np.zeros((1, 3)), np.ones(1),
+                           np.zeros(1), coeffs, None)
+    assert abs(F[0]) ** 2 == 1.0
+
+
+def test_unknown_element_kat():
+    # test_sim_utils.py:370-394: unknown element -> f = 0 (lobato), 1 (None)
+    coeffs, missing = O.atom_coeffs(["UnOb"], "lobato")
+    assert missing == ["UnOb"] and not coeffs.any()
+    assert O.element_of("Fe2+") == "Fe" and O.element_of("UnOb") == "UnOb"
+
+
+def _brute_F(hkl, g, xyz, occ, B, coeffs, table):
+    out = np.zeros(len(hkl), dtype=complex)
+    for i, h in enumerate(hkl):
+        for j in range(len(xyz)):
+            if table == "lobato":
+                f = sum(a * (2 + b * g[i] ** 2) / (1 + b * g[i] ** 2) ** 2 for a, b in coeffs[j])
+            else:
+                f = sum(a * np.exp(-b * g[i] ** 2 / 4) for a, b in coeffs[j])
+            out[i] += f * occ[j] * np.exp(-0.25 * g[i] ** 2 * B[j]) * np.exp(
+                2j * np.pi * np.dot(h, xyz[j]))
+    return out
+
+
+@pytest.mark.parametrize("table", ["lobato", "xtables"])
+def test_structure_factor_brute_force(table):
+    rng = np.random.default_rng(5)
+    syms = ["C", "O", "Fe", "Si", "Mg"]
+    xyz = rng.uniform(0, 1, (5, 3))
+    occ = rng.uniform(0.5, 1, 5)
+    B = rng.uniform(0.2, 1.0, 5)
+    coeffs, _ = O.atom_coeffs(syms, table)
+    hkl = rng.integers(-4, 5, (20, 3))
+    _, rec = O.lattice_basis(4.1, 5.2, 6.3, 85, 95, 100)
+    g = O.rnorm(hkl, rec)
+    for mode in ("reference", "exact"):
+        F = O.structure_factor(hkl, g, xyz, occ, B, coeffs, table, mode)
+        assert np.abs(F - _brute_F(hkl, g, xyz, occ, B, coeffs, table)).max() < 1e-10
+
+
+def test_friedel_and_extinctions():
+    rng = np.random.default_rng(6)
+    coeffs, _ = O.atom_coeffs(["Si", "O", "O"], "lobato")
+    xyz = rng.uniform(0, 1, (3, 3))
+    hkl = rng.integers(-5, 6, (30, 3))
+    g = np.linalg.norm(hkl, axis=1) / 5.0
+    F1 = O.structure_factor(hkl, g, xyz, np.ones(3), np.full(3, 0.4), coeffs, "lobato")
+    F2 = O.structure_factor(-hkl, g, xyz, np.ones(3), np.full(3, 0.4), coeffs, "lobato")
+    assert np.abs(np.abs(F1) - np.abs(F2)).max() < 1e-12
+    # FCC: mixed parity is extinct; diamond: all even with h+k+l = 4n+2 too
+    cf, _ = O.atom_coeffs(["Al"] * 4, "lobato")
+    fcc = np.array([[1, 0, 0], [2, 1, 0], [1, 1, 0], [3, 2, 1]])
+    F = O.structure_factor(fcc, np.ones(4), FCC_POS, np.ones(4), np.zeros(4), cf, "lobato", "exact")
+    assert np.all(F == 0)
+    cs, _ = O.atom_coeffs(["Si"] * 8, "lobato")
+    dia = np.array([[2, 0, 0], [2, 2, 2], [4, 2, 0]])
+    F = O.structure_factor(dia, np.ones(3), si_positions(), np.ones(8), np.zeros(8), cs, "lobato",
+                           "exact")
+    assert np.all(F == 0)
+    # the reference's own arithmetic leaves round-off there
+    Fr = O.structure_factor(fcc, np.ones(4), FCC_POS, np.ones(4), np.zeros(4), cf, "lobato",
+                            "reference")
+    assert np.all(np.abs(Fr) < 1e-13)
+
+
+# -- a4, a10, a11: full per-orientation KATs (test_simulation_generator.py) -------
+
+def _si(a=5.431):
+    return O.PhaseSpec((a, a, a), (90, 90, 90), ["Si"] * 8, si_positions(), np.ones(8), np.zeros(8))
+
+
+@pytest.mark.parametrize("prec, approx, expected", [(0, True, 70), (0.5, True, 250),
+                                                    (0.5, False, 250)])
+def test_si_spot_counts(prec, approx, expected):
+    # :137-161 (300 kV, r = 5, identity, with the direct beam, lorentzian)
+    ph = _si()
+    base, rec = ph.basis()
+    p = O.SimParams(kv=300, reciprocal_radius=5.0, precession_angle=prec, approximate_precession=approx)
+    rows = O.hkl_box(rec, 1 / 5.0, include_zero=True)
+    k = 1 / O.electron_wavelength(300)
+    assert len(O.simulate_one_reference(ph, rows, rec, np.eye(3), k, p).idx) == expected
+    if approx:
+        assert len(O.simulate_one(O.phase_tables(ph, p), np.eye(3), k, p).idx) == expected
+
+
+def test_si_count_needs_the_duplicated_direct_beam():
+    # planted mistake: without the second (000) row the 70-spot KAT fails
+    ph = _si()
+    base, rec = ph.basis()
+    p = O.SimParams(kv=300, reciprocal_radius=5.0, with_direct_beam=False)
+    rows = O.hkl_box(rec, 1 / 5.0, include_zero=True)
+    k = 1 / O.electron_wavelength(300)
+    assert len(O.simulate_one_reference(ph, rows, rec, np.eye(3), k, p).idx) == 69
+
+
+def test_direct_beam_strongest_and_absent():
+    # :177-208
+    ph = _si()
+    base, rec = ph.basis()
+    k = 1 / O.electron_wavelength(300)
+    p = O.SimParams(kv=300, reciprocal_radius=0.5, with_direct_beam=True)
+    tab = O.phase_tables(ph, p)
+    sp = O.simulate_one(tab, np.eye(3), k, p)
+    zero = np.all(tab.hkl[sp.idx] == 0, axis=1)
+    assert zero.sum() == 2 and sp.intensity[zero].max() >= sp.intensity.max()
+    p = O.SimParams(kv=300, reciprocal_radius=0.5, with_direct_beam=False)
+    tab = O.phase_tables(ph, p)
+    sp = O.simulate_one(tab, np.eye(3), k, p)
+    assert not np.all(tab.hkl[sp.idx] == 0, axis=1).any()
+
+
+def test_appropriate_scaling():
+    # :170-186: the (220) position halves when the cell doubles
+    k = 1 / O.electron_wavelength(300)
+    out = []
+    for a in (5.0, 10.0):
+        ph = _si(a)
+        p = O.SimParams(kv=300, reciprocal_radius=5.0)
+        tab = O.phase_tables(ph, p)
+        sp = O.simulate_one(tab, np.eye(3), k, p)
+        rows = tab.hkl[sp.idx]
+        i = int(np.nonzero(np.all(rows == [2, 2, 0], axis=1))[0][0])
+        out.append(sp.xyz[i])
+    np.testing.assert_allclose(out[0], 2 * out[1])
+
+
+# -- a12, a13: golden pattern and renderer KATs -------------------------------
+
+def _graphite():
+    return O.PhaseSpec((2.464, 2.464, 6.711), (90, 90, 120), ["C"] * 4,
+                       np.array([[0, 0, .25], [0, 0, .75], [1 / 3, 2 / 3, .25], [2 / 3, 1 / 3, .75]]),
+                       np.ones(4), np.zeros(4))
+
+
+def _golden_pattern(M_fn=O.bunge_matrix):
+    ph = _graphite()
+    p = O.SimParams(kv=200, reciprocal_radius=1.6768, max_excitation_error=0.1,
+                    shape_model="lorentzian", with_direct_beam=False)
+    M = M_fn(np.deg2rad(np.array([0.0, 90.0, 90.0])))
+    k = 1 / O.electron_wavelength(200)
+    sp = O.simulate_one(O.phase_tables(ph, p, "reference"), M, k, p)
+    return O.render_fast(sp.xyz[:, :2], sp.intensity, (128, 128), 1.4, 0.0262)
+
+
+def test_golden_old_simulation():
+    # test_simulation_generator.py:283-343 (asserts atol 1e-8)
+    old = np.load(GOLDEN)
+    assert np.abs(_golden_pattern() - old).max() < 1e-10
+
+
+def test_golden_catches_transposed_rotation():
+    # planted mistake: g_lab = M g instead of M^T g must fail the golden test
+    old = np.load(GOLDEN)
+    bad = _golden_pattern(lambda e: O.bunge_matrix(e).T)
+    assert np.abs(bad - old).max() > 0.1
+
+
+def test_render_total_intensity_and_edges():
+    # test_detector_functions.py:236-306
+    p = O.render_fast(np.array([[10.0 - 25, 10.0 - 25]]), np.array([100.0]), (50, 50), 3, 1.0,
+                      normalize=False)
+    assert abs(p.sum() - 100) < 1e-9
+    # float coordinates in a corner lose intensity, integer ones keep it
+    out = O.subpixel_gaussian(np.array([[0.3, 0.1]]), np.array([100.0]), (50, 50), 3)
+    assert out.sum() < 100
+    # low intensity below the clip threshold renders nothing
+    out = O.subpixel_gaussian(np.array([[10.0, 10.0], [20.0, 30.0]]), np.ones(2), (50, 50), 1)
+    assert out.sum() == 0.0
+    # large intensity: the clipped box keeps > 99.9 % of the total
+    out = O.subpixel_gaussian(np.array([[10.0, 10.0]]), np.array([1000.0]), (50, 50), 1)
+    assert out.sum() / 1000 > 0.999
+
+
+def test_gaussian_filter_is_reflected_tap_sum():
+    # the renderer's closed form: gaussian_filter of a delta = outer product of
+    # reflected 1-D taps (checked against scipy on small frames, edges included)
+    from scipy import ndimage as ndi
+    sigma = 1.4
+    R = int(4 * sigma + 0.5)
+    x = np.arange(-R, R + 1)
+    w = np.exp(-0.5 * x ** 2 / sigma ** 2)
+    w /= w.sum()
+
+    def K(y, c, n):
+        t = 0.0
+        for img in (c, -1 - c, 2 * n - 1 - c):
+            if abs(img - y) <= R:
+                t += w[R + abs(img - y)]
+        return t
+    H, W = 9, 12
+    for (cy, cx) in [(0, 0), (4, 6), (8, 11), (1, 10)]:
+        d = np.zeros((H, W))
+        d[cy, cx] = 1.0
+        ref = ndi.gaussian_filter(d, sigma)
+        mine = np.array([[K(y, cy, H) * K(xx, cx, W) for xx in range(W)] for y in range(H)])
+        assert np.abs(ref - mine).max() < 1e-15
