@@ -19,24 +19,44 @@
 //      last spot per pixel through an LDS hash (atomicMax of the spot's list
 //      position), the hash living in the strip buffer before it is needed;
 //   2. survivors are binned by the horizontal strips their footprint touches;
-//   3. pass A (normalize only): each strip is splatted into LDS with ds_add and
-//      reduced to the pattern maximum;
+//   3. pass A (normalize only): each strip is splatted into LDS and reduced to
+//      the pattern maximum;
 //   4. pass B: each strip is splatted again and streamed to HBM scaled by
 //      1/max with non-temporal 16-byte stores; strips without spots are
 //      written as zeros straight from registers.
+//
+// Accumulation is 64-bit fixed point (ds_add_u64): LDS float atomics run ~20x
+// slower than integer ones on gfx950.  A contribution I Kx Ky never exceeds the
+// pattern's largest spot weight P = max|I| (each separable kernel is at most 1;
+// the sub-pixel Gaussian at most I/(2 pi s^2)), so it is rounded to an int32 in
+// units of P / 2^30 and summed into int64 words that cannot overflow.  The
+// pattern maximum is at least P w0^2 (w0 the central tap), so one unit is at
+// most 2^-30 / w0^2 of it: 6e-9 at sigma = 1, 6e-7 at sigma = 10.
 #include <algorithm>
 
 #include "dsx_internal.h"
 
 namespace dsx {
 
-constexpr int kRenderThreads = 256;
+#ifndef DSX_RENDER_THREADS
+#define DSX_RENDER_THREADS 512
+#endif
+#ifndef DSX_STRIP_KB
+#define DSX_STRIP_KB 48
+#endif
+#ifndef DSX_RENDER_BLOCKS_PER_CU
+#define DSX_RENDER_BLOCKS_PER_CU 2
+#endif
+constexpr int kRenderThreads = DSX_RENDER_THREADS;
 constexpr int kMaxSpots = 1024;  // distinct in-frame spots per pattern
 constexpr int kHash = 2048;      // LDS hash slots (load factor <= 1/2)
 constexpr int kMaxStrips = 256;
 constexpr int kBinCap = 4096;    // strip-bin entries per pattern
 constexpr int kMaxR = 64;
-constexpr int kStripBytes = 32 * 1024;
+constexpr int kStripBytes = DSX_STRIP_KB * 1024;
+constexpr float kFixedOne = 1073741824.0f;  // 2^30
+
+typedef long long acc_t;
 
 enum { kModeFast = 0, kModeSubpixel = 1 };
 
@@ -53,6 +73,7 @@ struct RenderArgs {
   float clip;
   int strip_rows;
   int n_strips;
+  int lds_words;   // size of the dynamic LDS buffer in acc_t words
   float *out;
   int *status;
   float taps[kMaxR + 1];
@@ -105,6 +126,27 @@ __device__ __forceinline__ unsigned hash_slot(int key) {
   return (static_cast<unsigned>(key) * 2654435761u) >> (32 - 11);  // kHash = 2^11
 }
 
+__device__ __forceinline__ void acc_add(acc_t *p, float v, float scale) {
+  const long long q = __float2int_rn(v * scale);
+  atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(q));
+}
+
+// int64 fixed point -> float (exact to float rounding)
+__device__ __forceinline__ float acc_to_float(acc_t v) {
+  const int hi = static_cast<int>(v >> 32);
+  const unsigned lo = static_cast<unsigned>(v);
+  return fmaf(static_cast<float>(hi), 4294967296.0f, static_cast<float>(lo));
+}
+
+__device__ __forceinline__ acc_t wave_max_acc(acc_t v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const acc_t o = __shfl_xor(v, off, kWave);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
 template <int kMode>
 struct RenderShared {
   float I[kMaxSpots];
@@ -117,24 +159,30 @@ struct RenderShared {
   int bin_fill[kMaxStrips];
   int ns;
   int overflow;
-  int rows_max;
-  float red[kRenderThreads / kWave];
+  int cols_max;
+  float wsum[kRenderThreads / kWave];
+  acc_t wmax[kRenderThreads / kWave];
 };
 
-template <int kMode>
+// kR > 0: the fast-mode tap radius as a compile-time constant (the footprint
+// loops unroll and the taps stay in scalar registers); 0: taken from the args.
+template <int kMode, int kR>
 __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
-  extern __shared__ float s_dyn[];  // strip buffer; hash table before the passes
+  extern __shared__ acc_t s_acc[];  // strip accumulator; hash table before the passes
   __shared__ RenderShared<kMode> sh;
-  int *h_key = reinterpret_cast<int *>(s_dyn);
+  int *h_key = reinterpret_cast<int *>(s_acc);
   int *h_val = h_key + kHash;
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wid = tid / kWave;
   const int W = a.W, H = a.H;
-  const int R = a.R;
+  const int R = kR > 0 ? kR : a.R;
   const bool vec4 = (W & 3) == 0;
   const bool short_taps = R < min(H, W);
   for (int i = tid; i <= kMaxR; i += kRenderThreads) sh.taps[i] = a.taps[i];
+  // the strip buffer is kept zeroed between uses (each read-out clears it)
+  for (int i = tid; i < a.lds_words; i += kRenderThreads) s_acc[i] = 0;
+  lds_sync();
 
   for (int p = blockIdx.x; p < a.n; p += gridDim.x) {
     const int n = min(a.count[p], a.cap);
@@ -148,7 +196,7 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
     if (tid == 0) {
       sh.ns = 0;
       sh.overflow = 0;
-      sh.rows_max = 0;
+      sh.cols_max = 0;
     }
     for (int s = tid; s <= kMaxStrips; s += kRenderThreads) sh.bin_start[s] = 0;
     lds_sync();
@@ -175,11 +223,12 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
       lds_sync();
     }
     // ---- 1b. survivors -> LDS list with their footprint -----------------
+    float tmax = 0.0f;  // largest spot weight: bound on any single contribution
     for (int j = tid; j < n; j += kRenderThreads) {
       const float4 sp = spots[j];
       float X, Y;
       if (!to_pixel(sp, a, X, Y)) continue;
-      int lo, hi, packed;
+      int lo, hi, packed, width;
       float fxv = 0.0f, fyv = 0.0f;
       if (kMode == kModeFast) {
         const int yi = min(static_cast<int>(Y), H - 1), xi = min(static_cast<int>(X), W - 1);
@@ -199,6 +248,7 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
         lo = max(0, yi - R);
         hi = min(H - 1, yi + R);
         packed = (yi << 16) | xi;
+        width = 2 * R + 1;
       } else {
         const double amp = static_cast<double>(sp.w) * static_cast<double>(a.pref);
         // radius = sqrt(log(clip / amp) / ep); NaN (amp < clip) skips the spot
@@ -211,6 +261,7 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
         const int xhi = min(W, static_cast<int>(floor(static_cast<double>(X) + rad + 1.0))) - 1;
         if (hi < lo || xhi < xlo) continue;
         packed = (xlo << 16) | xhi;
+        width = xhi - xlo + 1;
         fxv = X;
         fyv = Y;
       }
@@ -219,6 +270,7 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
         sh.overflow = 1;
         continue;
       }
+      tmax = fmaxf(tmax, fabsf(sp.w) * (kMode == kModeFast ? 1.0f : a.pref));
       sh.I[q] = sp.w;
       sh.xy[q] = packed;
       if (kMode == kModeSubpixel) {  // fx/fy have one slot in fast mode
@@ -227,10 +279,16 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
       }
       sh.ylo[q] = static_cast<short>(lo);
       sh.yhi[q] = static_cast<short>(hi);
-      atomicMax(&sh.rows_max, hi - lo + 1);
+      atomicMax(&sh.cols_max, width);
       for (int s = lo / a.strip_rows; s <= hi / a.strip_rows; ++s) atomicAdd(&sh.bin_start[s + 1], 1);
     }
+    tmax = wave_max(tmax);
+    if (lane == 0) sh.wsum[wid] = tmax;
     lds_sync();
+    if (kMode == kModeFast) {
+      // the hash lived in the strip buffer: clear it for the splats
+      for (int i = tid; i < min(kHash, a.lds_words); i += kRenderThreads) s_acc[i] = 0;
+    }
     const int ns = min(sh.ns, kMaxSpots);
     if (tid == 0 && a.status) a.status[p] = sh.overflow;
     float *out = a.out + (size_t)p * H * W;
@@ -247,6 +305,10 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
       lds_sync();
       continue;
     }
+    float pmax = 0.0f;
+    for (int w = 0; w < kRenderThreads / kWave; ++w) pmax = fmaxf(pmax, sh.wsum[w]);
+    // units of P / 2^30, with a margin for the rounding of the scaled weights
+    const float scale = pmax > 0.0f ? kFixedOne / (pmax * 1.001f) : 1.0f;
 
     // ---- 2. strip bins ---------------------------------------------------
     if (tid == 0) {
@@ -269,14 +331,15 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
       }
     }
     lds_sync();
-    // footprint rows per item slot: next power of two of the tallest footprint
-    int rows_shift = 0;
-    while ((1 << rows_shift) < sh.rows_max) ++rows_shift;
-    const int rows_mask = (1 << rows_shift) - 1;
+    // splat work item = (spot in bin, footprint column); each item walks the
+    // footprint rows inside the strip, so the lanes of a wave add into
+    // consecutive LDS columns
+    const int ncols = sh.cols_max;
+    const float inv_ncols = 1.0f / static_cast<float>(ncols);
 
-    float scale = 1.0f;
+    float out_scale = 1.0f / scale;  // fixed point -> value (normalize: -> value/max)
     for (int pass = a.normalize ? 0 : 1; pass < 2; ++pass) {
-      float lmax = 0.0f;
+      acc_t lmax = 0;
       for (int s = 0; s < a.n_strips; ++s) {
         const int y0 = s * a.strip_rows;
         const int y1 = min(H, y0 + a.strip_rows);
@@ -299,84 +362,112 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
           }
           continue;
         }
-        // zero the LDS strip
-        if (vec4) {
-          for (int i = tid; i < cnt / 4; i += kRenderThreads)
-            reinterpret_cast<float4 *>(s_dyn)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        } else {
-          for (int i = tid; i < cnt; i += kRenderThreads) s_dyn[i] = 0.0f;
-        }
-        lds_sync();
-        // splat: item = (spot in bin, footprint row)
-        const int n_items = nb << rows_shift;
+        const int n_items = nb * ncols;
         for (int it = tid; it < n_items; it += kRenderThreads) {
-          const int bi = it >> rows_shift;
+          int bi = static_cast<int>(static_cast<float>(it) * inv_ncols);
+          int c = it - bi * ncols;
+          if (c < 0) {
+            --bi;
+            c += ncols;
+          } else if (c >= ncols) {
+            ++bi;
+            c -= ncols;
+          }
           const int q = binned ? sh.bin[b0 + bi] : bi;
-          const int y = sh.ylo[q] + (it & rows_mask);
-          if (y > sh.yhi[q] || y < y0 || y >= y1) continue;
+          const int ya = max(static_cast<int>(sh.ylo[q]), y0);
+          const int yb = min(static_cast<int>(sh.yhi[q]), y1 - 1);
+          if (ya > yb) continue;
           const float I = sh.I[q];
-          float *row = s_dyn + (y - y0) * W;
           if (kMode == kModeFast) {
             const int yi = sh.xy[q] >> 16, xi = sh.xy[q] & 0xffff;
-            const int xlo = max(0, xi - R), xhi = min(W - 1, xi + R);
-            if (short_taps) {
-              const float wy = I * reflect_weight3(y, yi, H, R, sh.taps);
-              if (xi >= R && xi + R < W) {
-                for (int x = xlo; x <= xhi; ++x) atomicAdd(row + x, wy * tap_at(x - xi, R, sh.taps));
-              } else {
-                for (int x = xlo; x <= xhi; ++x)
-                  atomicAdd(row + x, wy * reflect_weight3(x, xi, W, R, sh.taps));
+            const int x = xi - R + c;
+            if (x < 0 || x >= W) continue;
+            float wx;
+            if (!short_taps)
+              wx = reflect_weight(x, xi, W, R, sh.taps);
+            else if (xi >= R && xi + R < W)
+              wx = sh.taps[c > R ? c - R : R - c];
+            else
+              wx = reflect_weight3(x, xi, W, R, sh.taps);
+            const float iw = I * wx;
+            acc_t *col = s_acc + x;
+            if (short_taps && yi >= R && yi + R < H) {
+              // the tap index is uniform across the wave: taps come from
+              // scalar registers, not from a dependent LDS load per row
+#pragma unroll
+              for (int dd = -R; dd <= R; ++dd) {
+                const int y = yi + dd;
+                if (y >= ya && y <= yb) acc_add(col + (y - y0) * W, iw * a.taps[dd < 0 ? -dd : dd], scale);
               }
             } else {
-              const float wy = I * reflect_weight(y, yi, H, R, sh.taps);
-              for (int x = xlo; x <= xhi; ++x) atomicAdd(row + x, wy * reflect_weight(x, xi, W, R, sh.taps));
+              for (int y = ya; y <= yb; ++y) {
+                const float wy = short_taps ? reflect_weight3(y, yi, H, R, sh.taps)
+                                            : reflect_weight(y, yi, H, R, sh.taps);
+                acc_add(col + (y - y0) * W, iw * wy, scale);
+              }
             }
           } else {
-            const float cy = sh.fy[q], cx = sh.fx[q];
-            const float dy = static_cast<float>(y) - cy;
             const int xlo = sh.xy[q] >> 16, xhi = sh.xy[q] & 0xffff;
+            const int x = xlo + c;
+            if (x > xhi) continue;
+            const float dx = static_cast<float>(x) - sh.fx[q];
             const float amp = I * a.pref;
-            for (int x = xlo; x <= xhi; ++x) {
-              const float dx = static_cast<float>(x) - cx;
-              atomicAdd(row + x, amp * expf(a.ep * (dy * dy + dx * dx)));
+            const float cy = sh.fy[q];
+            acc_t *col = s_acc + x;
+            for (int y = ya; y <= yb; ++y) {
+              const float dy = static_cast<float>(y) - cy;
+              acc_add(col + (y - y0) * W, amp * expf(a.ep * (dy * dy + dx * dx)), scale);
             }
           }
         }
         lds_sync();
+        // read out and clear the strip
         if (pass == 0) {
-          if (vec4) {
-            for (int i = tid; i < cnt / 4; i += kRenderThreads) {
-              const float4 v = reinterpret_cast<const float4 *>(s_dyn)[i];
-              lmax = fmaxf(lmax, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
+          // cnt is even whenever W is (vec4); pairs are 16-byte LDS accesses
+          if ((cnt & 1) == 0) {
+#pragma unroll 4
+            for (int i = tid; i < cnt / 2; i += kRenderThreads) {
+              longlong2 *pv = reinterpret_cast<longlong2 *>(s_acc) + i;
+              const longlong2 v = *pv;
+              *pv = make_longlong2(0, 0);
+              const acc_t m = v.x > v.y ? v.x : v.y;
+              lmax = m > lmax ? m : lmax;
             }
           } else {
-            for (int i = tid; i < cnt; i += kRenderThreads) lmax = fmaxf(lmax, s_dyn[i]);
+            for (int i = tid; i < cnt; i += kRenderThreads) {
+              const acc_t v = s_acc[i];
+              s_acc[i] = 0;
+              lmax = v > lmax ? v : lmax;
+            }
+          }
+        } else if (vec4) {
+#pragma unroll 2
+          for (int i = tid; i < cnt / 4; i += kRenderThreads) {
+            longlong2 *pv = reinterpret_cast<longlong2 *>(s_acc) + 2 * i;
+            const longlong2 u = pv[0], v = pv[1];
+            pv[0] = make_longlong2(0, 0);
+            pv[1] = make_longlong2(0, 0);
+            const float4 o = make_float4(acc_to_float(u.x) * out_scale, acc_to_float(u.y) * out_scale,
+                                         acc_to_float(v.x) * out_scale, acc_to_float(v.y) * out_scale);
+            store_nt(o, reinterpret_cast<float4 *>(dst) + i);
           }
         } else {
-          if (vec4) {
-            for (int i = tid; i < cnt / 4; i += kRenderThreads) {
-              float4 v = reinterpret_cast<const float4 *>(s_dyn)[i];
-              v.x *= scale;
-              v.y *= scale;
-              v.z *= scale;
-              v.w *= scale;
-              store_nt(v, reinterpret_cast<float4 *>(dst) + i);
-            }
-          } else {
-            for (int i = tid; i < cnt; i += kRenderThreads) __builtin_nontemporal_store(s_dyn[i] * scale, dst + i);
+          for (int i = tid; i < cnt; i += kRenderThreads) {
+            __builtin_nontemporal_store(acc_to_float(s_acc[i]) * out_scale, dst + i);
+            s_acc[i] = 0;
           }
         }
         lds_sync();
       }
       if (pass == 0) {
-        lmax = wave_max(lmax);
-        if (lane == 0) sh.red[wid] = lmax;
+        lmax = wave_max_acc(lmax);
+        if (lane == 0) sh.wmax[wid] = lmax;
         lds_sync();
-        float m = 0.0f;
-        for (int w = 0; w < kRenderThreads / kWave; ++w) m = fmaxf(m, sh.red[w]);
+        acc_t m = 0;
+        for (int w = 0; w < kRenderThreads / kWave; ++w) m = sh.wmax[w] > m ? sh.wmax[w] : m;
         // a zero maximum (every spot has zero intensity) leaves the zeros as
         // they are instead of producing 0/0
-        scale = m > 0.0f ? 1.0f / m : 1.0f;
+        out_scale = m > 0 ? 1.0f / acc_to_float(m) : 1.0f / scale;
         lds_sync();
       }
     }
@@ -431,7 +522,7 @@ static int render_common(int mode, const float *spot_d, const int *count_d, int 
   a.ep = static_cast<float>(-1.0 / (2.0 * sg * sg));
   a.clip = p->clip_threshold;
   for (int i = 0; i <= kMaxR; ++i) a.taps[i] = (mode == kModeFast && i <= p->radius) ? p->taps[i] : 0.0f;
-  const int budget = kStripBytes / 4;
+  const int budget = kStripBytes / static_cast<int>(sizeof(acc_t));
   if (a.W > budget) {
     set_error("%s: pattern width %d exceeds the LDS strip budget", name, a.W);
     return DSX_EINVAL;
@@ -446,15 +537,32 @@ static int render_common(int mode, const float *spot_d, const int *count_d, int 
   a.n_strips = n_strips;
   a.out = out_d;
   a.status = status_d;
-  const size_t lds = std::max(static_cast<size_t>(rows) * a.W * sizeof(float),
+  const size_t lds = std::max(static_cast<size_t>(rows) * a.W * sizeof(acc_t),
                               static_cast<size_t>(2 * kHash) * sizeof(int));
-  const int blocks = std::min(n, cu_count() * 2);
-  if (mode == kModeFast)
-    hipLaunchKernelGGL(render_kernel<kModeFast>, dim3(blocks), dim3(kRenderThreads), lds,
-                       as_stream(stream), a);
-  else
-    hipLaunchKernelGGL(render_kernel<kModeSubpixel>, dim3(blocks), dim3(kRenderThreads), lds,
-                       as_stream(stream), a);
+  a.lds_words = static_cast<int>(lds / sizeof(acc_t));
+  const int blocks = std::min(n, cu_count() * DSX_RENDER_BLOCKS_PER_CU);
+  hipStream_t st = as_stream(stream);
+  if (mode == kModeSubpixel) {
+    hipLaunchKernelGGL((render_kernel<kModeSubpixel, 0>), dim3(blocks), dim3(kRenderThreads), lds, st, a);
+  } else {
+    switch (a.R) {  // radii of sigma <= 2 px get an unrolled kernel
+#define DSX_RENDER_CASE(r)                                                                     \
+  case r:                                                                                      \
+    hipLaunchKernelGGL((render_kernel<kModeFast, r>), dim3(blocks), dim3(kRenderThreads), lds, st, a); \
+    break;
+      DSX_RENDER_CASE(1)
+      DSX_RENDER_CASE(2)
+      DSX_RENDER_CASE(3)
+      DSX_RENDER_CASE(4)
+      DSX_RENDER_CASE(5)
+      DSX_RENDER_CASE(6)
+      DSX_RENDER_CASE(7)
+      DSX_RENDER_CASE(8)
+#undef DSX_RENDER_CASE
+      default:
+        hipLaunchKernelGGL((render_kernel<kModeFast, 0>), dim3(blocks), dim3(kRenderThreads), lds, st, a);
+    }
+  }
   return check_launch(name);
 }
 

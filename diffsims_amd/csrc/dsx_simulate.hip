@@ -58,16 +58,18 @@ __device__ __forceinline__ double shape_factor(double s, double r, const SimArgs
       const double t = a.sigma * s;
       return 1.0 / (1.0 + t * t);
     }
+    // the transcendental models take their FP64 argument rounded to FP32
+    // (relative error ~1e-7 of the factor)
     case DSX_SF_SINC:
     case DSX_SF_SIN2C: {
-      const double x = a.fac * s;
-      if (x == 0.0) return 0.0;  // the reference's where=denom != 0 leaves 0
-      const double v = fabs(sin(x) / x);
+      const float x = static_cast<float>(a.fac * s);
+      if (x == 0.0f) return 0.0;  // the reference's where=denom != 0 leaves 0
+      const double v = fabsf(sinf(x) / x);
       return a.model == DSX_SF_SINC ? v : v * v;
     }
     case DSX_SF_ATANC: {
-      const double x = a.fac * s;
-      return x == 0.0 ? 1.0 : atan(x) / x;  // nan_to_num(0/0) = 1
+      const float x = static_cast<float>(a.fac * s);
+      return x == 0.0f ? 1.0 : static_cast<double>(atanf(x) / x);  // nan_to_num(0/0) = 1
     }
     case DSX_SF_BINARY:
       return 1.0;
@@ -106,76 +108,120 @@ __device__ __forceinline__ bool intersects(double x, double y, double z, const S
   return (z - a.me <= up) && (z + a.me >= dn);
 }
 
+// Candidate queue per wave: (reflection row << 2) | orientation slot, in the
+// order the sweep produced them (row-major, so per orientation ascending rows).
+constexpr int kQueue = 64 + kOT * kWave;  // one batch plus one worst-case chunk
+constexpr int kWavesPerBlock = kSimThreads / kWave;
+
 __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
+  __shared__ int s_queue[kWavesPerBlock][kQueue];
+  __shared__ double s_R[kWavesPerBlock][kOT][9];
   const int lane = lane_id();
-  const int wave = (blockIdx.x * kSimThreads + threadIdx.x) / kWave;
-  const int n_waves = gridDim.x * (kSimThreads / kWave);
+  const int wib = threadIdx.x / kWave;
+  int *queue = s_queue[wib];
+  const int wave = blockIdx.x * kWavesPerBlock + wib;
+  const int n_waves = gridDim.x * kWavesPerBlock;
   const int n_tiles = (a.n_orient + kOT - 1) / kOT;
   const float two_k = 2.0f * a.k_f;
 
   for (int tile = wave; tile < n_tiles; tile += n_waves) {
     const int o0 = tile * kOT;
-    float R[kOT][9];
+    const int n_here = min(kOT, a.n_orient - o0);
+    float R[kOT][3];  // third rows (beam direction) for the FP32 pretest
     int cnt[kOT];
     float lmax[kOT], lmin[kOT];
+    if (lane < kOT * 9) {
+      const int o = lane / 9, j = lane - o * 9;
+      s_R[wib][o][j] = o < n_here ? a.Rt[(size_t)(o0 + o) * 9 + j] : 0.0;
+    }
 #pragma unroll
     for (int o = 0; o < kOT; ++o) {
-      const bool ok = o0 + o < a.n_orient;
+      const bool ok = o < n_here;
 #pragma unroll
-      for (int j = 0; j < 9; ++j) R[o][j] = ok ? static_cast<float>(a.Rt[(size_t)(o0 + o) * 9 + j]) : 0.0f;
+      for (int j = 0; j < 3; ++j) R[o][j] = ok ? static_cast<float>(a.Rt[(size_t)(o0 + o) * 9 + 6 + j]) : 0.0f;
       cnt[o] = 0;
       lmax[o] = 0.0f;
       lmin[o] = __builtin_huge_valf();
     }
+    int head = 0, qn = 0;  // ring buffer state (uniform)
 
-    for (int base = 0; base < a.n_refl; base += kWave) {
-      const int i = base + lane;
-      const bool v = i < a.n_refl;
-      const float4 g = v ? a.g4[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // Evaluate one batch of up to 64 queued candidates in FP64, all lanes busy.
+    auto drain = [&](int nb) {
+      bool in = false;
+      int row = 0, slot = 0;
+      double x = 0.0, y = 0.0, z = 0.0;
+      float I = 0.0f;
+      if (lane < nb) {
+        const int e = queue[(head + lane) % kQueue];
+        row = e >> 2;
+        slot = e & 3;
+        const int4 h = a.hkl[row];
+        const double hx = h.x, hy = h.y, hz = h.z;
+        const double gx = a.rec[0] * hx + a.rec[1] * hy + a.rec[2] * hz;
+        const double gy = a.rec[3] * hx + a.rec[4] * hy + a.rec[5] * hz;
+        const double gz = a.rec[6] * hx + a.rec[7] * hy + a.rec[8] * hz;
+        const double *Rd = s_R[wib][slot];
+        x = Rd[0] * gx + Rd[1] * gy + Rd[2] * gz;
+        y = Rd[3] * gx + Rd[4] * gy + Rd[5] * gz;
+        z = Rd[6] * gx + Rd[7] * gy + Rd[8] * gz;
+        double s, r;
+        in = intersects(x, y, z, a, s, r);
+        if (in) I = static_cast<float>(shape_factor(s, r, a) * static_cast<double>(a.F2[row]));
+      }
 #pragma unroll
       for (int o = 0; o < kOT; ++o) {
-        if (o0 + o >= a.n_orient) break;  // uniform
-        const float z = fmaf(R[o][6], g.x, fmaf(R[o][7], g.y, R[o][8] * g.z));
-        const bool pre = v && fabsf(fmaf(-two_k, z, g.w)) < a.pre_thr;
-        if (__ballot(pre) == 0) continue;
-        bool in = false;
-        double x = 0.0, y = 0.0, zd = 0.0;
-        float I = 0.0f;
-        if (pre) {
-          const int4 h = a.hkl[i];
-          const double hx = h.x, hy = h.y, hz = h.z;
-          const double gx = a.rec[0] * hx + a.rec[1] * hy + a.rec[2] * hz;
-          const double gy = a.rec[3] * hx + a.rec[4] * hy + a.rec[5] * hz;
-          const double gz = a.rec[6] * hx + a.rec[7] * hy + a.rec[8] * hz;
-          const double *Rd = a.Rt + (size_t)(o0 + o) * 9;  // uniform: scalar loads
-          x = Rd[0] * gx + Rd[1] * gy + Rd[2] * gz;
-          y = Rd[3] * gx + Rd[4] * gy + Rd[5] * gz;
-          zd = Rd[6] * gx + Rd[7] * gy + Rd[8] * gz;
-          double s, r;
-          in = intersects(x, y, zd, a, s, r);
-          if (in) I = static_cast<float>(shape_factor(s, r, a) * static_cast<double>(a.F2[i]));
-        }
-        const uint64_t m = __ballot(in);
+        const bool mine = in && slot == o;
+        const uint64_t m = __ballot(mine);
         if (m == 0) continue;
-        if (in) {
+        if (mine) {
           const int pos = cnt[o] + mask_rank(m);
           lmax[o] = fmaxf(lmax[o], I);
           lmin[o] = fminf(lmin[o], I);
           if (pos < a.cap) {
             const size_t off = (size_t)(o0 + o) * a.cap + pos;
-            a.idx[off] = i;
+            a.idx[off] = row;
             a.spots[off] = make_float4(static_cast<float>(x), static_cast<float>(y),
-                                       static_cast<float>(zd), I);
+                                       static_cast<float>(z), I);
           }
         }
         cnt[o] += __popcll(m);
       }
+      head = (head + nb) % kQueue;
+      qn -= nb;
+    };
+
+    for (int base = 0;; base += kWave) {
+      const bool more = base < a.n_refl;
+      if (more) {
+        const int i = base + lane;
+        const bool v = i < a.n_refl;
+        const float4 g = v ? a.g4[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int o = 0; o < kOT; ++o) {
+          if (o >= n_here) break;  // uniform
+          const float zf = fmaf(R[o][0], g.x, fmaf(R[o][1], g.y, R[o][2] * g.z));
+          const bool pre = v && fabsf(fmaf(-two_k, zf, g.w)) < a.pre_thr;
+          const uint64_t pm = __ballot(pre);
+          if (pm == 0) continue;
+          if (pre) queue[(head + qn + mask_rank(pm)) % kQueue] = (i << 2) | o;
+          qn += __popcll(pm);
+        }
+      }
+      // drain full batches during the sweep, everything at its end
+      const int need = more ? kWave : 1;
+      if (qn >= need) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        while (qn >= need) drain(min(qn, kWave));
+      }
+      if (!more) break;
     }
 
 #pragma unroll
     for (int o = 0; o < kOT; ++o) {
       const int orient = o0 + o;
-      if (orient >= a.n_orient) break;
+      if (o >= n_here) break;
       const float mx = wave_max(lmax[o]);
       const float mn = wave_min(lmin[o]);
       const int n = cnt[o];
@@ -215,6 +261,10 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
         a.maxI[orient] = mx;
       }
     }
+    // the next tile rewrites s_R and the queue
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
   }
 }
 
