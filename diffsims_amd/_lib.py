@@ -35,6 +35,9 @@ SCATTER_XTABLES = 2
 
 MAX_TAP_RADIUS = 64
 
+ADJ_L2 = 0
+ADJ_PATTERN_GRAD = 1
+
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTED = (
     "dsx_version",
@@ -46,6 +49,8 @@ EXPORTED = (
     "dsx_render_max_spots",
     "dsx_render_fast",
     "dsx_render_subpixel",
+    "dsx_render_adjoint",
+    "dsx_structure_factor_adjoint",
 )
 
 
@@ -98,6 +103,10 @@ _SIGS = {
     "dsx_render_max_spots": (_I, []),
     "dsx_render_fast": (_I, [_P, _P, _I, _I, ctypes.POINTER(RenderParams), _P, _P, _P]),
     "dsx_render_subpixel": (_I, [_P, _P, _I, _I, ctypes.POINTER(RenderParams), _P, _P, _P]),
+    "dsx_render_adjoint": (_I, [_P, _P, _P, _I, _I, _P, _P, _I, _P, ctypes.POINTER(SimParams),
+                                ctypes.POINTER(RenderParams), _I, _P, _P, _P, _P, _P, _P, _P]),
+    "dsx_structure_factor_adjoint": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _P,
+                                          _I, _P, _I, _I, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

@@ -1,0 +1,742 @@
+// Adjoint of the pattern path: gradients of a loss on the rendered sub-pixel
+// patterns with respect to the orientation matrices, the reciprocal basis
+// (hence the lattice parameters), the fractional atom positions and the
+// Debye-Waller factors.  diffsims has no gradients; this is the reverse mode
+// of dsx_simulate + dsx_render_subpixel + dsx_structure_factor.
+//
+// render_adjoint_kernel (one block per pattern, persistent over patterns)
+//   recomputes the pattern exactly as render_kernel<subpixel> does (same
+//   spots, clip boxes, 64-bit fixed-point LDS strips), so it needs no stored
+//   pattern:
+//     pass A  pattern maximum m and its first pixel p* (normalize);
+//     pass B  per strip: P = Q/m, then the upstream gradient G = dL/dP
+//             (2 (P - T) for the L2 loss, or given), kept in LDS as G/m;
+//             each spot gathers A = sum G/m K, B = sum G/m K (x - X) over its
+//             box (the transpose of the splat: no atomics on pixels);
+//     then    the max term of the normalisation, -(sum_q G P)/m at p*;
+//             per spot the chain through pixel position, intensity
+//             I = |F|^2 f(s, r) and the excitation error (recomputed from the
+//             integer hkl, the orientation and the reciprocal basis), giving
+//             dL/dRt for the pattern and dL/dg, dL/d|F|^2 per reflection.
+// structure_factor_adjoint_kernel (one thread per reflection)
+//   recomputes F(hkl) and reduces dL/d|F|^2 into dL/dxyz and dL/dB per atom and,
+//   with dL/dg, into dL/d(rec).
+#include <algorithm>
+
+#include "dsx_internal.h"
+#include "dsx_physics.h"
+#include "dsx_splat.h"
+
+namespace dsx {
+
+constexpr int kAdjThreads = 512;
+constexpr int kAdjWaves = kAdjThreads / kWave;
+constexpr int kAdjMaxSpots = 1024;
+constexpr int kAdjMaxStrips = 256;
+constexpr int kAdjBinCap = 4096;
+#ifndef DSX_ADJ_STRIP_KB
+#define DSX_ADJ_STRIP_KB 96
+#endif
+constexpr int kAdjStripBytes = DSX_ADJ_STRIP_KB * 1024;
+constexpr int kPixBits = 22;  // argmax key: value << 22 | (2^22 - 1 - pixel)
+constexpr int kNRed = 11;     // block sums: loss, sum G P, dRt[9]
+
+struct AdjArgs {
+  const int *idx;
+  const float4 *spots;
+  const int *count;
+  int cap;
+  int n;
+  const int4 *hkl;
+  const float *F2;
+  int n_refl;
+  const double *Rt;
+  double rec[9];
+  // physics (fill_physics)
+  double k, k2, me, width, sigma, fac;
+  int model, precession;
+  double phi, Pz, Pt;
+  // detector
+  int H, W;
+  float cal, cx, cy, ca, sa;
+  int mirrored, normalize;
+  float pref, ep, inv_s2, clip;
+  int strip_rows, n_strips, acc_words;
+  int mode;
+  int want_grad;
+  const float *target;
+  float *pattern;
+  float *loss;
+  double *dRt;
+  float *dtab;
+  int *status;
+};
+
+struct AdjShared {
+  float I[kAdjMaxSpots], X[kAdjMaxSpots], Y[kAdjMaxSpots];
+  float A[kAdjMaxSpots], BX[kAdjMaxSpots], BY[kAdjMaxSpots];
+  int xb[kAdjMaxSpots];    // (xlo << 16) | xhi
+  int jpos[kAdjMaxSpots];  // position in the pattern's spot list
+  short ylo[kAdjMaxSpots], yhi[kAdjMaxSpots];
+  short bin[kAdjBinCap];
+  int bin_start[kAdjMaxStrips + 1];
+  int bin_fill[kAdjMaxStrips];
+  int ns, overflow, cols_max;
+  float wmaxf[kAdjWaves];
+  acc_t wkey[kAdjWaves];
+  double red[kAdjWaves][kNRed];
+  double R[9];
+};
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Splat (kGather = false) or gather (true) of every binned spot footprint
+// inside strip rows [y0, y1).  Work item = (spot, footprint column) so a wave
+// touches consecutive LDS columns.
+template <bool kGather>
+__device__ __forceinline__ void strip_pass(const AdjArgs &a, AdjShared &sh, acc_t *acc,
+                                           const float *G, int b0, int nb, bool binned, int y0,
+                                           int y1, float scale, int tid) {
+  const int W = a.W;
+  const int ncols = sh.cols_max;
+  const int n_items = nb * ncols;
+  for (int it = tid; it < n_items; it += kAdjThreads) {
+    const int bi = it / ncols;
+    const int c = it - bi * ncols;
+    const int q = binned ? sh.bin[b0 + bi] : bi;
+    const int ya = max(static_cast<int>(sh.ylo[q]), y0);
+    const int yb = min(static_cast<int>(sh.yhi[q]), y1 - 1);
+    if (ya > yb) continue;
+    const int xlo = sh.xb[q] >> 16, xhi = sh.xb[q] & 0xffff;
+    const int x = xlo + c;
+    if (x > xhi) continue;
+    const float dx = static_cast<float>(x) - sh.X[q];
+    const float cy = sh.Y[q];
+    if (!kGather) {
+      const float amp = sh.I[q] * a.pref;
+      acc_t *col = acc + x;
+      for (int y = ya; y <= yb; ++y) {
+        const float dy = static_cast<float>(y) - cy;
+        acc_add(col + (y - y0) * W, amp * expf(a.ep * (dy * dy + dx * dx)), scale);
+      }
+    } else {
+      float sa = 0.0f, sy = 0.0f;
+      for (int y = ya; y <= yb; ++y) {
+        const float dy = static_cast<float>(y) - cy;
+        const float kg = a.pref * expf(a.ep * (dy * dy + dx * dx)) * G[(y - y0) * W + x];
+        sa += kg;
+        sy = fmaf(kg, dy, sy);
+      }
+      if (sa != 0.0f || sy != 0.0f) {
+        atomicAdd(&sh.A[q], sa);
+        atomicAdd(&sh.BX[q], sa * dx);
+        atomicAdd(&sh.BY[q], sy);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) {
+  extern __shared__ acc_t s_acc[];
+  __shared__ AdjShared sh;
+  float *s_G = reinterpret_cast<float *>(s_acc + a.acc_words);
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int wid = tid / kWave;
+  const int H = a.H, W = a.W;
+  const size_t HW = static_cast<size_t>(H) * W;
+  for (int i = tid; i < a.acc_words; i += kAdjThreads) s_acc[i] = 0;
+  lds_sync();
+
+  for (int p = blockIdx.x; p < a.n; p += gridDim.x) {
+    const int n = min(a.count[p], a.cap);
+    const float4 *spots = a.spots + (size_t)p * a.cap;
+    if (tid == 0) {
+      sh.ns = 0;
+      sh.overflow = 0;
+      sh.cols_max = 1;
+    }
+    if (tid < 9) sh.R[tid] = a.Rt[(size_t)p * 9 + tid];
+    for (int s = tid; s <= kAdjMaxStrips; s += kAdjThreads) sh.bin_start[s] = 0;
+    lds_sync();
+
+    // ---- 1. spots in frame with their clip boxes --------------------------
+    float tmax = 0.0f;
+    for (int j = tid; j < n; j += kAdjThreads) {
+      const float4 sp = spots[j];
+      float X, Y;
+      if (!to_pixel(sp, a, X, Y)) continue;
+      int lo, hi, xlo, xhi;
+      if (!subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, H, W, lo, hi, xlo, xhi)) continue;
+      const int q = atomicAdd(&sh.ns, 1);
+      if (q >= kAdjMaxSpots) {
+        sh.overflow = 1;
+        continue;
+      }
+      tmax = fmaxf(tmax, fabsf(sp.w) * a.pref);
+      sh.I[q] = sp.w;
+      sh.X[q] = X;
+      sh.Y[q] = Y;
+      sh.A[q] = 0.0f;
+      sh.BX[q] = 0.0f;
+      sh.BY[q] = 0.0f;
+      sh.xb[q] = (xlo << 16) | xhi;
+      sh.jpos[q] = j;
+      sh.ylo[q] = static_cast<short>(lo);
+      sh.yhi[q] = static_cast<short>(hi);
+      atomicMax(&sh.cols_max, xhi - xlo + 1);
+      for (int s = lo / a.strip_rows; s <= hi / a.strip_rows; ++s) atomicAdd(&sh.bin_start[s + 1], 1);
+    }
+    tmax = wave_max(tmax);
+    if (lane == 0) sh.wmaxf[wid] = tmax;
+    lds_sync();
+    const int ns = min(sh.ns, kAdjMaxSpots);
+    if (tid == 0 && a.status) a.status[p] = sh.overflow;
+    float pmax = 0.0f;
+    for (int w = 0; w < kAdjWaves; ++w) pmax = fmaxf(pmax, sh.wmaxf[w]);
+    const float scale = pmax > 0.0f ? kFixedOne / (pmax * 1.001f) : 1.0f;
+
+    // ---- 2. strip bins ----------------------------------------------------
+    if (tid == 0) {
+      int acc = 0;
+      for (int s = 0; s < a.n_strips; ++s) {
+        acc += sh.bin_start[s + 1];
+        sh.bin_start[s + 1] = acc;
+      }
+    }
+    lds_sync();
+    const bool binned = sh.bin_start[a.n_strips] <= kAdjBinCap;
+    for (int s = tid; s < a.n_strips; s += kAdjThreads) sh.bin_fill[s] = sh.bin_start[s];
+    lds_sync();
+    if (binned) {
+      for (int q = tid; q < ns; q += kAdjThreads) {
+        for (int s = sh.ylo[q] / a.strip_rows; s <= sh.yhi[q] / a.strip_rows; ++s) {
+          const int pos = atomicAdd(&sh.bin_fill[s], 1);
+          sh.bin[pos] = static_cast<short>(q);
+        }
+      }
+    }
+    lds_sync();
+
+    // ---- 3. pass A: maximum and its first pixel -----------------------------
+    acc_t key = 0;
+    if (a.normalize && ns > 0) {
+      for (int s = 0; s < a.n_strips; ++s) {
+        const int y0 = s * a.strip_rows;
+        const int y1 = min(H, y0 + a.strip_rows);
+        const int b0 = binned ? sh.bin_start[s] : 0;
+        const int nb = binned ? sh.bin_start[s + 1] - b0 : ns;
+        if (nb == 0) continue;
+        strip_pass<false>(a, sh, s_acc, nullptr, b0, nb, binned, y0, y1, scale, tid);
+        lds_sync();
+        const int cnt = (y1 - y0) * W;
+        const int pix0 = y0 * W;
+        for (int i = tid; i < cnt; i += kAdjThreads) {
+          const acc_t v = s_acc[i];
+          s_acc[i] = 0;
+          const acc_t kv = (max(v, static_cast<acc_t>(0)) << kPixBits) |
+                           static_cast<acc_t>((1 << kPixBits) - 1 - (pix0 + i));
+          key = kv > key ? kv : key;
+        }
+        lds_sync();
+      }
+      key = wave_max_acc(key);
+      if (lane == 0) sh.wkey[wid] = key;
+      lds_sync();
+      key = 0;
+      for (int w = 0; w < kAdjWaves; ++w) key = sh.wkey[w] > key ? sh.wkey[w] : key;
+    }
+    const acc_t m_fixed = key >> kPixBits;
+    const int pstar = (1 << kPixBits) - 1 - static_cast<int>(key & ((1 << kPixBits) - 1));
+    // a zero maximum leaves the pattern unnormalised (the reference returns the
+    // zeros as they are)
+    const bool norm = a.normalize && m_fixed > 0;
+    const float out_scale = norm ? 1.0f / acc_to_float(m_fixed) : 1.0f / scale;
+    const float gscale = scale * out_scale;  // dL/dQ = G / m  (1 without normalisation)
+
+    // ---- 4. pass B: pattern, loss, upstream gradient, gather ----------------
+    double lsum = 0.0, csum = 0.0;
+    const float *tgt = a.target + (size_t)p * HW;
+    float *pat = a.pattern ? a.pattern + (size_t)p * HW : nullptr;
+    const bool l2 = a.mode == DSX_ADJ_L2;
+    for (int s = 0; s < a.n_strips; ++s) {
+      const int y0 = s * a.strip_rows;
+      const int y1 = min(H, y0 + a.strip_rows);
+      const int cnt = (y1 - y0) * W;
+      const int b0 = binned ? sh.bin_start[s] : 0;
+      const int nb = binned ? sh.bin_start[s + 1] - b0 : ns;
+      if (nb > 0) {
+        strip_pass<false>(a, sh, s_acc, nullptr, b0, nb, binned, y0, y1, scale, tid);
+        lds_sync();
+      }
+      const size_t off = (size_t)y0 * W;
+      if ((W & 3) == 0) {
+        for (int i = tid; i < cnt / 4; i += kAdjThreads) {
+          longlong2 *pv = reinterpret_cast<longlong2 *>(s_acc) + 2 * i;
+          const longlong2 u = pv[0], v = pv[1];
+          if (nb > 0) {
+            pv[0] = make_longlong2(0, 0);
+            pv[1] = make_longlong2(0, 0);
+          }
+          const float4 P = make_float4(acc_to_float(u.x) * out_scale, acc_to_float(u.y) * out_scale,
+                                       acc_to_float(v.x) * out_scale, acc_to_float(v.y) * out_scale);
+          if (pat) store_nt(P, reinterpret_cast<float4 *>(pat + off) + i);
+          const native_float4 t =
+              __builtin_nontemporal_load(reinterpret_cast<const native_float4 *>(tgt + off) + i);
+          float4 g;
+          if (l2) {
+            const float4 d = make_float4(P.x - t.x, P.y - t.y, P.z - t.z, P.w - t.w);
+            lsum += static_cast<double>(d.x * d.x + d.y * d.y + d.z * d.z + d.w * d.w);
+            g = make_float4(2.0f * d.x, 2.0f * d.y, 2.0f * d.z, 2.0f * d.w);
+          } else {
+            g = make_float4(t.x, t.y, t.z, t.w);
+          }
+          csum += static_cast<double>(g.x * P.x + g.y * P.y + g.z * P.z + g.w * P.w);
+          reinterpret_cast<float4 *>(s_G)[i] =
+              make_float4(g.x * gscale, g.y * gscale, g.z * gscale, g.w * gscale);
+        }
+      } else {
+        for (int i = tid; i < cnt; i += kAdjThreads) {
+          const acc_t v = s_acc[i];
+          if (nb > 0) s_acc[i] = 0;
+          const float P = acc_to_float(v) * out_scale;
+          if (pat) __builtin_nontemporal_store(P, pat + off + i);
+          const float t = tgt[off + i];
+          float g;
+          if (l2) {
+            const float d = P - t;
+            lsum += static_cast<double>(d * d);
+            g = 2.0f * d;
+          } else {
+            g = t;
+          }
+          csum += static_cast<double>(g * P);
+          s_G[i] = g * gscale;
+        }
+      }
+      lds_sync();
+      if (a.want_grad && nb > 0) {
+        strip_pass<true>(a, sh, s_acc, s_G, b0, nb, binned, y0, y1, scale, tid);
+      }
+      lds_sync();
+    }
+    lsum = wave_sum_d(lsum);
+    csum = wave_sum_d(csum);
+    if (lane == 0) {
+      sh.red[wid][0] = lsum;
+      sh.red[wid][1] = csum;
+    }
+    lds_sync();
+    double ctot = 0.0, ltot = 0.0;
+    for (int w = 0; w < kAdjWaves; ++w) {
+      ltot += sh.red[w][0];
+      ctot += sh.red[w][1];
+    }
+
+    if (a.want_grad) {
+      // ---- 5. the max term of the normalisation: -(sum G P)/m at p* --------
+      if (norm) {
+        const float corr = static_cast<float>(ctot) * gscale;
+        const int px = pstar % W, py = pstar / W;
+        for (int q = tid; q < ns; q += kAdjThreads) {
+          const int xlo = sh.xb[q] >> 16, xhi = sh.xb[q] & 0xffff;
+          if (px < xlo || px > xhi || py < sh.ylo[q] || py > sh.yhi[q]) continue;
+          const float dx = static_cast<float>(px) - sh.X[q], dy = static_cast<float>(py) - sh.Y[q];
+          const float kc = corr * a.pref * expf(a.ep * (dx * dx + dy * dy));
+          sh.A[q] -= kc;
+          sh.BX[q] -= kc * dx;
+          sh.BY[q] -= kc * dy;
+        }
+        lds_sync();
+      }
+      // ---- 6. per-spot chain rule ---------------------------------------------
+      double dR[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) dR[i] = 0.0;
+      for (int q = tid; q < ns; q += kAdjThreads) {
+        const double dI = sh.A[q];
+        const double I = sh.I[q];
+        const double dX = I * static_cast<double>(sh.BX[q]) * a.inv_s2;
+        const double dY = I * static_cast<double>(sh.BY[q]) * a.inv_s2;
+        if (dI == 0.0 && dX == 0.0 && dY == 0.0) continue;
+        const double ca = a.ca, sa = a.sa, ic = 1.0 / static_cast<double>(a.cal);
+        double dx, dy;  // d/d(lab x, lab y)
+        if (a.mirrored) {
+          dx = (dX * ca + dY * sa) * ic;
+          dy = (dX * sa - dY * ca) * ic;
+        } else {
+          dx = (dX * ca + dY * sa) * ic;
+          dy = (-dX * sa + dY * ca) * ic;
+        }
+        const int row = a.idx[(size_t)p * a.cap + sh.jpos[q]];
+        const int4 h = a.hkl[row];
+        const double hx = h.x, hy = h.y, hz = h.z;
+        const double g0 = a.rec[0] * hx + a.rec[1] * hy + a.rec[2] * hz;
+        const double g1 = a.rec[3] * hx + a.rec[4] * hy + a.rec[5] * hz;
+        const double g2 = a.rec[6] * hx + a.rec[7] * hy + a.rec[8] * hz;
+        const double *R = sh.R;
+        const double x = R[0] * g0 + R[1] * g1 + R[2] * g2;
+        const double y = R[3] * g0 + R[4] * g1 + R[5] * g2;
+        const double z = R[6] * g0 + R[7] * g1 + R[8] * g2;
+        const double r2 = x * x + y * y;
+        const double r = sqrt(r2);
+        const double qk = sqrt(fmax(a.k2 - r2, 0.0));
+        const double sv = r2 / (a.k + qk) - z;
+        double fs, fr;
+        const double f = shape_factor_grad(sv, r, a, fs, fr);
+        const double F2 = a.F2[row];
+        const double dF2 = dI * f;
+        const double ds = dI * F2 * fs;
+        const double dr = dI * F2 * fr;
+        double gx = dx, gy = dy, gz = -ds;
+        if (qk > 0.0) {
+          gx += ds * x / qk;
+          gy += ds * y / qk;
+        }
+        if (r > 0.0 && dr != 0.0) {
+          gx += dr * x / r;
+          gy += dr * y / r;
+        }
+        // g_lab = Rt g: dL/dRt[i][c] = dL/dg_lab[i] g[c]; dL/dg = Rt^T dL/dg_lab
+        dR[0] += gx * g0; dR[1] += gx * g1; dR[2] += gx * g2;
+        dR[3] += gy * g0; dR[4] += gy * g1; dR[5] += gy * g2;
+        dR[6] += gz * g0; dR[7] += gz * g1; dR[8] += gz * g2;
+        float *dt = a.dtab + (size_t)row * 4;
+        unsafeAtomicAdd(dt + 0, static_cast<float>(R[0] * gx + R[3] * gy + R[6] * gz));
+        unsafeAtomicAdd(dt + 1, static_cast<float>(R[1] * gx + R[4] * gy + R[7] * gz));
+        unsafeAtomicAdd(dt + 2, static_cast<float>(R[2] * gx + R[5] * gy + R[8] * gz));
+        unsafeAtomicAdd(dt + 3, static_cast<float>(dF2));
+      }
+#pragma unroll
+      for (int i = 0; i < 9; ++i) dR[i] = wave_sum_d(dR[i]);
+      lds_sync();  // everyone has read sh.red above
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) sh.red[wid][2 + i] = dR[i];
+      }
+      lds_sync();
+      if (tid < 9) {
+        double v = 0.0;
+        for (int w = 0; w < kAdjWaves; ++w) v += sh.red[w][2 + tid];
+        a.dRt[(size_t)p * 9 + tid] = v;
+      }
+    }
+    if (tid == 0 && a.loss) a.loss[p] = static_cast<float>(ltot);
+    lds_sync();  // LDS state of this pattern is dead before the next one
+  }
+}
+
+// ---------------------------------------------------------------------------
+// structure factor adjoint
+// ---------------------------------------------------------------------------
+
+constexpr int kSfaThreads = 256;
+constexpr int kSfaWaves = kSfaThreads / kWave;
+
+struct SfAdjArgs {
+  const int4 *hkl;
+  int n;
+  double rec[9];
+  const uint4 *xyz_fx;
+  const int *elem;
+  const float *occ;
+  const float *B;
+  int n_atoms;
+  const double *coeff;
+  int n_elem;
+  int scatter;
+  const float4 *dtab;
+  float *dxyz;
+  float *dB;
+  double *drec;
+};
+
+__device__ __forceinline__ float turns_of(uint32_t uh, uint32_t uk, uint32_t ul, const uint4 &x) {
+  const uint32_t t = uh * x.x + uk * x.y + ul * x.z;
+  return static_cast<float>(static_cast<int32_t>(t)) * 2.3283064365386963e-10f;
+}
+
+__global__ __launch_bounds__(kSfaThreads) void structure_factor_adjoint_kernel(SfAdjArgs a) {
+  __shared__ uint4 s_xyz[kSfaThreads];
+  __shared__ float s_occ[kSfaThreads];
+  __shared__ float s_B[kSfaThreads];
+  __shared__ int s_el[kSfaThreads];
+  __shared__ float s_g[4][kSfaThreads];  // per-atom sums of the chunk: dx, dy, dz, dB
+  __shared__ double s_rec[kSfaWaves][9];
+
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int wid = tid / kWave;
+  const int i = blockIdx.x * kSfaThreads + tid;
+  const bool valid = i < a.n;
+  const int4 hkl = valid ? a.hkl[i] : make_int4(0, 0, 0, 0);
+  const float4 dt = valid ? a.dtab[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const double w = dt.w;
+  const double hx = hkl.x, hy = hkl.y, hz = hkl.z;
+  const double gx = a.rec[0] * hx + a.rec[1] * hy + a.rec[2] * hz;
+  const double gy = a.rec[3] * hx + a.rec[4] * hy + a.rec[5] * hz;
+  const double gz = a.rec[6] * hx + a.rec[7] * hy + a.rec[8] * hz;
+  const double g2 = gx * gx + gy * gy + gz * gz;
+  const float mq = -0.25f * static_cast<float>(g2);
+  const uint32_t uh = static_cast<uint32_t>(hkl.x), uk = static_cast<uint32_t>(hkl.y),
+                 ul = static_cast<uint32_t>(hkl.z);
+
+  // pass 1: F and dF/d(g^2)
+  double Fr = 0.0, Fi = 0.0, Qr = 0.0, Qi = 0.0;
+  int cur_el = -1;
+  double f = 0.0, df = 0.0;
+  for (int base = 0; base < a.n_atoms; base += kSfaThreads) {
+    __syncthreads();
+    const int j = base + tid;
+    if (j < a.n_atoms) {
+      s_xyz[tid] = a.xyz_fx[j];
+      s_occ[tid] = a.occ[j];
+      s_B[tid] = a.B[j];
+      s_el[tid] = a.elem[j];
+    }
+    __syncthreads();
+    const int m = min(kSfaThreads, a.n_atoms - base);
+    for (int jj = 0; jj < m; ++jj) {
+      const int el = min(max(s_el[jj], 0), a.n_elem - 1);
+      if (el != cur_el) {
+        f = atomic_f_grad(a.coeff + 10 * el, g2, a.scatter, df);
+        cur_el = el;
+      }
+      float sn, cs;
+      sincospif(2.0f * turns_of(uh, uk, ul, s_xyz[jj]), &sn, &cs);
+      const double od = static_cast<double>(s_occ[jj] * expf(mq * s_B[jj]));
+      const double amp = f * od;
+      // d/dq [f occ exp(-B q/4)] = (f' - f B/4) occ exp(-B q/4)
+      const double damp = (df - 0.25 * f * static_cast<double>(s_B[jj])) * od;
+      Fr = fma(amp, static_cast<double>(cs), Fr);
+      Fi = fma(amp, static_cast<double>(sn), Fi);
+      Qr = fma(damp, static_cast<double>(cs), Qr);
+      Qi = fma(damp, static_cast<double>(sn), Qi);
+    }
+  }
+  // dL/dg = dtab.xyz + 2 g dL/d(g^2), dL/d(g^2) = w 2 Re(conj(F) dF/dq)
+  const double dq = w * 2.0 * (Fr * Qr + Fi * Qi);
+  const double dgx = static_cast<double>(dt.x) + 2.0 * gx * dq;
+  const double dgy = static_cast<double>(dt.y) + 2.0 * gy * dq;
+  const double dgz = static_cast<double>(dt.z) + 2.0 * gz * dq;
+  double dr[9] = {dgx * hx, dgx * hy, dgx * hz, dgy * hx, dgy * hy, dgy * hz,
+                  dgz * hx, dgz * hy, dgz * hz};
+#pragma unroll
+  for (int t = 0; t < 9; ++t) dr[t] = wave_sum_d(dr[t]);
+  if (lane == 0) {
+#pragma unroll
+    for (int t = 0; t < 9; ++t) s_rec[wid][t] = dr[t];
+  }
+
+  // pass 2: per-atom terms, dF2/dx_a = -4 pi h Im(conj(F) t_a),
+  // dF2/dB_a = -(g^2/2) Re(conj(F) t_a)
+  const bool any = __ballot(w != 0.0) != 0;
+  cur_el = -1;
+  for (int base = 0; base < a.n_atoms; base += kSfaThreads) {
+    __syncthreads();
+    const int j = base + tid;
+    if (j < a.n_atoms) {
+      s_xyz[tid] = a.xyz_fx[j];
+      s_occ[tid] = a.occ[j];
+      s_B[tid] = a.B[j];
+      s_el[tid] = a.elem[j];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s_g[c][tid] = 0.0f;
+    __syncthreads();
+    const int m = min(kSfaThreads, a.n_atoms - base);
+    if (any) {
+      for (int jj = 0; jj < m; ++jj) {
+        const int el = min(max(s_el[jj], 0), a.n_elem - 1);
+        if (el != cur_el) {
+          f = atomic_f(a.coeff + 10 * el, g2, a.scatter);
+          cur_el = el;
+        }
+        float sn, cs;
+        sincospif(2.0f * turns_of(uh, uk, ul, s_xyz[jj]), &sn, &cs);
+        const double amp = f * static_cast<double>(s_occ[jj] * expf(mq * s_B[jj]));
+        const double tr = amp * cs, ti = amp * sn;
+        const double im = Fr * ti - Fi * tr;
+        const double re = Fr * tr + Fi * ti;
+        const float ci = static_cast<float>(-4.0 * M_PI * w * im);
+        float vx = ci * static_cast<float>(hx), vy = ci * static_cast<float>(hy),
+              vz = ci * static_cast<float>(hz);
+        float vb = static_cast<float>(-0.5 * g2 * w * re);
+        vx = wave_sum_f(vx);
+        vy = wave_sum_f(vy);
+        vz = wave_sum_f(vz);
+        vb = wave_sum_f(vb);
+        if (lane == 0) {
+          atomicAdd(&s_g[0][jj], vx);
+          atomicAdd(&s_g[1][jj], vy);
+          atomicAdd(&s_g[2][jj], vz);
+          atomicAdd(&s_g[3][jj], vb);
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < m) {
+      const int atom = base + tid;
+      if (s_g[0][tid] != 0.0f) unsafeAtomicAdd(&a.dxyz[3 * atom + 0], s_g[0][tid]);
+      if (s_g[1][tid] != 0.0f) unsafeAtomicAdd(&a.dxyz[3 * atom + 1], s_g[1][tid]);
+      if (s_g[2][tid] != 0.0f) unsafeAtomicAdd(&a.dxyz[3 * atom + 2], s_g[2][tid]);
+      if (s_g[3][tid] != 0.0f) unsafeAtomicAdd(&a.dB[atom], s_g[3][tid]);
+    }
+  }
+  __syncthreads();
+  if (tid < 9) {
+    double v = 0.0;
+    for (int ww = 0; ww < kSfaWaves; ++ww) v += s_rec[ww][tid];
+    if (v != 0.0) unsafeAtomicAdd(&a.drec[tid], v);
+  }
+}
+
+}  // namespace dsx
+
+using namespace dsx;
+
+extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const int *count_d, int cap,
+                                  int n, const int *hkl_d, const float *F2_d, int n_refl,
+                                  const double *Rt_d, const dsx_sim_params *sim,
+                                  const dsx_render_params *rp, int mode, const float *target_d,
+                                  float *pattern_d, float *loss_d, double *dRt_d, float *dtab_d,
+                                  int *status_d, void *stream) {
+  clear_error();
+  if (!sim || !rp || n < 0 || cap < 1 || n_refl < 0) {
+    set_error("dsx_render_adjoint: bad arguments (n=%d cap=%d n_refl=%d)", n, cap, n_refl);
+    return DSX_EINVAL;
+  }
+  if (mode != DSX_ADJ_L2 && mode != DSX_ADJ_PATTERN_GRAD) {
+    set_error("dsx_render_adjoint: unknown mode %d", mode);
+    return DSX_EINVAL;
+  }
+  if (n == 0) return DSX_OK;
+  const bool want_grad = dRt_d != nullptr || dtab_d != nullptr;
+  if (!idx_d || !spot_d || !count_d || !target_d || !Rt_d || (want_grad && (!dRt_d || !dtab_d)) ||
+      (want_grad && n_refl > 0 && (!hkl_d || !F2_d))) {
+    set_error("dsx_render_adjoint: null device pointer");
+    return DSX_EINVAL;
+  }
+  if (rp->height < 1 || rp->width < 1 || rp->height > 32767 || rp->width > 32767 ||
+      static_cast<long long>(rp->height) * rp->width > (1ll << kPixBits) ||
+      !(rp->calibration > 0.0f) || !(rp->sigma > 0.0f) || !(rp->clip_threshold > 0.0f)) {
+    set_error("dsx_render_adjoint: bad pattern shape %dx%d (at most 2^22 pixels), calibration, "
+              "sigma or clip_threshold", rp->height, rp->width);
+    return DSX_EINVAL;
+  }
+  if (sim->shape_model < DSX_SF_LINEAR || sim->shape_model > DSX_SF_LORENTZIAN_PRECESSION ||
+      !(sim->k > 0.0) || sim->shape_width == 0.0) {
+    set_error("dsx_render_adjoint: bad physical parameters");
+    return DSX_EINVAL;
+  }
+  AdjArgs a;
+  a.idx = idx_d;
+  a.spots = reinterpret_cast<const float4 *>(spot_d);
+  a.count = count_d;
+  a.cap = cap;
+  a.n = n;
+  a.hkl = reinterpret_cast<const int4 *>(hkl_d);
+  a.F2 = F2_d;
+  a.n_refl = n_refl;
+  a.Rt = Rt_d;
+  for (int i = 0; i < 9; ++i) a.rec[i] = sim->rec[i];
+  fill_physics(a, sim);
+  a.H = rp->height;
+  a.W = rp->width;
+  a.cal = rp->calibration;
+  a.cx = rp->center_x;
+  a.cy = rp->center_y;
+  a.ca = rp->cos_angle;
+  a.sa = rp->sin_angle;
+  a.mirrored = rp->mirrored;
+  a.normalize = rp->normalize;
+  const double sg = rp->sigma;
+  a.pref = static_cast<float>(1.0 / (2.0 * M_PI * sg * sg));
+  a.ep = static_cast<float>(-1.0 / (2.0 * sg * sg));
+  a.inv_s2 = static_cast<float>(1.0 / (sg * sg));
+  a.clip = rp->clip_threshold;
+  const int budget = kAdjStripBytes / static_cast<int>(sizeof(acc_t) + sizeof(float));
+  if (a.W > budget) {
+    set_error("dsx_render_adjoint: pattern width %d exceeds the LDS strip budget", a.W);
+    return DSX_EINVAL;
+  }
+  int rows = std::max(1, std::min(a.H, budget / a.W));
+  const int n_strips = (a.H + rows - 1) / rows;
+  if (n_strips > kAdjMaxStrips) {
+    set_error("dsx_render_adjoint: pattern height %d needs %d strips (> %d)", a.H, n_strips,
+              kAdjMaxStrips);
+    return DSX_EINVAL;
+  }
+  a.strip_rows = rows;
+  a.n_strips = n_strips;
+  a.acc_words = rows * a.W;
+  a.mode = mode;
+  a.want_grad = want_grad ? 1 : 0;
+  a.target = target_d;
+  a.pattern = pattern_d;
+  a.loss = loss_d;
+  a.dRt = dRt_d;
+  a.dtab = dtab_d;
+  a.status = status_d;
+  const size_t lds = static_cast<size_t>(a.acc_words) * (sizeof(acc_t) + sizeof(float));
+  const int blocks = std::min(n, cu_count());
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&render_adjoint_kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("dsx_render_adjoint: cannot reserve %zu bytes of LDS", lds);
+    return DSX_ELAUNCH;
+  }
+  hipLaunchKernelGGL(render_adjoint_kernel, dim3(blocks), dim3(kAdjThreads), lds, as_stream(stream), a);
+  return check_launch("dsx_render_adjoint");
+}
+
+extern "C" int dsx_structure_factor_adjoint(const int *hkl_d, int n, const double rec[9],
+                                            const uint32_t *xyz_fx_d, const int *elem_d,
+                                            const float *occ_d, const float *B_d, int n_atoms,
+                                            const double *coeff_d, int n_elem, int scatter,
+                                            const float *dtab_d, float *dxyz_d, float *dB_d,
+                                            double *drec_d, void *stream) {
+  clear_error();
+  if (n < 0 || n_atoms < 0 || n_elem < 0 || !rec) {
+    set_error("dsx_structure_factor_adjoint: bad sizes n=%d n_atoms=%d n_elem=%d", n, n_atoms, n_elem);
+    return DSX_EINVAL;
+  }
+  if (scatter != DSX_SCATTER_NONE && scatter != DSX_SCATTER_LOBATO && scatter != DSX_SCATTER_XTABLES) {
+    set_error("dsx_structure_factor_adjoint: unknown scattering parameterisation %d", scatter);
+    return DSX_EINVAL;
+  }
+  if (n == 0) return DSX_OK;
+  if (n_atoms > 0 && n_elem == 0) {
+    set_error("dsx_structure_factor_adjoint: atoms given but no element coefficients");
+    return DSX_EINVAL;
+  }
+  if (!hkl_d || !dtab_d || !drec_d ||
+      (n_atoms > 0 && (!xyz_fx_d || !elem_d || !occ_d || !B_d || !dxyz_d || !dB_d)) ||
+      (n_elem > 0 && !coeff_d)) {
+    set_error("dsx_structure_factor_adjoint: null device pointer");
+    return DSX_EINVAL;
+  }
+  SfAdjArgs a;
+  a.hkl = reinterpret_cast<const int4 *>(hkl_d);
+  a.n = n;
+  for (int i = 0; i < 9; ++i) a.rec[i] = rec[i];
+  a.xyz_fx = reinterpret_cast<const uint4 *>(xyz_fx_d);
+  a.elem = elem_d;
+  a.occ = occ_d;
+  a.B = B_d;
+  a.n_atoms = n_atoms;
+  a.coeff = coeff_d;
+  a.n_elem = n_elem;
+  a.scatter = scatter;
+  a.dtab = reinterpret_cast<const float4 *>(dtab_d);
+  a.dxyz = dxyz_d;
+  a.dB = dB_d;
+  a.drec = drec_d;
+  hipLaunchKernelGGL(structure_factor_adjoint_kernel, dim3((n + kSfaThreads - 1) / kSfaThreads),
+                     dim3(kSfaThreads), 0, as_stream(stream), a);
+  return check_launch("dsx_structure_factor_adjoint");
+}

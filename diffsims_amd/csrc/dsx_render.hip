@@ -35,6 +35,7 @@
 #include <algorithm>
 
 #include "dsx_internal.h"
+#include "dsx_splat.h"
 
 namespace dsx {
 
@@ -54,9 +55,6 @@ constexpr int kMaxStrips = 256;
 constexpr int kBinCap = 4096;    // strip-bin entries per pattern
 constexpr int kMaxR = 64;
 constexpr int kStripBytes = DSX_STRIP_KB * 1024;
-constexpr float kFixedOne = 1073741824.0f;  // 2^30
-
-typedef long long acc_t;
 
 enum { kModeFast = 0, kModeSubpixel = 1 };
 
@@ -108,43 +106,8 @@ __device__ __forceinline__ float reflect_weight3(int y, int c, int n, int R, con
   return tap_at(c - y, R, taps) + tap_at(-1 - c - y, R, taps) + tap_at(2 * n - 1 - c - y, R, taps);
 }
 
-// Pixel position of a lab-frame spot (Simulation2D._get_transformed_coordinates,
-// simulation2d.py:261-285, in pixel units).
-__device__ __forceinline__ bool to_pixel(const float4 &sp, const RenderArgs &a, float &X, float &Y) {
-  const float xp = sp.x / a.cal, yp = sp.y / a.cal;
-  if (a.mirrored) {
-    X = xp * a.ca + yp * a.sa + a.cx;
-    Y = xp * a.sa - yp * a.ca + a.cy;
-  } else {
-    X = xp * a.ca - yp * a.sa + a.cx;
-    Y = xp * a.sa + yp * a.ca + a.cy;
-  }
-  return X >= 0.0f && X < static_cast<float>(a.W) && Y >= 0.0f && Y < static_cast<float>(a.H);
-}
-
 __device__ __forceinline__ unsigned hash_slot(int key) {
   return (static_cast<unsigned>(key) * 2654435761u) >> (32 - 11);  // kHash = 2^11
-}
-
-__device__ __forceinline__ void acc_add(acc_t *p, float v, float scale) {
-  const long long q = __float2int_rn(v * scale);
-  atomicAdd(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(q));
-}
-
-// int64 fixed point -> float (exact to float rounding)
-__device__ __forceinline__ float acc_to_float(acc_t v) {
-  const int hi = static_cast<int>(v >> 32);
-  const unsigned lo = static_cast<unsigned>(v);
-  return fmaf(static_cast<float>(hi), 4294967296.0f, static_cast<float>(lo));
-}
-
-__device__ __forceinline__ acc_t wave_max_acc(acc_t v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    const acc_t o = __shfl_xor(v, off, kWave);
-    v = o > v ? o : v;
-  }
-  return v;
 }
 
 template <int kMode>
@@ -250,16 +213,8 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
         packed = (yi << 16) | xi;
         width = 2 * R + 1;
       } else {
-        const double amp = static_cast<double>(sp.w) * static_cast<double>(a.pref);
-        // radius = sqrt(log(clip / amp) / ep); NaN (amp < clip) skips the spot
-        const double rad2 = log(static_cast<double>(a.clip) / amp) / static_cast<double>(a.ep);
-        if (!(amp > 0.0) || !(rad2 >= 0.0)) continue;
-        const double rad = sqrt(fmin(rad2, 1e12));
-        lo = max(0, static_cast<int>(ceil(static_cast<double>(Y) - rad)));
-        hi = min(H, static_cast<int>(floor(static_cast<double>(Y) + rad + 1.0))) - 1;
-        const int xlo = max(0, static_cast<int>(ceil(static_cast<double>(X) - rad)));
-        const int xhi = min(W, static_cast<int>(floor(static_cast<double>(X) + rad + 1.0))) - 1;
-        if (hi < lo || xhi < xlo) continue;
+        int xlo, xhi;
+        if (!subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, H, W, lo, hi, xlo, xhi)) continue;
         packed = (xlo << 16) | xhi;
         width = xhi - xlo + 1;
         fxv = X;

@@ -16,6 +16,7 @@
 #include <algorithm>
 
 #include "dsx_internal.h"
+#include "dsx_physics.h"
 
 namespace dsx {
 
@@ -48,65 +49,6 @@ struct SimArgs {
   int *ncand;
   float *maxI;
 };
-
-// shape_factor_models.py in FP64 (evaluated for candidates only).
-__device__ __forceinline__ double shape_factor(double s, double r, const SimArgs &a) {
-  switch (a.model) {
-    case DSX_SF_LINEAR:
-      return fmax(1.0 - fabs(s) / a.width, 0.0);
-    case DSX_SF_LORENTZIAN: {
-      const double t = a.sigma * s;
-      return 1.0 / (1.0 + t * t);
-    }
-    // the transcendental models take their FP64 argument rounded to FP32
-    // (relative error ~1e-7 of the factor)
-    case DSX_SF_SINC:
-    case DSX_SF_SIN2C: {
-      const float x = static_cast<float>(a.fac * s);
-      if (x == 0.0f) return 0.0;  // the reference's where=denom != 0 leaves 0
-      const double v = fabsf(sinf(x) / x);
-      return a.model == DSX_SF_SINC ? v : v * v;
-    }
-    case DSX_SF_ATANC: {
-      const float x = static_cast<float>(a.fac * s);
-      return x == 0.0f ? 1.0 : static_cast<double>(atanf(x) / x);  // nan_to_num(0/0) = 1
-    }
-    case DSX_SF_BINARY:
-      return 1.0;
-    case DSX_SF_LORENTZIAN_PRECESSION: {
-      // (sigma/pi) sqrt(2 (u + z) / z^2), u = sigma^2 (r^2 phi^2 - s^2) + 1,
-      // z = sqrt(u^2 + 4 sigma^2 s^2); u + z rewritten for u < 0.
-      const double A = r * a.phi;
-      const double sg2 = a.sigma * a.sigma;
-      const double u = sg2 * ((A - s) * (A + s)) + 1.0;
-      const double ss = 4.0 * sg2 * s * s;
-      const double z = sqrt(u * u + ss);
-      const double upz = u >= 0.0 ? u + z : ss / (z - u);
-      return (a.sigma * 0.31830988618379067) * sqrt(2.0 * upz) / z;
-    }
-    default:
-      return 0.0;
-  }
-}
-
-// Membership test of simulation_generator.py:355-375 in FP64 with the
-// cancellations removed:
-//   k - sqrt(k^2 - r^2)            = r^2 / (k + sqrt(k^2 - r^2))
-//   Pz - sqrt(k^2 - (r +- Pt)^2)   = r (r +- 2 Pt) / (Pz + sqrt(k^2 - (r +- Pt)^2))
-__device__ __forceinline__ bool intersects(double x, double y, double z, const SimArgs &a,
-                                           double &s, double &r) {
-  const double r2 = x * x + y * y;
-  r = sqrt(r2);
-  if (!(r2 < a.k2)) return false;
-  s = r2 / (a.k + sqrt(a.k2 - r2)) - z;
-  if (!a.precession) return fabs(s) < a.me;
-  const double rp = r + a.Pt, rm = r - a.Pt;
-  const double qp = a.k2 - rp * rp, qm = a.k2 - rm * rm;
-  if (!(qp >= 0.0) || !(qm >= 0.0)) return false;
-  const double up = r * (r + 2.0 * a.Pt) / (a.Pz + sqrt(qp));
-  const double dn = r * (r - 2.0 * a.Pt) / (a.Pz + sqrt(qm));
-  return (z - a.me <= up) && (z + a.me >= dn);
-}
 
 // Candidate queue per wave: (reflection row << 2) | orientation slot, in the
 // order the sweep produced them (row-major, so per orientation ascending rows).
@@ -312,21 +254,10 @@ extern "C" int dsx_simulate(const int *hkl_d, const float *g4_d, const float *F2
   a.Rt = Rt_d;
   a.n_orient = n_orient;
   for (int i = 0; i < 9; ++i) a.rec[i] = p->rec[i];
+  fill_physics(a, p);
   const double k = p->k;
-  a.k = k;
-  a.k2 = k * k;
   a.k_f = static_cast<float>(k);
-  a.me = p->max_excitation_error;
-  const double w = p->shape_width;
-  a.width = w;
-  a.sigma = M_PI / w;
-  a.fac = M_PI * p->minima_number / (p->shape_model == DSX_SF_ATANC ? fabs(w) : w);
-  a.model = p->shape_model;
   const double phi = p->precession_angle;
-  a.precession = phi > 0.0;
-  a.phi = phi;
-  a.Pz = k * cos(phi);
-  a.Pt = k * sin(phi);
   a.min_I = static_cast<float>(p->minimum_intensity);
   // Pretest bound: e = |g|^2 - 2 k z = s (2k - z_sphere - z), and |s| <= me
   // (+ r sin(phi) with precession); the slack covers FP32 rounding of e.

@@ -9,6 +9,7 @@
 // (diffsims/utils/sim_utils.py:256-304), computed once per reflection instead of
 // once per orientation.
 #include "dsx_internal.h"
+#include "dsx_physics.h"
 
 namespace dsx {
 
@@ -162,26 +163,6 @@ struct SfArgs {
   float *F2;
   float2 *F;
 };
-
-// f(g^2) in FP64: the Lobato parameterisation sums terms of opposite sign that
-// are up to ~100x larger than the result (e.g. He: a2 = -62.0, a3 = +64.0), so
-// FP32 evaluation loses 2+ digits.  Once per element per reflection; cheap.
-__device__ __forceinline__ double atomic_f(const double *c, double g2, int scatter) {
-  if (scatter == DSX_SCATTER_NONE) return 1.0;
-  double f = 0.0;
-  if (scatter == DSX_SCATTER_LOBATO) {
-#pragma unroll
-    for (int t = 0; t < 5; ++t) {
-      const double bg = c[2 * t + 1] * g2;
-      const double d = 1.0 + bg;
-      f += c[2 * t] * (2.0 + bg) / (d * d);
-    }
-  } else {
-#pragma unroll
-    for (int t = 0; t < 5; ++t) f += c[2 * t] * exp(-0.25 * c[2 * t + 1] * g2);
-  }
-  return f;
-}
 
 // One thread per reflection; atoms are staged through LDS in chunks.  The
 // atomic scattering factor is recomputed only when the element changes along

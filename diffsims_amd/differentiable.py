@@ -1,0 +1,345 @@
+"""Differentiable pattern simulation: hand-written adjoint kernels behind
+``torch.autograd.Function``.
+
+diffsims computes no gradients; this module makes the rendered sub-pixel
+patterns of the pattern path (``calculate_diffraction2d`` followed by
+``get_diffraction_pattern(fast=False)``, simulation_generator.py:134-259,
+simulation2d.py:357-442) differentiable with respect to
+
+* the orientation of every pattern (``Rt``: float64 [n, 9], rows of M^T with
+  M = orix ``to_matrix()``; build it from Euler angles with
+  :func:`bunge_matrices` / :func:`rt_from_matrices` to get Euler gradients),
+* the reciprocal basis ``rec`` (float64 [3, 3], CPU; build it from
+  a, b, c, alpha, beta, gamma with :func:`reciprocal_basis`),
+* the fractional atom positions ``xyz`` (float64 [n_atoms, 3]) and
+* the Debye-Waller factors ``B`` (float64 [n_atoms], A^2).
+
+The discrete structure (the hkl set, which reflections intersect the Ewald
+shell, which pixels each spot's clip box covers) is fixed by the forward
+evaluation and is piecewise constant in the parameters, as in any
+differentiable renderer.
+
+Two functions:
+
+* :meth:`RefinementModel.l2_loss` -- sum over patterns and pixels of
+  (P - T)^2.  When gradients are needed the forward launch computes the loss
+  and every gradient in one pass (fused forward+backward): the render adjoint
+  re-renders each pattern in LDS, forms 2 (P - T) there and pulls it back to
+  the spots without writing the pattern to HBM.
+* :meth:`RefinementModel.patterns` -- the rendered patterns; its backward
+  re-runs the spot kernel and the render adjoint with the incoming dL/dP.
+
+Both recompute F(hkl) and the excitation errors in the backward instead of
+storing them per orientation.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import _lib, engine
+from .crystallography import (PhaseArrays, element_of, highest_hkl, lattice_setup,
+                              reciprocal_metric)
+from .scattering import table
+
+__all__ = ["bunge_matrices", "rt_from_matrices", "reciprocal_basis", "RefinementModel"]
+
+_EXACT_COS = {0.0: 1.0, 60.0: 0.5, 90.0: 0.0, 120.0: -0.5, 180.0: -1.0,
+              240.0: -0.5, 270.0: 0.0, 300.0: 0.5}
+
+
+# ---------------------------------------------------------------------------
+# differentiable parameterisations (torch, float64)
+# ---------------------------------------------------------------------------
+
+
+def bunge_matrices(euler: torch.Tensor) -> torch.Tensor:
+    """Passive Bunge (ZXZ) matrices M(phi1, Phi, phi2) from radians, shape
+    (..., 3, 3); M equals orix ``Rotation.from_euler(euler).to_matrix()``."""
+    p1, P, p2 = euler.unbind(-1)
+    c1, s1 = torch.cos(p1), torch.sin(p1)
+    c, s = torch.cos(P), torch.sin(P)
+    c2, s2 = torch.cos(p2), torch.sin(p2)
+    rows = [
+        torch.stack([c1 * c2 - s1 * s2 * c, s1 * c2 + c1 * s2 * c, s2 * s], -1),
+        torch.stack([-c1 * s2 - s1 * c2 * c, -s1 * s2 + c1 * c2 * c, c2 * s], -1),
+        torch.stack([s1 * s, -c1 * s, c], -1),
+    ]
+    return torch.stack(rows, -2)
+
+
+def rt_from_matrices(M: torch.Tensor) -> torch.Tensor:
+    """(n, 3, 3) orientation matrices -> contiguous [n, 9] rows of M^T, the
+    layout the kernels read (g_lab = M^T g)."""
+    return M.transpose(-1, -2).reshape(-1, 9).contiguous()
+
+
+def _cosd(x: torch.Tensor) -> torch.Tensor:
+    """cos of degrees with the exact values of the lattice setup at multiples
+    of 30/60/90 degrees; the derivative is that of cos."""
+    c = torch.cos(torch.deg2rad(x))
+    r = math.fmod(float(x.detach()), 360.0)
+    if r < 0:
+        r += 360.0
+    if r in _EXACT_COS:
+        c = c + (_EXACT_COS[r] - c).detach()
+    return c
+
+
+def reciprocal_basis(abc, angles) -> torch.Tensor:
+    """rec (3x3, columns a*, b*, c*; g = rec @ hkl) in the x || a, z || c*
+    frame from a, b, c (A) and alpha, beta, gamma (degrees): the closed-form
+    inverse of the lower-triangular direct basis, differentiable."""
+    a, b, c = abc[0], abc[1], abc[2]
+    ca, cb, cg = _cosd(angles[0]), _cosd(angles[1]), _cosd(angles[2])
+    sg = _cosd(90.0 - angles[2])
+    b1, b2 = b * cg, b * sg
+    c1 = c * cb
+    t = (ca - cb * cg) / sg
+    c2 = c * t
+    c3 = c * torch.sqrt(1.0 - cb * cb - t * t)
+    z = torch.zeros((), dtype=a.dtype)
+    return torch.stack([
+        torch.stack([1.0 / a, z, z]),
+        torch.stack([-b1 / (a * b2), 1.0 / b2, z]),
+        torch.stack([(b1 * c2 - b2 * c1) / (a * b2 * c3), -c2 / (b2 * c3), 1.0 / c3]),
+    ])
+
+
+# ---------------------------------------------------------------------------
+# the model
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class _Tables:
+    hkl: torch.Tensor   # int32 [N, 4]
+    g4: torch.Tensor
+    F2: torch.Tensor
+    F: torch.Tensor
+    xyz_fx: torch.Tensor
+    B: torch.Tensor
+    rec: np.ndarray
+    g_max: float
+
+
+def _fixed_point_t(xyz: torch.Tensor) -> torch.Tensor:
+    """float64 fractional positions -> uint32 turns (as int32 bits), on device."""
+    f = xyz.detach().to(torch.float64)
+    f = f - torch.floor(f)
+    u = torch.round(f * 4294967296.0).to(torch.int64) & 0xFFFFFFFF
+    u = torch.where(u >= 2 ** 31, u - 2 ** 32, u)
+    return u.to(torch.int32)
+
+
+class RefinementModel:
+    """Fixed part of a differentiable simulation: one phase's elements and
+    occupancies, the hkl set (from the initial lattice and
+    ``reciprocal_radius``), the physical settings and the detector."""
+
+    def __init__(self, phase: PhaseArrays, reciprocal_radius: float, settings: engine.SimSettings,
+                 shape, sigma: float, calibration: float, clip_threshold: float = 1.0,
+                 normalize: bool = True, in_plane_angle: float = 0.0, mirrored: bool = False,
+                 direct_beam_position=None, scattering_params: Optional[str] = "lobato",
+                 cap: Optional[int] = None, device=None):
+        self.dev = engine._device(device)
+        self.L = _lib.load(require_gpu=True)
+        if not settings.approximate_precession and settings.precession_angle:
+            raise NotImplementedError("gradients of the full precession integral are not implemented")
+        self.settings = settings
+        self.rr = float(reciprocal_radius)
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.sigma = float(sigma)
+        self.calibration = float(calibration)
+        self.render = engine.render_params(self.shape, sigma, calibration, direct_beam_position,
+                                           in_plane_angle, mirrored, normalize, fast=False,
+                                           clip_threshold=clip_threshold)
+        self.render_kw = dict(direct_beam_position=direct_beam_position,
+                              in_plane_angle=in_plane_angle, mirrored=mirrored,
+                              normalize=normalize, fast=False, clip_threshold=clip_threshold)
+        self.scatter = scattering_params
+        # hkl set of the initial lattice, on the device
+        init = engine.build_phase_table(phase, reciprocal_radius, False, scattering_params,
+                                        device=self.dev)
+        self.hkl = init.hkl
+        self.hkl_np = init.hkl[:, :3].cpu().numpy().astype(np.float64)
+        self.n_refl = init.n
+        # atoms sorted by element (the structure-factor kernel's order)
+        elements = [element_of(s) for s in phase.symbols]
+        uniq = sorted(set(elements))
+        tab = table(scattering_params) if scattering_params is not None else {}
+        coeff = np.zeros((max(len(uniq), 1), 10), dtype=np.float64)
+        for i, el in enumerate(uniq):
+            if el in tab:
+                coeff[i] = tab[el]
+        el_idx = np.array([uniq.index(e) for e in elements], dtype=np.int32)
+        self.order = torch.from_numpy(np.argsort(el_idx, kind="stable").astype(np.int64))
+        self.n_atoms = len(elements)
+        self.n_elem = len(uniq)
+        self.elem = torch.from_numpy(np.ascontiguousarray(el_idx[self.order.numpy()])).to(self.dev)
+        self.occ = torch.from_numpy(np.ascontiguousarray(
+            np.asarray(phase.occ, np.float32)[self.order.numpy()])).to(self.dev)
+        self.coeff = torch.from_numpy(coeff).to(self.dev)
+        self.order_dev = self.order.to(self.dev)
+        self.inv_order_dev = torch.argsort(self.order).to(self.dev)
+        if cap is None:
+            cap = min(max(self.n_refl, 1),
+                      2 * engine.expected_spots(init, settings, reciprocal_radius) + 128)
+        self.cap = int(cap)
+
+    # -- forward pieces ------------------------------------------------------
+
+    def tables(self, rec: torch.Tensor, xyz: torch.Tensor, B: torch.Tensor) -> _Tables:
+        """g vectors and |F|^2 of the fixed hkl set for the given parameters."""
+        rec_np = np.ascontiguousarray(rec.detach().cpu().numpy(), dtype=np.float64).reshape(3, 3)
+        xs = xyz.detach().to(self.dev, torch.float64)[self.order_dev]
+        fx = torch.zeros((max(self.n_atoms, 1), 4), dtype=torch.int32, device=self.dev)
+        if self.n_atoms:
+            fx[:self.n_atoms, :3] = _fixed_point_t(xs)
+        Bs = B.detach().to(self.dev, torch.float32)[self.order_dev].contiguous()
+        N = self.n_refl
+        g4 = torch.empty((N, 4), dtype=torch.float32, device=self.dev)
+        F2 = torch.empty(N, dtype=torch.float32, device=self.dev)
+        F = torch.empty((N, 2), dtype=torch.float32, device=self.dev)
+        rec9 = (ctypes.c_double * 9)(*rec_np.reshape(-1).tolist())
+        _lib.check(self.L.dsx_structure_factor(
+            _lib.ptr(self.hkl), N, rec9, _lib.ptr(fx), _lib.ptr(self.elem), _lib.ptr(self.occ),
+            _lib.ptr(Bs), self.n_atoms, _lib.ptr(self.coeff), self.n_elem,
+            engine.SCATTER_IDS[self.scatter], _lib.ptr(g4), _lib.ptr(F2), _lib.ptr(F),
+            _lib.stream_handle(self.dev)), "dsx_structure_factor")
+        g_max = float(np.sqrt(np.max(np.sum((self.hkl_np @ rec_np.T) ** 2, axis=1)))) if N else 0.0
+        return _Tables(self.hkl, g4, F2, F, fx, Bs, rec_np, g_max)
+
+    def _phase_table(self, t: _Tables) -> engine.PhaseTable:
+        return engine.PhaseTable(t.hkl, t.g4, t.F2, t.F, self.n_refl, False, t.g_max, t.rec,
+                                 None, "")
+
+    def spots(self, t: _Tables, Rt: torch.Tensor, batch: Optional[engine.SpotBatch] = None,
+              check: bool = True) -> engine.SpotBatch:
+        """Spot lists of every orientation (dsx_simulate)."""
+        tab = self._phase_table(t)
+        Rt = Rt.detach()
+        if batch is None:
+            batch = engine.empty_batch(tab, int(Rt.shape[0]), self.cap, self.dev)
+        engine.simulate_into(tab, Rt.contiguous(), self.settings, batch)
+        if check:
+            need = int(batch.ncand.max().item()) if batch.n else 0
+            if need > batch.cap:
+                self.cap = need + 64
+                batch = engine.empty_batch(tab, batch.n, self.cap, self.dev)
+                engine.simulate_into(tab, Rt.contiguous(), self.settings, batch)
+        return batch
+
+    def _sim_params(self, t: _Tables) -> _lib.SimParams:
+        return self.settings.params(t.g_max, t.rec)
+
+    def adjoint(self, t: _Tables, Rt: torch.Tensor, batch: engine.SpotBatch, mode: int,
+                target: torch.Tensor, want_grad: bool, pattern: Optional[torch.Tensor] = None,
+                status: Optional[torch.Tensor] = None):
+        """One launch of the render adjoint (+ the structure-factor adjoint when
+        gradients are wanted).  Returns (loss[n], dRt, dtab, dxyz, dB, drec)."""
+        n = batch.n
+        H, W = self.shape
+        if target.shape != (n, H, W) or target.dtype != torch.float32 or not target.is_contiguous():
+            raise ValueError("target must be a contiguous float32 [n, H, W] tensor")
+        loss = torch.empty(n, dtype=torch.float32, device=self.dev)
+        dRt = dtab = None
+        if want_grad:
+            dRt = torch.empty((n, 9), dtype=torch.float64, device=self.dev)
+            dtab = torch.zeros((self.n_refl, 4), dtype=torch.float32, device=self.dev)
+        sync_status = status is None
+        if status is None:
+            status = torch.zeros(n, dtype=torch.int32, device=self.dev)
+        sp = self._sim_params(t)
+        stream = _lib.stream_handle(self.dev)
+        _lib.check(self.L.dsx_render_adjoint(
+            _lib.ptr(batch.idx), _lib.ptr(batch.spots), _lib.ptr(batch.count), batch.cap, n,
+            _lib.ptr(t.hkl), _lib.ptr(t.F2), self.n_refl, _lib.ptr(Rt), ctypes.byref(sp),
+            ctypes.byref(self.render), mode, _lib.ptr(target), _lib.ptr(pattern), _lib.ptr(loss),
+            _lib.ptr(dRt), _lib.ptr(dtab), _lib.ptr(status), stream), "dsx_render_adjoint")
+        if sync_status:
+            engine.check_render_status(status)
+        dxyz = dB = drec = None
+        if want_grad:
+            dxyz_s = torch.zeros((max(self.n_atoms, 1), 3), dtype=torch.float32, device=self.dev)
+            dB_s = torch.zeros(max(self.n_atoms, 1), dtype=torch.float32, device=self.dev)
+            drec = torch.zeros(9, dtype=torch.float64, device=self.dev)
+            rec9 = (ctypes.c_double * 9)(*t.rec.reshape(-1).tolist())
+            _lib.check(self.L.dsx_structure_factor_adjoint(
+                _lib.ptr(t.hkl), self.n_refl, rec9, _lib.ptr(t.xyz_fx), _lib.ptr(self.elem),
+                _lib.ptr(self.occ), _lib.ptr(t.B), self.n_atoms, _lib.ptr(self.coeff), self.n_elem,
+                engine.SCATTER_IDS[self.scatter], _lib.ptr(dtab), _lib.ptr(dxyz_s), _lib.ptr(dB_s),
+                _lib.ptr(drec), stream), "dsx_structure_factor_adjoint")
+            dxyz = dxyz_s[:self.n_atoms][self.inv_order_dev]
+            dB = dB_s[:self.n_atoms][self.inv_order_dev]
+        return loss, dRt, dtab, dxyz, dB, drec
+
+    # -- public differentiable functions --------------------------------------
+
+    def l2_loss(self, Rt: torch.Tensor, rec: torch.Tensor, xyz: torch.Tensor, B: torch.Tensor,
+                target: torch.Tensor) -> torch.Tensor:
+        """sum_patterns sum_pixels (P - T)^2 as a float64 scalar on the device."""
+        return _L2Loss.apply(Rt, rec, xyz, B, self, target)
+
+    def patterns(self, Rt: torch.Tensor, rec: torch.Tensor, xyz: torch.Tensor,
+                 B: torch.Tensor) -> torch.Tensor:
+        """Rendered sub-pixel patterns [n, H, W] (float32), differentiable."""
+        return _Patterns.apply(Rt, rec, xyz, B, self)
+
+
+def _grads_like(ctx_inputs, dRt, drec, dxyz, dB):
+    Rt, rec, xyz, B = ctx_inputs
+    out = []
+    for need, g, ref in ((True, dRt, Rt), (True, drec, rec), (True, dxyz, xyz), (True, dB, B)):
+        out.append(None if g is None else g.to(ref.device, ref.dtype).reshape(ref.shape))
+    return out
+
+
+class _L2Loss(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, Rt, rec, xyz, B, model: RefinementModel, target):
+        want = any(ctx.needs_input_grad[:4])
+        t = model.tables(rec, xyz, B)
+        batch = model.spots(t, Rt)
+        loss, dRt, _, dxyz, dB, drec = model.adjoint(t, Rt.detach(), batch, _lib.ADJ_L2, target, want)
+        if want:
+            g = _grads_like((Rt, rec, xyz, B), dRt, drec, dxyz, dB)
+            ctx.save_for_backward(*g)
+        return loss.to(torch.float64).sum()
+
+    @staticmethod
+    def backward(ctx, gl):
+        dRt, drec, dxyz, dB = ctx.saved_tensors
+        return (gl.to(dRt.device) * dRt, gl.to(drec.device) * drec, gl.to(dxyz.device) * dxyz,
+                gl.to(dB.device) * dB, None, None)
+
+
+class _Patterns(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, Rt, rec, xyz, B, model: RefinementModel):
+        t = model.tables(rec, xyz, B)
+        batch = model.spots(t, Rt)
+        H, W = model.shape
+        out = engine.render_spots(batch.spots, batch.count, batch.cap, model.shape, model.sigma,
+                                  model.calibration, **model.render_kw)
+        ctx.model = model
+        ctx.save_for_backward(Rt.detach(), rec.detach(), xyz.detach(), B.detach())
+        return out
+
+    @staticmethod
+    def backward(ctx, gP):
+        model: RefinementModel = ctx.model
+        Rt, rec, xyz, B = ctx.saved_tensors
+        # recompute the tables and spot lists instead of keeping them alive
+        t = model.tables(rec, xyz, B)
+        batch = model.spots(t, Rt)
+        _, dRt, _, dxyz, dB, drec = model.adjoint(t, Rt, batch, _lib.ADJ_PATTERN_GRAD,
+                                                  gP.contiguous().to(torch.float32), True)
+        g = _grads_like((Rt, rec, xyz, B), dRt, drec, dxyz, dB)
+        return (*g, None)

@@ -185,6 +185,53 @@ int dsx_render_subpixel(const float *spot_d, const int *count_d, int cap, int n,
                         const dsx_render_params *params, float *out_d,
                         int *status_d, void *stream);
 
+/*
+ * Adjoint of dsx_simulate + dsx_render_subpixel.  No counterpart in the
+ * reference (diffsims computes no gradients); it serves gradient-based
+ * refinement of orientation, lattice, positions and Debye-Waller factors.
+ *
+ * For each pattern the sub-pixel pattern P (normalised when params->normalize)
+ * is recomputed from the spot lists exactly as dsx_render_subpixel renders it,
+ * and the upstream gradient G = dL/dP is
+ *   mode DSX_ADJ_L2            G = 2 (P - target), loss_d[o] = sum (P - target)^2
+ *   mode DSX_ADJ_PATTERN_GRAD  G = target (a caller-supplied dL/dP)
+ * G is pulled back through the render, the intensity |F|^2 f(s, r) and the
+ * excitation error, which are recomputed from idx_d, hkl_d, F2_d, Rt_d and
+ * sim->rec (the same inputs dsx_simulate used):
+ *   dRt_d   double [n][9]        dL/dRt of each pattern (overwritten)
+ *   dtab_d  float  [n_refl][4]   {dL/dg (crystal frame, 3), dL/d|F(hkl)|^2},
+ *                                ACCUMULATED (zero it first)
+ * dRt_d and dtab_d may both be NULL (loss / pattern only).  pattern_d
+ * (float [n][H][W], may be NULL) receives P; loss_d (float [n], may be NULL).
+ * status_d as for the renderers.  At most 2^22 pixels per pattern.
+ */
+#define DSX_ADJ_L2 0
+#define DSX_ADJ_PATTERN_GRAD 1
+int dsx_render_adjoint(const int *idx_d, const float *spot_d, const int *count_d, int cap,
+                       int n, const int *hkl_d, const float *F2_d, int n_refl,
+                       const double *Rt_d, const dsx_sim_params *sim,
+                       const dsx_render_params *params, int mode, const float *target_d,
+                       float *pattern_d, float *loss_d, double *dRt_d, float *dtab_d,
+                       int *status_d, void *stream);
+
+/*
+ * Adjoint of dsx_structure_factor.  No counterpart in the reference.  Given
+ * dtab_d = {dL/dg, dL/d|F|^2} per reflection (as accumulated by
+ * dsx_render_adjoint), recomputes F(hkl) and ACCUMULATES (zero first):
+ *   dxyz_d  float  [n_atoms][3]  dL/d(fractional position), atoms in the order
+ *                                given (the order dsx_structure_factor took)
+ *   dB_d    float  [n_atoms]     dL/dB (A^2)
+ *   drec_d  double [9]           dL/d rec (g = rec hkl, row-major), including
+ *                                the |g|-dependence of f and of exp(-B g^2/4)
+ * Other arguments as for dsx_structure_factor.
+ */
+int dsx_structure_factor_adjoint(const int *hkl_d, int n, const double rec[9],
+                                 const uint32_t *xyz_fx_d, const int *elem_d,
+                                 const float *occ_d, const float *B_d, int n_atoms,
+                                 const double *coeff_d, int n_elem, int scatter,
+                                 const float *dtab_d, float *dxyz_d, float *dB_d,
+                                 double *drec_d, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,168 @@
+"""Hand-written adjoint (render + spot kernel + structure factor) against the
+FP64 torch reference gradients of oracle/diffsims_grad.py (GPU).
+
+The reference is pinned to central finite differences of the NumPy oracle in
+tests/test_grad_oracle.py; here the GPU's FP32 loss and gradients are compared
+with it on the same seeded inputs and the same frozen structure.
+Tolerances: loss within 1e-5 relative; each gradient group (orientation,
+reciprocal basis, positions, B) within 2e-5 of that group's largest |component|
+(FP32 pixel sums and FP32 atomics over patterns).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import diffsims_grad as G
+from oracle import diffsims_oracle as O
+from tests.test_grad_oracle import mono_problem
+
+pytestmark = pytest.mark.gpu
+
+TOL_LOSS = 1e-5
+TOL_GRAD = 2e-5
+
+
+def _model(pr, **kw):
+    from diffsims_amd import engine
+    from diffsims_amd.crystallography import PhaseArrays
+    from diffsims_amd.differentiable import RefinementModel
+
+    ph = pr.ph
+    pa = PhaseArrays(tuple(ph.abc), tuple(ph.angles), list(ph.symbols), np.asarray(ph.xyz, float),
+                     np.asarray(ph.occ, float), np.asarray(ph.B, float), "t")
+    p = pr.p
+    st = engine.SimSettings(accelerating_voltage=p.kv, max_excitation_error=p.max_excitation_error,
+                            shape_factor_width=p.shape_factor_width,
+                            shape_model=p.shape_model if isinstance(p.shape_model, str) else "lorentzian",
+                            minima_number=p.shape_kwargs.get("minima_number", 5.0),
+                            precession_angle=p.precession_angle,
+                            minimum_intensity=p.minimum_intensity)
+    return RefinementModel(pa, p.reciprocal_radius, st, pr.shape, pr.sigma, pr.calibration,
+                           clip_threshold=pr.clip, normalize=pr.normalize,
+                           in_plane_angle=pr.in_plane_angle, mirrored=pr.mirrored,
+                           scattering_params=p.scattering_params, **kw)
+
+
+def _params(pr, eu, device):
+    from diffsims_amd.differentiable import bunge_matrices, reciprocal_basis, rt_from_matrices
+
+    e = torch.tensor(eu, dtype=torch.float64, device=device, requires_grad=True)
+    abc = torch.tensor(pr.ph.abc, dtype=torch.float64, requires_grad=True)
+    ang = torch.tensor(pr.ph.angles, dtype=torch.float64, requires_grad=True)
+    xyz = torch.tensor(np.asarray(pr.ph.xyz), dtype=torch.float64, requires_grad=True)
+    B = torch.tensor(np.asarray(pr.ph.B), dtype=torch.float64, requires_grad=True)
+    Rt = rt_from_matrices(bunge_matrices(e))
+    rec = reciprocal_basis(abc, ang)
+    return (e, abc, ang, xyz, B), (Rt, rec, xyz, B)
+
+
+def _reference(pr, eu, weight_fn):
+    """FP64 value and gradients of weight_fn(P) with the oracle's frozen structure."""
+    M = O.bunge_matrix(eu)
+    pats, frozen, tab = G.numpy_patterns(pr, M)
+    e, abc, ang, xyz, B = (torch.tensor(np.asarray(v, dtype=np.float64), requires_grad=True)
+                           for v in (eu, pr.ph.abc, pr.ph.angles, pr.ph.xyz, pr.ph.B))
+    P = G.torch_patterns(pr, G.bunge(e).transpose(-1, -2), G.reciprocal_basis(abc, ang), xyz, B,
+                         frozen, tab.hkl)
+    L = weight_fn(P)
+    L.backward()
+    return float(L.detach()), [t.grad.numpy() for t in (e, abc, ang, xyz, B)], frozen, pats
+
+
+def _compare_grads(got, ref, names=("euler", "abc", "angles", "xyz", "B")):
+    for name, g, r in zip(names, got, ref):
+        g = g.detach().cpu().numpy()
+        scale = max(np.abs(r).max(), 1e-30)
+        err = np.abs(g - r).max() / scale
+        print(f"  {name:7s} max|ref| {scale:.3e}  rel err {err:.2e}", flush=True)
+        assert err <= TOL_GRAD, (name, err)
+
+
+def _check_survivors(model, args, frozen):
+    Rt, rec, xyz, B = args
+    b = model.spots(model.tables(rec, xyz, B), Rt.detach())
+    cnt = b.count.cpu().numpy()
+    idx = b.idx.cpu().numpy()
+    for o, fz in enumerate(frozen):
+        assert np.array_equal(np.sort(idx[o, :cnt[o]]), np.sort(fz.idx)), o
+
+
+CASES = {
+    "lorentzian": dict(model="lorentzian"),
+    "linear": dict(model="linear"),
+    "sin2c": dict(model="sin2c"),
+    "atanc": dict(model="atanc"),
+    "precession": dict(model="lorentzian", precession=1.0),
+    "unnormalised": dict(model="lorentzian", normalize=False),
+    "rotated_mirrored": dict(model="lorentzian", in_plane_angle=23.0, mirrored=True),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_l2_loss_gradients_match_fp64(case):
+    print(f"adjoint L2 case={case}", flush=True)
+    pr = mono_problem(**CASES[case])
+    n = 6
+    eu = O.random_euler(n, 21)
+    target = np.random.default_rng(2).uniform(0, 0.3, (n,) + pr.shape)
+    L_ref, g_ref, frozen, _ = _reference(
+        pr, eu, lambda P: ((P - torch.as_tensor(target)) ** 2).sum())
+    model = _model(pr)
+    leaves, args = _params(pr, eu, "cuda")
+    _check_survivors(model, args, frozen)
+    L = model.l2_loss(*args, torch.tensor(target, dtype=torch.float32, device="cuda"))
+    L.backward()
+    assert abs(float(L) - L_ref) <= TOL_LOSS * abs(L_ref), (float(L), L_ref)
+    _compare_grads([t.grad for t in leaves], g_ref)
+
+
+def test_pattern_gradients_match_fp64():
+    print("adjoint pattern-grad case", flush=True)
+    pr = mono_problem(model="lorentzian")
+    n = 5
+    eu = O.random_euler(n, 33)
+    Wt = np.random.default_rng(4).normal(size=(n,) + pr.shape)
+    L_ref, g_ref, frozen, pats = _reference(pr, eu, lambda P: (P * torch.as_tensor(Wt)).sum())
+    model = _model(pr)
+    leaves, args = _params(pr, eu, "cuda")
+    P = model.patterns(*args)
+    assert np.abs(P.detach().cpu().numpy() - pats).max() <= 1e-5 * pats.max()
+    L = (P * torch.tensor(Wt, dtype=torch.float32, device="cuda")).sum()
+    L.backward()
+    # per-pixel pattern error <= 1e-5 (pattern maximum 1)
+    assert abs(float(L) - L_ref) <= 1e-5 * np.abs(Wt).sum()
+    _compare_grads([t.grad for t in leaves], g_ref)
+
+
+def test_loss_without_grad_and_fused_pattern_output():
+    print("adjoint no-grad case", flush=True)
+    pr = mono_problem()
+    n = 4
+    eu = O.random_euler(n, 8)
+    target = torch.rand((n,) + pr.shape, device="cuda", generator=torch.Generator("cuda").manual_seed(3))
+    model = _model(pr)
+    _, args = _params(pr, eu, "cuda")
+    L1 = model.l2_loss(*args, target)
+    with torch.no_grad():
+        L0 = model.l2_loss(*args, target)
+        P = model.patterns(*args)
+    assert torch.allclose(L0, L1.detach(), rtol=1e-6)
+    assert abs(float(L0) - float(((P - target).double() ** 2).sum())) <= 1e-5 * float(L0)
+
+
+def test_empty_frame_gives_zero_gradients():
+    """Every spot off the detector: P = 0, loss = sum T^2, no gradient."""
+    print("adjoint empty-frame case", flush=True)
+    pr = mono_problem()
+    pr.calibration = 1e-4          # frame spans 0.006 1/A: no reflection lands in it
+    n = 3
+    eu = O.random_euler(n, 9)
+    model = _model(pr)
+    leaves, args = _params(pr, eu, "cuda")
+    target = torch.full((n,) + pr.shape, 0.25, device="cuda")
+    L = model.l2_loss(*args, target)
+    L.backward()
+    assert abs(float(L) - 0.0625 * n * pr.shape[0] * pr.shape[1]) < 1e-6 * float(L)
+    for t in leaves:
+        assert float(t.grad.abs().max()) == 0.0
