@@ -48,7 +48,8 @@ from .crystallography import (PhaseArrays, element_of, highest_hkl, lattice_setu
                               reciprocal_metric)
 from .scattering import table
 
-__all__ = ["bunge_matrices", "rt_from_matrices", "reciprocal_basis", "RefinementModel"]
+__all__ = ["bunge_matrices", "rt_from_matrices", "reciprocal_basis", "RefinementModel",
+           "leaf_parameters", "forward_args"]
 
 _EXACT_COS = {0.0: 1.0, 60.0: 0.5, 90.0: 0.0, 120.0: -0.5, 180.0: -1.0,
               240.0: -0.5, 270.0: 0.0, 300.0: 0.5}
@@ -110,6 +111,27 @@ def reciprocal_basis(abc, angles) -> torch.Tensor:
         torch.stack([-b1 / (a * b2), 1.0 / b2, z]),
         torch.stack([(b1 * c2 - b2 * c1) / (a * b2 * c3), -c2 / (b2 * c3), 1.0 / c3]),
     ])
+
+
+def leaf_parameters(phase: PhaseArrays, euler: np.ndarray, device=None,
+                    requires_grad: bool = True) -> dict:
+    """Leaf tensors of a refinement: Bunge angles (radians, on the device),
+    a/b/c and alpha/beta/gamma (CPU), positions and B (on the device)."""
+    dev = engine._device(device)
+
+    def leaf(v, d):
+        return torch.tensor(np.asarray(v, dtype=np.float64), dtype=torch.float64, device=d,
+                            requires_grad=requires_grad)
+
+    return {"euler": leaf(euler, dev), "abc": leaf(phase.abc, "cpu"),
+            "angles": leaf(phase.angles, "cpu"), "xyz": leaf(phase.xyz, dev),
+            "B": leaf(phase.B, dev)}
+
+
+def forward_args(leaves: dict):
+    """(Rt, rec, xyz, B) for RefinementModel from :func:`leaf_parameters`."""
+    return (rt_from_matrices(bunge_matrices(leaves["euler"])),
+            reciprocal_basis(leaves["abc"], leaves["angles"]), leaves["xyz"], leaves["B"])
 
 
 # ---------------------------------------------------------------------------

@@ -5,6 +5,7 @@ benchmark, the tests and the documentation use one definition:
 
 * ``small_fcc_al``      FCC Al, 500 random orientations, 144x144 (CPU-runnable case)
 * ``template_library``  orthorhombic 40-atom cell, 200,000-orientation SO(3) grid, 256x256
+* ``refinement_monoclinic``  24-atom monoclinic cell, 4,096 perturbed targets (fwd+bwd)
 """
 
 from __future__ import annotations
@@ -85,3 +86,52 @@ def template_library(n_orient: int = 200_000) -> Workload:
     return Workload("ortho40-library", [pa], [M], [2.0], False, st, (256, 256), 0.015, 1.0,
                     notes={"grid": "super-Fibonacci SO(3) spiral", "sigma_px": 1.0,
                            "with_direct_beam": False})
+
+
+@dataclass
+class RefinementWorkload:
+    """Differentiable refinement: a model phase/orientation set and the
+    perturbed ground truth the targets are rendered from."""
+    name: str
+    phase: PhaseArrays              # the model's starting point
+    euler: np.ndarray               # (n, 3) radians, model orientations
+    truth_phase: PhaseArrays        # perturbed parameters that render the targets
+    truth_euler: np.ndarray
+    settings: SimSettings
+    reciprocal_radius: float
+    shape: tuple
+    calibration: float
+    sigma: float
+    noise: float                    # Gaussian noise sigma as a fraction of each pattern's max
+    seed: int
+
+
+def refinement_monoclinic(n_patterns: int = 4096, seed: int = SEED + 3) -> RefinementWorkload:
+    """24-atom monoclinic cell: a, b, c ~ U[5, 10] A, beta ~ U[95, 115] deg,
+    fractional positions U[0, 1), Z in {8, 12, 14, 26} equiprobable,
+    B ~ U[0.3, 1.0] A^2; r = 1.5 1/A, max_excitation_error 0.02, lorentzian,
+    200 kV, no direct beam; n random orientations.  Targets: 256x256 sub-pixel
+    patterns (0.015 1/A/px, sigma 1 px, clip 1, normalised) of the ground truth
+    perturbed by orientation U[-0.5, 0.5] deg per Euler angle, lattice
+    (a, b, c, beta) x (1 + U[-1 %, 1 %]), positions U[-0.02, 0.02], B
+    U[-0.2, 0.2] (kept >= 0.05), plus Gaussian noise of 1 % of each pattern's
+    maximum."""
+    rng = np.random.default_rng(seed)
+    abc = tuple(float(v) for v in rng.uniform(5, 10, 3))
+    beta = float(rng.uniform(95, 115))
+    Z = rng.choice([8, 12, 14, 26], 24)
+    xyz = rng.uniform(0, 1, (24, 3))
+    B = rng.uniform(0.3, 1.0, 24)
+    symbols = [symbol(int(z)) for z in Z]
+    model = PhaseArrays(abc, (90.0, beta, 90.0), symbols, xyz, np.ones(24), B, "mono24")
+    euler = random_euler(n_patterns, rng)
+    f = 1 + rng.uniform(-0.01, 0.01, 4)
+    t_abc = tuple(float(v) for v in np.array(abc) * f[:3])
+    truth = PhaseArrays(t_abc, (90.0, beta * float(f[3]), 90.0), symbols,
+                        xyz + rng.uniform(-0.02, 0.02, xyz.shape), np.ones(24),
+                        np.maximum(B + rng.uniform(-0.2, 0.2, 24), 0.05), "mono24-truth")
+    t_euler = euler + np.deg2rad(rng.uniform(-0.5, 0.5, euler.shape))
+    st = SimSettings(accelerating_voltage=200, max_excitation_error=0.02, shape_model="lorentzian",
+                     minimum_intensity=1e-20)
+    return RefinementWorkload("mono24-refinement", model, euler, truth, t_euler, st, 1.5,
+                              (256, 256), 0.015, 1.0, 0.01, seed)
