@@ -38,6 +38,9 @@ MAX_TAP_RADIUS = 64
 ADJ_L2 = 0
 ADJ_PATTERN_GRAD = 1
 
+RENDER_SPLAT = 0
+RENDER_DENSE = 1
+
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTED = (
     "dsx_version",
@@ -51,6 +54,10 @@ EXPORTED = (
     "dsx_render_subpixel",
     "dsx_render_adjoint",
     "dsx_structure_factor_adjoint",
+    "dsx_simulate_tile_size",
+    "dsx_simulate_ragged",
+    "dsx_render_dense_max_pixels",
+    "dsx_render_fast_ragged",
 )
 
 
@@ -66,6 +73,27 @@ class SimParams(ctypes.Structure):
         ("minimum_intensity", ctypes.c_double),
         ("g_max", ctypes.c_double),
         ("rec", ctypes.c_double * 9),
+    ]
+
+
+class PhaseDesc(ctypes.Structure):
+    """dsx_phase_desc (packed on the host, copied to the device as bytes)."""
+    _fields_ = [
+        ("hkl", ctypes.c_void_p),
+        ("g4", ctypes.c_void_p),
+        ("F2", ctypes.c_void_p),
+        ("n_refl", ctypes.c_int),
+        ("cap", ctypes.c_int),
+        ("rec", ctypes.c_double * 9),
+        ("g_max", ctypes.c_double),
+        ("Rt", ctypes.c_void_p),
+        ("n_orient", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+        ("idx", ctypes.c_void_p),
+        ("spots", ctypes.c_void_p),
+        ("count", ctypes.c_void_p),
+        ("ncand", ctypes.c_void_p),
+        ("maxI", ctypes.c_void_p),
     ]
 
 
@@ -107,6 +135,11 @@ _SIGS = {
                                 ctypes.POINTER(RenderParams), _I, _P, _P, _P, _P, _P, _P, _P]),
     "dsx_structure_factor_adjoint": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _P,
                                           _I, _P, _I, _I, _P, _P, _P, _P, _P]),
+    "dsx_simulate_tile_size": (_I, []),
+    "dsx_simulate_ragged": (_I, [_P, _I, _P, _I, ctypes.POINTER(SimParams), _I, _P]),
+    "dsx_render_dense_max_pixels": (_I, []),
+    "dsx_render_fast_ragged": (_I, [_P, _P, _P, _P, _P, _I, ctypes.POINTER(RenderParams), _I, _I,
+                                    _P, _P, _P]),
 }
 
 _lib = None

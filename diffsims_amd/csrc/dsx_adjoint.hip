@@ -684,11 +684,15 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   a.status = status_d;
   const size_t lds = static_cast<size_t>(a.acc_words) * (sizeof(acc_t) + sizeof(float));
   const int blocks = std::min(n, cu_count());
-  if (hipFuncSetAttribute(reinterpret_cast<const void *>(&render_adjoint_kernel),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)) != hipSuccess) {
-    (void)hipGetLastError();
-    set_error("dsx_render_adjoint: cannot reserve %zu bytes of LDS", lds);
-    return DSX_ELAUNCH;
+  static int reserved = -1;  // raised once, to the largest strip this build uses
+  if (reserved < 0) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&render_adjoint_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kAdjStripBytes) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("dsx_render_adjoint: cannot reserve %zu bytes of LDS", lds);
+      return DSX_ELAUNCH;
+    }
+    reserved = 1;
   }
   hipLaunchKernelGGL(render_adjoint_kernel, dim3(blocks), dim3(kAdjThreads), lds, as_stream(stream), a);
   return check_launch("dsx_render_adjoint");

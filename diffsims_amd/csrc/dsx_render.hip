@@ -62,7 +62,10 @@ struct RenderArgs {
   const float4 *spots;
   const int *count;
   int cap;
-  int n;
+  int n;                     // patterns to render (entries of list, or 0..n-1)
+  const long long *spot_off; // ragged: first spot of pattern p (NULL: p * cap)
+  const int *capv;           // ragged: capacity of pattern p (NULL: cap)
+  const int *list;           // ragged: pattern ids to render (NULL: 0..n-1)
   int H, W;
   float cal, cx, cy, ca, sa;
   int mirrored, normalize;
@@ -147,9 +150,10 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
   for (int i = tid; i < a.lds_words; i += kRenderThreads) s_acc[i] = 0;
   lds_sync();
 
-  for (int p = blockIdx.x; p < a.n; p += gridDim.x) {
-    const int n = min(a.count[p], a.cap);
-    const float4 *spots = a.spots + (size_t)p * a.cap;
+  for (int pi = blockIdx.x; pi < a.n; pi += gridDim.x) {
+    const int p = a.list ? a.list[pi] : pi;
+    const int n = min(a.count[p], a.capv ? a.capv[p] : a.cap);
+    const float4 *spots = a.spots + (a.spot_off ? a.spot_off[p] : (long long)p * a.cap);
     if (kMode == kModeFast) {
       for (int i = tid; i < kHash; i += kRenderThreads) {
         h_key[i] = -1;
@@ -431,19 +435,134 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
   }
 }
 
-static int render_common(int mode, const float *spot_d, const int *count_d, int cap, int n,
-                         const dsx_render_params *p, float *out_d, int *status_d, void *stream,
-                         const char *name) {
-  clear_error();
-  if (!p || n < 0 || cap < 1) {
-    set_error("%s: bad arguments (n=%d cap=%d)", name, n, cap);
-    return DSX_EINVAL;
+// ---------------------------------------------------------------------------
+// Dense fast-mode renderer: the whole pattern lives in LDS.  For patterns
+// with many spots per pixel area (large cells, wide reciprocal radii) it is
+// cheaper than splatting footprints: last write wins through an LDS owner map
+// (atomicMax of the spot's list position), then scipy's separable Gaussian
+// with 'reflect' boundaries as two LDS passes.  No limit on spots per pattern.
+// ---------------------------------------------------------------------------
+
+constexpr int kDenseThreads = 1024;
+constexpr int kDenseMaxPix = 16384;  // two FP32 H x W buffers in 128 KB of LDS
+
+// scipy.ndimage 'reflect' (half-sample symmetric) index for any offset
+__device__ __forceinline__ int reflect_index(int x, int n) {
+  const int period = 2 * n;
+  int m = x % period;
+  if (m < 0) m += period;
+  return m < n ? m : period - 1 - m;
+}
+
+template <int kR>
+__global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs a) {
+  extern __shared__ float s_buf[];
+  __shared__ float s_taps[kMaxR + 1];
+  __shared__ float s_wmax[kDenseThreads / kWave];
+  __shared__ int s_any;
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int wid = tid / kWave;
+  const int H = a.H, W = a.W, HW = a.H * a.W;
+  const int R = kR > 0 ? kR : a.R;
+  const int HWp = (HW + 3) & ~3;
+  int *A = reinterpret_cast<int *>(s_buf);  // owner map, then the row-filtered image
+  float *Af = s_buf;
+  float *B = s_buf + HWp;                   // spot image, then the filtered pattern
+  for (int i = tid; i <= kMaxR; i += kDenseThreads) s_taps[i] = a.taps[i];
+
+  for (int pi = blockIdx.x; pi < a.n; pi += gridDim.x) {
+    const int p = a.list ? a.list[pi] : pi;
+    const int n = min(a.count[p], a.capv ? a.capv[p] : a.cap);
+    const float4 *spots = a.spots + (a.spot_off ? a.spot_off[p] : (long long)p * a.cap);
+    for (int i = tid; i < HW; i += kDenseThreads) {
+      A[i] = -1;
+      B[i] = 0.0f;
+    }
+    if (tid == 0) s_any = 0;
+    lds_sync();
+    // owner of each pixel = largest list position (last write wins)
+    bool any = false;
+    for (int j = tid; j < n; j += kDenseThreads) {
+      float X, Y;
+      if (!to_pixel(spots[j], a, X, Y)) continue;
+      any = true;
+      atomicMax(&A[min(static_cast<int>(Y), H - 1) * W + min(static_cast<int>(X), W - 1)], j);
+    }
+    if (__ballot(any) != 0 && lane == 0) s_any = 1;
+    lds_sync();
+    for (int j = tid; j < n; j += kDenseThreads) {
+      const float4 sp = spots[j];
+      float X, Y;
+      if (!to_pixel(sp, a, X, Y)) continue;
+      const int key = min(static_cast<int>(Y), H - 1) * W + min(static_cast<int>(X), W - 1);
+      if (A[key] == j) B[key] = sp.w;
+    }
+    lds_sync();
+    float *out = a.out + (size_t)p * HW;
+    if (tid == 0 && a.status) a.status[p] = 0;
+    if (s_any == 0) {  // no spot in frame: np.zeros(shape), not normalised
+      for (int i = tid; i < HW; i += kDenseThreads) __builtin_nontemporal_store(0.0f, out + i);
+      lds_sync();
+      continue;
+    }
+    // rows: Af = B (*) taps along x
+    for (int i = tid; i < HW; i += kDenseThreads) {
+      const int y = i / W, x = i - y * W;
+      const float *row = B + y * W;
+      float v = s_taps[0] * row[x];
+      if (x >= R && x + R < W) {
+#pragma unroll
+        for (int d = 1; d <= (kR > 0 ? kR : kMaxR); ++d) {
+          if (kR == 0 && d > R) break;
+          v = fmaf(s_taps[d], row[x - d] + row[x + d], v);
+        }
+      } else {
+        for (int d = 1; d <= R; ++d)
+          v = fmaf(s_taps[d], row[reflect_index(x - d, W)] + row[reflect_index(x + d, W)], v);
+      }
+      Af[i] = v;
+    }
+    lds_sync();
+    // columns: B = Af (*) taps along y, and the maximum
+    float lmax = 0.0f;
+    for (int i = tid; i < HW; i += kDenseThreads) {
+      const int y = i / W, x = i - y * W;
+      const float *col = Af + x;
+      float v = s_taps[0] * col[y * W];
+      if (y >= R && y + R < H) {
+#pragma unroll
+        for (int d = 1; d <= (kR > 0 ? kR : kMaxR); ++d) {
+          if (kR == 0 && d > R) break;
+          v = fmaf(s_taps[d], col[(y - d) * W] + col[(y + d) * W], v);
+        }
+      } else {
+        for (int d = 1; d <= R; ++d)
+          v = fmaf(s_taps[d], col[reflect_index(y - d, H) * W] + col[reflect_index(y + d, H) * W], v);
+      }
+      B[i] = v;
+      lmax = fmaxf(lmax, v);
+    }
+    lmax = wave_max(lmax);
+    if (lane == 0) s_wmax[wid] = lmax;
+    lds_sync();
+    float m = 0.0f;
+    for (int w = 0; w < kDenseThreads / kWave; ++w) m = fmaxf(m, s_wmax[w]);
+    // a zero maximum leaves the zeros as they are instead of producing 0/0
+    const float sc = (a.normalize && m > 0.0f) ? 1.0f / m : 1.0f;
+    if ((HW & 3) == 0) {
+      for (int i = tid; i < HW / 4; i += kDenseThreads) {
+        const float4 v = reinterpret_cast<const float4 *>(B)[i];
+        store_nt(make_float4(v.x * sc, v.y * sc, v.z * sc, v.w * sc), reinterpret_cast<float4 *>(out) + i);
+      }
+    } else {
+      for (int i = tid; i < HW; i += kDenseThreads) __builtin_nontemporal_store(B[i] * sc, out + i);
+    }
+    lds_sync();
   }
-  if (n == 0) return DSX_OK;
-  if (!spot_d || !count_d || !out_d) {
-    set_error("%s: null device pointer", name);
-    return DSX_EINVAL;
-  }
+}
+
+static int fill_render_args(int mode, const dsx_render_params *p, RenderArgs &a, const char *name) {
   if (p->height < 1 || p->width < 1 || p->height > 32767 || p->width > 32767 ||
       !(p->calibration > 0.0f) || !(p->sigma > 0.0f)) {
     set_error("%s: bad pattern shape %dx%d, calibration or sigma", name, p->height, p->width);
@@ -457,11 +576,9 @@ static int render_common(int mode, const float *spot_d, const int *count_d, int 
     set_error("%s: Gaussian radius %d exceeds the supported maximum %d", name, p->radius, kMaxR);
     return DSX_EINVAL;
   }
-  RenderArgs a;
-  a.spots = reinterpret_cast<const float4 *>(spot_d);
-  a.count = count_d;
-  a.cap = cap;
-  a.n = n;
+  a.spot_off = nullptr;
+  a.capv = nullptr;
+  a.list = nullptr;
   a.H = p->height;
   a.W = p->width;
   a.cal = p->calibration;
@@ -490,13 +607,17 @@ static int render_common(int mode, const float *spot_d, const int *count_d, int 
   }
   a.strip_rows = rows;
   a.n_strips = n_strips;
-  a.out = out_d;
-  a.status = status_d;
   const size_t lds = std::max(static_cast<size_t>(rows) * a.W * sizeof(acc_t),
                               static_cast<size_t>(2 * kHash) * sizeof(int));
   a.lds_words = static_cast<int>(lds / sizeof(acc_t));
-  const int blocks = std::min(n, cu_count() * DSX_RENDER_BLOCKS_PER_CU);
-  hipStream_t st = as_stream(stream);
+  return DSX_OK;
+}
+
+static int launch_splat(int mode, const RenderArgs &a, int blocks_per_cu, hipStream_t st,
+                        const char *name) {
+  const size_t lds = static_cast<size_t>(a.lds_words) * sizeof(acc_t);
+  const int bpc = blocks_per_cu > 0 ? blocks_per_cu : DSX_RENDER_BLOCKS_PER_CU;
+  const int blocks = std::min(a.n, cu_count() * bpc);
   if (mode == kModeSubpixel) {
     hipLaunchKernelGGL((render_kernel<kModeSubpixel, 0>), dim3(blocks), dim3(kRenderThreads), lds, st, a);
   } else {
@@ -521,6 +642,76 @@ static int render_common(int mode, const float *spot_d, const int *count_d, int 
   return check_launch(name);
 }
 
+template <int kR>
+static int launch_dense_r(const RenderArgs &a, int blocks, size_t lds, hipStream_t st) {
+  // the LDS limit is raised once per kernel to its maximum, so launches that
+  // a caller records in a graph issue no attribute calls
+  static int reserved = -1;
+  if (reserved < 0) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&render_dense_kernel<kR>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            static_cast<int>(2 * kDenseMaxPix * sizeof(float))) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("dsx_render_fast_ragged: cannot reserve %zu bytes of LDS", lds);
+      return DSX_ELAUNCH;
+    }
+    reserved = 1;
+  }
+  hipLaunchKernelGGL(render_dense_kernel<kR>, dim3(blocks), dim3(kDenseThreads), lds, st, a);
+  return DSX_OK;
+}
+
+static int launch_dense(const RenderArgs &a, int blocks_per_cu, hipStream_t st, const char *name) {
+  const int HW = a.H * a.W;
+  if (HW > kDenseMaxPix) {
+    set_error("%s: the dense renderer holds at most %d pixels (got %dx%d)", name, kDenseMaxPix, a.H, a.W);
+    return DSX_EINVAL;
+  }
+  const size_t lds = 2 * static_cast<size_t>((HW + 3) & ~3) * sizeof(float);
+  const int per_cu = std::max(1, std::min(blocks_per_cu > 0 ? blocks_per_cu : 1,
+                                          static_cast<int>((160 * 1024) / (lds + 1024))));
+  const int blocks = std::min(a.n, cu_count() * per_cu);
+  int st_ = DSX_OK;
+  switch (a.R) {
+    case 1: st_ = launch_dense_r<1>(a, blocks, lds, st); break;
+    case 2: st_ = launch_dense_r<2>(a, blocks, lds, st); break;
+    case 3: st_ = launch_dense_r<3>(a, blocks, lds, st); break;
+    case 4: st_ = launch_dense_r<4>(a, blocks, lds, st); break;
+    case 5: st_ = launch_dense_r<5>(a, blocks, lds, st); break;
+    case 6: st_ = launch_dense_r<6>(a, blocks, lds, st); break;
+    case 7: st_ = launch_dense_r<7>(a, blocks, lds, st); break;
+    case 8: st_ = launch_dense_r<8>(a, blocks, lds, st); break;
+    default: st_ = launch_dense_r<0>(a, blocks, lds, st);
+  }
+  if (st_ != DSX_OK) return st_;
+  return check_launch(name);
+}
+
+static int render_common(int mode, const float *spot_d, const int *count_d, int cap, int n,
+                         const dsx_render_params *p, float *out_d, int *status_d, void *stream,
+                         const char *name) {
+  clear_error();
+  if (!p || n < 0 || cap < 1) {
+    set_error("%s: bad arguments (n=%d cap=%d)", name, n, cap);
+    return DSX_EINVAL;
+  }
+  if (n == 0) return DSX_OK;
+  if (!spot_d || !count_d || !out_d) {
+    set_error("%s: null device pointer", name);
+    return DSX_EINVAL;
+  }
+  RenderArgs a;
+  const int st = fill_render_args(mode, p, a, name);
+  if (st != DSX_OK) return st;
+  a.spots = reinterpret_cast<const float4 *>(spot_d);
+  a.count = count_d;
+  a.cap = cap;
+  a.n = n;
+  a.out = out_d;
+  a.status = status_d;
+  return launch_splat(mode, a, 0, as_stream(stream), name);
+}
+
 }  // namespace dsx
 
 using namespace dsx;
@@ -533,6 +724,40 @@ extern "C" int dsx_render_fast(const float *spot_d, const int *count_d, int cap,
   return render_common(kModeFast, spot_d, count_d, cap, n, params, out_d, status_d, stream,
                        "dsx_render_fast");
 }
+
+extern "C" int dsx_render_fast_ragged(const float *spot_d, const long long *spot_off_d,
+                                      const int *cap_d, const int *count_d, const int *list_d, int n,
+                                      const dsx_render_params *params, int method, int blocks_per_cu,
+                                      float *out_d, int *status_d, void *stream) {
+  const char *name = "dsx_render_fast_ragged";
+  clear_error();
+  if (!params || n < 0 || blocks_per_cu < 0 ||
+      (method != DSX_RENDER_SPLAT && method != DSX_RENDER_DENSE)) {
+    set_error("%s: bad arguments (n=%d method=%d)", name, n, method);
+    return DSX_EINVAL;
+  }
+  if (n == 0) return DSX_OK;
+  if (!spot_d || !spot_off_d || !cap_d || !count_d || !list_d || !out_d) {
+    set_error("%s: null device pointer", name);
+    return DSX_EINVAL;
+  }
+  RenderArgs a;
+  const int st = fill_render_args(kModeFast, params, a, name);
+  if (st != DSX_OK) return st;
+  a.spots = reinterpret_cast<const float4 *>(spot_d);
+  a.count = count_d;
+  a.cap = 1;
+  a.n = n;
+  a.spot_off = spot_off_d;
+  a.capv = cap_d;
+  a.list = list_d;
+  a.out = out_d;
+  a.status = status_d;
+  return method == DSX_RENDER_DENSE ? launch_dense(a, blocks_per_cu, as_stream(stream), name)
+                                    : launch_splat(kModeFast, a, blocks_per_cu, as_stream(stream), name);
+}
+
+extern "C" int dsx_render_dense_max_pixels(void) { return kDenseMaxPix; }
 
 extern "C" int dsx_render_subpixel(const float *spot_d, const int *count_d, int cap, int n,
                                    const dsx_render_params *params, float *out_d, int *status_d,

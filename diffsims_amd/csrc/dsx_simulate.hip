@@ -24,13 +24,6 @@ constexpr int kOT = 4;            // orientations per wave
 constexpr int kSimThreads = 256;  // 4 waves per block
 
 struct SimArgs {
-  const int4 *hkl;
-  const float4 *g4;  // {gx, gy, gz, |g|^2}
-  const float *F2;
-  int n_refl;
-  const double *Rt;  // [n_orient][9], g_lab = Rt g
-  int n_orient;
-  double rec[9];
   double k, k2;
   double me;         // max_excitation_error
   double width;      // shape width
@@ -39,22 +32,30 @@ struct SimArgs {
   int model;
   int precession;
   double phi, Pz, Pt;  // precession angle (rad), k cos(phi), k sin(phi)
+  double sin_phi;
   float min_I;
   float k_f;
-  float pre_thr;
-  int cap;
-  int *idx;
-  float4 *spots;
-  int *count;
-  int *ncand;
-  float *maxI;
+  int n_items;                  // tiles of kOT orientations
+  dsx_phase_desc single;        // plain launch: the one phase
+  const dsx_phase_desc *phases; // ragged launch: device array of phases
+  const int2 *items;            // ragged launch: {phase, first orientation} per tile
 };
+
+// Pretest bound: e = |g|^2 - 2 k z = s (2k - z_sphere - z), and |s| <= me
+// (+ r sin(phi) with precession); the slack covers FP32 rounding of e.
+__host__ __device__ __forceinline__ float pretest_threshold(double me, double k, double gmax,
+                                                            double sin_phi) {
+  const double me_eff = me + (sin_phi > 0.0 ? 1.1 * gmax * sin_phi : 0.0);
+  return static_cast<float>(1.05 * me_eff * (2.0 * k + gmax) + 1e-3 +
+                            1e-6 * (2.0 * k * gmax + gmax * gmax));
+}
 
 // Candidate queue per wave: (reflection row << 2) | orientation slot, in the
 // order the sweep produced them (row-major, so per orientation ascending rows).
 constexpr int kQueue = 64 + kOT * kWave;  // one batch plus one worst-case chunk
 constexpr int kWavesPerBlock = kSimThreads / kWave;
 
+template <bool kRagged>
 __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
   __shared__ int s_queue[kWavesPerBlock][kQueue];
   __shared__ double s_R[kWavesPerBlock][kOT][9];
@@ -63,24 +64,42 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
   int *queue = s_queue[wib];
   const int wave = blockIdx.x * kWavesPerBlock + wib;
   const int n_waves = gridDim.x * kWavesPerBlock;
-  const int n_tiles = (a.n_orient + kOT - 1) / kOT;
   const float two_k = 2.0f * a.k_f;
 
-  for (int tile = wave; tile < n_tiles; tile += n_waves) {
-    const int o0 = tile * kOT;
-    const int n_here = min(kOT, a.n_orient - o0);
+  for (int tile = wave; tile < a.n_items; tile += n_waves) {
+    const dsx_phase_desc *d;
+    int o0;
+    if (kRagged) {
+      const int2 it = a.items[tile];
+      d = a.phases + it.x;
+      o0 = it.y;
+    } else {
+      d = &a.single;
+      o0 = tile * kOT;
+    }
+    const int n_here = min(kOT, d->n_orient - o0);
+    const int4 *hkl = reinterpret_cast<const int4 *>(d->hkl);
+    const float4 *g4 = reinterpret_cast<const float4 *>(d->g4);
+    const float *F2 = d->F2;
+    const double *Rt = d->Rt;
+    const double *rec = d->rec;
+    const int n_refl = d->n_refl;
+    const int cap = d->cap;
+    int *idx_out = d->idx;
+    float4 *spot_out = reinterpret_cast<float4 *>(d->spots);
+    const float pre_thr = pretest_threshold(a.me, a.k, d->g_max, a.sin_phi);
     float R[kOT][3];  // third rows (beam direction) for the FP32 pretest
     int cnt[kOT];
     float lmax[kOT], lmin[kOT];
     if (lane < kOT * 9) {
       const int o = lane / 9, j = lane - o * 9;
-      s_R[wib][o][j] = o < n_here ? a.Rt[(size_t)(o0 + o) * 9 + j] : 0.0;
+      s_R[wib][o][j] = o < n_here ? Rt[(size_t)(o0 + o) * 9 + j] : 0.0;
     }
 #pragma unroll
     for (int o = 0; o < kOT; ++o) {
       const bool ok = o < n_here;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) R[o][j] = ok ? static_cast<float>(a.Rt[(size_t)(o0 + o) * 9 + 6 + j]) : 0.0f;
+      for (int j = 0; j < 3; ++j) R[o][j] = ok ? static_cast<float>(Rt[(size_t)(o0 + o) * 9 + 6 + j]) : 0.0f;
       cnt[o] = 0;
       lmax[o] = 0.0f;
       lmin[o] = __builtin_huge_valf();
@@ -97,18 +116,18 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
         const int e = queue[(head + lane) % kQueue];
         row = e >> 2;
         slot = e & 3;
-        const int4 h = a.hkl[row];
+        const int4 h = hkl[row];
         const double hx = h.x, hy = h.y, hz = h.z;
-        const double gx = a.rec[0] * hx + a.rec[1] * hy + a.rec[2] * hz;
-        const double gy = a.rec[3] * hx + a.rec[4] * hy + a.rec[5] * hz;
-        const double gz = a.rec[6] * hx + a.rec[7] * hy + a.rec[8] * hz;
+        const double gx = rec[0] * hx + rec[1] * hy + rec[2] * hz;
+        const double gy = rec[3] * hx + rec[4] * hy + rec[5] * hz;
+        const double gz = rec[6] * hx + rec[7] * hy + rec[8] * hz;
         const double *Rd = s_R[wib][slot];
         x = Rd[0] * gx + Rd[1] * gy + Rd[2] * gz;
         y = Rd[3] * gx + Rd[4] * gy + Rd[5] * gz;
         z = Rd[6] * gx + Rd[7] * gy + Rd[8] * gz;
         double s, r;
         in = intersects(x, y, z, a, s, r);
-        if (in) I = static_cast<float>(shape_factor(s, r, a) * static_cast<double>(a.F2[row]));
+        if (in) I = static_cast<float>(shape_factor(s, r, a) * static_cast<double>(F2[row]));
       }
 #pragma unroll
       for (int o = 0; o < kOT; ++o) {
@@ -119,10 +138,10 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
           const int pos = cnt[o] + mask_rank(m);
           lmax[o] = fmaxf(lmax[o], I);
           lmin[o] = fminf(lmin[o], I);
-          if (pos < a.cap) {
-            const size_t off = (size_t)(o0 + o) * a.cap + pos;
-            a.idx[off] = row;
-            a.spots[off] = make_float4(static_cast<float>(x), static_cast<float>(y),
+          if (pos < cap) {
+            const size_t off = (size_t)(o0 + o) * cap + pos;
+            idx_out[off] = row;
+            spot_out[off] = make_float4(static_cast<float>(x), static_cast<float>(y),
                                        static_cast<float>(z), I);
           }
         }
@@ -133,16 +152,16 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
     };
 
     for (int base = 0;; base += kWave) {
-      const bool more = base < a.n_refl;
+      const bool more = base < n_refl;
       if (more) {
         const int i = base + lane;
-        const bool v = i < a.n_refl;
-        const float4 g = v ? a.g4[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const bool v = i < n_refl;
+        const float4 g = v ? g4[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
         for (int o = 0; o < kOT; ++o) {
           if (o >= n_here) break;  // uniform
           const float zf = fmaf(R[o][0], g.x, fmaf(R[o][1], g.y, R[o][2] * g.z));
-          const bool pre = v && fabsf(fmaf(-two_k, zf, g.w)) < a.pre_thr;
+          const bool pre = v && fabsf(fmaf(-two_k, zf, g.w)) < pre_thr;
           const uint64_t pm = __ballot(pre);
           if (pm == 0) continue;
           if (pre) queue[(head + qn + mask_rank(pm)) % kQueue] = (i << 2) | o;
@@ -167,7 +186,7 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
       const float mx = wave_max(lmax[o]);
       const float mn = wave_min(lmin[o]);
       const int n = cnt[o];
-      const int nw = min(n, a.cap);
+      const int nw = min(n, cap);
       int final_count = nw;
       const float thr = mx * a.min_I;
       // mx == 0 with n > 0: every intersecting reflection is kinematically
@@ -175,7 +194,7 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
       // round-off passes the relative cut), so they are kept with I = 0.
       if (n > 0 && mx > 0.0f && !(mn > thr)) {
         __threadfence_block();
-        const size_t off = (size_t)orient * a.cap;
+        const size_t off = (size_t)orient * cap;
         int w = 0;
         for (int b = 0; b < nw; b += kWave) {
           const int j = b + lane;
@@ -183,24 +202,24 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
           int id = 0;
           bool keep = false;
           if (j < nw) {
-            sp = a.spots[off + j];
-            id = a.idx[off + j];
+            sp = spot_out[off + j];
+            id = idx_out[off + j];
             keep = sp.w > thr;
           }
           const uint64_t km = __ballot(keep);
           if (keep) {
             const int pos = w + mask_rank(km);
-            a.spots[off + pos] = sp;
-            a.idx[off + pos] = id;
+            spot_out[off + pos] = sp;
+            idx_out[off + pos] = id;
           }
           w += __popcll(km);
         }
         final_count = w;
       }
       if (lane == 0) {
-        a.count[orient] = final_count;
-        a.ncand[orient] = n;
-        a.maxI[orient] = mx;
+        d->count[orient] = final_count;
+        d->ncand[orient] = n;
+        d->maxI[orient] = mx;
       }
     }
     // the next tile rewrites s_R and the queue
@@ -208,6 +227,38 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
   }
+}
+
+static int check_physics(const dsx_sim_params *p, const char *name) {
+  if (p->shape_model < DSX_SF_LINEAR || p->shape_model > DSX_SF_LORENTZIAN_PRECESSION) {
+    set_error("%s: unknown shape model %d", name, p->shape_model);
+    return DSX_EINVAL;
+  }
+  if (!(p->k > 0.0) || !(p->max_excitation_error > 0.0) || p->shape_width == 0.0 ||
+      !(p->precession_angle >= 0.0) || !(p->g_max >= 0.0)) {
+    set_error("%s: bad physical parameters", name);
+    return DSX_EINVAL;
+  }
+  if (p->shape_model == DSX_SF_LORENTZIAN_PRECESSION && !(p->precession_angle > 0.0)) {
+    set_error("%s: the precession shape factor needs a positive precession angle", name);
+    return DSX_EINVAL;
+  }
+  return DSX_OK;
+}
+
+static void fill_args(SimArgs &a, const dsx_sim_params *p) {
+  fill_physics(a, p);
+  a.k_f = static_cast<float>(p->k);
+  a.sin_phi = p->precession_angle > 0.0 ? sin(p->precession_angle) : 0.0;
+  a.min_I = static_cast<float>(p->minimum_intensity);
+  a.phases = nullptr;
+  a.items = nullptr;
+}
+
+static int launch_blocks(int n_items, int blocks_per_cu) {
+  const int waves_per_block = kSimThreads / kWave;
+  const int blocks = (n_items + waves_per_block - 1) / waves_per_block;
+  return std::max(1, std::min(blocks, cu_count() * (blocks_per_cu > 0 ? blocks_per_cu : 8)));
 }
 
 }  // namespace dsx
@@ -229,52 +280,59 @@ extern "C" int dsx_simulate(const int *hkl_d, const float *g4_d, const float *F2
     set_error("dsx_simulate: null device pointer");
     return DSX_EINVAL;
   }
-  if (p->shape_model < DSX_SF_LINEAR || p->shape_model > DSX_SF_LORENTZIAN_PRECESSION) {
-    set_error("dsx_simulate: unknown shape model %d", p->shape_model);
-    return DSX_EINVAL;
-  }
-  if (!(p->k > 0.0) || !(p->max_excitation_error > 0.0) || p->shape_width == 0.0 ||
-      !(p->precession_angle >= 0.0) || !(p->g_max >= 0.0)) {
-    set_error("dsx_simulate: bad physical parameters");
-    return DSX_EINVAL;
-  }
-  if (p->shape_model == DSX_SF_LORENTZIAN_PRECESSION && !(p->precession_angle > 0.0)) {
-    set_error("dsx_simulate: the precession shape factor needs a positive precession angle");
-    return DSX_EINVAL;
-  }
+  const int st = check_physics(p, "dsx_simulate");
+  if (st != DSX_OK) return st;
   if ((size_t)n_orient * (size_t)cap > (size_t)1 << 40) {
     set_error("dsx_simulate: n_orient*cap too large");
     return DSX_EINVAL;
   }
   SimArgs a;
-  a.hkl = reinterpret_cast<const int4 *>(hkl_d);
-  a.g4 = reinterpret_cast<const float4 *>(g4_d);
-  a.F2 = F2_d;
-  a.n_refl = n_refl;
-  a.Rt = Rt_d;
-  a.n_orient = n_orient;
-  for (int i = 0; i < 9; ++i) a.rec[i] = p->rec[i];
-  fill_physics(a, p);
-  const double k = p->k;
-  a.k_f = static_cast<float>(k);
-  const double phi = p->precession_angle;
-  a.min_I = static_cast<float>(p->minimum_intensity);
-  // Pretest bound: e = |g|^2 - 2 k z = s (2k - z_sphere - z), and |s| <= me
-  // (+ r sin(phi) with precession); the slack covers FP32 rounding of e.
-  const double gmax = p->g_max;
-  const double me_eff = p->max_excitation_error + (phi > 0.0 ? 1.1 * gmax * sin(phi) : 0.0);
-  a.pre_thr = static_cast<float>(1.05 * me_eff * (2.0 * k + gmax) + 1e-3 +
-                                 1e-6 * (2.0 * k * gmax + gmax * gmax));
-  a.cap = cap;
-  a.idx = idx_d;
-  a.spots = reinterpret_cast<float4 *>(spot_d);
-  a.count = count_d;
-  a.ncand = ncand_d;
-  a.maxI = maxI_d;
-  const int n_tiles = (n_orient + kOT - 1) / kOT;
-  const int waves_per_block = kSimThreads / kWave;
-  int blocks = (n_tiles + waves_per_block - 1) / waves_per_block;
-  blocks = std::min(blocks, cu_count() * 8);
-  hipLaunchKernelGGL(simulate_kernel, dim3(blocks), dim3(kSimThreads), 0, as_stream(stream), a);
+  fill_args(a, p);
+  dsx_phase_desc &d = a.single;
+  d.hkl = hkl_d;
+  d.g4 = g4_d;
+  d.F2 = F2_d;
+  d.n_refl = n_refl;
+  d.cap = cap;
+  for (int i = 0; i < 9; ++i) d.rec[i] = p->rec[i];
+  d.g_max = p->g_max;
+  d.Rt = Rt_d;
+  d.n_orient = n_orient;
+  d.reserved = 0;
+  d.idx = idx_d;
+  d.spots = spot_d;
+  d.count = count_d;
+  d.ncand = ncand_d;
+  d.maxI = maxI_d;
+  a.n_items = (n_orient + kOT - 1) / kOT;
+  hipLaunchKernelGGL(simulate_kernel<false>, dim3(launch_blocks(a.n_items, 0)), dim3(kSimThreads), 0,
+                     as_stream(stream), a);
   return check_launch("dsx_simulate");
+}
+
+extern "C" int dsx_simulate_tile_size(void) { return kOT; }
+
+extern "C" int dsx_simulate_ragged(const dsx_phase_desc *phases_d, int n_phases, const int *items_d,
+                                   int n_items, const dsx_sim_params *p, int blocks_per_cu,
+                                   void *stream) {
+  clear_error();
+  if (!p || n_phases < 0 || n_items < 0 || blocks_per_cu < 0) {
+    set_error("dsx_simulate_ragged: bad arguments (n_phases=%d n_items=%d)", n_phases, n_items);
+    return DSX_EINVAL;
+  }
+  if (n_items == 0) return DSX_OK;
+  if (!phases_d || !items_d || n_phases == 0) {
+    set_error("dsx_simulate_ragged: null device pointer or no phases");
+    return DSX_EINVAL;
+  }
+  const int st = check_physics(p, "dsx_simulate_ragged");
+  if (st != DSX_OK) return st;
+  SimArgs a;
+  fill_args(a, p);
+  a.phases = phases_d;
+  a.items = reinterpret_cast<const int2 *>(items_d);
+  a.n_items = n_items;
+  hipLaunchKernelGGL(simulate_kernel<true>, dim3(launch_blocks(n_items, blocks_per_cu)),
+                     dim3(kSimThreads), 0, as_stream(stream), a);
+  return check_launch("dsx_simulate_ragged");
 }

@@ -6,6 +6,7 @@ benchmark, the tests and the documentation use one definition:
 * ``small_fcc_al``      FCC Al, 500 random orientations, 144x144 (CPU-runnable case)
 * ``template_library``  orthorhombic 40-atom cell, 200,000-orientation SO(3) grid, 256x256
 * ``refinement_monoclinic``  24-atom monoclinic cell, 4,096 perturbed targets (fwd+bwd)
+* ``ragged_library``  64 triclinic phases of uneven size, ~10^6 orientations, 128x128
 """
 
 from __future__ import annotations
@@ -135,3 +136,31 @@ def refinement_monoclinic(n_patterns: int = 4096, seed: int = SEED + 3) -> Refin
                      minimum_intensity=1e-20)
     return RefinementWorkload("mono24-refinement", model, euler, truth, t_euler, st, 1.5,
                               (256, 256), 0.015, 1.0, 0.01, seed)
+
+
+def ragged_library(n_phases: int = 64, orient_scale: float = 1.0, seed: int = SEED + 2) -> Workload:
+    """Ragged multi-phase batch: per phase, atom count log-uniform in [1, 200],
+    triclinic cell with edges U[3, 25] A and angles U[80, 110] deg, Z uniform
+    in [3, 42], B U[0.2, 1.5] A^2, occupancy 1, a log-uniform 2,000-50,000
+    Haar-random orientations (times ``orient_scale``) and reciprocal_radius
+    U[1.0, 2.5] 1/A.  200 kV, max_excitation_error 0.02, lorentzian, no direct
+    beam; 128x128 f32 patterns at 0.04 1/A/px (the frame holds r <= 2.56),
+    sigma 1 px, fast render, normalised."""
+    rng = np.random.default_rng(seed)
+    phases, rots, radii = [], [], []
+    for i in range(n_phases):
+        n_atoms = int(round(np.exp(rng.uniform(np.log(1), np.log(200)))))
+        abc = tuple(float(v) for v in rng.uniform(3, 25, 3))
+        angles = tuple(float(v) for v in rng.uniform(80, 110, 3))
+        Z = rng.integers(3, 43, n_atoms)
+        xyz = rng.uniform(0, 1, (n_atoms, 3))
+        B = rng.uniform(0.2, 1.5, n_atoms)
+        phases.append(PhaseArrays(abc, angles, [symbol(int(z)) for z in Z], xyz, np.ones(n_atoms),
+                                  B, f"ragged{i:02d}"))
+        n_o = max(1, int(round(np.exp(rng.uniform(np.log(2000), np.log(50000))) * orient_scale)))
+        rots.append(Rotation.from_euler(random_euler(n_o, rng)).to_matrix())
+        radii.append(float(rng.uniform(1.0, 2.5)))
+    st = SimSettings(accelerating_voltage=200, max_excitation_error=0.02, shape_model="lorentzian",
+                     minimum_intensity=1e-20)
+    return Workload("ragged-64", phases, rots, radii, False, st, (128, 128), 0.04, 1.0,
+                    notes={"orient_scale": orient_scale})

@@ -140,6 +140,49 @@ int dsx_simulate(const int *hkl_d, const float *g4_d, const float *F2_d, int n_r
                  float *maxI_d, void *stream);
 
 /*
+ * One phase of a ragged multi-phase launch (device-resident array of these).
+ * Each phase has its own reflection table (as for dsx_simulate), reciprocal
+ * basis, largest |g|, orientations and output buffers with capacity cap.
+ */
+typedef struct dsx_phase_desc {
+  const int *hkl;          /* int32 [n_refl][4]                                */
+  const float *g4;         /* float [n_refl][4]                                */
+  const float *F2;         /* float [n_refl]                                   */
+  int n_refl;
+  int cap;
+  double rec[9];
+  double g_max;
+  const double *Rt;        /* double [n_orient][9]                             */
+  int n_orient;
+  int reserved;
+  int *idx;                /* int32 [n_orient][cap]                            */
+  float *spots;            /* float [n_orient][cap][4]                         */
+  int *count;              /* int32 [n_orient]                                 */
+  int *ncand;              /* int32 [n_orient]                                 */
+  float *maxI;             /* float [n_orient]                                 */
+} dsx_phase_desc;
+
+/* Orientations per work item (tile) of dsx_simulate_ragged. */
+int dsx_simulate_tile_size(void);
+
+/*
+ * Ragged multi-phase form of dsx_simulate — replaces the per-phase loop of
+ * calculate_diffraction2d (simulation_generator.py:198-242) with one
+ * persistent launch over every phase.
+ *   phases_d  device array of n_phases dsx_phase_desc
+ *   items_d   device int32 [n_items][2] {phase, first orientation}: one work
+ *             item per dsx_simulate_tile_size() orientations, processed in
+ *             the given order (put the most expensive first)
+ *   params    physics shared by all phases (rec and g_max are ignored: each
+ *             phase carries its own)
+ *   blocks_per_cu  launch width (0 = default)
+ * Outputs per phase as for dsx_simulate.
+ */
+int dsx_simulate_ragged(const dsx_phase_desc *phases_d, int n_phases, const int *items_d,
+                        int n_items, const dsx_sim_params *params, int blocks_per_cu,
+                        void *stream);
+
+/*
  * Render parameters (Simulation2D.get_diffraction_pattern,
  * diffsims/simulations/simulation2d.py:357-442).
  */
@@ -184,6 +227,31 @@ int dsx_render_fast(const float *spot_d, const int *count_d, int cap, int n,
 int dsx_render_subpixel(const float *spot_d, const int *count_d, int cap, int n,
                         const dsx_render_params *params, float *out_d,
                         int *status_d, void *stream);
+
+/*
+ * Ragged fast rendering: the patterns of many phases, each with its own spot
+ * capacity, from one spot pool — the render half of the multi-phase
+ * calculate_diffraction2d + get_diffraction_pattern loop
+ * (simulation_generator.py:198-242, simulation2d.py:357-442).
+ *   spot_d      float4 spot pool; pattern p's list starts at spot_off_d[p]
+ *               (int64, in spots) and holds min(count_d[p], cap_d[p]) spots
+ *   list_d      int32 [n] pattern ids to render; out_d/status_d are indexed by
+ *               pattern id (out_d float [*][height][width])
+ *   method      DSX_RENDER_SPLAT: footprint splatting (at most
+ *               dsx_render_max_spots() distinct in-frame spots per pattern);
+ *               DSX_RENDER_DENSE: whole pattern in LDS, owner map + separable
+ *               filter, any number of spots, at most
+ *               dsx_render_dense_max_pixels() pixels
+ *   blocks_per_cu  launch width (0 = default)
+ * Same results as dsx_render_fast.
+ */
+#define DSX_RENDER_SPLAT 0
+#define DSX_RENDER_DENSE 1
+int dsx_render_dense_max_pixels(void);
+int dsx_render_fast_ragged(const float *spot_d, const long long *spot_off_d, const int *cap_d,
+                           const int *count_d, const int *list_d, int n,
+                           const dsx_render_params *params, int method, int blocks_per_cu,
+                           float *out_d, int *status_d, void *stream);
 
 /*
  * Adjoint of dsx_simulate + dsx_render_subpixel.  No counterpart in the
