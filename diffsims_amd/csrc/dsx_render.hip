@@ -54,6 +54,11 @@ constexpr int kHash = 2048;      // LDS hash slots (load factor <= 1/2)
 constexpr int kMaxStrips = 256;
 constexpr int kBinCap = 4096;    // strip-bin entries per pattern
 constexpr int kMaxR = 64;
+// bounded search for the pattern maximum (fast mode): 16x16 pixel tiles
+constexpr int kTile = 16;
+constexpr int kMaxTiles = 1024;   // up to 512x512 patterns
+constexpr int kMaxCand = 16;      // candidate tiles evaluated exactly
+constexpr int kCandSpots = 64;    // spots touching one candidate tile
 constexpr int kStripBytes = DSX_STRIP_KB * 1024;
 
 enum { kModeFast = 0, kModeSubpixel = 1 };
@@ -128,7 +133,148 @@ struct RenderShared {
   int cols_max;
   float wsum[kRenderThreads / kWave];
   acc_t wmax[kRenderThreads / kWave];
+  // bounded maximum search (fast mode)
+  float tile_u[kMode == kModeFast ? kMaxTiles : 1];
+  short cand_q[kMode == kModeFast ? kMaxCand : 1][kCandSpots];
+  int cand_tile[kMaxCand];
+  int cand_n[kMaxCand];
+  int n_cand;
+  int fail;
+  int qstar;
+  acc_t lfix;
 };
+
+// Fast-mode weight of a source pixel c at output index y along one axis, the
+// value the splat uses (reflect_weight3 equals the interior tap exactly when
+// the footprint does not reach an edge).
+__device__ __forceinline__ float fast_weight(int y, int c, int n, int R, bool short_taps,
+                                             const float *taps) {
+  return short_taps ? reflect_weight3(y, c, n, R, taps) : reflect_weight(y, c, n, R, taps);
+}
+
+// Fixed-point contribution of a survivor (I at pixel xi, yi; rows lo..hi) to
+// pixel (x, y), rounded exactly as the splat rounds it; 0 outside its footprint.
+__device__ __forceinline__ long long fast_term(float I, int xi, int yi, int lo, int hi, int x, int y,
+                                               int H, int W, int R, bool short_taps,
+                                               const float *taps, float scale) {
+  if (x < xi - R || x > xi + R || y < lo || y > hi) return 0;
+  const float iw = I * fast_weight(x, xi, W, R, short_taps, taps);
+  const float v = iw * fast_weight(y, yi, H, R, short_taps, taps);
+  return __float2int_rn(v * scale);
+}
+
+// Exact pattern maximum (fixed point) without rendering the whole pattern:
+// every contribution is at most I_j bx_j by_j (the largest tap along each
+// axis), so a 16x16 tile whose summed bounds stay below the value already
+// known at the brightest spot's pixel cannot hold the maximum.  Only the few
+// remaining tiles are evaluated, pixel by pixel, with the splat's exact
+// fixed-point terms, so the result equals the full pass bit for bit.
+// Returns false (uniformly) when too many tiles remain; the caller then runs
+// the full pass.
+template <int kMode>
+__device__ bool bounded_max(RenderShared<kMode> &sh, int ns, int H, int W, int R, bool short_taps,
+                            float scale, float pmax, int tid, int lane, int wid, acc_t &m_out) {
+  const int ntx = (W + kTile - 1) / kTile, nty = (H + kTile - 1) / kTile;
+  const int n_tiles = ntx * nty;
+  if (n_tiles > kMaxTiles) return false;
+  for (int t = tid; t < n_tiles; t += kRenderThreads) sh.tile_u[t] = 0.0f;
+  if (tid == 0) {
+    sh.n_cand = 0;
+    sh.fail = 0;
+    sh.qstar = 0;
+  }
+  lds_sync();
+  const float w0 = sh.taps[0];
+  for (int q = tid; q < ns; q += kRenderThreads) {
+    const float I = sh.I[q];
+    if (fabsf(I) == pmax) sh.qstar = q;
+    const int yi = sh.xy[q] >> 16, xi = sh.xy[q] & 0xffff;
+    const float bx = !short_taps ? 1.0f : ((xi >= R && xi + R < W) ? w0 : fminf(1.0f, 3.0f * w0));
+    const float by = !short_taps ? 1.0f : ((yi >= R && yi + R < H) ? w0 : fminf(1.0f, 3.0f * w0));
+    const float u = fabsf(I) * bx * by;
+    const int tx0 = max(0, xi - R) / kTile, tx1 = min(W - 1, xi + R) / kTile;
+    const int ty0 = sh.ylo[q] / kTile, ty1 = sh.yhi[q] / kTile;
+    for (int ty = ty0; ty <= ty1; ++ty)
+      for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&sh.tile_u[ty * ntx + tx], u);
+  }
+  lds_sync();
+  // lower bound: the exact value at the brightest survivor's pixel (wave 0)
+  if (wid == 0) {
+    const int qs = sh.qstar;
+    const int ys = sh.xy[qs] >> 16, xs = sh.xy[qs] & 0xffff;
+    long long acc = 0;
+    for (int q = lane; q < ns; q += kWave)
+      acc += fast_term(sh.I[q], sh.xy[q] & 0xffff, sh.xy[q] >> 16, sh.ylo[q], sh.yhi[q], xs, ys, H,
+                       W, R, short_taps, sh.taps, scale);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+    if (lane == 0) sh.lfix = acc;
+  }
+  lds_sync();
+  const acc_t lfix = sh.lfix;
+  // candidate tiles: bound (with a rounding margin per term) >= lower bound;
+  // tile_u is reused to hold each tile's candidate slot (-1: pruned)
+  int *slot_of = reinterpret_cast<int *>(sh.tile_u);
+  const float margin = 0.5f * static_cast<float>(ns) + 1.0f;
+  for (int t = tid; t < n_tiles; t += kRenderThreads) {
+    const float bound = sh.tile_u[t] * scale * 1.00001f + margin;
+    int slot = -1;
+    if (bound >= static_cast<float>(lfix)) {
+      slot = atomicAdd(&sh.n_cand, 1);
+      if (slot < kMaxCand) {
+        sh.cand_tile[slot] = t;
+        sh.cand_n[slot] = 0;
+      } else {
+        slot = -1;
+        sh.fail = 1;
+      }
+    }
+    slot_of[t] = slot;
+  }
+  lds_sync();
+  if (sh.fail) return false;
+  const int n_cand = sh.n_cand;
+  for (int q = tid; q < ns; q += kRenderThreads) {
+    const int yi = sh.xy[q] >> 16, xi = sh.xy[q] & 0xffff;
+    const int tx0 = max(0, xi - R) / kTile, tx1 = min(W - 1, xi + R) / kTile;
+    const int ty0 = sh.ylo[q] / kTile, ty1 = sh.yhi[q] / kTile;
+    for (int ty = ty0; ty <= ty1; ++ty)
+      for (int tx = tx0; tx <= tx1; ++tx) {
+        const int sl = slot_of[ty * ntx + tx];
+        if (sl < 0) continue;
+        const int pos = atomicAdd(&sh.cand_n[sl], 1);
+        if (pos < kCandSpots)
+          sh.cand_q[sl][pos] = static_cast<short>(q);
+        else
+          sh.fail = 1;
+      }
+  }
+  lds_sync();
+  if (sh.fail) return false;
+  acc_t lmax = lfix;
+  for (int i = tid; i < n_cand * kTile * kTile; i += kRenderThreads) {
+    const int sl = i / (kTile * kTile);
+    const int t = sh.cand_tile[sl];
+    const int x = (t % ntx) * kTile + (i % kTile), y = (t / ntx) * kTile + (i / kTile) % kTile;
+    if (x >= W || y >= H) continue;
+    long long acc = 0;
+    const int nq = sh.cand_n[sl];
+    for (int k = 0; k < nq; ++k) {
+      const int q = sh.cand_q[sl][k];
+      acc += fast_term(sh.I[q], sh.xy[q] & 0xffff, sh.xy[q] >> 16, sh.ylo[q], sh.yhi[q], x, y, H, W,
+                       R, short_taps, sh.taps, scale);
+    }
+    lmax = acc > lmax ? acc : lmax;
+  }
+  lmax = wave_max_acc(lmax);
+  if (lane == 0) sh.wmax[wid] = lmax;
+  lds_sync();
+  acc_t m = 0;
+  for (int w = 0; w < kRenderThreads / kWave; ++w) m = sh.wmax[w] > m ? sh.wmax[w] : m;
+  m_out = m;
+  lds_sync();  // wmax and the candidate state are reused
+  return true;
+}
 
 // kR > 0: the fast-mode tap radius as a compile-time constant (the footprint
 // loops unroll and the taps stay in scalar registers); 0: taken from the args.
@@ -297,7 +443,15 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
     const float inv_ncols = 1.0f / static_cast<float>(ncols);
 
     float out_scale = 1.0f / scale;  // fixed point -> value (normalize: -> value/max)
-    for (int pass = a.normalize ? 0 : 1; pass < 2; ++pass) {
+    int first_pass = a.normalize ? 0 : 1;
+    if (kMode == kModeFast && a.normalize) {
+      acc_t m;
+      if (bounded_max(sh, ns, H, W, R, short_taps, scale, pmax, tid, lane, wid, m)) {
+        out_scale = m > 0 ? 1.0f / acc_to_float(m) : 1.0f / scale;
+        first_pass = 1;
+      }
+    }
+    for (int pass = first_pass; pass < 2; ++pass) {
       acc_t lmax = 0;
       for (int s = 0; s < a.n_strips; ++s) {
         const int y0 = s * a.strip_rows;

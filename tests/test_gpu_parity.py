@@ -245,3 +245,43 @@ def test_render_no_spots_and_zero_intensity():
     count = torch.tensor([1, 2, 0], dtype=torch.int32, device=dev)
     out = e.render_spots(spots, count, 4, (50, 50), 1.0, 0.001).cpu().numpy()
     assert np.all(out == 0.0) and not np.isnan(out).any()
+
+
+@pytest.mark.parametrize("case", ["equal", "dominant", "border", "clustered"])
+def test_render_fast_maximum_search(case):
+    """The fast renderer finds each pattern's maximum from tile bounds and
+    falls back to a full pass when too many tiles remain; both must give the
+    oracle's normalised pattern.  Spots sit at pixel centres (no truncation
+    ambiguity) with intensity layouts that drive either branch."""
+    import torch
+
+    e = eng()
+    rng = np.random.default_rng({"equal": 1, "dominant": 2, "border": 3, "clustered": 4}[case])
+    n_pat, n_spots, shape, cal, sigma = 24, 300, (200, 160), 0.01, 1.0
+    H, W = shape
+    cx, cy = W // 2, H // 2
+    px = rng.integers(0, W, (n_pat, n_spots))
+    py = rng.integers(0, H, (n_pat, n_spots))
+    if case == "border":
+        px[:, ::2] = rng.choice([0, 1, W - 2, W - 1], (n_pat, (n_spots + 1) // 2))
+        py[:, 1::2] = rng.choice([0, 1, H - 2, H - 1], (n_pat, n_spots // 2))
+    if case == "clustered":
+        px = np.clip(W // 3 + rng.integers(-6, 7, (n_pat, n_spots)), 0, W - 1)
+        py = np.clip(H // 2 + rng.integers(-6, 7, (n_pat, n_spots)), 0, H - 1)
+    I = {"equal": np.ones((n_pat, n_spots)),
+         "dominant": rng.exponential(1.0, (n_pat, n_spots)) * (1 + 50 * (np.arange(n_spots) == 7)),
+         "border": rng.exponential(1.0, (n_pat, n_spots)),
+         "clustered": rng.uniform(0.5, 1.0, (n_pat, n_spots))}[case]
+    x = (px + 0.5 - cx) * cal
+    y = (py + 0.5 - cy) * cal
+    spots = np.zeros((n_pat, n_spots, 4), np.float32)
+    spots[..., 0], spots[..., 1], spots[..., 3] = x, y, I
+    dev = torch.device("cuda")
+    st = torch.from_numpy(spots).to(dev)
+    cnt = torch.full((n_pat,), n_spots, dtype=torch.int32, device=dev)
+    out = e.render_spots(st, cnt, n_spots, shape, sigma, cal).cpu().numpy()
+    for o in range(n_pat):
+        X = (spots[o, :, 0] / np.float32(cal) + np.float32(cx)).astype(np.float64)
+        Y = (spots[o, :, 1] / np.float32(cal) + np.float32(cy)).astype(np.float64)
+        rp = P.render_from_pixels(X, Y, spots[o, :, 3], shape, sigma)
+        assert np.abs(out[o] - rp).max() <= P.TOL_P * rp.max(), (case, o)
