@@ -141,7 +141,7 @@ __device__ __forceinline__ void strip_pass(const AdjArgs &a, AdjShared &sh, acc_
 }
 
 __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) {
-  extern __shared__ acc_t s_acc[];
+  extern __shared__ __align__(16) acc_t s_acc[];  // 16-byte aligned for the longlong2 read-out
   __shared__ AdjShared sh;
   float *s_G = reinterpret_cast<float *>(s_acc + a.acc_words);
   const int tid = threadIdx.x;

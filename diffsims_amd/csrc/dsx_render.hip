@@ -133,9 +133,8 @@ struct RenderShared {
   int cols_max;
   float wsum[kRenderThreads / kWave];
   acc_t wmax[kRenderThreads / kWave];
-  // bounded maximum search (fast mode)
-  float tile_u[kMode == kModeFast ? kMaxTiles : 1];
-  short cand_q[kMode == kModeFast ? kMaxCand : 1][kCandSpots];
+  // bounded maximum search (fast mode); its per-tile arrays borrow the
+  // (zeroed) strip buffer so the block keeps two-blocks-per-CU occupancy
   int cand_tile[kMaxCand];
   int cand_n[kMaxCand];
   int n_cand;
@@ -172,12 +171,17 @@ __device__ __forceinline__ long long fast_term(float I, int xi, int yi, int lo, 
 // Returns false (uniformly) when too many tiles remain; the caller then runs
 // the full pass.
 template <int kMode>
-__device__ bool bounded_max(RenderShared<kMode> &sh, int ns, int H, int W, int R, bool short_taps,
-                            float scale, float pmax, int tid, int lane, int wid, acc_t &m_out) {
+__device__ bool bounded_max(RenderShared<kMode> &sh, acc_t *scratch, int scratch_words, int ns,
+                            int H, int W, int R, bool short_taps, float scale, float pmax, int tid,
+                            int lane, int wid, acc_t &m_out) {
   const int ntx = (W + kTile - 1) / kTile, nty = (H + kTile - 1) / kTile;
   const int n_tiles = ntx * nty;
-  if (n_tiles > kMaxTiles) return false;
-  for (int t = tid; t < n_tiles; t += kRenderThreads) sh.tile_u[t] = 0.0f;
+  // scratch layout: float tile_u[n_tiles] | short cand_q[kMaxCand][kCandSpots]
+  const int tile_words = (n_tiles + 1) / 2;
+  const int used_words = tile_words + (kMaxCand * kCandSpots * 2 + 7) / 8;
+  if (n_tiles > kMaxTiles || used_words > scratch_words) return false;
+  float *tile_u = reinterpret_cast<float *>(scratch);
+  short(*cand_q)[kCandSpots] = reinterpret_cast<short(*)[kCandSpots]>(scratch + tile_words);
   if (tid == 0) {
     sh.n_cand = 0;
     sh.fail = 0;
@@ -195,7 +199,7 @@ __device__ bool bounded_max(RenderShared<kMode> &sh, int ns, int H, int W, int R
     const int tx0 = max(0, xi - R) / kTile, tx1 = min(W - 1, xi + R) / kTile;
     const int ty0 = sh.ylo[q] / kTile, ty1 = sh.yhi[q] / kTile;
     for (int ty = ty0; ty <= ty1; ++ty)
-      for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&sh.tile_u[ty * ntx + tx], u);
+      for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&tile_u[ty * ntx + tx], u);
   }
   lds_sync();
   // lower bound: the exact value at the brightest survivor's pixel (wave 0)
@@ -214,10 +218,10 @@ __device__ bool bounded_max(RenderShared<kMode> &sh, int ns, int H, int W, int R
   const acc_t lfix = sh.lfix;
   // candidate tiles: bound (with a rounding margin per term) >= lower bound;
   // tile_u is reused to hold each tile's candidate slot (-1: pruned)
-  int *slot_of = reinterpret_cast<int *>(sh.tile_u);
+  int *slot_of = reinterpret_cast<int *>(tile_u);
   const float margin = 0.5f * static_cast<float>(ns) + 1.0f;
   for (int t = tid; t < n_tiles; t += kRenderThreads) {
-    const float bound = sh.tile_u[t] * scale * 1.00001f + margin;
+    const float bound = tile_u[t] * scale * 1.00001f + margin;
     int slot = -1;
     if (bound >= static_cast<float>(lfix)) {
       slot = atomicAdd(&sh.n_cand, 1);
@@ -232,7 +236,11 @@ __device__ bool bounded_max(RenderShared<kMode> &sh, int ns, int H, int W, int R
     slot_of[t] = slot;
   }
   lds_sync();
-  if (sh.fail) return false;
+  if (sh.fail) {
+    for (int i = tid; i < used_words; i += kRenderThreads) scratch[i] = 0;
+    lds_sync();
+    return false;
+  }
   const int n_cand = sh.n_cand;
   for (int q = tid; q < ns; q += kRenderThreads) {
     const int yi = sh.xy[q] >> 16, xi = sh.xy[q] & 0xffff;
@@ -244,13 +252,17 @@ __device__ bool bounded_max(RenderShared<kMode> &sh, int ns, int H, int W, int R
         if (sl < 0) continue;
         const int pos = atomicAdd(&sh.cand_n[sl], 1);
         if (pos < kCandSpots)
-          sh.cand_q[sl][pos] = static_cast<short>(q);
+          cand_q[sl][pos] = static_cast<short>(q);
         else
           sh.fail = 1;
       }
   }
   lds_sync();
-  if (sh.fail) return false;
+  if (sh.fail) {
+    for (int i = tid; i < used_words; i += kRenderThreads) scratch[i] = 0;
+    lds_sync();
+    return false;
+  }
   acc_t lmax = lfix;
   for (int i = tid; i < n_cand * kTile * kTile; i += kRenderThreads) {
     const int sl = i / (kTile * kTile);
@@ -260,7 +272,7 @@ __device__ bool bounded_max(RenderShared<kMode> &sh, int ns, int H, int W, int R
     long long acc = 0;
     const int nq = sh.cand_n[sl];
     for (int k = 0; k < nq; ++k) {
-      const int q = sh.cand_q[sl][k];
+      const int q = cand_q[sl][k];
       acc += fast_term(sh.I[q], sh.xy[q] & 0xffff, sh.xy[q] >> 16, sh.ylo[q], sh.yhi[q], x, y, H, W,
                        R, short_taps, sh.taps, scale);
     }
@@ -272,7 +284,9 @@ __device__ bool bounded_max(RenderShared<kMode> &sh, int ns, int H, int W, int R
   acc_t m = 0;
   for (int w = 0; w < kRenderThreads / kWave; ++w) m = sh.wmax[w] > m ? sh.wmax[w] : m;
   m_out = m;
-  lds_sync();  // wmax and the candidate state are reused
+  // hand the strip buffer back zeroed
+  for (int i = tid; i < used_words; i += kRenderThreads) scratch[i] = 0;
+  lds_sync();  // wmax, the candidate state and the strip buffer are reused
   return true;
 }
 
@@ -280,7 +294,9 @@ __device__ bool bounded_max(RenderShared<kMode> &sh, int ns, int H, int W, int R
 // loops unroll and the taps stay in scalar registers); 0: taken from the args.
 template <int kMode, int kR>
 __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
-  extern __shared__ acc_t s_acc[];  // strip accumulator; hash table before the passes
+  // 16-byte aligned: the read-outs use 16-byte LDS accesses, which run at a
+  // fraction of their rate on an 8-byte-aligned base
+  extern __shared__ __align__(16) acc_t s_acc[];  // strip accumulator; hash table before the passes
   __shared__ RenderShared<kMode> sh;
   int *h_key = reinterpret_cast<int *>(s_acc);
   int *h_val = h_key + kHash;
@@ -446,7 +462,7 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
     int first_pass = a.normalize ? 0 : 1;
     if (kMode == kModeFast && a.normalize) {
       acc_t m;
-      if (bounded_max(sh, ns, H, W, R, short_taps, scale, pmax, tid, lane, wid, m)) {
+      if (bounded_max(sh, s_acc, a.lds_words, ns, H, W, R, short_taps, scale, pmax, tid, lane, wid, m)) {
         out_scale = m > 0 ? 1.0f / acc_to_float(m) : 1.0f / scale;
         first_pass = 1;
       }
@@ -610,7 +626,7 @@ __device__ __forceinline__ int reflect_index(int x, int n) {
 
 template <int kR>
 __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs a) {
-  extern __shared__ float s_buf[];
+  extern __shared__ __align__(16) float s_buf[];
   __shared__ float s_taps[kMaxR + 1];
   __shared__ float s_wmax[kDenseThreads / kWave];
   __shared__ int s_any;
