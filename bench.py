@@ -40,6 +40,9 @@ def parse():
                     help="timed hipGraph replays of the 64-phase ragged batch (0 disables it)")
     ap.add_argument("--ragged-scale", type=float, default=1.0,
                     help="fraction of the ragged batch's orientations to use")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (the reference "
+                         "kernel for scaling the HBM byte counters of a profiler) and exit")
     return ap.parse_args()
 
 
@@ -203,6 +206,15 @@ def main():
     from diffsims_amd import engine, workloads
 
     dev = torch.device("cuda", 0)
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
+        return
     wl = workloads.template_library(args.orientations)
     pa = wl.phases[0]
     rr = wl.reciprocal_radius[0]
