@@ -26,6 +26,23 @@ METRIC = "patterns/s: batched fwd & fwd+bwd vs plain fwd, 1 MI355X, rel err ≤1
 HBM_PEAK_GBS = 8000.0
 
 
+PROFILE = os.path.join(ROOT, "profiles", "render_fast_ortho40_library_kernel_trace.md")
+
+
+def measured_traffic(render_bytes):
+    """HBM bytes of one render launch: the FETCH_SIZE/WRITE_SIZE measurement
+    committed in profiles/ (taken on the 200,000-orientation library), scaled
+    to this run's algorithmic bytes; None when the summary is absent."""
+    import re
+    try:
+        text = open(PROFILE).read()
+        t = float(re.search(r"traffic_bytes_per_launch: (\d+)", text).group(1))
+        a = float(re.search(r"algorithmic_bytes_per_launch: (\d+)", text).group(1))
+    except (OSError, AttributeError):
+        return None
+    return round(t / a * render_bytes)
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -290,7 +307,8 @@ def main():
             "kernel_ms": {"simulate": round(sim_ms, 3), "render": round(ren_ms, 3)},
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(render_bytes)},
     }
     if args.cpu_sample > 0:
         v, t, err, compared = cpu_baseline(wl, args.cpu_sample, batch, out)
