@@ -37,6 +37,7 @@ MAX_TAP_RADIUS = 64
 
 ADJ_L2 = 0
 ADJ_PATTERN_GRAD = 1
+ADJ_FORWARD = 2
 
 RENDER_SPLAT = 0
 RENDER_DENSE = 1
@@ -53,6 +54,7 @@ EXPORTED = (
     "dsx_render_fast",
     "dsx_render_subpixel",
     "dsx_render_adjoint",
+    "dsx_render_adjoint_workspace_ints",
     "dsx_structure_factor_adjoint",
     "dsx_simulate_tile_size",
     "dsx_simulate_ragged",
@@ -132,7 +134,9 @@ _SIGS = {
     "dsx_render_fast": (_I, [_P, _P, _I, _I, ctypes.POINTER(RenderParams), _P, _P, _P]),
     "dsx_render_subpixel": (_I, [_P, _P, _I, _I, ctypes.POINTER(RenderParams), _P, _P, _P]),
     "dsx_render_adjoint": (_I, [_P, _P, _P, _I, _I, _P, _P, _I, _P, ctypes.POINTER(SimParams),
-                                ctypes.POINTER(RenderParams), _I, _P, _P, _P, _P, _P, _P, _P]),
+                                ctypes.POINTER(RenderParams), _I, _P, ctypes.c_longlong, _P, _P,
+                                _P, _P, _P, ctypes.c_longlong, _P, _P]),
+    "dsx_render_adjoint_workspace_ints": (ctypes.c_longlong, [_I]),
     "dsx_structure_factor_adjoint": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _P,
                                           _I, _P, _I, _I, _P, _P, _P, _P, _P]),
     "dsx_simulate_tile_size": (_I, []),

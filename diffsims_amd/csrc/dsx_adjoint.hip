@@ -64,6 +64,10 @@ struct AdjArgs {
   int strip_rows, n_strips, acc_words;
   int mode;
   int want_grad;
+  long long target_stride;  // elements between targets (0: one target for every pattern)
+  int *work;                // large patterns: per-block bins + per-spot sums (may be NULL)
+  long long work_per_block; // ints
+  int work_entries;         // bin entries per block
   const float *target;
   float *pattern;
   float *loss;
@@ -86,6 +90,15 @@ struct AdjShared {
   acc_t wkey[kAdjWaves];
   double red[kAdjWaves][kNRed];
   double R[9];
+};
+
+// Reflection-side constants of the chain rule, staged in LDS once per block so
+// the kernel does not hold them in scalar registers throughout.
+struct ChainConsts {
+  double rec[9];
+  double k, k2, me, width, sigma, fac;
+  int model, precession;
+  double phi, Pz, Pt;
 };
 
 __device__ __forceinline__ double wave_sum_d(double v) {
@@ -140,15 +153,130 @@ __device__ __forceinline__ void strip_pass(const AdjArgs &a, AdjShared &sh, acc_
   }
 }
 
+// Load entries [base, base + m) of a strip's global bin into LDS slots 0..m-1
+// (large patterns: more in-frame spots than the LDS holds).
+__device__ __forceinline__ void load_chunk(const AdjArgs &a, AdjShared &sh, const float4 *spots,
+                                           const int *bins, int base, int m, int tid) {
+  for (int i = tid; i < m; i += kAdjThreads) {
+    const int j = bins[base + i];
+    const float4 sp = spots[j];
+    float X, Y;
+    int lo = 0, hi = -1, xlo = 0, xhi = -1;
+    // the same tests as in the scan that binned j (cannot fail; kept for safety)
+    if (!to_pixel(sp, a, X, Y) ||
+        !subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, a.H, a.W, lo, hi, xlo, xhi)) {
+      lo = 1;
+      hi = 0;  // empty rows: the slot contributes nothing
+      xlo = 0;
+      xhi = 0;
+    }
+    sh.I[i] = sp.w;
+    sh.X[i] = X;
+    sh.Y[i] = Y;
+    sh.A[i] = 0.0f;
+    sh.BX[i] = 0.0f;
+    sh.BY[i] = 0.0f;
+    sh.xb[i] = (xlo << 16) | xhi;
+    sh.jpos[i] = j;
+    sh.ylo[i] = static_cast<short>(lo);
+    sh.yhi[i] = static_cast<short>(hi);
+  }
+}
+
+// Chain rule of one spot: (dL/dI, dL/dX, dL/dY) -> dL/dRt of the pattern (dR,
+// accumulated) and dL/dg, dL/d|F|^2 of its reflection (global atomics).
+__device__ __forceinline__ void spot_chain(const AdjArgs &a, const ChainConsts &c, const double *R,
+                                           int row, double I, double dI, double BX, double BY,
+                                           double *dR) {
+  const double dX = I * BX * a.inv_s2;
+  const double dY = I * BY * a.inv_s2;
+  if (dI == 0.0 && dX == 0.0 && dY == 0.0) return;
+  const double ca = a.ca, sa = a.sa, ic = 1.0 / static_cast<double>(a.cal);
+  double dx, dy;  // d/d(lab x, lab y)
+  if (a.mirrored) {
+    dx = (dX * ca + dY * sa) * ic;
+    dy = (dX * sa - dY * ca) * ic;
+  } else {
+    dx = (dX * ca + dY * sa) * ic;
+    dy = (-dX * sa + dY * ca) * ic;
+  }
+  const int4 h = a.hkl[row];
+  const double hx = h.x, hy = h.y, hz = h.z;
+  const double g0 = c.rec[0] * hx + c.rec[1] * hy + c.rec[2] * hz;
+  const double g1 = c.rec[3] * hx + c.rec[4] * hy + c.rec[5] * hz;
+  const double g2 = c.rec[6] * hx + c.rec[7] * hy + c.rec[8] * hz;
+  const double x = R[0] * g0 + R[1] * g1 + R[2] * g2;
+  const double y = R[3] * g0 + R[4] * g1 + R[5] * g2;
+  const double z = R[6] * g0 + R[7] * g1 + R[8] * g2;
+  const double r2 = x * x + y * y;
+  const double r = sqrt(r2);
+  const double qk = sqrt(fmax(c.k2 - r2, 0.0));
+  const double sv = r2 / (c.k + qk) - z;
+  double fs, fr;
+  const double f = shape_factor_grad(sv, r, c, fs, fr);
+  const double F2 = a.F2[row];
+  const double dF2 = dI * f;
+  const double ds = dI * F2 * fs;
+  const double dr = dI * F2 * fr;
+  double gx = dx, gy = dy, gz = -ds;
+  if (qk > 0.0) {
+    gx += ds * x / qk;
+    gy += ds * y / qk;
+  }
+  if (r > 0.0 && dr != 0.0) {
+    gx += dr * x / r;
+    gy += dr * y / r;
+  }
+  // g_lab = Rt g: dL/dRt[i][c] = dL/dg_lab[i] g[c]; dL/dg = Rt^T dL/dg_lab
+  dR[0] += gx * g0; dR[1] += gx * g1; dR[2] += gx * g2;
+  dR[3] += gy * g0; dR[4] += gy * g1; dR[5] += gy * g2;
+  dR[6] += gz * g0; dR[7] += gz * g1; dR[8] += gz * g2;
+  float *dt = a.dtab + (size_t)row * 4;
+  unsafeAtomicAdd(dt + 0, static_cast<float>(R[0] * gx + R[3] * gy + R[6] * gz));
+  unsafeAtomicAdd(dt + 1, static_cast<float>(R[1] * gx + R[4] * gy + R[7] * gz));
+  unsafeAtomicAdd(dt + 2, static_cast<float>(R[2] * gx + R[5] * gy + R[8] * gz));
+  unsafeAtomicAdd(dt + 3, static_cast<float>(dF2));
+}
+
+// The max term of the normalisation (-(sum G P)/m at the maximum's pixel p*)
+// for a spot whose box holds p*, added to its gathered sums.
+__device__ __forceinline__ void max_term(const AdjArgs &a, float corr, int px, int py, float X,
+                                         float Y, int xlo, int xhi, int lo, int hi, float &A,
+                                         float &BX, float &BY) {
+  if (px < xlo || px > xhi || py < lo || py > hi) return;
+  const float dx = static_cast<float>(px) - X, dy = static_cast<float>(py) - Y;
+  const float kc = corr * a.pref * expf(a.ep * (dx * dx + dy * dy));
+  A -= kc;
+  BX -= kc * dx;
+  BY -= kc * dy;
+}
+
 __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) {
   extern __shared__ __align__(16) acc_t s_acc[];  // 16-byte aligned for the longlong2 read-out
   __shared__ AdjShared sh;
+  __shared__ ChainConsts s_cc;
   float *s_G = reinterpret_cast<float *>(s_acc + a.acc_words);
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wid = tid / kWave;
   const int H = a.H, W = a.W;
   const size_t HW = static_cast<size_t>(H) * W;
+  if (tid == 0) {
+    for (int i = 0; i < 9; ++i) s_cc.rec[i] = a.rec[i];
+    s_cc.k = a.k;
+    s_cc.k2 = a.k2;
+    s_cc.me = a.me;
+    s_cc.width = a.width;
+    s_cc.sigma = a.sigma;
+    s_cc.fac = a.fac;
+    s_cc.model = a.model;
+    s_cc.precession = a.precession;
+    s_cc.phi = a.phi;
+    s_cc.Pz = a.Pz;
+    s_cc.Pt = a.Pt;
+  }
+  int *wbins = a.work ? a.work + (size_t)blockIdx.x * a.work_per_block : nullptr;
+  float *wacc = a.work ? reinterpret_cast<float *>(wbins + a.work_entries) : nullptr;
   for (int i = tid; i < a.acc_words; i += kAdjThreads) s_acc[i] = 0;
   lds_sync();
 
@@ -160,11 +288,13 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
       sh.overflow = 0;
       sh.cols_max = 1;
     }
-    if (tid < 9) sh.R[tid] = a.Rt[(size_t)p * 9 + tid];
+    if (tid < 9 && a.Rt) sh.R[tid] = a.Rt[(size_t)p * 9 + tid];
     for (int s = tid; s <= kAdjMaxStrips; s += kAdjThreads) sh.bin_start[s] = 0;
     lds_sync();
 
     // ---- 1. spots in frame with their clip boxes --------------------------
+    // every such spot counts in the strip bins and the maximum; the first
+    // kAdjMaxSpots also go to LDS (all of them for the usual pattern)
     float tmax = 0.0f;
     for (int j = tid; j < n; j += kAdjThreads) {
       const float4 sp = spots[j];
@@ -172,12 +302,11 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
       if (!to_pixel(sp, a, X, Y)) continue;
       int lo, hi, xlo, xhi;
       if (!subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, H, W, lo, hi, xlo, xhi)) continue;
-      const int q = atomicAdd(&sh.ns, 1);
-      if (q >= kAdjMaxSpots) {
-        sh.overflow = 1;
-        continue;
-      }
       tmax = fmaxf(tmax, fabsf(sp.w) * a.pref);
+      atomicMax(&sh.cols_max, xhi - xlo + 1);
+      for (int s = lo / a.strip_rows; s <= hi / a.strip_rows; ++s) atomicAdd(&sh.bin_start[s + 1], 1);
+      const int q = atomicAdd(&sh.ns, 1);
+      if (q >= kAdjMaxSpots) continue;
       sh.I[q] = sp.w;
       sh.X[q] = X;
       sh.Y[q] = Y;
@@ -188,14 +317,11 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
       sh.jpos[q] = j;
       sh.ylo[q] = static_cast<short>(lo);
       sh.yhi[q] = static_cast<short>(hi);
-      atomicMax(&sh.cols_max, xhi - xlo + 1);
-      for (int s = lo / a.strip_rows; s <= hi / a.strip_rows; ++s) atomicAdd(&sh.bin_start[s + 1], 1);
     }
     tmax = wave_max(tmax);
     if (lane == 0) sh.wmaxf[wid] = tmax;
     lds_sync();
-    const int ns = min(sh.ns, kAdjMaxSpots);
-    if (tid == 0 && a.status) a.status[p] = sh.overflow;
+    const int ns_all = sh.ns;
     float pmax = 0.0f;
     for (int w = 0; w < kAdjWaves; ++w) pmax = fmaxf(pmax, sh.wmaxf[w]);
     const float scale = pmax > 0.0f ? kFixedOne / (pmax * 1.001f) : 1.0f;
@@ -209,10 +335,42 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
       }
     }
     lds_sync();
-    const bool binned = sh.bin_start[a.n_strips] <= kAdjBinCap;
+    const int entries = sh.bin_start[a.n_strips];
+    // large: more in-frame spots than LDS slots; their bins live in the
+    // caller's workspace and the spots pass through LDS in chunks
+    const bool large = ns_all > kAdjMaxSpots;
+    const bool fits = !large || (wbins != nullptr && entries <= a.work_entries);
+    if (tid == 0 && a.status) a.status[p] = fits ? 0 : 1;
+    const int ns = large ? 0 : ns_all;  // spots held in LDS for the whole pattern
+    const bool binned = !large && entries <= kAdjBinCap;
     for (int s = tid; s < a.n_strips; s += kAdjThreads) sh.bin_fill[s] = sh.bin_start[s];
     lds_sync();
-    if (binned) {
+    if (!fits) {
+      // no room: the pattern is reported in status and left as zeros
+      if (a.pattern)
+        for (size_t i = tid; i < HW; i += kAdjThreads) a.pattern[(size_t)p * HW + i] = 0.0f;
+      if (tid < 9 && a.want_grad) a.dRt[(size_t)p * 9 + tid] = 0.0;
+      if (tid == 0 && a.loss) a.loss[p] = 0.0f;
+      lds_sync();
+      continue;
+    }
+    if (large) {
+      for (int j = tid; j < n; j += kAdjThreads) {
+        const float4 sp = spots[j];
+        float X, Y;
+        if (!to_pixel(sp, a, X, Y)) continue;
+        int lo, hi, xlo, xhi;
+        if (!subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, H, W, lo, hi, xlo, xhi)) continue;
+        for (int s = lo / a.strip_rows; s <= hi / a.strip_rows; ++s)
+          wbins[atomicAdd(&sh.bin_fill[s], 1)] = j;
+        if (a.want_grad) {
+          wacc[3 * j + 0] = 0.0f;
+          wacc[3 * j + 1] = 0.0f;
+          wacc[3 * j + 2] = 0.0f;
+        }
+      }
+      __syncthreads();  // bins and accumulators in global memory, block-visible
+    } else if (binned) {
       for (int q = tid; q < ns; q += kAdjThreads) {
         for (int s = sh.ylo[q] / a.strip_rows; s <= sh.yhi[q] / a.strip_rows; ++s) {
           const int pos = atomicAdd(&sh.bin_fill[s], 1);
@@ -222,16 +380,51 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
     }
     lds_sync();
 
+    // splat (or gather) strip s: every LDS-resident spot of its bin, or, for a
+    // large pattern, its global bin chunk by chunk
+    auto strip_spots = [&](int s, bool gather, int y0, int y1) {
+      if (!large) {
+        const int b0 = binned ? sh.bin_start[s] : 0;
+        const int nb = binned ? sh.bin_start[s + 1] - b0 : ns;
+        if (nb == 0) return;
+        if (gather)
+          strip_pass<true>(a, sh, s_acc, s_G, b0, nb, binned, y0, y1, scale, tid);
+        else
+          strip_pass<false>(a, sh, s_acc, nullptr, b0, nb, binned, y0, y1, scale, tid);
+        return;
+      }
+      const int e0 = sh.bin_start[s], e1 = sh.bin_start[s + 1];
+      for (int c0 = e0; c0 < e1; c0 += kAdjMaxSpots) {
+        const int m = min(kAdjMaxSpots, e1 - c0);
+        load_chunk(a, sh, spots, wbins, c0, m, tid);
+        lds_sync();
+        if (gather) {
+          strip_pass<true>(a, sh, s_acc, s_G, 0, m, false, y0, y1, scale, tid);
+          lds_sync();
+          for (int i = tid; i < m; i += kAdjThreads) {
+            // one entry per (spot, strip) and strips run in order: a plain
+            // read-modify-write, made visible by the barrier below
+            float *w = wacc + 3 * sh.jpos[i];
+            w[0] += sh.A[i];
+            w[1] += sh.BX[i];
+            w[2] += sh.BY[i];
+          }
+          __syncthreads();
+        } else {
+          strip_pass<false>(a, sh, s_acc, nullptr, 0, m, false, y0, y1, scale, tid);
+          lds_sync();  // the next chunk overwrites the slots
+        }
+      }
+    };
+
     // ---- 3. pass A: maximum and its first pixel -----------------------------
     acc_t key = 0;
-    if (a.normalize && ns > 0) {
+    if (a.normalize && ns_all > 0) {
       for (int s = 0; s < a.n_strips; ++s) {
         const int y0 = s * a.strip_rows;
         const int y1 = min(H, y0 + a.strip_rows);
-        const int b0 = binned ? sh.bin_start[s] : 0;
-        const int nb = binned ? sh.bin_start[s + 1] - b0 : ns;
-        if (nb == 0) continue;
-        strip_pass<false>(a, sh, s_acc, nullptr, b0, nb, binned, y0, y1, scale, tid);
+        if (sh.bin_start[s + 1] == sh.bin_start[s]) continue;
+        strip_spots(s, false, y0, y1);
         lds_sync();
         const int cnt = (y1 - y0) * W;
         const int pix0 = y0 * W;
@@ -260,17 +453,17 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
 
     // ---- 4. pass B: pattern, loss, upstream gradient, gather ----------------
     double lsum = 0.0, csum = 0.0;
-    const float *tgt = a.target + (size_t)p * HW;
+    const float *tgt = a.target ? a.target + (size_t)p * a.target_stride : nullptr;
     float *pat = a.pattern ? a.pattern + (size_t)p * HW : nullptr;
     const bool l2 = a.mode == DSX_ADJ_L2;
+    const bool fwd_only = a.mode == DSX_ADJ_FORWARD;
     for (int s = 0; s < a.n_strips; ++s) {
       const int y0 = s * a.strip_rows;
       const int y1 = min(H, y0 + a.strip_rows);
       const int cnt = (y1 - y0) * W;
-      const int b0 = binned ? sh.bin_start[s] : 0;
-      const int nb = binned ? sh.bin_start[s + 1] - b0 : ns;
-      if (nb > 0) {
-        strip_pass<false>(a, sh, s_acc, nullptr, b0, nb, binned, y0, y1, scale, tid);
+      const bool any = sh.bin_start[s + 1] > sh.bin_start[s];
+      if (any) {
+        strip_spots(s, false, y0, y1);
         lds_sync();
       }
       const size_t off = (size_t)y0 * W;
@@ -278,13 +471,14 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
         for (int i = tid; i < cnt / 4; i += kAdjThreads) {
           longlong2 *pv = reinterpret_cast<longlong2 *>(s_acc) + 2 * i;
           const longlong2 u = pv[0], v = pv[1];
-          if (nb > 0) {
+          if (any) {
             pv[0] = make_longlong2(0, 0);
             pv[1] = make_longlong2(0, 0);
           }
           const float4 P = make_float4(acc_to_float(u.x) * out_scale, acc_to_float(u.y) * out_scale,
                                        acc_to_float(v.x) * out_scale, acc_to_float(v.y) * out_scale);
           if (pat) store_nt(P, reinterpret_cast<float4 *>(pat + off) + i);
+          if (fwd_only) continue;
           const native_float4 t =
               __builtin_nontemporal_load(reinterpret_cast<const native_float4 *>(tgt + off) + i);
           float4 g;
@@ -302,9 +496,10 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
       } else {
         for (int i = tid; i < cnt; i += kAdjThreads) {
           const acc_t v = s_acc[i];
-          if (nb > 0) s_acc[i] = 0;
+          if (any) s_acc[i] = 0;
           const float P = acc_to_float(v) * out_scale;
           if (pat) __builtin_nontemporal_store(P, pat + off + i);
+          if (fwd_only) continue;
           const float t = tgt[off + i];
           float g;
           if (l2) {
@@ -319,9 +514,7 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
         }
       }
       lds_sync();
-      if (a.want_grad && nb > 0) {
-        strip_pass<true>(a, sh, s_acc, s_G, b0, nb, binned, y0, y1, scale, tid);
-      }
+      if (a.want_grad && any) strip_spots(s, true, y0, y1);
       lds_sync();
     }
     lsum = wave_sum_d(lsum);
@@ -338,82 +531,35 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
     }
 
     if (a.want_grad) {
-      // ---- 5. the max term of the normalisation: -(sum G P)/m at p* --------
-      if (norm) {
-        const float corr = static_cast<float>(ctot) * gscale;
-        const int px = pstar % W, py = pstar / W;
-        for (int q = tid; q < ns; q += kAdjThreads) {
-          const int xlo = sh.xb[q] >> 16, xhi = sh.xb[q] & 0xffff;
-          if (px < xlo || px > xhi || py < sh.ylo[q] || py > sh.yhi[q]) continue;
-          const float dx = static_cast<float>(px) - sh.X[q], dy = static_cast<float>(py) - sh.Y[q];
-          const float kc = corr * a.pref * expf(a.ep * (dx * dx + dy * dy));
-          sh.A[q] -= kc;
-          sh.BX[q] -= kc * dx;
-          sh.BY[q] -= kc * dy;
-        }
-        lds_sync();
-      }
-      // ---- 6. per-spot chain rule ---------------------------------------------
+      // ---- 5. per spot: the normalisation's max term, then the chain rule ----
+      const float corr = norm ? static_cast<float>(ctot) * gscale : 0.0f;
+      const int px = pstar % W, py = pstar / W;
       double dR[9];
 #pragma unroll
       for (int i = 0; i < 9; ++i) dR[i] = 0.0;
-      for (int q = tid; q < ns; q += kAdjThreads) {
-        const double dI = sh.A[q];
-        const double I = sh.I[q];
-        const double dX = I * static_cast<double>(sh.BX[q]) * a.inv_s2;
-        const double dY = I * static_cast<double>(sh.BY[q]) * a.inv_s2;
-        if (dI == 0.0 && dX == 0.0 && dY == 0.0) continue;
-        const double ca = a.ca, sa = a.sa, ic = 1.0 / static_cast<double>(a.cal);
-        double dx, dy;  // d/d(lab x, lab y)
-        if (a.mirrored) {
-          dx = (dX * ca + dY * sa) * ic;
-          dy = (dX * sa - dY * ca) * ic;
-        } else {
-          dx = (dX * ca + dY * sa) * ic;
-          dy = (-dX * sa + dY * ca) * ic;
+      const int *idx = a.idx + (size_t)p * a.cap;
+      if (!large) {
+        for (int q = tid; q < ns; q += kAdjThreads) {
+          float A = sh.A[q], BX = sh.BX[q], BY = sh.BY[q];
+          if (norm)
+            max_term(a, corr, px, py, sh.X[q], sh.Y[q], sh.xb[q] >> 16, sh.xb[q] & 0xffff, sh.ylo[q],
+                     sh.yhi[q], A, BX, BY);
+          spot_chain(a, s_cc, sh.R, idx[sh.jpos[q]], sh.I[q], A, BX, BY, dR);
         }
-        const int row = a.idx[(size_t)p * a.cap + sh.jpos[q]];
-        const int4 h = a.hkl[row];
-        const double hx = h.x, hy = h.y, hz = h.z;
-        const double g0 = a.rec[0] * hx + a.rec[1] * hy + a.rec[2] * hz;
-        const double g1 = a.rec[3] * hx + a.rec[4] * hy + a.rec[5] * hz;
-        const double g2 = a.rec[6] * hx + a.rec[7] * hy + a.rec[8] * hz;
-        const double *R = sh.R;
-        const double x = R[0] * g0 + R[1] * g1 + R[2] * g2;
-        const double y = R[3] * g0 + R[4] * g1 + R[5] * g2;
-        const double z = R[6] * g0 + R[7] * g1 + R[8] * g2;
-        const double r2 = x * x + y * y;
-        const double r = sqrt(r2);
-        const double qk = sqrt(fmax(a.k2 - r2, 0.0));
-        const double sv = r2 / (a.k + qk) - z;
-        double fs, fr;
-        const double f = shape_factor_grad(sv, r, a, fs, fr);
-        const double F2 = a.F2[row];
-        const double dF2 = dI * f;
-        const double ds = dI * F2 * fs;
-        const double dr = dI * F2 * fr;
-        double gx = dx, gy = dy, gz = -ds;
-        if (qk > 0.0) {
-          gx += ds * x / qk;
-          gy += ds * y / qk;
+      } else {
+        for (int j = tid; j < n; j += kAdjThreads) {
+          const float4 sp = spots[j];
+          float X, Y;
+          if (!to_pixel(sp, a, X, Y)) continue;
+          int lo, hi, xlo, xhi;
+          if (!subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, H, W, lo, hi, xlo, xhi)) continue;
+          float A = wacc[3 * j], BX = wacc[3 * j + 1], BY = wacc[3 * j + 2];
+          if (norm) max_term(a, corr, px, py, X, Y, xlo, xhi, lo, hi, A, BX, BY);
+          spot_chain(a, s_cc, sh.R, idx[j], sp.w, A, BX, BY, dR);
         }
-        if (r > 0.0 && dr != 0.0) {
-          gx += dr * x / r;
-          gy += dr * y / r;
-        }
-        // g_lab = Rt g: dL/dRt[i][c] = dL/dg_lab[i] g[c]; dL/dg = Rt^T dL/dg_lab
-        dR[0] += gx * g0; dR[1] += gx * g1; dR[2] += gx * g2;
-        dR[3] += gy * g0; dR[4] += gy * g1; dR[5] += gy * g2;
-        dR[6] += gz * g0; dR[7] += gz * g1; dR[8] += gz * g2;
-        float *dt = a.dtab + (size_t)row * 4;
-        unsafeAtomicAdd(dt + 0, static_cast<float>(R[0] * gx + R[3] * gy + R[6] * gz));
-        unsafeAtomicAdd(dt + 1, static_cast<float>(R[1] * gx + R[4] * gy + R[7] * gz));
-        unsafeAtomicAdd(dt + 2, static_cast<float>(R[2] * gx + R[5] * gy + R[8] * gz));
-        unsafeAtomicAdd(dt + 3, static_cast<float>(dF2));
       }
 #pragma unroll
       for (int i = 0; i < 9; ++i) dR[i] = wave_sum_d(dR[i]);
-      lds_sync();  // everyone has read sh.red above
       if (lane == 0) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) sh.red[wid][2 + i] = dR[i];
@@ -426,7 +572,7 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
       }
     }
     if (tid == 0 && a.loss) a.loss[p] = static_cast<float>(ltot);
-    lds_sync();  // LDS state of this pattern is dead before the next one
+    __syncthreads();  // LDS and workspace state of this pattern is dead before the next one
   }
 }
 
@@ -599,24 +745,38 @@ __global__ __launch_bounds__(kSfaThreads) void structure_factor_adjoint_kernel(S
 
 using namespace dsx;
 
+// Workspace layout per block: int bins[kWorkEntriesPerSpot * cap] | float sums[3 * cap].
+constexpr int kWorkEntriesPerSpot = 4;
+
+extern "C" long long dsx_render_adjoint_workspace_ints(int cap) {
+  if (cap < 1) return 0;
+  return static_cast<long long>(cu_count()) * (kWorkEntriesPerSpot + 3) * static_cast<long long>(cap);
+}
+
 extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const int *count_d, int cap,
                                   int n, const int *hkl_d, const float *F2_d, int n_refl,
                                   const double *Rt_d, const dsx_sim_params *sim,
                                   const dsx_render_params *rp, int mode, const float *target_d,
-                                  float *pattern_d, float *loss_d, double *dRt_d, float *dtab_d,
+                                  long long target_stride, float *pattern_d, float *loss_d,
+                                  double *dRt_d, float *dtab_d, int *work_d, long long work_ints,
                                   int *status_d, void *stream) {
   clear_error();
-  if (!sim || !rp || n < 0 || cap < 1 || n_refl < 0) {
+  if (!sim || !rp || n < 0 || cap < 1 || n_refl < 0 || work_ints < 0 || target_stride < 0) {
     set_error("dsx_render_adjoint: bad arguments (n=%d cap=%d n_refl=%d)", n, cap, n_refl);
     return DSX_EINVAL;
   }
-  if (mode != DSX_ADJ_L2 && mode != DSX_ADJ_PATTERN_GRAD) {
+  if (mode != DSX_ADJ_L2 && mode != DSX_ADJ_PATTERN_GRAD && mode != DSX_ADJ_FORWARD) {
     set_error("dsx_render_adjoint: unknown mode %d", mode);
     return DSX_EINVAL;
   }
   if (n == 0) return DSX_OK;
   const bool want_grad = dRt_d != nullptr || dtab_d != nullptr;
-  if (!idx_d || !spot_d || !count_d || !target_d || !Rt_d || (want_grad && (!dRt_d || !dtab_d)) ||
+  if (mode == DSX_ADJ_FORWARD && (want_grad || !pattern_d)) {
+    set_error("dsx_render_adjoint: forward mode writes patterns and no gradients");
+    return DSX_EINVAL;
+  }
+  if (!spot_d || !count_d || (mode != DSX_ADJ_FORWARD && !target_d) ||
+      (want_grad && (!dRt_d || !dtab_d || !idx_d || !Rt_d)) ||
       (want_grad && n_refl > 0 && (!hkl_d || !F2_d))) {
     set_error("dsx_render_adjoint: null device pointer");
     return DSX_EINVAL;
@@ -664,7 +824,7 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
     set_error("dsx_render_adjoint: pattern width %d exceeds the LDS strip budget", a.W);
     return DSX_EINVAL;
   }
-  int rows = std::max(1, std::min(a.H, budget / a.W));
+  const int rows = std::max(1, std::min(a.H, budget / a.W));
   const int n_strips = (a.H + rows - 1) / rows;
   if (n_strips > kAdjMaxStrips) {
     set_error("dsx_render_adjoint: pattern height %d needs %d strips (> %d)", a.H, n_strips,
@@ -677,13 +837,28 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   a.mode = mode;
   a.want_grad = want_grad ? 1 : 0;
   a.target = target_d;
+  a.target_stride = target_stride;
   a.pattern = pattern_d;
   a.loss = loss_d;
   a.dRt = dRt_d;
   a.dtab = dtab_d;
   a.status = status_d;
-  const size_t lds = static_cast<size_t>(a.acc_words) * (sizeof(acc_t) + sizeof(float));
   const int blocks = std::min(n, cu_count());
+  const long long per_block = (kWorkEntriesPerSpot + 3) * static_cast<long long>(cap);
+  if (work_d && work_ints >= per_block * blocks && kWorkEntriesPerSpot * static_cast<long long>(cap) < (1ll << 31)) {
+    a.work = work_d;
+    a.work_per_block = per_block;
+    a.work_entries = kWorkEntriesPerSpot * cap;
+  } else if (work_d) {
+    set_error("dsx_render_adjoint: workspace of %lld ints is smaller than %lld "
+              "(dsx_render_adjoint_workspace_ints)", work_ints, per_block * blocks);
+    return DSX_EINVAL;
+  } else {
+    a.work = nullptr;
+    a.work_per_block = 0;
+    a.work_entries = 0;
+  }
+  const size_t lds = static_cast<size_t>(a.acc_words) * (sizeof(acc_t) + sizeof(float));
   static int reserved = -1;  // raised once, to the largest strip this build uses
   if (reserved < 0) {
     if (hipFuncSetAttribute(reinterpret_cast<const void *>(&render_adjoint_kernel),

@@ -261,15 +261,32 @@ class RefinementModel:
     def _sim_params(self, t: _Tables) -> _lib.SimParams:
         return self.settings.params(t.g_max, t.rec)
 
+    def workspace(self, cap: int) -> Optional[torch.Tensor]:
+        """Scratch for patterns with more in-frame spots than the render LDS
+        holds (None when the spot capacity cannot exceed it)."""
+        if cap <= int(self.L.dsx_render_max_spots()):
+            return None
+        need = int(self.L.dsx_render_adjoint_workspace_ints(int(cap)))
+        if getattr(self, "_work", None) is None or self._work.numel() < need:
+            self._work = torch.empty(need, dtype=torch.int32, device=self.dev)
+        return self._work
+
     def adjoint(self, t: _Tables, Rt: torch.Tensor, batch: engine.SpotBatch, mode: int,
-                target: torch.Tensor, want_grad: bool, pattern: Optional[torch.Tensor] = None,
-                status: Optional[torch.Tensor] = None):
+                target: Optional[torch.Tensor], want_grad: bool,
+                pattern: Optional[torch.Tensor] = None, status: Optional[torch.Tensor] = None):
         """One launch of the render adjoint (+ the structure-factor adjoint when
-        gradients are wanted).  Returns (loss[n], dRt, dtab, dxyz, dB, drec)."""
+        gradients are wanted).  ``target`` is [n, H, W] or one [H, W] pattern
+        shared by every orientation.  Returns (loss[n], dRt, dtab, dxyz, dB, drec)."""
         n = batch.n
         H, W = self.shape
-        if target.shape != (n, H, W) or target.dtype != torch.float32 or not target.is_contiguous():
-            raise ValueError("target must be a contiguous float32 [n, H, W] tensor")
+        stride = 0
+        if target is not None:
+            if target.dtype != torch.float32 or not target.is_contiguous():
+                raise ValueError("target must be a contiguous float32 tensor")
+            if tuple(target.shape) == (n, H, W):
+                stride = H * W
+            elif tuple(target.shape) != (H, W):
+                raise ValueError(f"target must be [n, {H}, {W}] or [{H}, {W}]")
         loss = torch.empty(n, dtype=torch.float32, device=self.dev)
         dRt = dtab = None
         if want_grad:
@@ -278,13 +295,15 @@ class RefinementModel:
         sync_status = status is None
         if status is None:
             status = torch.zeros(n, dtype=torch.int32, device=self.dev)
+        work = self.workspace(batch.cap)
         sp = self._sim_params(t)
         stream = _lib.stream_handle(self.dev)
         _lib.check(self.L.dsx_render_adjoint(
             _lib.ptr(batch.idx), _lib.ptr(batch.spots), _lib.ptr(batch.count), batch.cap, n,
             _lib.ptr(t.hkl), _lib.ptr(t.F2), self.n_refl, _lib.ptr(Rt), ctypes.byref(sp),
-            ctypes.byref(self.render), mode, _lib.ptr(target), _lib.ptr(pattern), _lib.ptr(loss),
-            _lib.ptr(dRt), _lib.ptr(dtab), _lib.ptr(status), stream), "dsx_render_adjoint")
+            ctypes.byref(self.render), mode, _lib.ptr(target), stride, _lib.ptr(pattern),
+            _lib.ptr(loss), _lib.ptr(dRt), _lib.ptr(dtab), _lib.ptr(work),
+            0 if work is None else work.numel(), _lib.ptr(status), stream), "dsx_render_adjoint")
         if sync_status:
             engine.check_render_status(status)
         dxyz = dB = drec = None

@@ -350,12 +350,35 @@ def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigm
     p = render_params(shape, sigma, calibration, direct_beam_position, in_plane_angle, mirrored,
                       normalize, fast, clip_threshold)
     stream = _lib.stream_handle(dev) if stream is None else stream
-    fn = L.dsx_render_fast if fast else L.dsx_render_subpixel
-    _lib.check(fn(_lib.ptr(spots), _lib.ptr(count), cap, n, ctypes.byref(p), _lib.ptr(out),
-                  _lib.ptr(status), stream), "dsx_render")
+    if not fast and cap > int(L.dsx_render_max_spots()):
+        # more spots than the render LDS holds: the adjoint kernel's forward
+        # mode renders them chunk by chunk (same pixels as dsx_render_subpixel)
+        work = _workspace(dev, int(L.dsx_render_adjoint_workspace_ints(int(cap))))
+        sim = _lib.SimParams()
+        sim.k, sim.shape_width, sim.shape_model = 1.0, 1.0, _lib.SF_LINEAR
+        _lib.check(L.dsx_render_adjoint(
+            None, _lib.ptr(spots), _lib.ptr(count), cap, n, None, None, 0, None, ctypes.byref(sim),
+            ctypes.byref(p), _lib.ADJ_FORWARD, None, 0, _lib.ptr(out), None, None, None,
+            _lib.ptr(work), work.numel(), _lib.ptr(status), stream), "dsx_render_adjoint")
+    else:
+        fn = L.dsx_render_fast if fast else L.dsx_render_subpixel
+        _lib.check(fn(_lib.ptr(spots), _lib.ptr(count), cap, n, ctypes.byref(p), _lib.ptr(out),
+                      _lib.ptr(status), stream), "dsx_render")
     if sync_check:
         check_render_status(status)
     return out
+
+
+_WORK = {}
+
+
+def _workspace(dev, ints: int) -> torch.Tensor:
+    """Reusable device scratch of at least ``ints`` int32 words per device."""
+    w = _WORK.get(dev)
+    if w is None or w.numel() < ints:
+        w = torch.empty(ints, dtype=torch.int32, device=dev)
+        _WORK[dev] = w
+    return w
 
 
 def check_render_status(status: torch.Tensor) -> None:

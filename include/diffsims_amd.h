@@ -261,25 +261,35 @@ int dsx_render_fast_ragged(const float *spot_d, const long long *spot_off_d, con
  * For each pattern the sub-pixel pattern P (normalised when params->normalize)
  * is recomputed from the spot lists exactly as dsx_render_subpixel renders it,
  * and the upstream gradient G = dL/dP is
- *   mode DSX_ADJ_L2            G = 2 (P - target), loss_d[o] = sum (P - target)^2
- *   mode DSX_ADJ_PATTERN_GRAD  G = target (a caller-supplied dL/dP)
- * G is pulled back through the render, the intensity |F|^2 f(s, r) and the
- * excitation error, which are recomputed from idx_d, hkl_d, F2_d, Rt_d and
- * sim->rec (the same inputs dsx_simulate used):
+ *   mode DSX_ADJ_L2            G = 2 (P - T), loss_d[o] = sum (P - T)^2
+ *   mode DSX_ADJ_PATTERN_GRAD  G = T (a caller-supplied dL/dP)
+ *   mode DSX_ADJ_FORWARD       no G: only P is written (target_d unused)
+ * where T = target_d + o * target_stride (float [H][W]; stride 0 compares
+ * every pattern with one target).  G is pulled back through the render, the
+ * intensity |F|^2 f(s, r) and the excitation error, which are recomputed from
+ * idx_d, hkl_d, F2_d, Rt_d and sim->rec (the inputs dsx_simulate used):
  *   dRt_d   double [n][9]        dL/dRt of each pattern (overwritten)
  *   dtab_d  float  [n_refl][4]   {dL/dg (crystal frame, 3), dL/d|F(hkl)|^2},
  *                                ACCUMULATED (zero it first)
  * dRt_d and dtab_d may both be NULL (loss / pattern only).  pattern_d
  * (float [n][H][W], may be NULL) receives P; loss_d (float [n], may be NULL).
- * status_d as for the renderers.  At most 2^22 pixels per pattern.
+ * Patterns with more than dsx_render_max_spots() in-frame spots need the
+ * workspace work_d of dsx_render_adjoint_workspace_ints(cap) ints (may be NULL
+ * otherwise); without it, or when it is too small for a pattern, that
+ * pattern's status_d entry is 1.  At most 2^22 pixels per pattern.
+ * This entry point also serves as the sub-pixel renderer for such large
+ * patterns (mode DSX_ADJ_FORWARD).
  */
 #define DSX_ADJ_L2 0
 #define DSX_ADJ_PATTERN_GRAD 1
+#define DSX_ADJ_FORWARD 2
+long long dsx_render_adjoint_workspace_ints(int cap);
 int dsx_render_adjoint(const int *idx_d, const float *spot_d, const int *count_d, int cap,
                        int n, const int *hkl_d, const float *F2_d, int n_refl,
                        const double *Rt_d, const dsx_sim_params *sim,
                        const dsx_render_params *params, int mode, const float *target_d,
-                       float *pattern_d, float *loss_d, double *dRt_d, float *dtab_d,
+                       long long target_stride, float *pattern_d, float *loss_d,
+                       double *dRt_d, float *dtab_d, int *work_d, long long work_ints,
                        int *status_d, void *stream);
 
 /*

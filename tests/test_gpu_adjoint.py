@@ -196,3 +196,84 @@ def test_refinement_workload_gradients_match_fp64():
     L.backward()
     assert abs(float(L.detach()) - L_ref) <= TOL_LOSS * abs(L_ref), (float(L.detach()), L_ref)
     _compare_grads([leaves[k].grad for k in ("euler", "abc", "angles", "xyz", "B")], g_ref)
+
+
+def _big_problem(**kw):
+    """A 19-21 A triclinic cell: ~2,000 spots per pattern, all in a 128x128
+    frame, so the render and its adjoint take the chunked large-pattern path."""
+    rng = np.random.default_rng(17)
+    n_atoms = 4
+    ph = O.PhaseSpec((19.5, 20.5, 21.0), (88.0, 96.0, 91.0),
+                     list(rng.choice(["O", "Si", "Fe"], n_atoms)), rng.uniform(0, 1, (n_atoms, 3)),
+                     np.ones(n_atoms), rng.uniform(0.3, 1.0, n_atoms))
+    p = O.SimParams(kv=200, reciprocal_radius=1.2, max_excitation_error=0.03,
+                    shape_model=kw.pop("model", "lorentzian"), with_direct_beam=False,
+                    minimum_intensity=1e-20)
+    return G.Problem(ph, p, (128, 128), 1.2, 0.02, clip=1e-3, **kw)
+
+
+def test_large_pattern_subpixel_render_and_gradients():
+    print("adjoint large-pattern case", flush=True)
+    pr = _big_problem()
+    n = 3
+    eu = O.random_euler(n, 41)
+    target = np.random.default_rng(6).uniform(0, 0.3, (n,) + pr.shape)
+    L_ref, g_ref, frozen, pats = _reference(
+        pr, eu, lambda P: ((P - torch.as_tensor(target)) ** 2).sum())
+    assert min(len(f.idx) for f in frozen) > 1500
+    model = _model(pr)
+    leaves, args = _params(pr, eu, "cuda")
+    _check_survivors(model, args, frozen)
+    with torch.no_grad():
+        P = model.patterns(*args)                     # forward mode of the chunked kernel
+    assert np.abs(P.cpu().numpy() - pats).max() <= 1e-5 * pats.max()
+    L = model.l2_loss(*args, torch.tensor(target, dtype=torch.float32, device="cuda"))
+    L.backward()
+    assert abs(float(L.detach()) - L_ref) <= TOL_LOSS * abs(L_ref), (float(L.detach()), L_ref)
+    _compare_grads([t.grad for t in leaves], g_ref)
+
+
+def test_forward_mode_equals_subpixel_renderer():
+    """dsx_render_adjoint's forward mode and dsx_render_subpixel give the same
+    bits on patterns both can hold."""
+    import ctypes
+
+    from diffsims_amd import _lib, engine
+
+    pr = mono_problem()
+    eu = O.random_euler(7, 12)
+    model = _model(pr)
+    _, args = _params(pr, eu, "cuda")
+    t = model.tables(*args[1:])
+    b = model.spots(t, args[0].detach())
+    ref = engine.render_spots(b.spots, b.count, b.cap, pr.shape, pr.sigma, pr.calibration,
+                              fast=False, clip_threshold=pr.clip)
+    out = torch.empty_like(ref)
+    L = _lib.load()
+    sim = model._sim_params(t)
+    status = torch.zeros(b.n, dtype=torch.int32, device="cuda")
+    _lib.check(L.dsx_render_adjoint(
+        None, _lib.ptr(b.spots), _lib.ptr(b.count), b.cap, b.n, None, None, 0, None,
+        ctypes.byref(sim), ctypes.byref(model.render), _lib.ADJ_FORWARD, None, 0, _lib.ptr(out),
+        None, None, None, None, 0, _lib.ptr(status), _lib.stream_handle()), "forward")
+    torch.cuda.synchronize()
+    assert int(status.max()) == 0
+    assert torch.equal(out, ref)
+
+
+def test_broadcast_target():
+    """One target shared by every orientation (stride 0) equals the repeated target."""
+    pr = mono_problem()
+    n = 5
+    eu = O.random_euler(n, 13)
+    model = _model(pr)
+    T1 = torch.rand(pr.shape, device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+    res = []
+    for T in (T1, T1.expand((n,) + pr.shape).contiguous()):
+        leaves, args = _params(pr, eu, "cuda")
+        L = model.l2_loss(*args, T)
+        L.backward()
+        res.append((float(L.detach()), [x.grad.detach().cpu().numpy() for x in leaves]))
+    assert abs(res[0][0] - res[1][0]) <= 1e-6 * res[1][0]
+    for a, b in zip(res[0][1], res[1][1]):
+        assert np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1e-30)
