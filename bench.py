@@ -57,6 +57,9 @@ def parse():
                     help="timed hipGraph replays of the 64-phase ragged batch (0 disables it)")
     ap.add_argument("--ragged-scale", type=float, default=1.0,
                     help="fraction of the ragged batch's orientations to use")
+    ap.add_argument("--large-chunks", type=int, default=1,
+                    help="65,536-orientation chunks of the 500-atom precession case to run "
+                         "fused forward+backward (16 = the full 10^6 orientations; 0 disables)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the reference "
                          "kernel for scaling the HBM byte counters of a profiler) and exit")
@@ -212,6 +215,64 @@ def ragged_leg(dev, steps, scale):
     return res
 
 
+def large_precession_leg(dev, n_chunks):
+    """500-atom triclinic cell, r = 3.0 (~1.6e6 reflections), precession 1 deg,
+    512x512 sub-pixel patterns; each 65,536-orientation chunk computes the summed
+    L2 loss against one target and its orientation and B gradients."""
+    import torch
+
+    from diffsims_amd import differentiable as D
+    from diffsims_amd import workloads
+
+    d = workloads.large_precession()
+    pa = d["phase"]
+    t0 = time.perf_counter()
+    model = D.RefinementModel(pa, d["reciprocal_radius"], d["settings"], d["shape"], d["sigma"],
+                              d["calibration"], clip_threshold=d["clip"], device=dev)
+    setup = time.perf_counter() - t0
+    with torch.no_grad():
+        T = model.patterns(*D.forward_args(D.leaf_parameters(pa, d["target_euler"], dev,
+                                                             requires_grad=False)))[0].contiguous()
+    chunk = d["chunk"]
+    base = D.leaf_parameters(pa, d["euler"][:chunk], dev)
+    for name in ("abc", "angles", "xyz"):
+        base[name].requires_grad_(False)
+    times, spots, losses = [], [], []
+    n_done = 0
+    for c in range(n_chunks):
+        e = d["euler"][c * chunk:(c + 1) * chunk]
+        leaves = dict(base)
+        leaves["euler"] = torch.tensor(e, dtype=torch.float64, device=dev, requires_grad=True)
+        leaves["B"] = base["B"].detach().clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L = model.l2_loss(*D.forward_args(leaves), T)
+        L.backward()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        losses.append(float(L.detach()))
+        n_done += e.shape[0]
+    # mean spots of the last chunk (its spot lists are rebuilt for the count)
+    t = model.tables(*D.forward_args(leaves)[1:])
+    b = model.spots(t, D.forward_args(leaves)[0].detach())
+    mean_spots = float(b.count.float().mean())
+    steady = times[1:] if len(times) > 1 else times
+    per_chunk = float(np.mean(steady))
+    return {
+        "workload": "500-atom triclinic cell (edges U[15,30] A, angles U[80,110] deg, Z U[1,30], "
+                    "B U[0.5,2]), r=3.0 1/A, precession 1 deg (precession-integrated lorentzian), "
+                    "max_exc 0.02, 200 kV; chunks of 65,536 random orientations, 512x512 sub-pixel "
+                    "(0.012 1/A/px, sigma 1 px, clip 1); summed L2 loss vs one target, orientation "
+                    "and B gradients",
+        "n_hkl": model.n_refl, "chunks": n_chunks, "orientations": n_done,
+        "mean_spots": round(mean_spots, 1), "spot_cap": model.cap,
+        "ms_per_chunk_first": round(1e3 * times[0], 1),
+        "ms_per_chunk": round(1e3 * per_chunk, 1),
+        "fwd_bwd_patterns_per_s": round(chunk / per_chunk, 1) if n_done >= chunk else None,
+        "setup_s": round(setup, 2), "loss_first_chunk": losses[0],
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -328,6 +389,9 @@ def main():
         torch.cuda.empty_cache()
     if args.ragged_steps > 0:
         res["ragged"] = ragged_leg(dev, args.ragged_steps, args.ragged_scale)
+        torch.cuda.empty_cache()
+    if args.large_chunks > 0:
+        res["large_precession"] = large_precession_leg(dev, args.large_chunks)
     print(json.dumps(res), flush=True)
 
 

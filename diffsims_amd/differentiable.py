@@ -211,8 +211,7 @@ class RefinementModel:
         self.order_dev = self.order.to(self.dev)
         self.inv_order_dev = torch.argsort(self.order).to(self.dev)
         if cap is None:
-            cap = min(max(self.n_refl, 1),
-                      2 * engine.expected_spots(init, settings, reciprocal_radius) + 128)
+            cap = engine.default_cap(init, settings, reciprocal_radius)
         self.cap = int(cap)
 
     # -- forward pieces ------------------------------------------------------

@@ -253,12 +253,23 @@ def rotation_matrices_to_device(matrices: np.ndarray, device=None) -> torch.Tens
 
 
 def expected_spots(tab: PhaseTable, settings: SimSettings, reciprocal_radius: float) -> int:
-    """Mean number of intersecting reflections for a random orientation: the
-    table fraction inside a shell of half-thickness max_excitation_error (plus
-    r sin(phi) with precession) around the Ewald sphere."""
-    me = settings.max_excitation_error + tab.g_max * math.sin(math.radians(abs(settings.precession_angle)))
+    """Mean number of intersecting reflections for a random orientation.
+
+    The reflections fill a ball of radius R (the reciprocal radius); the Ewald
+    shell is, near the origin, a slab |z| <= me + rho sin(phi) (rho the distance
+    from the beam axis, phi the precession angle), which holds the fraction
+    1.5 me / R + sin(phi) of the ball."""
     rr = max(reciprocal_radius, 1e-6)
-    return int(math.ceil(tab.n * min(1.0, 1.5 * me / rr)))
+    frac = 1.5 * settings.max_excitation_error / rr + math.sin(math.radians(abs(settings.precession_angle)))
+    return int(math.ceil(tab.n * min(1.0, frac)))
+
+
+def default_cap(tab: PhaseTable, settings: SimSettings, reciprocal_radius: float) -> int:
+    """Spot-list capacity per orientation: generous for small lists (their
+    relative spread is large), 25 % above the mean for large ones."""
+    e = expected_spots(tab, settings, reciprocal_radius)
+    cap = 2 * e + 128 if e < 4096 else int(1.25 * e) + 256
+    return max(1, min(max(tab.n, 1), cap))
 
 
 def simulate(tab: PhaseTable, Rt: torch.Tensor, settings: SimSettings, reciprocal_radius: float,
@@ -271,7 +282,7 @@ def simulate(tab: PhaseTable, Rt: torch.Tensor, settings: SimSettings, reciproca
         raise ValueError("Rt must be a contiguous float64 [n, 9] tensor")
     params = settings.params(tab.g_max, tab.rec)
     if cap is None:
-        cap = min(max(tab.n, 1), 2 * expected_spots(tab, settings, reciprocal_radius) + 128)
+        cap = default_cap(tab, settings, reciprocal_radius)
     stream = _lib.stream_handle(dev) if stream is None else stream
     for attempt in range(2):
         idx = torch.empty((n, cap), dtype=torch.int32, device=dev)

@@ -164,3 +164,29 @@ def ragged_library(n_phases: int = 64, orient_scale: float = 1.0, seed: int = SE
                      minimum_intensity=1e-20)
     return Workload("ragged-64", phases, rots, radii, False, st, (128, 128), 0.04, 1.0,
                     notes={"orient_scale": orient_scale})
+
+
+def large_precession(n_orient: int = 1_000_000, seed: int = SEED + 4):
+    """Large triclinic cell with precession: 500 atoms, edges U[15, 30] A,
+    angles U[80, 110] deg, Z uniform in [1, 30], B U[0.5, 2.0] A^2, occupancy 1;
+    reciprocal_radius 3.0 1/A, precession 1 deg with the precession-integrated
+    Lorentzian (approximate_precession), max_excitation_error 0.02, 200 kV, no
+    direct beam; ``n_orient`` Haar-random orientations processed in chunks of
+    65,536; 512x512 sub-pixel patterns (0.012 1/A/px: the frame holds r <= 3.07,
+    sigma 1 px, clip 1, normalised).  The loss of a chunk is
+    sum_o sum_px (P_o - T)^2 against one target T (the pattern of a held-out
+    orientation), and each chunk yields the orientation and B gradients."""
+    rng = np.random.default_rng(seed)
+    n_atoms = 500
+    abc = tuple(float(v) for v in rng.uniform(15, 30, 3))
+    angles = tuple(float(v) for v in rng.uniform(80, 110, 3))
+    Z = rng.integers(1, 31, n_atoms)
+    pa = PhaseArrays(abc, angles, [symbol(int(z)) for z in Z], rng.uniform(0, 1, (n_atoms, 3)),
+                     np.ones(n_atoms), rng.uniform(0.5, 2.0, n_atoms), "tri500")
+    euler = random_euler(n_orient, rng)
+    target_euler = random_euler(1, rng)
+    st = SimSettings(accelerating_voltage=200, max_excitation_error=0.02, shape_model="lorentzian",
+                     precession_angle=1.0, approximate_precession=True, minimum_intensity=1e-20)
+    return {"phase": pa, "euler": euler, "target_euler": target_euler, "settings": st,
+            "reciprocal_radius": 3.0, "shape": (512, 512), "calibration": 0.012, "sigma": 1.0,
+            "clip": 1.0, "chunk": 65536}

@@ -207,21 +207,29 @@ def torch_patterns(pr: Problem, Rt: torch.Tensor, rec: torch.Tensor, xyz: torch.
     out = []
     for o, fz in enumerate(frozen):
         Q = torch.zeros((H, W), dtype=torch.float64)
-        if len(fz.idx):
-            gl = g[fz.idx] @ Rt[o].T                                # M^T g
+        keep = np.nonzero(fz.boxes[:, 0] >= 0)[0] if len(fz.idx) else np.zeros(0, np.int64)
+        if keep.size:
+            rows = fz.idx[keep]
+            gl = g[rows] @ Rt[o].T                                  # M^T g
             r2 = gl[:, 0] ** 2 + gl[:, 1] ** 2
             s = (k - torch.sqrt(k * k - r2)) - gl[:, 2]
-            I = F2[fz.idx] * shape_factor(s, torch.sqrt(r2), p)
+            I = F2[rows] * shape_factor(s, torch.sqrt(r2), p)
             X, Y = _pixel_xy(gl[:, :2], pr)
-            for j in range(len(fz.idx)):
-                y0, y1, x0, x1 = (int(v) for v in fz.boxes[j])
-                if y0 < 0:
-                    continue
-                yy = torch.arange(y0, y1, dtype=torch.float64)[:, None]
-                xx = torch.arange(x0, x1, dtype=torch.float64)[None, :]
-                blk = I[j] * pref * torch.exp(ep * ((yy - Y[j]) ** 2 + (xx - X[j]) ** 2))
-                Q = Q.index_put((torch.arange(y0, y1)[:, None], torch.arange(x0, x1)[None, :]),
-                                blk, accumulate=True)
+            # every clip box as a masked patch of the largest box size
+            bx = fz.boxes[keep]
+            hmax = int((bx[:, 1] - bx[:, 0]).max())
+            wmax = int((bx[:, 3] - bx[:, 2]).max())
+            dy = np.arange(hmax)[None, :, None]
+            dx = np.arange(wmax)[None, None, :]
+            yy = bx[:, 0][:, None, None] + dy + 0 * dx
+            xx = bx[:, 2][:, None, None] + dx + 0 * dy
+            m = (yy < bx[:, 1][:, None, None]) & (xx < bx[:, 3][:, None, None])
+            sel = np.nonzero(m)
+            ys = torch.as_tensor(yy[sel])
+            xs = torch.as_tensor(xx[sel])
+            j = torch.as_tensor(sel[0])
+            d2 = (ys.double() - Y[j]) ** 2 + (xs.double() - X[j]) ** 2
+            Q = Q.index_put((ys, xs), I[j] * pref * torch.exp(ep * d2), accumulate=True)
         if pr.normalize and float(Q.detach().max()) > 0:
             Q = Q / Q.max()
         out.append(Q)

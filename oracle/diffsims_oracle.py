@@ -441,15 +441,24 @@ def structure_factor(hkl, g, xyz, occ, B, coeffs, table, phase_mode="reference")
     exactly zero (the GPU kernel's behaviour).
     """
     hkl = np.asarray(hkl, dtype=np.float64)
-    g2 = np.asarray(g, dtype=np.float64) ** 2
-    f = atomic_f(g2, coeffs, table)                      # (M, n)
-    t = hkl @ np.asarray(xyz, dtype=np.float64).T        # (M, n) turns
-    dw = np.exp(-0.25 * np.outer(g2, np.asarray(B, dtype=np.float64)))
-    if phase_mode == "reference":
-        ph = np.exp(2j * np.pi * t)
-    else:
-        ph = _exact_phase_expi(t)
-    return np.sum(f * np.asarray(occ, dtype=np.float64) * dw * ph, axis=-1)
+    g2_all = np.asarray(g, dtype=np.float64) ** 2
+    xyz = np.asarray(xyz, dtype=np.float64)
+    occ = np.asarray(occ, dtype=np.float64)
+    B = np.asarray(B, dtype=np.float64)
+    out = np.empty(hkl.shape[0], dtype=np.complex128)
+    # rows in blocks so the (rows, atoms) temporaries stay small for big cells
+    step = max(1, (1 << 22) // max(1, xyz.shape[0]))
+    for r0 in range(0, hkl.shape[0], step):
+        g2 = g2_all[r0:r0 + step]
+        f = atomic_f(g2, coeffs, table)                  # (M, n)
+        t = hkl[r0:r0 + step] @ xyz.T                    # (M, n) turns
+        dw = np.exp(-0.25 * np.outer(g2, B))
+        if phase_mode == "reference":
+            ph = np.exp(2j * np.pi * t)
+        else:
+            ph = _exact_phase_expi(t)
+        out[r0:r0 + step] = np.sum(f * occ * dw * ph, axis=-1)
+    return out
 
 
 # --------------------------------------------------------------------------

@@ -15,6 +15,7 @@ import torch
 
 from oracle import diffsims_grad as G
 from oracle import diffsims_oracle as O
+from tests import parity as P
 from tests.test_grad_oracle import mono_problem
 
 pytestmark = pytest.mark.gpu
@@ -277,3 +278,86 @@ def test_broadcast_target():
     assert abs(res[0][0] - res[1][0]) <= 1e-6 * res[1][0]
     for a, b in zip(res[0][1], res[1][1]):
         assert np.abs(a - b).max() <= 1e-5 * max(np.abs(b).max(), 1e-30)
+
+
+def _precession_problem(rr, shape, cal):
+    from diffsims_amd import workloads
+
+    d = workloads.large_precession(4)
+    pa = d["phase"]
+    ph = O.PhaseSpec(pa.abc, pa.angles, pa.symbols, pa.xyz, pa.occ, pa.B)
+    p = O.SimParams(kv=200, reciprocal_radius=rr, max_excitation_error=0.02,
+                    shape_model="lorentzian", precession_angle=1.0, with_direct_beam=False,
+                    minimum_intensity=1e-20)
+    return d, G.Problem(ph, p, shape, 1.0, cal, clip=1.0)
+
+
+def test_large_precession_cell_gradients_reduced_radius():
+    """The 500-atom triclinic precession cell at reciprocal radius 1.0
+    (~3,000 spots per 256x256 pattern): fused loss and gradients against FP64."""
+    print("adjoint 500-atom precession cell, r = 1.0", flush=True)
+    d, pr = _precession_problem(1.0, (256, 256), 0.008)
+    eu = d["euler"][:2]
+    target = np.random.default_rng(8).uniform(0, 0.2, pr.shape)
+    L_ref, g_ref, frozen, pats = _reference(
+        pr, eu, lambda P: ((P - torch.as_tensor(target)[None]) ** 2).sum())
+    print("  spots per pattern:", [len(f.idx) for f in frozen], flush=True)
+    assert min(len(f.idx) for f in frozen) > 1024
+    model = _model(pr)
+    leaves, args = _params(pr, eu, "cuda")
+    _check_survivors(model, args, frozen)
+    L = model.l2_loss(*args, torch.tensor(target, dtype=torch.float32, device="cuda"))
+    L.backward()
+    assert abs(float(L.detach()) - L_ref) <= TOL_LOSS * abs(L_ref), (float(L.detach()), L_ref)
+    _compare_grads([t.grad for t in leaves], g_ref)
+
+
+def test_large_precession_cell_full_radius_forward():
+    """The full 500-atom precession case (r = 3.0, ~1.6e6 reflections, ~4e4
+    spots per 512x512 pattern): spot lists and sub-pixel patterns of two
+    orientations against the FP64 oracle (F evaluated for the intersecting rows)."""
+    print("500-atom precession cell, r = 3.0 forward", flush=True)
+    from diffsims_amd import engine
+
+    d, pr = _precession_problem(3.0, (512, 512), 0.012)
+    p = pr.p
+    M = O.bunge_matrix(d["euler"][:2])
+    tab = engine.build_phase_table(d["phase"], 3.0, False, "lobato")
+    base, rec = pr.ph.basis()
+    hkl = tab.hkl_host()
+    g = O.g_cartesian(hkl, rec)
+    k = 1 / O.electron_wavelength(200)
+    rows = set()
+    for o in range(2):
+        rows |= set(np.nonzero(O.intersect(g @ M[o], k, p)[2])[0].tolist())
+    rows = np.array(sorted(rows))
+    F2 = np.full(len(hkl), np.nan)
+    coeffs, _ = O.atom_coeffs(pr.ph.symbols, "lobato")
+    F = O.structure_factor(hkl[rows], np.linalg.norm(g[rows], axis=1), pr.ph.xyz, pr.ph.occ,
+                           pr.ph.B, coeffs, "lobato", "exact")
+    F2[rows] = (F * F.conjugate()).real
+    otab = O.PhaseTables(hkl, g, np.linalg.norm(g, axis=1), F2, len(hkl))
+    print(f"  {len(hkl)} reflections, {len(rows)} intersecting rows", flush=True)
+    b = engine.simulate(tab, engine.rotation_matrices_to_device(M), d["settings"], 3.0)
+    pats = engine.render(b, pr.shape, 1.0, pr.calibration, fast=False, clip_threshold=1.0)
+    cnt = b.count.cpu().numpy()
+    pats = pats.cpu().numpy()
+    for o in range(2):
+        ref = O.simulate_one(otab, M[o], k, p)
+        s_all = P.excitation_errors(otab, M[o], k)
+        gi = b.idx[o, :cnt[o]].cpu().numpy()
+        print(f"  orientation {o}: {cnt[o]} spots", flush=True)
+        assert cnt[o] > 20000
+        if not np.array_equal(gi, ref.idx):
+            # only reflections at the edge of the precession band may differ
+            diff = np.setxor1d(gi, ref.idx)
+            assert diff.size <= 4, diff.size
+            continue
+        raw = O.render_subpixel(ref.xyz[:, :2], ref.intensity, pr.shape, 1.0, pr.calibration,
+                                normalize=False, clip_threshold=1.0)
+        rp = raw / raw.max()
+        # a clip box whose edge moves by a pixel under FP32 coordinates changes
+        # pixels worth at most clip_threshold
+        edge = 1.0 / raw.max()
+        err = np.abs(pats[o] - rp).max()
+        assert err <= P.TOL_P + edge, err
