@@ -60,6 +60,7 @@ EXPORTED = (
     "dsx_simulate_ragged",
     "dsx_render_dense_max_pixels",
     "dsx_render_fast_ragged",
+    "dsx_polar_templates",
 )
 
 
@@ -142,6 +143,7 @@ _SIGS = {
     "dsx_simulate_tile_size": (_I, []),
     "dsx_simulate_ragged": (_I, [_P, _I, _P, _I, ctypes.POINTER(SimParams), _I, _P]),
     "dsx_render_dense_max_pixels": (_I, []),
+    "dsx_polar_templates": (_I, [_P, _P, _I, _I, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P]),
     "dsx_render_fast_ragged": (_I, [_P, _P, _P, _P, _P, _I, ctypes.POINTER(RenderParams), _I, _I,
                                     _P, _P, _P]),
 }

@@ -422,5 +422,39 @@ def empty_batch(tab: PhaseTable, n: int, cap: int, device) -> SpotBatch:
                      torch.empty(n, dtype=torch.float32, device=dev), cap)
 
 
+def polar_templates(spots: torch.Tensor, count: torch.Tensor, cap: int, radial_axes=None,
+                    azimuthal_axes=None, width: Optional[int] = None, stream=None):
+    """(r, theta, intensity) templates of every spot list on the device:
+    Simulation2D.polar_flatten_simulations (simulation2d.py:313-355) without a
+    host round trip.  Returns torch tensors of shape [n, width] (width = the
+    largest spot count unless given): float32 r and theta, or int32 nearest
+    indices when both axes are given (entries past the last index dropped)."""
+    L = _lib.load(require_gpu=True)
+    dev = spots.device
+    n = int(count.shape[0])
+    if width is None:
+        width = int(count.max().item()) if n else 0
+    axes = radial_axes is not None and azimuthal_axes is not None
+    I = torch.zeros((n, width), dtype=torch.float32, device=dev)
+    n_out = torch.zeros(n, dtype=torch.int32, device=dev)
+    stream = _lib.stream_handle(dev) if stream is None else stream
+    if axes:
+        ra = torch.as_tensor(np.asarray(radial_axes, dtype=np.float64)).to(dev)
+        az = torch.as_tensor(np.asarray(azimuthal_axes, dtype=np.float64)).to(dev)
+        r = torch.zeros((n, width), dtype=torch.int32, device=dev)
+        t = torch.zeros((n, width), dtype=torch.int32, device=dev)
+        _lib.check(L.dsx_polar_templates(_lib.ptr(spots), _lib.ptr(count), cap, n, width,
+                                         _lib.ptr(ra), int(ra.numel()), _lib.ptr(az),
+                                         int(az.numel()), None, None, _lib.ptr(r), _lib.ptr(t),
+                                         _lib.ptr(I), _lib.ptr(n_out), stream), "dsx_polar_templates")
+    else:
+        r = torch.zeros((n, width), dtype=torch.float32, device=dev)
+        t = torch.zeros((n, width), dtype=torch.float32, device=dev)
+        _lib.check(L.dsx_polar_templates(_lib.ptr(spots), _lib.ptr(count), cap, n, width, None, 0,
+                                         None, 0, _lib.ptr(r), _lib.ptr(t), None, None, _lib.ptr(I),
+                                         _lib.ptr(n_out), stream), "dsx_polar_templates")
+    return r, t, I
+
+
 def render(batch: SpotBatch, shape, sigma, calibration, **kw) -> torch.Tensor:
     return render_spots(batch.spots, batch.count, batch.cap, shape, sigma, calibration, **kw)

@@ -310,6 +310,24 @@ int dsx_structure_factor_adjoint(const int *hkl_d, int n, const double rec[9],
                                  const float *dtab_d, float *dxyz_d, float *dB_d,
                                  double *drec_d, void *stream);
 
+/*
+ * Polar templates from spot lists — replaces Simulation2D.polar_flatten_simulations
+ * (simulation2d.py:313-355) with DiffractingVector.to_flat_polar
+ * (_diffracting_vector.py:186-194) and get_closest (simulation2d.py:767-781).
+ * For orientation o, the spots of dsx_simulate give r = |(x, y)| and
+ * theta = atan2(y, x); rows of `width` entries (zero-padded, in spot order):
+ *   without axes (radial_d or azimuthal_d NULL): r_d, theta_d (float [n][width])
+ *   with both axes (ascending double arrays): r_idx_d, theta_idx_d (int32
+ *     [n][width]) hold the nearest axis indices, entries with an index on or
+ *     past the last axis entry are dropped
+ *   I_d float [n][width] intensities; n_out_d int32 [n] (may be NULL) entries
+ *   kept per orientation.
+ */
+int dsx_polar_templates(const float *spot_d, const int *count_d, int cap, int n, int width,
+                        const double *radial_d, int n_radial, const double *azimuthal_d,
+                        int n_azimuthal, float *r_d, float *theta_d, int *r_idx_d,
+                        int *theta_idx_d, float *I_d, int *n_out_d, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -726,3 +726,48 @@ def simulate_library(ph: PhaseSpec, rotations_M: np.ndarray, p: SimParams,
             pats[i] = render_fast(sp.xyz[:, :2], sp.intensity, render["shape"],
                                   render["sigma"], render["calibration"])
     return spots, pats
+
+
+# --------------------------------------------------------------------------
+# Polar templates (simulation2d.py:313-355, 767-781; _diffracting_vector.py:186-194)
+# --------------------------------------------------------------------------
+
+
+def nearest_indices(axis, values):
+    """For each value, the index of the closest axis entry; the lower neighbour
+    wins only when strictly closer (get_closest, simulation2d.py:767-781)."""
+    axis = np.asarray(axis, dtype=np.float64)
+    values = np.asarray(values, dtype=np.float64)
+    out = np.empty(values.shape, dtype=np.int64)
+    for i, v in enumerate(values.ravel()):
+        hi = int(np.sum(axis < v))            # first index with axis >= v
+        if hi == len(axis):
+            out.flat[i] = hi - 1
+            continue
+        lo = max(hi - 1, 0)
+        out.flat[i] = hi - 1 if abs(v - axis[lo]) < abs(v - axis[hi]) else hi
+    return out
+
+
+def polar_flatten(spot_lists, radial_axes=None, azimuthal_axes=None):
+    """(r, theta, intensity) arrays (n, max spots) of lab-frame spot lists
+    [(xyz (S, 3), intensity (S,))] as polar_flatten_simulations builds them."""
+    width = max(len(I) for _, I in spot_lists)
+    n = len(spot_lists)
+    r_t, t_t, i_t = np.zeros((n, width)), np.zeros((n, width)), np.zeros((n, width))
+    for i, (xyz, I) in enumerate(spot_lists):
+        xyz = np.asarray(xyz, dtype=np.float64).reshape(-1, 3)
+        r = np.sqrt(xyz[:, 0] ** 2 + xyz[:, 1] ** 2)
+        t = np.arctan2(xyz[:, 1], xyz[:, 0])
+        I = np.asarray(I, dtype=np.float64)
+        if radial_axes is not None and azimuthal_axes is not None:
+            r = nearest_indices(radial_axes, r)
+            t = nearest_indices(azimuthal_axes, t)
+            keep = (r < len(radial_axes) - 1) & (t < len(azimuthal_axes) - 1)
+            r, t, I = r[keep], t[keep], I[keep]
+        r_t[i, :len(r)] = r
+        t_t[i, :len(t)] = t
+        i_t[i, :len(I)] = I
+    if radial_axes is not None and azimuthal_axes is not None:
+        r_t, t_t = r_t.astype(int), t_t.astype(int)
+    return r_t, t_t, i_t

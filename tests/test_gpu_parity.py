@@ -285,3 +285,38 @@ def test_render_fast_maximum_search(case):
         Y = (spots[o, :, 1] / np.float32(cal) + np.float32(cy)).astype(np.float64)
         rp = P.render_from_pixels(X, Y, spots[o, :, 3], shape, sigma)
         assert np.abs(out[o] - rp).max() <= P.TOL_P * rp.max(), (case, o)
+
+
+@pytest.mark.parametrize("axes", [False, True])
+def test_polar_templates_on_device(axes):
+    """dsx_polar_templates against the oracle's polar_flatten on the same spot
+    lists (FP64 coordinates); nearest-index ties within 1e-6 of a midpoint may
+    fall on either side."""
+    import torch
+
+    e = eng()
+    st = e.SimSettings(max_excitation_error=0.02, shape_model="lorentzian")
+    ph = P.triclinic(6, 8)
+    tab, b, M, refs, _ = run_parity(ph, st, 1.5, False, 40, 9)
+    cnt = b.count.cpu().numpy()
+    ok = [o for o in range(len(refs)) if np.array_equal(b.idx[o, :cnt[o]].cpu().numpy(), refs[o].idx)]
+    lists = [(refs[o].xyz, refs[o].intensity) for o in ok]
+    radial = np.linspace(0, 1.6, 81) if axes else None
+    az = np.linspace(-np.pi, np.pi, 361) if axes else None
+    r_ref, t_ref, i_ref = O.polar_flatten(lists, radial, az)
+    r, t, I = e.polar_templates(b.spots[ok], b.count[ok], b.cap, radial, az,
+                                width=r_ref.shape[1])
+    r, t, I = r.cpu().numpy(), t.cpu().numpy(), I.cpu().numpy()
+    assert np.abs(I - i_ref).max() <= P.TOL_I * max(i_ref.max(), 1e-300)
+    if not axes:
+        assert np.abs(r - r_ref).max() <= 1e-6 and np.abs(t - t_ref).max() <= 1e-5
+        return
+    for row in range(len(ok)):
+        xyz = refs[ok[row]].xyz
+        rv = np.hypot(xyz[:, 0], xyz[:, 1])
+        tv = np.arctan2(xyz[:, 1], xyz[:, 0])
+        near = np.any(np.abs(np.abs(rv[:, None] - (radial[:-1] + radial[1:])[None] / 2)) < 1e-6) or \
+            np.any(np.abs(np.abs(tv[:, None] - (az[:-1] + az[1:])[None] / 2)) < 1e-6)
+        if near:
+            continue
+        assert np.array_equal(r[row], r_ref[row]) and np.array_equal(t[row], t_ref[row]), row

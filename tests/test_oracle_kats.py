@@ -380,3 +380,20 @@ def test_gaussian_filter_is_reflected_tap_sum():
         ref = ndi.gaussian_filter(d, sigma)
         mine = np.array([[K(y, cy, H) * K(xx, cx, W) for xx in range(W)] for y in range(H)])
         assert np.abs(ref - mine).max() < 1e-15
+
+
+def test_polar_flatten_oracle_nearest_index_rules():
+    """get_closest semantics: ties go to the upper neighbour, values past either
+    end clamp, and the last index is dropped by polar_flatten."""
+    ax = np.array([0.0, 1.0, 2.0, 3.0])
+    v = np.array([-5.0, 0.4, 0.5, 0.6, 1.5, 2.49, 2.5, 9.0])
+    assert O.nearest_indices(ax, v).tolist() == [0, 0, 1, 1, 2, 2, 3, 3]
+    xyz = np.array([[1.0, 0.0, 0.0], [0.0, 2.0, 0.0], [3.0, 0.0, 0.0]])
+    r, t, I = O.polar_flatten([(xyz, np.array([1.0, 2.0, 3.0])), (xyz[:1], np.array([5.0]))],
+                              ax, np.linspace(-np.pi, np.pi, 9))
+    assert r.dtype.kind == "i" and r.shape == (2, 3)
+    # the spot at r = 3 (index 3 = last) is dropped; theta pi/2 -> index 6 of 9
+    assert r[0].tolist() == [1, 2, 0] and t[0].tolist() == [4, 6, 0] and I[0].tolist() == [1, 2, 0]
+    assert I[1].tolist() == [5, 0, 0]
+    rr, tt, _ = O.polar_flatten([(xyz, np.ones(3))])
+    assert np.allclose(rr[0], [1, 2, 3]) and np.allclose(tt[0], [0, np.pi / 2, 0])
