@@ -61,6 +61,8 @@ EXPORTED = (
     "dsx_render_dense_max_pixels",
     "dsx_render_fast_ragged",
     "dsx_polar_templates",
+    "dsx_render_fast_large_workspace_ints",
+    "dsx_render_fast_large",
 )
 
 
@@ -143,6 +145,9 @@ _SIGS = {
     "dsx_simulate_tile_size": (_I, []),
     "dsx_simulate_ragged": (_I, [_P, _I, _P, _I, ctypes.POINTER(SimParams), _I, _P]),
     "dsx_render_dense_max_pixels": (_I, []),
+    "dsx_render_fast_large_workspace_ints": (ctypes.c_longlong, [_I]),
+    "dsx_render_fast_large": (_I, [_P, _P, _I, _I, ctypes.POINTER(RenderParams), _P,
+                                   ctypes.c_longlong, _P, _P, _P]),
     "dsx_polar_templates": (_I, [_P, _P, _I, _I, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P]),
     "dsx_render_fast_ragged": (_I, [_P, _P, _P, _P, _P, _I, ctypes.POINTER(RenderParams), _I, _I,
                                     _P, _P, _P]),

@@ -732,6 +732,158 @@ __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Windowed dense fast-mode renderer for large patterns with many spots (more
+// pixels than the dense kernel holds and more spots than the splat kernel
+// holds).  In-frame spots are binned by their source row strip into a
+// caller workspace; every output strip then rebuilds, in LDS, the spot image
+// of its window (the strip plus R rows of halo each side: all source rows a
+// reflected tap can reach), resolves last-write-wins with an owner map,
+// filters it along x and then along y (scipy's separable Gaussian with
+// 'reflect' boundaries) and writes the strip.  Pass A finds the maximum.
+// ---------------------------------------------------------------------------
+
+constexpr int kLargeThreads = 1024;
+#ifndef DSX_LARGE_WINDOW_KB
+#define DSX_LARGE_WINDOW_KB 128
+#endif
+constexpr int kLargeWindowBytes = DSX_LARGE_WINDOW_KB * 1024;
+
+struct LargeArgs {
+  RenderArgs r;
+  int *bins;             // workspace: cap ints per block
+  long long per_block;
+  int rows;              // output rows per strip
+  int n_src;             // source strips (= output strips)
+};
+
+template <int kR>
+__global__ __launch_bounds__(kLargeThreads) void render_fast_large_kernel(LargeArgs la) {
+  extern __shared__ __align__(16) float s_win[];
+  __shared__ float s_taps[kMaxR + 1];
+  __shared__ int s_start[kMaxStrips + 1];
+  __shared__ int s_fill[kMaxStrips];
+  __shared__ float s_wmax[kLargeThreads / kWave];
+  __shared__ int s_any;
+  const RenderArgs &a = la.r;
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int wid = tid / kWave;
+  const int H = a.H, W = a.W;
+  const int R = kR > 0 ? kR : a.R;
+  const int rows = la.rows;
+  const int win_max = rows + 2 * R;
+  int *owner = reinterpret_cast<int *>(s_win);  // window spot image (owner, then value)
+  float *img = s_win;
+  float *tmp = s_win + win_max * W;              // row-filtered window
+  int *bins = la.bins + (size_t)blockIdx.x * la.per_block;
+  for (int i = tid; i <= kMaxR; i += kLargeThreads) s_taps[i] = a.taps[i];
+
+  for (int pi = blockIdx.x; pi < a.n; pi += gridDim.x) {
+    const int p = a.list ? a.list[pi] : pi;
+    const int n = min(a.count[p], a.capv ? a.capv[p] : a.cap);
+    const float4 *spots = a.spots + (a.spot_off ? a.spot_off[p] : (long long)p * a.cap);
+    for (int s = tid; s <= la.n_src; s += kLargeThreads) s_start[s] = 0;
+    if (tid == 0) s_any = 0;
+    __syncthreads();
+    // bins of in-frame spots by source row strip
+    bool any = false;
+    for (int j = tid; j < n; j += kLargeThreads) {
+      float X, Y;
+      if (!to_pixel(spots[j], a, X, Y)) continue;
+      any = true;
+      atomicAdd(&s_start[min(static_cast<int>(Y), H - 1) / rows + 1], 1);
+    }
+    if (__ballot(any) != 0 && lane == 0) s_any = 1;
+    __syncthreads();
+    if (tid == 0) {
+      int acc = 0;
+      for (int s = 0; s < la.n_src; ++s) {
+        acc += s_start[s + 1];
+        s_start[s + 1] = acc;
+      }
+    }
+    __syncthreads();
+    for (int s = tid; s < la.n_src; s += kLargeThreads) s_fill[s] = s_start[s];
+    __syncthreads();
+    for (int j = tid; j < n; j += kLargeThreads) {
+      float X, Y;
+      if (!to_pixel(spots[j], a, X, Y)) continue;
+      bins[atomicAdd(&s_fill[min(static_cast<int>(Y), H - 1) / rows], 1)] = j;
+    }
+    __syncthreads();  // bins in global memory, block-visible
+    float *out = a.out + (size_t)p * H * W;
+    if (tid == 0 && a.status) a.status[p] = 0;
+    if (s_any == 0) {  // no spot in frame: np.zeros(shape), not normalised
+      for (int i = tid; i < H * W; i += kLargeThreads) __builtin_nontemporal_store(0.0f, out + i);
+      __syncthreads();
+      continue;
+    }
+    float sc = 1.0f;
+    for (int pass = a.normalize ? 0 : 1; pass < 2; ++pass) {
+      float lmax = 0.0f;
+      for (int s = 0; s < la.n_src; ++s) {
+        const int y0 = s * rows, y1 = min(H, y0 + rows);
+        const int wy0 = max(0, y0 - R), wy1 = min(H, y1 + R);
+        const int wn = (wy1 - wy0) * W;
+        for (int i = tid; i < wn; i += kLargeThreads) owner[i] = -1;
+        __syncthreads();
+        const int e0 = s_start[wy0 / rows], e1 = s_start[(wy1 - 1) / rows + 1];
+        for (int e = e0 + tid; e < e1; e += kLargeThreads) {
+          const int j = bins[e];
+          float X, Y;
+          to_pixel(spots[j], a, X, Y);
+          const int yi = min(static_cast<int>(Y), H - 1), xi = min(static_cast<int>(X), W - 1);
+          if (yi >= wy0 && yi < wy1) atomicMax(&owner[(yi - wy0) * W + xi], j);
+        }
+        __syncthreads();
+        for (int i = tid; i < wn; i += kLargeThreads) {
+          const int o = owner[i];
+          img[i] = o >= 0 ? spots[o].w : 0.0f;
+        }
+        __syncthreads();
+        for (int i = tid; i < wn; i += kLargeThreads) {
+          const int r = i / W, x = i - r * W;
+          const float *row = img + r * W;
+          float v = s_taps[0] * row[x];
+          if (x >= R && x + R < W) {
+            for (int d = 1; d <= R; ++d) v = fmaf(s_taps[d], row[x - d] + row[x + d], v);
+          } else {
+            for (int d = 1; d <= R; ++d)
+              v = fmaf(s_taps[d], row[reflect_index(x - d, W)] + row[reflect_index(x + d, W)], v);
+          }
+          tmp[i] = v;
+        }
+        __syncthreads();
+        const int cnt = (y1 - y0) * W;
+        for (int i = tid; i < cnt; i += kLargeThreads) {
+          const int y = y0 + i / W, x = i % W;
+          const float *col = tmp + x;
+          float v = s_taps[0] * col[(y - wy0) * W];
+          for (int d = 1; d <= R; ++d)
+            v = fmaf(s_taps[d], col[(reflect_index(y - d, H) - wy0) * W] +
+                                    col[(reflect_index(y + d, H) - wy0) * W], v);
+          if (pass == 0)
+            lmax = fmaxf(lmax, v);
+          else
+            __builtin_nontemporal_store(v * sc, out + (size_t)y0 * W + i);
+        }
+        __syncthreads();
+      }
+      if (pass == 0) {
+        lmax = wave_max(lmax);
+        if (lane == 0) s_wmax[wid] = lmax;
+        __syncthreads();
+        float m = 0.0f;
+        for (int w = 0; w < kLargeThreads / kWave; ++w) m = fmaxf(m, s_wmax[w]);
+        // a zero maximum leaves the zeros as they are instead of producing 0/0
+        sc = m > 0.0f ? 1.0f / m : 1.0f;
+        __syncthreads();
+      }
+    }
+  }
+}
+
 static int fill_render_args(int mode, const dsx_render_params *p, RenderArgs &a, const char *name) {
   if (p->height < 1 || p->width < 1 || p->height > 32767 || p->width > 32767 ||
       !(p->calibration > 0.0f) || !(p->sigma > 0.0f)) {
@@ -928,6 +1080,91 @@ extern "C" int dsx_render_fast_ragged(const float *spot_d, const long long *spot
 }
 
 extern "C" int dsx_render_dense_max_pixels(void) { return kDenseMaxPix; }
+
+extern "C" long long dsx_render_fast_large_workspace_ints(int cap) {
+  return cap < 1 ? 0 : static_cast<long long>(cu_count()) * cap;
+}
+
+template <int kR>
+static int launch_large_r(const LargeArgs &la, int blocks, size_t lds, hipStream_t st) {
+  static int reserved = -1;  // raised once, outside any graph capture
+  if (reserved < 0) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&render_fast_large_kernel<kR>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kLargeWindowBytes) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("dsx_render_fast_large: cannot reserve %zu bytes of LDS", lds);
+      return DSX_ELAUNCH;
+    }
+    reserved = 1;
+  }
+  hipLaunchKernelGGL(render_fast_large_kernel<kR>, dim3(blocks), dim3(kLargeThreads), lds, st, la);
+  return DSX_OK;
+}
+
+extern "C" int dsx_render_fast_large(const float *spot_d, const int *count_d, int cap, int n,
+                                     const dsx_render_params *params, int *work_d,
+                                     long long work_ints, float *out_d, int *status_d,
+                                     void *stream) {
+  const char *name = "dsx_render_fast_large";
+  clear_error();
+  if (!params || n < 0 || cap < 1 || work_ints < 0) {
+    set_error("%s: bad arguments (n=%d cap=%d)", name, n, cap);
+    return DSX_EINVAL;
+  }
+  if (n == 0) return DSX_OK;
+  if (!spot_d || !count_d || !out_d || !work_d) {
+    set_error("%s: null device pointer", name);
+    return DSX_EINVAL;
+  }
+  LargeArgs la;
+  RenderArgs &a = la.r;
+  const int st = fill_render_args(kModeFast, params, a, name);
+  if (st != DSX_OK) return st;
+  a.spots = reinterpret_cast<const float4 *>(spot_d);
+  a.count = count_d;
+  a.cap = cap;
+  a.n = n;
+  a.out = out_d;
+  a.status = status_d;
+  // window = strip rows + 2R halo rows, two FP32 buffers of the full width
+  const int W = a.W, R = a.R;
+  const int win_rows = kLargeWindowBytes / (2 * 4 * W);
+  const int rows = win_rows - 2 * R;
+  if (rows < 1) {
+    set_error("%s: width %d with tap radius %d leaves no room for a strip", name, W, R);
+    return DSX_EINVAL;
+  }
+  if (R >= a.H || R >= W) {
+    set_error("%s: tap radius %d must be smaller than the pattern", name, R);
+    return DSX_EINVAL;
+  }
+  la.rows = std::min(rows, a.H);
+  la.n_src = (a.H + la.rows - 1) / la.rows;
+  if (la.n_src > kMaxStrips) {
+    set_error("%s: pattern height %d needs %d strips (> %d)", name, a.H, la.n_src, kMaxStrips);
+    return DSX_EINVAL;
+  }
+  const int blocks = std::min(n, cu_count());
+  if (work_ints < static_cast<long long>(blocks) * cap) {
+    set_error("%s: workspace of %lld ints is smaller than %lld", name, work_ints,
+              static_cast<long long>(blocks) * cap);
+    return DSX_EINVAL;
+  }
+  la.bins = work_d;
+  la.per_block = cap;
+  const size_t lds = 2 * static_cast<size_t>(la.rows + 2 * R) * W * sizeof(float);
+  hipStream_t s = as_stream(stream);
+  int rc;
+  switch (R) {
+    case 1: rc = launch_large_r<1>(la, blocks, lds, s); break;
+    case 2: rc = launch_large_r<2>(la, blocks, lds, s); break;
+    case 3: rc = launch_large_r<3>(la, blocks, lds, s); break;
+    case 4: rc = launch_large_r<4>(la, blocks, lds, s); break;
+    default: rc = launch_large_r<0>(la, blocks, lds, s);
+  }
+  if (rc != DSX_OK) return rc;
+  return check_launch(name);
+}
 
 extern "C" int dsx_render_subpixel(const float *spot_d, const int *count_d, int cap, int n,
                                    const dsx_render_params *params, float *out_d, int *status_d,

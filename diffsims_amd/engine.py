@@ -371,6 +371,22 @@ def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigm
             None, _lib.ptr(spots), _lib.ptr(count), cap, n, None, None, 0, None, ctypes.byref(sim),
             ctypes.byref(p), _lib.ADJ_FORWARD, None, 0, _lib.ptr(out), None, None, None,
             _lib.ptr(work), work.numel(), _lib.ptr(status), stream), "dsx_render_adjoint")
+    elif fast and cap > int(L.dsx_render_max_spots()):
+        # more spots than the splat renderer holds: whole pattern in LDS when
+        # it fits, otherwise strip windows with spot bins in a workspace
+        if H * W <= int(L.dsx_render_dense_max_pixels()):
+            lst = torch.arange(n, dtype=torch.int32, device=dev)
+            off = torch.arange(n, dtype=torch.int64, device=dev) * cap
+            capv = torch.full((n,), cap, dtype=torch.int32, device=dev)
+            _lib.check(L.dsx_render_fast_ragged(
+                _lib.ptr(spots), _lib.ptr(off), _lib.ptr(capv), _lib.ptr(count), _lib.ptr(lst), n,
+                ctypes.byref(p), _lib.RENDER_DENSE, 0, _lib.ptr(out), _lib.ptr(status), stream),
+                "dsx_render_fast_ragged")
+        else:
+            work = _workspace(dev, int(L.dsx_render_fast_large_workspace_ints(int(cap))))
+            _lib.check(L.dsx_render_fast_large(
+                _lib.ptr(spots), _lib.ptr(count), cap, n, ctypes.byref(p), _lib.ptr(work),
+                work.numel(), _lib.ptr(out), _lib.ptr(status), stream), "dsx_render_fast_large")
     else:
         fn = L.dsx_render_fast if fast else L.dsx_render_subpixel
         _lib.check(fn(_lib.ptr(spots), _lib.ptr(count), cap, n, ctypes.byref(p), _lib.ptr(out),

@@ -218,6 +218,18 @@ int dsx_render_fast(const float *spot_d, const int *count_d, int cap, int n,
                     void *stream);
 
 /*
+ * fast=True rendering of large patterns with many spots (the same result as
+ * dsx_render_fast, no limit on spots per pattern): each output strip rebuilds
+ * the spot image of its window in LDS and filters it; in-frame spots are
+ * binned by source row strip in work_d, dsx_render_fast_large_workspace_ints(cap)
+ * ints.  The tap radius must be smaller than both pattern dimensions.
+ */
+long long dsx_render_fast_large_workspace_ints(int cap);
+int dsx_render_fast_large(const float *spot_d, const int *count_d, int cap, int n,
+                          const dsx_render_params *params, int *work_d, long long work_ints,
+                          float *out_d, int *status_d, void *stream);
+
+/*
  * fast=False rendering — replaces _subpixel_gaussian
  * (detector_functions.py:314-359): per spot, the clipped box of
  * I/(2 pi s^2) exp(-d^2 / (2 s^2)) (skipped when I/(2 pi s^2) < clip_threshold),

@@ -320,3 +320,25 @@ def test_polar_templates_on_device(axes):
         if near:
             continue
         assert np.array_equal(r[row], r_ref[row]) and np.array_equal(t[row], t_ref[row]), row
+
+
+@pytest.mark.parametrize("shape,cal", [((512, 480), 0.01), ((100, 120), 0.03)])
+def test_render_fast_many_spots(shape, cal):
+    """Fast rendering of patterns with more than 1,024 spots: the windowed
+    strip renderer (512x480) and the whole-pattern dense renderer (100x120),
+    against the oracle (flipped-pixel carve-out as usual)."""
+    e = eng()
+    rng = np.random.default_rng(12)
+    ph = O.PhaseSpec((19.5, 20.5, 21.0), (88.0, 96.0, 91.0), ["O", "Si", "Fe", "Mg"],
+                     rng.uniform(0, 1, (4, 3)), np.ones(4), rng.uniform(0.3, 1.0, 4))
+    st = e.SimSettings(max_excitation_error=0.03, shape_model="lorentzian")
+    tab, b, M, refs, _ = run_parity(ph, st, 1.2, False, 4, 31)
+    cnt = b.count.cpu().numpy()
+    assert b.cap > 1024 and cnt.min() > 1024
+    pats = e.render(b, shape, 1.0, cal).cpu().numpy()
+    sp = b.spots.cpu().numpy()
+    for o, ref in enumerate(refs):
+        if not np.array_equal(b.idx[o, :cnt[o]].cpu().numpy(), ref.idx):
+            continue
+        rp = P.flipped_reference_pattern(ref, sp[o, :cnt[o], :2], shape, 1.0, cal)
+        assert np.abs(pats[o] - rp).max() <= P.TOL_P * rp.max(), o
