@@ -28,6 +28,7 @@ SF_SIN2C = 3
 SF_ATANC = 4
 SF_BINARY = 5
 SF_LORENTZIAN_PRECESSION = 6
+SF_PRECESSION_AVG = 16
 
 SCATTER_NONE = 0
 SCATTER_LOBATO = 1

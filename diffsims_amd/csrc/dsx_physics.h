@@ -23,7 +23,8 @@ inline void fill_physics(Args &a, const dsx_sim_params *p) {
   const double w = p->shape_width;
   a.width = w;
   a.sigma = M_PI / w;
-  a.fac = M_PI * p->minima_number / (p->shape_model == DSX_SF_ATANC ? fabs(w) : w);
+  const bool atanc = p->shape_model == DSX_SF_ATANC || p->shape_model == DSX_SF_PRECESSION_AVG + DSX_SF_ATANC;
+  a.fac = M_PI * p->minima_number / (atanc ? fabs(w) : w);
   a.model = p->shape_model;
   const double phi = p->precession_angle;
   a.precession = phi > 0.0;
@@ -32,8 +33,144 @@ inline void fill_physics(Args &a, const dsx_sim_params *p) {
   a.Pt = k * sin(phi);
 }
 
-// shape_factor_models.py in FP64 (evaluated for candidates only).
+// The models of shape_factor_models.py in plain FP64 (the inner function of
+// the full precession average).
 template <class Args>
+__device__ __forceinline__ double base_model_f64(int m, double x, const Args &a) {
+  switch (m) {
+    case DSX_SF_LINEAR:
+      return fmax(1.0 - fabs(x) / a.width, 0.0);
+    case DSX_SF_LORENTZIAN: {
+      const double t = a.sigma * x;
+      return 1.0 / (1.0 + t * t);
+    }
+    case DSX_SF_SINC:
+    case DSX_SF_SIN2C: {
+      const double y = a.fac * x;
+      if (y == 0.0) return 0.0;  // the reference's where=denom != 0 leaves 0
+      const double v = fabs(sin(y) / y);
+      return m == DSX_SF_SINC ? v : v * v;
+    }
+    case DSX_SF_ATANC: {
+      const double y = a.fac * x;
+      return y == 0.0 ? 1.0 : atan(y) / y;
+    }
+    case DSX_SF_BINARY:
+      return 1.0;
+    default:
+      return 0.0;
+  }
+}
+
+// (1/pi) * integral over [t0, t1] of f(s + A cos t) dt by 8-point Gauss-Legendre.
+template <class Args>
+__device__ __forceinline__ double gl8_piece(int m, double s, double A, double t0, double t1,
+                                            const Args &a) {
+  const double xs[4] = {0.18343464249564980, 0.52553240991632899, 0.79666647741362674,
+                        0.96028985649753623};
+  const double ws[4] = {0.36268378337836198, 0.31370664587788729, 0.22238103445337447,
+                        0.10122853629037626};
+  const double c = 0.5 * (t0 + t1), h = 0.5 * (t1 - t0);
+  double acc = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    acc += ws[i] * (base_model_f64(m, s + A * cos(c - h * xs[i]), a) +
+                    base_model_f64(m, s + A * cos(c + h * xs[i]), a));
+  }
+  return acc * h;
+}
+
+// Full precession average (1/2pi) * integral_0^2pi f(s + A cos t) dt, A = r phi
+// (_shape_factor_precession, shape_factor_models.py:222-268, which integrates
+// with scipy.integrate.quad).  By symmetry it is (1/pi) * integral_0^pi.
+//  * lorentzian: closed form Re[1 / sqrt((1 + i sigma s)^2 + sigma^2 A^2)]
+//    (the average of Re[1 / (1 + i sigma x)] with x = s + A cos t);
+//  * linear: exact, piece by piece between the angles where x = -w, 0, w;
+//  * sinc, sin2c: Gauss-Legendre on each lobe between consecutive zeros of
+//    sin(fac x) (the kinks of |sinc| and the oscillation of both);
+//  * atanc: Gauss-Legendre on 8 equal pieces (smooth integrand);
+//  * binary: 1.
+template <class Args>
+__device__ double precession_average(int m, double s, double A, const Args &a) {
+  if (!(A > 0.0)) return base_model_f64(m, s, a);
+  switch (m) {
+    case DSX_SF_BINARY:
+      return 1.0;
+    case DSX_SF_LORENTZIAN: {
+      const double sg = a.sigma;
+      const double re = 1.0 - sg * sg * s * s + sg * sg * A * A, im = 2.0 * sg * s;
+      const double mod = hypot(re, im);
+      // principal square root w of re + i im without cancellation; Re(1/w) = Re(w) / |z|
+      double wr;
+      if (re >= 0.0) {
+        wr = sqrt(0.5 * (mod + re));
+      } else {
+        const double wi = sqrt(0.5 * (mod - re));
+        wr = fabs(im) / (2.0 * wi);
+      }
+      return wr / mod;
+    }
+    case DSX_SF_LINEAR: {
+      const double w = fabs(a.width);
+      double brk[3];
+      int nb = 0;
+      const double xb[3] = {w, 0.0, -w};  // x decreases as t grows: t ascending
+      for (int i = 0; i < 3; ++i) {
+        const double c = (xb[i] - s) / A;
+        if (c > -1.0 && c < 1.0) brk[nb++] = acos(c);
+      }
+      double total = 0.0, t0 = 0.0;
+      for (int i = 0; i <= nb; ++i) {
+        const double t1 = i < nb ? brk[i] : M_PI;
+        if (t1 > t0) {
+          const double xm = s + A * cos(0.5 * (t0 + t1));
+          if (fabs(xm) < w) {
+            // 1 -+ x / w with x = s + A cos t: c0 + c1 cos t
+            const double sgn = xm >= 0.0 ? -1.0 : 1.0;
+            const double c0 = 1.0 + sgn * s / w, c1 = sgn * A / w;
+            total += c0 * (t1 - t0) + c1 * (sin(t1) - sin(t0));
+          }
+        }
+        t0 = t1;
+      }
+      return total / M_PI;
+    }
+    case DSX_SF_SINC:
+    case DSX_SF_SIN2C: {
+      const double fa = fabs(a.fac);
+      const double kmax = floor((s + A) * fa / M_PI), kmin = ceil((s - A) * fa / M_PI);
+      double total = 0.0, t0 = 0.0;
+      if (kmax - kmin > 8192.0) {  // an extreme excursion: uniform pieces
+        for (int i = 0; i < 4096; ++i) {
+          const double t1 = M_PI * (i + 1) / 4096.0;
+          total += gl8_piece(m, s, A, t0, t1, a);
+          t0 = t1;
+        }
+        return total / M_PI;
+      }
+      for (double k = kmax; k >= kmin; k -= 1.0) {
+        const double c = fmin(1.0, fmax(-1.0, (k * M_PI / fa - s) / A));
+        const double t1 = acos(c);
+        if (t1 > t0) total += gl8_piece(m, s, A, t0, t1, a);
+        t0 = fmax(t0, t1);
+      }
+      if (M_PI > t0) total += gl8_piece(m, s, A, t0, M_PI, a);
+      return total / M_PI;
+    }
+    case DSX_SF_ATANC: {
+      double total = 0.0;
+      for (int i = 0; i < 8; ++i) total += gl8_piece(m, s, A, M_PI * i / 8.0, M_PI * (i + 1) / 8.0, a);
+      return total / M_PI;
+    }
+    default:
+      return 0.0;
+  }
+}
+
+// shape_factor_models.py in FP64 (evaluated for candidates only).  kAvg
+// compiles in the full precession averages (DSX_SF_PRECESSION_AVG + m), kept
+// out of the common kernels for their register pressure.
+template <bool kAvg = false, class Args>
 __device__ __forceinline__ double shape_factor(double s, double r, const Args &a) {
   switch (a.model) {
     case DSX_SF_LINEAR:
@@ -69,6 +206,8 @@ __device__ __forceinline__ double shape_factor(double s, double r, const Args &a
       return (a.sigma * 0.31830988618379067) * sqrt(2.0 * upz) / z;
     }
     default:
+      if (kAvg && a.model >= DSX_SF_PRECESSION_AVG)
+        return precession_average(a.model - DSX_SF_PRECESSION_AVG, s, r * a.phi, a);
       return 0.0;
   }
 }

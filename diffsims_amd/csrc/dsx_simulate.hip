@@ -55,7 +55,7 @@ __host__ __device__ __forceinline__ float pretest_threshold(double me, double k,
 constexpr int kQueue = 64 + kOT * kWave;  // one batch plus one worst-case chunk
 constexpr int kWavesPerBlock = kSimThreads / kWave;
 
-template <bool kRagged>
+template <bool kRagged, bool kAvg>
 __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
   __shared__ int s_queue[kWavesPerBlock][kQueue];
   __shared__ double s_R[kWavesPerBlock][kOT][9];
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
         z = Rd[6] * gx + Rd[7] * gy + Rd[8] * gz;
         double s, r;
         in = intersects(x, y, z, a, s, r);
-        if (in) I = static_cast<float>(shape_factor(s, r, a) * static_cast<double>(F2[row]));
+        if (in) I = static_cast<float>(shape_factor<kAvg>(s, r, a) * static_cast<double>(F2[row]));
       }
 #pragma unroll
       for (int o = 0; o < kOT; ++o) {
@@ -230,8 +230,14 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
 }
 
 static int check_physics(const dsx_sim_params *p, const char *name) {
-  if (p->shape_model < DSX_SF_LINEAR || p->shape_model > DSX_SF_LORENTZIAN_PRECESSION) {
+  const bool averaged = p->shape_model >= DSX_SF_PRECESSION_AVG + DSX_SF_LINEAR &&
+                        p->shape_model <= DSX_SF_PRECESSION_AVG + DSX_SF_BINARY;
+  if (!averaged && (p->shape_model < DSX_SF_LINEAR || p->shape_model > DSX_SF_LORENTZIAN_PRECESSION)) {
     set_error("%s: unknown shape model %d", name, p->shape_model);
+    return DSX_EINVAL;
+  }
+  if (averaged && !(p->precession_angle > 0.0)) {
+    set_error("%s: the full precession average needs a positive precession angle", name);
     return DSX_EINVAL;
   }
   if (!(p->k > 0.0) || !(p->max_excitation_error > 0.0) || p->shape_width == 0.0 ||
@@ -305,8 +311,12 @@ extern "C" int dsx_simulate(const int *hkl_d, const float *g4_d, const float *F2
   d.ncand = ncand_d;
   d.maxI = maxI_d;
   a.n_items = (n_orient + kOT - 1) / kOT;
-  hipLaunchKernelGGL(simulate_kernel<false>, dim3(launch_blocks(a.n_items, 0)), dim3(kSimThreads), 0,
-                     as_stream(stream), a);
+  if (p->shape_model >= DSX_SF_PRECESSION_AVG)
+    hipLaunchKernelGGL((simulate_kernel<false, true>), dim3(launch_blocks(a.n_items, 0)),
+                       dim3(kSimThreads), 0, as_stream(stream), a);
+  else
+    hipLaunchKernelGGL((simulate_kernel<false, false>), dim3(launch_blocks(a.n_items, 0)),
+                       dim3(kSimThreads), 0, as_stream(stream), a);
   return check_launch("dsx_simulate");
 }
 
@@ -332,7 +342,11 @@ extern "C" int dsx_simulate_ragged(const dsx_phase_desc *phases_d, int n_phases,
   a.phases = phases_d;
   a.items = reinterpret_cast<const int2 *>(items_d);
   a.n_items = n_items;
-  hipLaunchKernelGGL(simulate_kernel<true>, dim3(launch_blocks(n_items, blocks_per_cu)),
-                     dim3(kSimThreads), 0, as_stream(stream), a);
+  if (p->shape_model >= DSX_SF_PRECESSION_AVG)
+    hipLaunchKernelGGL((simulate_kernel<true, true>), dim3(launch_blocks(n_items, blocks_per_cu)),
+                       dim3(kSimThreads), 0, as_stream(stream), a);
+  else
+    hipLaunchKernelGGL((simulate_kernel<true, false>), dim3(launch_blocks(n_items, blocks_per_cu)),
+                       dim3(kSimThreads), 0, as_stream(stream), a);
   return check_launch("dsx_simulate_ragged");
 }

@@ -206,11 +206,15 @@ class SimSettings:
         width = self.shape_factor_width if self.shape_factor_width is not None else self.max_excitation_error
         phi = abs(self.precession_angle)
         if phi > 0 and self.shape_model != "binary":
-            # the precession band test applies to every model; with precession
-            # the reference's shape factor is the precession-averaged Lorentzian
-            if not self.approximate_precession:
-                raise NotImplementedError("full precession integration is not implemented on the GPU")
-            model = _lib.SF_LORENTZIAN_PRECESSION
+            # the precession band test applies to every model; the shape factor
+            # is the reference's approximate precession Lorentzian, or with
+            # approximate_precession=False the full average of the model
+            if self.approximate_precession:
+                model = _lib.SF_LORENTZIAN_PRECESSION
+            else:
+                if self.shape_model not in SHAPE_MODEL_IDS:
+                    raise NotImplementedError(f"shape factor model {self.shape_model!r} has no GPU kernel")
+                model = _lib.SF_PRECESSION_AVG + SHAPE_MODEL_IDS[self.shape_model]
         else:
             if self.shape_model not in SHAPE_MODEL_IDS:
                 raise NotImplementedError(f"shape factor model {self.shape_model!r} has no GPU kernel")

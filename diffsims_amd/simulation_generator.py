@@ -81,8 +81,9 @@ class SimulationGenerator:
 
     # -- settings -----------------------------------------------------------------
     def _device_model(self) -> Optional[str]:
-        if self.precession_angle != 0:
-            return "lorentzian_precession" if self.approximate_precession else None
+        if self.precession_angle != 0 and self.approximate_precession:
+            return "lorentzian_precession"
+        # built-in models run on the GPU (with precession: their full average)
         return _DEVICE_MODELS.get(self.shape_factor_model)
 
     def _settings(self, max_excitation_error, shape_factor_width, model=None,
@@ -99,7 +100,7 @@ class SimulationGenerator:
             shape_model=model if model != "lorentzian_precession" else "lorentzian",
             minima_number=minima,
             precession_angle=float(self.precession_angle),
-            approximate_precession=True,
+            approximate_precession=self.approximate_precession or model == "lorentzian_precession",
             minimum_intensity=self.minimum_intensity if minimum_intensity is None else minimum_intensity)
 
     # -- the drop-in entry ----------------------------------------------------------

@@ -37,6 +37,10 @@ extern "C" {
 #define DSX_SF_BINARY 5     /* :33-49                                      */
 /* precession forms (simulation_generator.py:396-411) */
 #define DSX_SF_LORENTZIAN_PRECESSION 6 /* shape_factor_models.py:183-219   */
+/* Full precession average of model m (approximate_precession=False,
+ * _shape_factor_precession, shape_factor_models.py:222-268): DSX_SF_PRECESSION_AVG + m
+ * for m in DSX_SF_LINEAR..DSX_SF_BINARY; needs precession_angle > 0. */
+#define DSX_SF_PRECESSION_AVG 16
 
 /* Scattering-factor parameterisations (sim_utils.py:227-253, :281-286). */
 #define DSX_SCATTER_NONE 0    /* f = 1                                     */

@@ -28,6 +28,7 @@ def _oracle_params(settings, rr, wdb, scattering="lobato"):
                        shape_kwargs=({"minima_number": settings.minima_number}
                                      if settings.shape_model in ("sinc", "sin2c", "atanc") else {}),
                        precession_angle=settings.precession_angle,
+                       approximate_precession=settings.approximate_precession,
                        minimum_intensity=settings.minimum_intensity, with_direct_beam=wdb,
                        scattering_params=scattering)
 
@@ -138,6 +139,18 @@ def test_precession_lorentzian():
                            shape_model="lorentzian", precession_angle=1.0)
     tab, b, M, refs, stats = run_parity(P.triclinic(10, 41), st, 1.8, False, 64, 5)
     assert stats["max_rel_err"] <= P.TOL_I
+
+
+@pytest.mark.parametrize("model", ["linear", "lorentzian", "sinc", "sin2c", "atanc", "binary"])
+def test_full_precession_average(model):
+    """approximate_precession=False: the device average of every built-in
+    model against scipy.integrate.quad per spot in the oracle."""
+    st = eng().SimSettings(accelerating_voltage=200, max_excitation_error=0.02,
+                           shape_factor_width=0.025, shape_model=model, minima_number=4,
+                           precession_angle=1.2, approximate_precession=False)
+    tab, b, M, refs, stats = run_parity(P.triclinic(6, 43), st, 1.4, False, 6, 7)
+    assert stats.get("max_rel_err", 0.0) <= P.TOL_I
+    assert np.mean([r.idx.size for r in refs]) > 20
 
 
 def test_cap_overflow_rerun_gives_the_same_result():

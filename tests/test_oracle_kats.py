@@ -397,3 +397,24 @@ def test_polar_flatten_oracle_nearest_index_rules():
     assert I[1].tolist() == [5, 0, 0]
     rr, tt, _ = O.polar_flatten([(xyz, np.ones(3))])
     assert np.allclose(rr[0], [1, 2, 3]) and np.allclose(tt[0], [0, np.pi / 2, 0])
+
+
+def test_precession_integral_closed_forms():
+    """The converged precession average equals the closed forms the device
+    uses: Re[1/sqrt((1 + i sigma s)^2 + sigma^2 A^2)] for the Lorentzian and
+    the piecewise-exact average of the linear model."""
+    w, phi = 0.02, np.deg2rad(1.0)
+    sg = np.pi / w
+    rng = np.random.default_rng(3)
+    s = rng.uniform(-0.03, 0.03, 12)
+    r = rng.uniform(0.2, 2.5, 12)
+    A = r * phi
+    lor = O.sf_precession_integral(s, r, phi, O.sf_lorentzian, w)
+    closed = np.array([(1 / np.sqrt(complex((1 + 1j * sg * si) ** 2 + (sg * ai) ** 2))).real
+                       for si, ai in zip(s, A)])
+    assert np.allclose(lor, closed, rtol=1e-9, atol=0)
+    lin = O.sf_precession_integral(s, r, phi, O.sf_linear, w)
+    t = np.linspace(0, np.pi, 200001)
+    brute = [np.trapezoid(np.maximum(1 - np.abs(si + ai * np.cos(t)) / w, 0), t) / np.pi
+             for si, ai in zip(s, A)]
+    assert np.allclose(lin, brute, rtol=1e-7, atol=1e-12)
