@@ -25,13 +25,15 @@
 //      1/max with non-temporal 16-byte stores; strips without spots are
 //      written as zeros straight from registers.
 //
-// Accumulation is 64-bit fixed point (ds_add_u64): LDS float atomics run ~20x
-// slower than integer ones on gfx950.  A contribution I Kx Ky never exceeds the
-// pattern's largest spot weight P = max|I| (each separable kernel is at most 1;
-// the sub-pixel Gaussian at most I/(2 pi s^2)), so it is rounded to an int32 in
-// units of P / 2^30 and summed into int64 words that cannot overflow.  The
-// pattern maximum is at least P w0^2 (w0 the central tap), so one unit is at
-// most 2^-30 / w0^2 of it: 6e-9 at sigma = 1, 6e-7 at sigma = 10.
+// Accumulation is 32-bit fixed point (ds_add_u32): LDS float atomics run ~20x
+// slower than integer ones on gfx950, and 32-bit words let a strip hold twice
+// the rows of 64-bit ones (half the barriers per pattern).  Before splatting,
+// every spot adds the largest value its footprint can take anywhere (I times
+// the largest tap along each axis; I / (2 pi s^2) for the sub-pixel Gaussian)
+// to each 16x16 tile it touches; the largest tile sum U bounds every pixel, so
+// with one unit = U / (1.9 * 2^30) no pixel sum can overflow.  For the usual
+// sparse pattern U is a few times the pattern maximum, so one unit is ~1e-9 of
+// it.
 #include <algorithm>
 
 #include "dsx_internal.h"
@@ -59,6 +61,9 @@ constexpr int kTile = 16;
 constexpr int kMaxTiles = 1024;   // up to 512x512 patterns
 constexpr int kMaxCand = 16;      // candidate tiles evaluated exactly
 constexpr int kCandSpots = 64;    // spots touching one candidate tile
+constexpr float kScaleRange = 2040109465.6f;  // 1.9 * 2^30: fixed-point units per tile bound
+
+typedef int racc_t;  // strip accumulator word
 constexpr int kStripBytes = DSX_STRIP_KB * 1024;
 
 enum { kModeFast = 0, kModeSubpixel = 1 };
@@ -79,7 +84,7 @@ struct RenderArgs {
   float clip;
   int strip_rows;
   int n_strips;
-  int lds_words;   // size of the dynamic LDS buffer in acc_t words
+  int lds_words;   // size of the dynamic LDS buffer in racc_t words
   float *out;
   int *status;
   float taps[kMaxR + 1];
@@ -132,7 +137,8 @@ struct RenderShared {
   int overflow;
   int cols_max;
   float wsum[kRenderThreads / kWave];
-  acc_t wmax[kRenderThreads / kWave];
+  int wu[kRenderThreads / kWave];
+  racc_t wmax[kRenderThreads / kWave];
   // bounded maximum search (fast mode); its per-tile arrays borrow the
   // (zeroed) strip buffer so the block keeps two-blocks-per-CU occupancy
   int cand_tile[kMaxCand];
@@ -140,7 +146,7 @@ struct RenderShared {
   int n_cand;
   int fail;
   int qstar;
-  acc_t lfix;
+  long long lfix;
 };
 
 // Fast-mode weight of a source pixel c at output index y along one axis, the
@@ -153,55 +159,107 @@ __device__ __forceinline__ float fast_weight(int y, int c, int n, int R, bool sh
 
 // Fixed-point contribution of a survivor (I at pixel xi, yi; rows lo..hi) to
 // pixel (x, y), rounded exactly as the splat rounds it; 0 outside its footprint.
-__device__ __forceinline__ long long fast_term(float I, int xi, int yi, int lo, int hi, int x, int y,
-                                               int H, int W, int R, bool short_taps,
-                                               const float *taps, float scale) {
+__device__ __forceinline__ int fast_term(float I, int xi, int yi, int lo, int hi, int x, int y,
+                                         int H, int W, int R, bool short_taps, const float *taps,
+                                         float scale) {
   if (x < xi - R || x > xi + R || y < lo || y > hi) return 0;
   const float iw = I * fast_weight(x, xi, W, R, short_taps, taps);
   const float v = iw * fast_weight(y, yi, H, R, short_taps, taps);
   return __float2int_rn(v * scale);
 }
 
-// Exact pattern maximum (fixed point) without rendering the whole pattern:
-// every contribution is at most I_j bx_j by_j (the largest tap along each
-// axis), so a 16x16 tile whose summed bounds stay below the value already
-// known at the brightest spot's pixel cannot hold the maximum.  Only the few
-// remaining tiles are evaluated, pixel by pixel, with the splat's exact
-// fixed-point terms, so the result equals the full pass bit for bit.
-// Returns false (uniformly) when too many tiles remain; the caller then runs
-// the full pass.
+__device__ __forceinline__ void racc_add(racc_t *p, float v, float scale) {
+  atomicAdd(p, __float2int_rn(v * scale));
+}
+
+// Scratch words the tile bounds and the maximum search borrow from the strip
+// buffer: int tile_u[n_tiles] | short cand_q[kMaxCand][kCandSpots].
+__device__ __forceinline__ int bound_words(int n_tiles) {
+  return n_tiles + (kMaxCand * kCandSpots * 2 + 3) / 4;
+}
+
+// Upper bound U of every pixel of the pattern: per 16x16 tile, the sum over
+// the spots touching it of the largest value each can take there, counted in
+// integer units of P / 2^20 (P = the largest spot weight, rounded up) so the
+// bound -- and with it the fixed-point scale -- does not depend on the order
+// of the atomic adds.  Leaves the per-tile sums in scratch (for the maximum
+// search) when they fit; otherwise U is the sum over all spots.  Also notes
+// the brightest spot (fast mode).  Returns U in units of P / 2^20.
+constexpr float kBoundUnits = 1048576.0f;  // 2^20
+
 template <int kMode>
-__device__ bool bounded_max(RenderShared<kMode> &sh, acc_t *scratch, int scratch_words, int ns,
-                            int H, int W, int R, bool short_taps, float scale, float pmax, int tid,
-                            int lane, int wid, acc_t &m_out) {
-  const int ntx = (W + kTile - 1) / kTile, nty = (H + kTile - 1) / kTile;
-  const int n_tiles = ntx * nty;
-  // scratch layout: float tile_u[n_tiles] | short cand_q[kMaxCand][kCandSpots]
-  const int tile_words = (n_tiles + 1) / 2;
-  const int used_words = tile_words + (kMaxCand * kCandSpots * 2 + 7) / 8;
-  if (n_tiles > kMaxTiles || used_words > scratch_words) return false;
-  float *tile_u = reinterpret_cast<float *>(scratch);
-  short(*cand_q)[kCandSpots] = reinterpret_cast<short(*)[kCandSpots]>(scratch + tile_words);
+__device__ int tile_bounds(RenderShared<kMode> &sh, int *tile_u, bool tiles, int ntx, int n_tiles,
+                           int ns, int H, int W, int R, bool short_taps, float pref, float pmax,
+                           int tid, int lane, int wid) {
+  if (tiles)
+    for (int t = tid; t < n_tiles; t += kRenderThreads) tile_u[t] = 0;
+  if (tid == 0) sh.qstar = 0;
+  lds_sync();
+  const float w0 = sh.taps[0];
+  const float inv_p = pmax > 0.0f ? kBoundUnits / pmax : 0.0f;
+  int usum = 0;
+  for (int q = tid; q < ns; q += kRenderThreads) {
+    const float I = sh.I[q];
+    int tx0, tx1;
+    float uf;
+    if (kMode == kModeFast) {
+      if (fabsf(I) == pmax) sh.qstar = q;
+      const int yi = sh.xy[q] >> 16, xi = sh.xy[q] & 0xffff;
+      const float bx = !short_taps ? 1.0f : ((xi >= R && xi + R < W) ? w0 : fminf(1.0f, 3.0f * w0));
+      const float by = !short_taps ? 1.0f : ((yi >= R && yi + R < H) ? w0 : fminf(1.0f, 3.0f * w0));
+      uf = fabsf(I) * bx * by;
+      tx0 = max(0, xi - R) / kTile;
+      tx1 = min(W - 1, xi + R) / kTile;
+    } else {
+      uf = fabsf(I) * pref;
+      tx0 = (sh.xy[q] >> 16) / kTile;
+      tx1 = (sh.xy[q] & 0xffff) / kTile;
+    }
+    // uf <= P: at most 2^20 units, rounded up (+1 covers the float product)
+    const int u = min(static_cast<int>(ceilf(uf * inv_p)) + 1, static_cast<int>(kBoundUnits) + 1);
+    usum += u;
+    if (tiles) {
+      const int ty0 = sh.ylo[q] / kTile, ty1 = sh.yhi[q] / kTile;
+      for (int ty = ty0; ty <= ty1; ++ty)
+        for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&tile_u[ty * ntx + tx], u);
+    }
+  }
+  if (!tiles) {
+    usum = wave_sum(usum);
+    if (lane == 0) sh.wu[wid] = usum;
+    lds_sync();
+    int U = 0;
+    for (int w = 0; w < kRenderThreads / kWave; ++w) U += sh.wu[w];
+    lds_sync();
+    return U;
+  }
+  lds_sync();
+  int lu = 0;
+  for (int t = tid; t < n_tiles; t += kRenderThreads) lu = max(lu, tile_u[t]);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) lu = max(lu, __shfl_xor(lu, off, kWave));
+  if (lane == 0) sh.wu[wid] = lu;
+  lds_sync();
+  int U = 0;
+  for (int w = 0; w < kRenderThreads / kWave; ++w) U = max(U, sh.wu[w]);
+  lds_sync();
+  return U;
+}
+
+// Exact pattern maximum (fixed point) without rendering the whole pattern:
+// a tile whose bound stays below the value already known at the brightest
+// spot's pixel cannot hold the maximum.  Only the few remaining tiles are
+// evaluated, pixel by pixel, with the splat's exact fixed-point terms, so the
+// result equals a full pass bit for bit.  Returns false (uniformly) when too
+// many tiles remain; the caller then runs the full pass.
+template <int kMode>
+__device__ bool bounded_max(RenderShared<kMode> &sh, int *tile_u, short (*cand_q)[kCandSpots],
+                            int ntx, int n_tiles, int ns, int H, int W, int R, bool short_taps,
+                            float scale, float unit, int tid, int lane, int wid, racc_t &m_out) {
   if (tid == 0) {
     sh.n_cand = 0;
     sh.fail = 0;
-    sh.qstar = 0;
   }
-  lds_sync();
-  const float w0 = sh.taps[0];
-  for (int q = tid; q < ns; q += kRenderThreads) {
-    const float I = sh.I[q];
-    if (fabsf(I) == pmax) sh.qstar = q;
-    const int yi = sh.xy[q] >> 16, xi = sh.xy[q] & 0xffff;
-    const float bx = !short_taps ? 1.0f : ((xi >= R && xi + R < W) ? w0 : fminf(1.0f, 3.0f * w0));
-    const float by = !short_taps ? 1.0f : ((yi >= R && yi + R < H) ? w0 : fminf(1.0f, 3.0f * w0));
-    const float u = fabsf(I) * bx * by;
-    const int tx0 = max(0, xi - R) / kTile, tx1 = min(W - 1, xi + R) / kTile;
-    const int ty0 = sh.ylo[q] / kTile, ty1 = sh.yhi[q] / kTile;
-    for (int ty = ty0; ty <= ty1; ++ty)
-      for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&tile_u[ty * ntx + tx], u);
-  }
-  lds_sync();
   // lower bound: the exact value at the brightest survivor's pixel (wave 0)
   if (wid == 0) {
     const int qs = sh.qstar;
@@ -215,13 +273,13 @@ __device__ bool bounded_max(RenderShared<kMode> &sh, acc_t *scratch, int scratch
     if (lane == 0) sh.lfix = acc;
   }
   lds_sync();
-  const acc_t lfix = sh.lfix;
+  const long long lfix = sh.lfix;
   // candidate tiles: bound (with a rounding margin per term) >= lower bound;
   // tile_u is reused to hold each tile's candidate slot (-1: pruned)
-  int *slot_of = reinterpret_cast<int *>(tile_u);
+  int *slot_of = tile_u;
   const float margin = 0.5f * static_cast<float>(ns) + 1.0f;
   for (int t = tid; t < n_tiles; t += kRenderThreads) {
-    const float bound = tile_u[t] * scale * 1.00001f + margin;
+    const float bound = static_cast<float>(tile_u[t]) * unit * scale * 1.00001f + margin;
     int slot = -1;
     if (bound >= static_cast<float>(lfix)) {
       slot = atomicAdd(&sh.n_cand, 1);
@@ -236,11 +294,7 @@ __device__ bool bounded_max(RenderShared<kMode> &sh, acc_t *scratch, int scratch
     slot_of[t] = slot;
   }
   lds_sync();
-  if (sh.fail) {
-    for (int i = tid; i < used_words; i += kRenderThreads) scratch[i] = 0;
-    lds_sync();
-    return false;
-  }
+  if (sh.fail) return false;
   const int n_cand = sh.n_cand;
   for (int q = tid; q < ns; q += kRenderThreads) {
     const int yi = sh.xy[q] >> 16, xi = sh.xy[q] & 0xffff;
@@ -258,12 +312,8 @@ __device__ bool bounded_max(RenderShared<kMode> &sh, acc_t *scratch, int scratch
       }
   }
   lds_sync();
-  if (sh.fail) {
-    for (int i = tid; i < used_words; i += kRenderThreads) scratch[i] = 0;
-    lds_sync();
-    return false;
-  }
-  acc_t lmax = lfix;
+  if (sh.fail) return false;
+  long long lmax = lfix;
   for (int i = tid; i < n_cand * kTile * kTile; i += kRenderThreads) {
     const int sl = i / (kTile * kTile);
     const int t = sh.cand_tile[sl];
@@ -278,15 +328,16 @@ __device__ bool bounded_max(RenderShared<kMode> &sh, acc_t *scratch, int scratch
     }
     lmax = acc > lmax ? acc : lmax;
   }
-  lmax = wave_max_acc(lmax);
-  if (lane == 0) sh.wmax[wid] = lmax;
+  // the bound keeps every pixel sum below 2^31
+  racc_t lm = static_cast<racc_t>(lmax);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) lm = max(lm, __shfl_xor(lm, off, kWave));
+  if (lane == 0) sh.wmax[wid] = lm;
   lds_sync();
-  acc_t m = 0;
-  for (int w = 0; w < kRenderThreads / kWave; ++w) m = sh.wmax[w] > m ? sh.wmax[w] : m;
+  racc_t m = 0;
+  for (int w = 0; w < kRenderThreads / kWave; ++w) m = max(m, sh.wmax[w]);
   m_out = m;
-  // hand the strip buffer back zeroed
-  for (int i = tid; i < used_words; i += kRenderThreads) scratch[i] = 0;
-  lds_sync();  // wmax, the candidate state and the strip buffer are reused
+  lds_sync();  // wmax and the candidate state are reused
   return true;
 }
 
@@ -296,7 +347,7 @@ template <int kMode, int kR>
 __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
   // 16-byte aligned: the read-outs use 16-byte LDS accesses, which run at a
   // fraction of their rate on an 8-byte-aligned base
-  extern __shared__ __align__(16) acc_t s_acc[];  // strip accumulator; hash table before the passes
+  extern __shared__ __align__(16) racc_t s_acc[];  // strip accumulator; hash table before the passes
   __shared__ RenderShared<kMode> sh;
   int *h_key = reinterpret_cast<int *>(s_acc);
   int *h_val = h_key + kHash;
@@ -408,7 +459,7 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
     lds_sync();
     if (kMode == kModeFast) {
       // the hash lived in the strip buffer: clear it for the splats
-      for (int i = tid; i < min(kHash, a.lds_words); i += kRenderThreads) s_acc[i] = 0;
+      for (int i = tid; i < min(2 * kHash, a.lds_words); i += kRenderThreads) s_acc[i] = 0;
     }
     const int ns = min(sh.ns, kMaxSpots);
     if (tid == 0 && a.status) a.status[p] = sh.overflow;
@@ -428,8 +479,6 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
     }
     float pmax = 0.0f;
     for (int w = 0; w < kRenderThreads / kWave; ++w) pmax = fmaxf(pmax, sh.wsum[w]);
-    // units of P / 2^30, with a margin for the rounding of the scaled weights
-    const float scale = pmax > 0.0f ? kFixedOne / (pmax * 1.001f) : 1.0f;
 
     // ---- 2. strip bins ---------------------------------------------------
     if (tid == 0) {
@@ -458,17 +507,33 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
     const int ncols = sh.cols_max;
     const float inv_ncols = 1.0f / static_cast<float>(ncols);
 
+    // ---- 3. fixed-point scale from the tile bounds; maximum (fast mode) -----
+    const int ntx = (W + kTile - 1) / kTile;
+    const int n_tiles = ntx * ((H + kTile - 1) / kTile);
+    const bool tiles = n_tiles <= kMaxTiles && bound_words(n_tiles) <= a.lds_words;
+    int *tile_u = s_acc;
+    short(*cand_q)[kCandSpots] = reinterpret_cast<short(*)[kCandSpots]>(s_acc + n_tiles);
+    const int Ui = tile_bounds(sh, tile_u, tiles, ntx, n_tiles, ns, H, W, R, short_taps, a.pref,
+                               pmax, tid, lane, wid);
+    const float unit = pmax / kBoundUnits;  // value of one bound unit
+    const float U = static_cast<float>(Ui) * unit;
+    const float scale = U > 0.0f ? kScaleRange / (U * 1.00001f) : 1.0f;
     float out_scale = 1.0f / scale;  // fixed point -> value (normalize: -> value/max)
     int first_pass = a.normalize ? 0 : 1;
-    if (kMode == kModeFast && a.normalize) {
-      acc_t m;
-      if (bounded_max(sh, s_acc, a.lds_words, ns, H, W, R, short_taps, scale, pmax, tid, lane, wid, m)) {
-        out_scale = m > 0 ? 1.0f / acc_to_float(m) : 1.0f / scale;
+    if (kMode == kModeFast && a.normalize && tiles) {
+      racc_t m;
+      if (bounded_max(sh, tile_u, cand_q, ntx, n_tiles, ns, H, W, R, short_taps, scale, unit, tid,
+                      lane, wid, m)) {
+        out_scale = m > 0 ? 1.0f / static_cast<float>(m) : 1.0f / scale;
         first_pass = 1;
       }
     }
+    if (tiles) {  // hand the borrowed words back zeroed
+      for (int i = tid; i < bound_words(n_tiles); i += kRenderThreads) s_acc[i] = 0;
+      lds_sync();
+    }
     for (int pass = first_pass; pass < 2; ++pass) {
-      acc_t lmax = 0;
+      racc_t lmax = 0;
       for (int s = 0; s < a.n_strips; ++s) {
         const int y0 = s * a.strip_rows;
         const int y1 = min(H, y0 + a.strip_rows);
@@ -519,20 +584,20 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
             else
               wx = reflect_weight3(x, xi, W, R, sh.taps);
             const float iw = I * wx;
-            acc_t *col = s_acc + x;
+            racc_t *col = s_acc + x;
             if (short_taps && yi >= R && yi + R < H) {
               // the tap index is uniform across the wave: taps come from
               // scalar registers, not from a dependent LDS load per row
 #pragma unroll
               for (int dd = -R; dd <= R; ++dd) {
                 const int y = yi + dd;
-                if (y >= ya && y <= yb) acc_add(col + (y - y0) * W, iw * a.taps[dd < 0 ? -dd : dd], scale);
+                if (y >= ya && y <= yb) racc_add(col + (y - y0) * W, iw * a.taps[dd < 0 ? -dd : dd], scale);
               }
             } else {
               for (int y = ya; y <= yb; ++y) {
                 const float wy = short_taps ? reflect_weight3(y, yi, H, R, sh.taps)
                                             : reflect_weight(y, yi, H, R, sh.taps);
-                acc_add(col + (y - y0) * W, iw * wy, scale);
+                racc_add(col + (y - y0) * W, iw * wy, scale);
               }
             }
           } else {
@@ -542,61 +607,59 @@ __global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
             const float dx = static_cast<float>(x) - sh.fx[q];
             const float amp = I * a.pref;
             const float cy = sh.fy[q];
-            acc_t *col = s_acc + x;
+            racc_t *col = s_acc + x;
             for (int y = ya; y <= yb; ++y) {
               const float dy = static_cast<float>(y) - cy;
-              acc_add(col + (y - y0) * W, amp * expf(a.ep * (dy * dy + dx * dx)), scale);
+              racc_add(col + (y - y0) * W, amp * expf(a.ep * (dy * dy + dx * dx)), scale);
             }
           }
         }
         lds_sync();
-        // read out and clear the strip
+        // read out and clear the strip (16-byte LDS accesses: 4 words)
         if (pass == 0) {
-          // cnt is even whenever W is (vec4); pairs are 16-byte LDS accesses
-          if ((cnt & 1) == 0) {
+          if ((cnt & 3) == 0) {
 #pragma unroll 4
-            for (int i = tid; i < cnt / 2; i += kRenderThreads) {
-              longlong2 *pv = reinterpret_cast<longlong2 *>(s_acc) + i;
-              const longlong2 v = *pv;
-              *pv = make_longlong2(0, 0);
-              const acc_t m = v.x > v.y ? v.x : v.y;
-              lmax = m > lmax ? m : lmax;
+            for (int i = tid; i < cnt / 4; i += kRenderThreads) {
+              int4 *pv = reinterpret_cast<int4 *>(s_acc) + i;
+              const int4 v = *pv;
+              *pv = make_int4(0, 0, 0, 0);
+              lmax = max(lmax, max(max(v.x, v.y), max(v.z, v.w)));
             }
           } else {
             for (int i = tid; i < cnt; i += kRenderThreads) {
-              const acc_t v = s_acc[i];
+              const racc_t v = s_acc[i];
               s_acc[i] = 0;
-              lmax = v > lmax ? v : lmax;
+              lmax = max(lmax, v);
             }
           }
         } else if (vec4) {
 #pragma unroll 2
           for (int i = tid; i < cnt / 4; i += kRenderThreads) {
-            longlong2 *pv = reinterpret_cast<longlong2 *>(s_acc) + 2 * i;
-            const longlong2 u = pv[0], v = pv[1];
-            pv[0] = make_longlong2(0, 0);
-            pv[1] = make_longlong2(0, 0);
-            const float4 o = make_float4(acc_to_float(u.x) * out_scale, acc_to_float(u.y) * out_scale,
-                                         acc_to_float(v.x) * out_scale, acc_to_float(v.y) * out_scale);
+            int4 *pv = reinterpret_cast<int4 *>(s_acc) + i;
+            const int4 u = *pv;
+            *pv = make_int4(0, 0, 0, 0);
+            const float4 o = make_float4(static_cast<float>(u.x) * out_scale, static_cast<float>(u.y) * out_scale,
+                                         static_cast<float>(u.z) * out_scale, static_cast<float>(u.w) * out_scale);
             store_nt(o, reinterpret_cast<float4 *>(dst) + i);
           }
         } else {
           for (int i = tid; i < cnt; i += kRenderThreads) {
-            __builtin_nontemporal_store(acc_to_float(s_acc[i]) * out_scale, dst + i);
+            __builtin_nontemporal_store(static_cast<float>(s_acc[i]) * out_scale, dst + i);
             s_acc[i] = 0;
           }
         }
         lds_sync();
       }
       if (pass == 0) {
-        lmax = wave_max_acc(lmax);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) lmax = max(lmax, __shfl_xor(lmax, off, kWave));
         if (lane == 0) sh.wmax[wid] = lmax;
         lds_sync();
-        acc_t m = 0;
-        for (int w = 0; w < kRenderThreads / kWave; ++w) m = sh.wmax[w] > m ? sh.wmax[w] : m;
+        racc_t m = 0;
+        for (int w = 0; w < kRenderThreads / kWave; ++w) m = max(m, sh.wmax[w]);
         // a zero maximum (every spot has zero intensity) leaves the zeros as
         // they are instead of producing 0/0
-        out_scale = m > 0 ? 1.0f / acc_to_float(m) : 1.0f / scale;
+        out_scale = m > 0 ? 1.0f / static_cast<float>(m) : 1.0f / scale;
         lds_sync();
       }
     }
@@ -916,7 +979,7 @@ static int fill_render_args(int mode, const dsx_render_params *p, RenderArgs &a,
   a.ep = static_cast<float>(-1.0 / (2.0 * sg * sg));
   a.clip = p->clip_threshold;
   for (int i = 0; i <= kMaxR; ++i) a.taps[i] = (mode == kModeFast && i <= p->radius) ? p->taps[i] : 0.0f;
-  const int budget = kStripBytes / static_cast<int>(sizeof(acc_t));
+  const int budget = kStripBytes / static_cast<int>(sizeof(racc_t));
   if (a.W > budget) {
     set_error("%s: pattern width %d exceeds the LDS strip budget", name, a.W);
     return DSX_EINVAL;
@@ -929,15 +992,15 @@ static int fill_render_args(int mode, const dsx_render_params *p, RenderArgs &a,
   }
   a.strip_rows = rows;
   a.n_strips = n_strips;
-  const size_t lds = std::max(static_cast<size_t>(rows) * a.W * sizeof(acc_t),
+  const size_t lds = std::max(static_cast<size_t>(rows) * a.W * sizeof(racc_t),
                               static_cast<size_t>(2 * kHash) * sizeof(int));
-  a.lds_words = static_cast<int>(lds / sizeof(acc_t));
+  a.lds_words = static_cast<int>(lds / sizeof(racc_t));
   return DSX_OK;
 }
 
 static int launch_splat(int mode, const RenderArgs &a, int blocks_per_cu, hipStream_t st,
                         const char *name) {
-  const size_t lds = static_cast<size_t>(a.lds_words) * sizeof(acc_t);
+  const size_t lds = static_cast<size_t>(a.lds_words) * sizeof(racc_t);
   const int bpc = blocks_per_cu > 0 ? blocks_per_cu : DSX_RENDER_BLOCKS_PER_CU;
   const int blocks = std::min(a.n, cu_count() * bpc);
   if (mode == kModeSubpixel) {

@@ -235,8 +235,9 @@ def test_large_pattern_subpixel_render_and_gradients():
 
 
 def test_forward_mode_equals_subpixel_renderer():
-    """dsx_render_adjoint's forward mode and dsx_render_subpixel give the same
-    bits on patterns both can hold."""
+    """dsx_render_adjoint's forward mode (64-bit fixed point) and
+    dsx_render_subpixel (32-bit fixed point) agree to 1e-6 of the pattern
+    maximum on patterns both can hold."""
     import ctypes
 
     from diffsims_amd import _lib, engine
@@ -259,7 +260,7 @@ def test_forward_mode_equals_subpixel_renderer():
         None, None, None, None, 0, _lib.ptr(status), _lib.stream_handle()), "forward")
     torch.cuda.synchronize()
     assert int(status.max()) == 0
-    assert torch.equal(out, ref)
+    assert float((out - ref).abs().max()) <= 1e-6 * float(ref.abs().max())
 
 
 def test_broadcast_target():
