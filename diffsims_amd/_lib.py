@@ -64,6 +64,8 @@ EXPORTED = (
     "dsx_polar_templates",
     "dsx_render_fast_large_workspace_ints",
     "dsx_render_fast_large",
+    "dsx_euler_to_rt",
+    "dsx_euler_to_rt_backward",
 )
 
 
@@ -147,6 +149,8 @@ _SIGS = {
     "dsx_simulate_ragged": (_I, [_P, _I, _P, _I, ctypes.POINTER(SimParams), _I, _P]),
     "dsx_render_dense_max_pixels": (_I, []),
     "dsx_render_fast_large_workspace_ints": (ctypes.c_longlong, [_I]),
+    "dsx_euler_to_rt": (_I, [_P, _I, _P, _P]),
+    "dsx_euler_to_rt_backward": (_I, [_P, _P, _I, _P, _P]),
     "dsx_render_fast_large": (_I, [_P, _P, _I, _I, ctypes.POINTER(RenderParams), _P,
                                    ctypes.c_longlong, _P, _P, _P]),
     "dsx_polar_templates": (_I, [_P, _P, _I, _I, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P]),

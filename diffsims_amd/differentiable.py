@@ -48,8 +48,8 @@ from .crystallography import (PhaseArrays, element_of, highest_hkl, lattice_setu
                               reciprocal_metric)
 from .scattering import table
 
-__all__ = ["bunge_matrices", "rt_from_matrices", "reciprocal_basis", "RefinementModel",
-           "leaf_parameters", "forward_args"]
+__all__ = ["bunge_matrices", "rt_from_matrices", "euler_to_rt", "reciprocal_basis",
+           "RefinementModel", "leaf_parameters", "forward_args"]
 
 _EXACT_COS = {0.0: 1.0, 60.0: 0.5, 90.0: 0.0, 120.0: -0.5, 180.0: -1.0,
               240.0: -0.5, 270.0: 0.0, 300.0: 0.5}
@@ -79,6 +79,38 @@ def rt_from_matrices(M: torch.Tensor) -> torch.Tensor:
     """(n, 3, 3) orientation matrices -> contiguous [n, 9] rows of M^T, the
     layout the kernels read (g_lab = M^T g)."""
     return M.transpose(-1, -2).reshape(-1, 9).contiguous()
+
+
+class _EulerToRt(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, euler):
+        e = euler.detach().contiguous()
+        rt = torch.empty((e.shape[0], 9), dtype=torch.float64, device=e.device)
+        L = _lib.load(require_gpu=True)
+        _lib.check(L.dsx_euler_to_rt(_lib.ptr(e), int(e.shape[0]), _lib.ptr(rt),
+                                     _lib.stream_handle(e.device)), "dsx_euler_to_rt")
+        ctx.save_for_backward(e)
+        return rt
+
+    @staticmethod
+    def backward(ctx, g):
+        (e,) = ctx.saved_tensors
+        g = g.contiguous().to(torch.float64)
+        de = torch.empty_like(e)
+        L = _lib.load()
+        _lib.check(L.dsx_euler_to_rt_backward(_lib.ptr(e), _lib.ptr(g), int(e.shape[0]),
+                                              _lib.ptr(de), _lib.stream_handle(e.device)),
+                   "dsx_euler_to_rt_backward")
+        return de
+
+
+def euler_to_rt(euler: torch.Tensor) -> torch.Tensor:
+    """Bunge angles [n, 3] (radians, float64, on the device) -> Rt [n, 9] in one
+    kernel, with a hand-written backward; equals
+    ``rt_from_matrices(bunge_matrices(euler))``."""
+    if euler.dtype != torch.float64 or euler.dim() != 2 or euler.shape[1] != 3 or not euler.is_cuda:
+        raise ValueError("euler must be a float64 [n, 3] device tensor")
+    return _EulerToRt.apply(euler)
 
 
 def _cosd(x: torch.Tensor) -> torch.Tensor:
@@ -130,7 +162,7 @@ def leaf_parameters(phase: PhaseArrays, euler: np.ndarray, device=None,
 
 def forward_args(leaves: dict):
     """(Rt, rec, xyz, B) for RefinementModel from :func:`leaf_parameters`."""
-    return (rt_from_matrices(bunge_matrices(leaves["euler"])),
+    return (euler_to_rt(leaves["euler"]),
             reciprocal_basis(leaves["abc"], leaves["angles"]), leaves["xyz"], leaves["B"])
 
 
@@ -341,13 +373,33 @@ def _grads_like(ctx_inputs, dRt, drec, dxyz, dB):
     return out
 
 
+def _grow_if_needed(model: "RefinementModel", batch: engine.SpotBatch, status: torch.Tensor) -> bool:
+    """One synchronisation per step: True when a spot list overflowed its
+    capacity (the caller re-runs with the grown capacity); raises when a
+    pattern could not be rendered."""
+    if batch.n == 0:
+        return False
+    flags = torch.stack([batch.ncand.max(), status.max()]).cpu()
+    if int(flags[1]) != 0:
+        engine.check_render_status(status)
+    if int(flags[0]) > batch.cap:
+        model.cap = int(flags[0]) + 64
+        return True
+    return False
+
+
 class _L2Loss(torch.autograd.Function):
     @staticmethod
     def forward(ctx, Rt, rec, xyz, B, model: RefinementModel, target):
         want = any(ctx.needs_input_grad[:4])
         t = model.tables(rec, xyz, B)
-        batch = model.spots(t, Rt)
-        loss, dRt, _, dxyz, dB, drec = model.adjoint(t, Rt.detach(), batch, _lib.ADJ_L2, target, want)
+        while True:
+            batch = model.spots(t, Rt, check=False)
+            status = torch.zeros(batch.n, dtype=torch.int32, device=model.dev)
+            loss, dRt, _, dxyz, dB, drec = model.adjoint(t, Rt.detach(), batch, _lib.ADJ_L2, target,
+                                                         want, status=status)
+            if not _grow_if_needed(model, batch, status):
+                break
         if want:
             g = _grads_like((Rt, rec, xyz, B), dRt, drec, dxyz, dB)
             ctx.save_for_backward(*g)
@@ -364,10 +416,13 @@ class _Patterns(torch.autograd.Function):
     @staticmethod
     def forward(ctx, Rt, rec, xyz, B, model: RefinementModel):
         t = model.tables(rec, xyz, B)
-        batch = model.spots(t, Rt)
-        H, W = model.shape
-        out = engine.render_spots(batch.spots, batch.count, batch.cap, model.shape, model.sigma,
-                                  model.calibration, **model.render_kw)
+        while True:
+            batch = model.spots(t, Rt, check=False)
+            status = torch.zeros(batch.n, dtype=torch.int32, device=model.dev)
+            out = engine.render_spots(batch.spots, batch.count, batch.cap, model.shape, model.sigma,
+                                      model.calibration, status=status, **model.render_kw)
+            if not _grow_if_needed(model, batch, status):
+                break
         ctx.model = model
         ctx.save_for_backward(Rt.detach(), rec.detach(), xyz.detach(), B.detach())
         return out
@@ -378,8 +433,13 @@ class _Patterns(torch.autograd.Function):
         Rt, rec, xyz, B = ctx.saved_tensors
         # recompute the tables and spot lists instead of keeping them alive
         t = model.tables(rec, xyz, B)
-        batch = model.spots(t, Rt)
-        _, dRt, _, dxyz, dB, drec = model.adjoint(t, Rt, batch, _lib.ADJ_PATTERN_GRAD,
-                                                  gP.contiguous().to(torch.float32), True)
+        while True:
+            batch = model.spots(t, Rt, check=False)
+            status = torch.zeros(batch.n, dtype=torch.int32, device=model.dev)
+            _, dRt, _, dxyz, dB, drec = model.adjoint(t, Rt, batch, _lib.ADJ_PATTERN_GRAD,
+                                                      gP.contiguous().to(torch.float32), True,
+                                                      status=status)
+            if not _grow_if_needed(model, batch, status):
+                break
         g = _grads_like((Rt, rec, xyz, B), dRt, drec, dxyz, dB)
         return (*g, None)

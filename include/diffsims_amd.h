@@ -344,6 +344,18 @@ int dsx_polar_templates(const float *spot_d, const int *count_d, int cap, int n,
                         int n_azimuthal, float *r_d, float *theta_d, int *r_idx_d,
                         int *theta_idx_d, float *I_d, int *n_out_d, void *stream);
 
+/*
+ * Bunge Euler angles (phi1, Phi, phi2 in radians, double [n][3]) -> rows of
+ * M^T (double [n][9]) with M the passive Bunge matrix (orix
+ * Rotation.from_euler(...).to_matrix(); the reference rotates with
+ * ~rotation * miller, _diffracting_vector.py:157-160), and its adjoint
+ * deuler[n][3] = sum_ij drt[n][ij] d rt_ij / d angle.  No counterpart in the
+ * reference beyond the matrix itself; serves the differentiable path.
+ */
+int dsx_euler_to_rt(const double *euler_d, int n, double *rt_d, void *stream);
+int dsx_euler_to_rt_backward(const double *euler_d, const double *drt_d, int n,
+                             double *deuler_d, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

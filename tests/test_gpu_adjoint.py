@@ -362,3 +362,22 @@ def test_large_precession_cell_full_radius_forward():
         edge = 1.0 / raw.max()
         err = np.abs(pats[o] - rp).max()
         assert err <= P.TOL_P + edge, err
+
+
+def test_euler_to_rt_kernel_and_backward():
+    """The one-kernel Euler -> Rt map and its hand-written backward against the
+    generic torch autograd of bunge_matrices (float64)."""
+    from diffsims_amd import differentiable as D
+
+    eu = torch.tensor(O.random_euler(257, 5), dtype=torch.float64, device="cuda")
+    e1 = eu.clone().requires_grad_(True)
+    e2 = eu.clone().requires_grad_(True)
+    r1 = D.euler_to_rt(e1)
+    r2 = D.rt_from_matrices(D.bunge_matrices(e2))
+    assert float((r1 - r2).abs().max()) <= 1e-14
+    assert np.allclose(r1.detach().cpu().numpy().reshape(-1, 3, 3).transpose(0, 2, 1),
+                       O.bunge_matrix(eu.cpu().numpy()), atol=1e-14)
+    w = torch.randn(257, 9, dtype=torch.float64, device="cuda")
+    (r1 * w).sum().backward()
+    (r2 * w).sum().backward()
+    assert float((e1.grad - e2.grad).abs().max()) <= 1e-12
