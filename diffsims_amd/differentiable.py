@@ -125,24 +125,67 @@ def _cosd(x: torch.Tensor) -> torch.Tensor:
     return c
 
 
+def _basis_and_jacobian(abc: np.ndarray, angles: np.ndarray):
+    """Direct basis rows (x || a, z || c*) and d base / d(a, b, c, alpha, beta,
+    gamma) (angles in degrees), shape (3, 3, 6)."""
+    a, b, c = (float(v) for v in abc)
+    al, be, ga = (math.radians(float(v)) for v in angles)
+    ex = [_EXACT_COS.get(math.fmod(float(v), 360.0) % 360.0) for v in angles]
+    ca = ex[0] if ex[0] is not None else math.cos(al)
+    cb = ex[1] if ex[1] is not None else math.cos(be)
+    cg = ex[2] if ex[2] is not None else math.cos(ga)
+    sa, sb = math.sin(al), math.sin(be)
+    sgx = _EXACT_COS.get(math.fmod(90.0 - float(angles[2]), 360.0) % 360.0)
+    sg = sgx if sgx is not None else math.sin(ga)
+    t = (ca - cb * cg) / sg
+    root = math.sqrt(1.0 - cb * cb - t * t)
+    base = np.array([[a, 0.0, 0.0], [b * cg, b * sg, 0.0], [c * cb, c * t, c * root]])
+    J = np.zeros((3, 3, 6))
+    d = math.pi / 180.0                           # per degree
+    J[0, 0, 0] = 1.0
+    J[1, 0, 1], J[1, 1, 1] = cg, sg
+    J[2, 0, 2], J[2, 1, 2], J[2, 2, 2] = cb, t, root
+    # t = (ca - cb cg) / sg
+    dt = np.array([-sa / sg, sb * cg / sg, (cb * sg) / sg - (ca - cb * cg) * math.cos(ga) / sg ** 2])
+    dt *= d
+    droot = np.array([-t * dt[0], (cb * sb * d - t * dt[1]), -t * dt[2]]) / root
+    J[1, 0, 5], J[1, 1, 5] = -b * math.sin(ga) * d, b * math.cos(ga) * d
+    J[2, 0, 4] = -c * sb * d
+    J[2, 1, 3:6] = c * dt
+    J[2, 2, 3:6] = c * droot
+    return base, J
+
+
+class _ReciprocalBasis(torch.autograd.Function):
+    """rec = inv(base) with d rec = -rec (d base) rec, so
+    dL/d base = -rec^T G rec^T for G = dL/d rec."""
+
+    @staticmethod
+    def forward(ctx, abc, angles):
+        a_np, g_np = abc.detach().cpu().numpy(), angles.detach().cpu().numpy()
+        _, J = _basis_and_jacobian(a_np, g_np)
+        # the same closed-form inverse the tables are built with
+        rec = lattice_setup(tuple(a_np.tolist()), tuple(g_np.tolist()))[1]
+        ctx.rec, ctx.J = rec, J
+        return torch.as_tensor(rec, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, G):
+        rec, J = ctx.rec, ctx.J
+        dbase = -rec.T @ G.detach().cpu().numpy() @ rec.T
+        g = np.tensordot(dbase, J, axes=([0, 1], [0, 1]))
+        return (torch.as_tensor(g[:3], dtype=torch.float64),
+                torch.as_tensor(g[3:], dtype=torch.float64))
+
+
 def reciprocal_basis(abc, angles) -> torch.Tensor:
     """rec (3x3, columns a*, b*, c*; g = rec @ hkl) in the x || a, z || c*
-    frame from a, b, c (A) and alpha, beta, gamma (degrees): the closed-form
-    inverse of the lower-triangular direct basis, differentiable."""
-    a, b, c = abc[0], abc[1], abc[2]
-    ca, cb, cg = _cosd(angles[0]), _cosd(angles[1]), _cosd(angles[2])
-    sg = _cosd(90.0 - angles[2])
-    b1, b2 = b * cg, b * sg
-    c1 = c * cb
-    t = (ca - cb * cg) / sg
-    c2 = c * t
-    c3 = c * torch.sqrt(1.0 - cb * cb - t * t)
-    z = torch.zeros((), dtype=a.dtype)
-    return torch.stack([
-        torch.stack([1.0 / a, z, z]),
-        torch.stack([-b1 / (a * b2), 1.0 / b2, z]),
-        torch.stack([(b1 * c2 - b2 * c1) / (a * b2 * c3), -c2 / (b2 * c3), 1.0 / c3]),
-    ])
+    frame from a, b, c (A) and alpha, beta, gamma (degrees), as CPU float64
+    tensors; differentiable (closed-form Jacobian of the lower-triangular
+    direct basis and d inv(B) = -inv(B) dB inv(B))."""
+    abc = torch.as_tensor(abc, dtype=torch.float64)
+    angles = torch.as_tensor(angles, dtype=torch.float64)
+    return _ReciprocalBasis.apply(abc, angles)
 
 
 def leaf_parameters(phase: PhaseArrays, euler: np.ndarray, device=None,
