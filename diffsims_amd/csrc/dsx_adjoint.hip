@@ -40,6 +40,15 @@ constexpr int kAdjBinCap = 4096;
 constexpr int kAdjStripBytes = DSX_ADJ_STRIP_KB * 1024;
 constexpr int kPixBits = 22;  // argmax key: value << 22 | (2^22 - 1 - pixel)
 constexpr int kNRed = 11;     // block sums: loss, sum G P, dRt[9]
+constexpr int kAdjTile = 16;      // tile edge of the fixed-point bound
+constexpr int kAdjMaxTiles = 1024;
+constexpr float kAdjRange = 2040109465.6f;  // 1.9 * 2^30 units per bound
+constexpr float kAdjBoundUnits = 1048576.0f;  // 2^20 bound units per largest spot
+
+// 32-bit fixed-point strip words: the largest per-tile sum U of the spots'
+// peak values bounds every pixel, so one unit = U / (1.9 * 2^30) cannot
+// overflow (the bound is an integer sum, independent of the atomics' order).
+typedef int aacc_t;
 
 struct AdjArgs {
   const int *idx;
@@ -87,7 +96,8 @@ struct AdjShared {
   int bin_fill[kAdjMaxStrips];
   int ns, overflow, cols_max;
   float wmaxf[kAdjWaves];
-  acc_t wkey[kAdjWaves];
+  long long wkey[kAdjWaves];
+  int wu[kAdjWaves];
   double red[kAdjWaves][kNRed];
   double R[9];
 };
@@ -111,7 +121,7 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 // inside strip rows [y0, y1).  Work item = (spot, footprint column) so a wave
 // touches consecutive LDS columns.
 template <bool kGather>
-__device__ __forceinline__ void strip_pass(const AdjArgs &a, AdjShared &sh, acc_t *acc,
+__device__ __forceinline__ void strip_pass(const AdjArgs &a, AdjShared &sh, aacc_t *acc,
                                            const float *G, int b0, int nb, bool binned, int y0,
                                            int y1, float scale, int tid) {
   const int W = a.W;
@@ -131,10 +141,10 @@ __device__ __forceinline__ void strip_pass(const AdjArgs &a, AdjShared &sh, acc_
     const float cy = sh.Y[q];
     if (!kGather) {
       const float amp = sh.I[q] * a.pref;
-      acc_t *col = acc + x;
+      aacc_t *col = acc + x;
       for (int y = ya; y <= yb; ++y) {
         const float dy = static_cast<float>(y) - cy;
-        acc_add(col + (y - y0) * W, amp * expf(a.ep * (dy * dy + dx * dx)), scale);
+        atomicAdd(col + (y - y0) * W, __float2int_rn(amp * expf(a.ep * (dy * dy + dx * dx)) * scale));
       }
     } else {
       float sa = 0.0f, sy = 0.0f;
@@ -252,7 +262,7 @@ __device__ __forceinline__ void max_term(const AdjArgs &a, float corr, int px, i
 }
 
 __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) {
-  extern __shared__ __align__(16) acc_t s_acc[];  // 16-byte aligned for the longlong2 read-out
+  extern __shared__ __align__(16) aacc_t s_acc[];  // 16-byte aligned for the 16-byte read-out
   __shared__ AdjShared sh;
   __shared__ ChainConsts s_cc;
   float *s_G = reinterpret_cast<float *>(s_acc + a.acc_words);
@@ -324,7 +334,7 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
     const int ns_all = sh.ns;
     float pmax = 0.0f;
     for (int w = 0; w < kAdjWaves; ++w) pmax = fmaxf(pmax, sh.wmaxf[w]);
-    const float scale = pmax > 0.0f ? kFixedOne / (pmax * 1.001f) : 1.0f;
+    float scale = 1.0f;  // set from the tile bounds below
 
     // ---- 2. strip bins ----------------------------------------------------
     if (tid == 0) {
@@ -380,6 +390,55 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
     }
     lds_sync();
 
+    // ---- 2b. fixed-point scale: per-tile sums of the spots' peak values ------
+    {
+      const int ntx = (W + kAdjTile - 1) / kAdjTile;
+      const int n_tiles = ntx * ((H + kAdjTile - 1) / kAdjTile);
+      const bool tiles = n_tiles <= kAdjMaxTiles && n_tiles <= a.acc_words;
+      int *tile_u = s_acc;  // the strip buffer is zero here and zeroed again below
+      const float inv_p = pmax > 0.0f ? kAdjBoundUnits / pmax : 0.0f;
+      int usum = 0;
+      auto add_bound = [&](float I, int lo, int hi, int xlo, int xhi) {
+        const int u = min(static_cast<int>(ceilf(fabsf(I) * a.pref * inv_p)) + 1,
+                          static_cast<int>(kAdjBoundUnits) + 1);
+        usum += u;
+        if (!tiles) return;
+        for (int ty = lo / kAdjTile; ty <= hi / kAdjTile; ++ty)
+          for (int tx = xlo / kAdjTile; tx <= xhi / kAdjTile; ++tx) atomicAdd(&tile_u[ty * ntx + tx], u);
+      };
+      if (!large) {
+        for (int q = tid; q < ns; q += kAdjThreads)
+          add_bound(sh.I[q], sh.ylo[q], sh.yhi[q], sh.xb[q] >> 16, sh.xb[q] & 0xffff);
+      } else {
+        for (int j = tid; j < n; j += kAdjThreads) {
+          const float4 sp = spots[j];
+          float X, Y;
+          if (!to_pixel(sp, a, X, Y)) continue;
+          int lo, hi, xlo, xhi;
+          if (!subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, H, W, lo, hi, xlo, xhi)) continue;
+          add_bound(sp.w, lo, hi, xlo, xhi);
+        }
+      }
+      int lu = 0;
+      if (tiles) {
+        lds_sync();
+        for (int t = tid; t < n_tiles; t += kAdjThreads) lu = max(lu, tile_u[t]);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) lu = max(lu, __shfl_xor(lu, off, kWave));
+      } else {
+        lu = wave_sum(usum);
+      }
+      if (lane == 0) sh.wu[wid] = lu;
+      lds_sync();
+      int Ui = 0;
+      for (int w = 0; w < kAdjWaves; ++w) Ui = tiles ? max(Ui, sh.wu[w]) : Ui + sh.wu[w];
+      if (tiles)
+        for (int t = tid; t < n_tiles; t += kAdjThreads) tile_u[t] = 0;
+      const float U = static_cast<float>(Ui) * (pmax / kAdjBoundUnits);
+      scale = U > 0.0f ? kAdjRange / (U * 1.00001f) : 1.0f;
+      lds_sync();
+    }
+
     // splat (or gather) strip s: every LDS-resident spot of its bin, or, for a
     // large pattern, its global bin chunk by chunk
     auto strip_spots = [&](int s, bool gather, int y0, int y1) {
@@ -418,7 +477,7 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
     };
 
     // ---- 3. pass A: maximum and its first pixel -----------------------------
-    acc_t key = 0;
+    long long key = 0;
     if (a.normalize && ns_all > 0) {
       for (int s = 0; s < a.n_strips; ++s) {
         const int y0 = s * a.strip_rows;
@@ -429,10 +488,10 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
         const int cnt = (y1 - y0) * W;
         const int pix0 = y0 * W;
         for (int i = tid; i < cnt; i += kAdjThreads) {
-          const acc_t v = s_acc[i];
+          const aacc_t v = s_acc[i];
           s_acc[i] = 0;
-          const acc_t kv = (max(v, static_cast<acc_t>(0)) << kPixBits) |
-                           static_cast<acc_t>((1 << kPixBits) - 1 - (pix0 + i));
+          const long long kv = (static_cast<long long>(max(v, 0)) << kPixBits) |
+                               static_cast<long long>((1 << kPixBits) - 1 - (pix0 + i));
           key = kv > key ? kv : key;
         }
         lds_sync();
@@ -443,12 +502,12 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
       key = 0;
       for (int w = 0; w < kAdjWaves; ++w) key = sh.wkey[w] > key ? sh.wkey[w] : key;
     }
-    const acc_t m_fixed = key >> kPixBits;
+    const long long m_fixed = key >> kPixBits;
     const int pstar = (1 << kPixBits) - 1 - static_cast<int>(key & ((1 << kPixBits) - 1));
     // a zero maximum leaves the pattern unnormalised (the reference returns the
     // zeros as they are)
     const bool norm = a.normalize && m_fixed > 0;
-    const float out_scale = norm ? 1.0f / acc_to_float(m_fixed) : 1.0f / scale;
+    const float out_scale = norm ? 1.0f / static_cast<float>(m_fixed) : 1.0f / scale;
     const float gscale = scale * out_scale;  // dL/dQ = G / m  (1 without normalisation)
 
     // ---- 4. pass B: pattern, loss, upstream gradient, gather ----------------
@@ -469,14 +528,11 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
       const size_t off = (size_t)y0 * W;
       if ((W & 3) == 0) {
         for (int i = tid; i < cnt / 4; i += kAdjThreads) {
-          longlong2 *pv = reinterpret_cast<longlong2 *>(s_acc) + 2 * i;
-          const longlong2 u = pv[0], v = pv[1];
-          if (any) {
-            pv[0] = make_longlong2(0, 0);
-            pv[1] = make_longlong2(0, 0);
-          }
-          const float4 P = make_float4(acc_to_float(u.x) * out_scale, acc_to_float(u.y) * out_scale,
-                                       acc_to_float(v.x) * out_scale, acc_to_float(v.y) * out_scale);
+          int4 *pv = reinterpret_cast<int4 *>(s_acc) + i;
+          const int4 u = *pv;
+          if (any) *pv = make_int4(0, 0, 0, 0);
+          const float4 P = make_float4(static_cast<float>(u.x) * out_scale, static_cast<float>(u.y) * out_scale,
+                                       static_cast<float>(u.z) * out_scale, static_cast<float>(u.w) * out_scale);
           if (pat) store_nt(P, reinterpret_cast<float4 *>(pat + off) + i);
           if (fwd_only) continue;
           const native_float4 t =
@@ -495,9 +551,9 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
         }
       } else {
         for (int i = tid; i < cnt; i += kAdjThreads) {
-          const acc_t v = s_acc[i];
+          const aacc_t v = s_acc[i];
           if (any) s_acc[i] = 0;
-          const float P = acc_to_float(v) * out_scale;
+          const float P = static_cast<float>(v) * out_scale;
           if (pat) __builtin_nontemporal_store(P, pat + off + i);
           if (fwd_only) continue;
           const float t = tgt[off + i];
@@ -819,7 +875,7 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   a.ep = static_cast<float>(-1.0 / (2.0 * sg * sg));
   a.inv_s2 = static_cast<float>(1.0 / (sg * sg));
   a.clip = rp->clip_threshold;
-  const int budget = kAdjStripBytes / static_cast<int>(sizeof(acc_t) + sizeof(float));
+  const int budget = kAdjStripBytes / static_cast<int>(sizeof(aacc_t) + sizeof(float));
   if (a.W > budget) {
     set_error("dsx_render_adjoint: pattern width %d exceeds the LDS strip budget", a.W);
     return DSX_EINVAL;
@@ -858,7 +914,7 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
     a.work_per_block = 0;
     a.work_entries = 0;
   }
-  const size_t lds = static_cast<size_t>(a.acc_words) * (sizeof(acc_t) + sizeof(float));
+  const size_t lds = static_cast<size_t>(a.acc_words) * (sizeof(aacc_t) + sizeof(float));
   static int reserved = -1;  // raised once, to the largest strip this build uses
   if (reserved < 0) {
     if (hipFuncSetAttribute(reinterpret_cast<const void *>(&render_adjoint_kernel),
