@@ -42,6 +42,13 @@ constexpr int kPixBits = 22;  // argmax key: value << 22 | (2^22 - 1 - pixel)
 constexpr int kNRed = 11;     // block sums: loss, sum G P, dRt[9]
 constexpr int kAdjTile = 16;      // tile edge of the fixed-point bound
 constexpr int kAdjMaxTiles = 1024;
+constexpr int kAdjMaxCand = 16;    // candidate tiles of the maximum search
+constexpr int kAdjCandSpots = 64;  // spots touching one candidate tile
+// float4 target values each thread of the two-blocks-per-CU kernel prefetches
+// for one strip of at most kAdjSmallStripBytes / 8 pixels (the large-pattern
+// kernel, whose strips take long to splat, loads them in the read-out)
+constexpr int kAdjPre = 4;
+constexpr int kAdjSmallStripBytes = kAdjPre * 4 * kAdjThreads * 8;
 constexpr float kAdjRange = 2040109465.6f;  // 1.9 * 2^30 units per bound
 constexpr float kAdjBoundUnits = 1048576.0f;  // 2^20 bound units per largest spot
 
@@ -71,12 +78,15 @@ struct AdjArgs {
   int mirrored, normalize;
   float pref, ep, inv_s2, clip;
   int strip_rows, n_strips, acc_words;
+  int slots, bin_cap;       // LDS spot slots and strip-bin entries
   int mode;
   int want_grad;
   long long target_stride;  // elements between targets (0: one target for every pattern)
   int *work;                // large patterns: per-block bins + per-spot sums (may be NULL)
   long long work_per_block; // ints
   int work_entries;         // bin entries per block
+  float4 *gsum;             // per pattern and slot: A, BX, BY, list position (chain kernel input)
+  int *gcount;              // per pattern: slots used
   const float *target;
   float *pattern;
   float *loss;
@@ -86,24 +96,52 @@ struct AdjArgs {
 };
 
 struct AdjShared {
-  float I[kAdjMaxSpots], X[kAdjMaxSpots], Y[kAdjMaxSpots];
-  float A[kAdjMaxSpots], BX[kAdjMaxSpots], BY[kAdjMaxSpots];
-  int xb[kAdjMaxSpots];    // (xlo << 16) | xhi
-  int jpos[kAdjMaxSpots];  // position in the pattern's spot list
-  short ylo[kAdjMaxSpots], yhi[kAdjMaxSpots];
-  short bin[kAdjBinCap];
   int bin_start[kAdjMaxStrips + 1];
   int bin_fill[kAdjMaxStrips];
-  int ns, overflow, cols_max;
+  int ns, cols_max;
   float wmaxf[kAdjWaves];
   long long wkey[kAdjWaves];
   int wu[kAdjWaves];
+  int qstar, n_cand, fail;
+  long long lkey;
+  int cand_tile[kAdjMaxCand], cand_n[kAdjMaxCand];
   double red[kAdjWaves][kNRed];
   double R[9];
 };
 
+// The spot table lives in dynamic LDS after the two strip buffers (acc_words
+// fixed-point words, acc_words gradient floats), sized by the host: `slots`
+// entries of each field, then `bin_cap` strip-bin entries.
+struct SpotTab {
+  float *I, *X, *Y, *A, *BX, *BY;
+  int *xb;    // (xlo << 16) | xhi
+  int *jpos;  // position in the pattern's spot list
+  short *ylo, *yhi, *bin;
+};
+
+constexpr int kSpotTabBytes = 6 * 4 + 2 * 4 + 2 * 2;  // per slot
+
+__device__ __forceinline__ SpotTab spot_tab(aacc_t *s_acc, int acc_words, int slots) {
+  float *f = reinterpret_cast<float *>(s_acc + 2 * static_cast<size_t>(acc_words));
+  SpotTab t;
+  t.I = f;
+  t.X = f + slots;
+  t.Y = f + 2 * slots;
+  t.A = f + 3 * slots;
+  t.BX = f + 4 * slots;
+  t.BY = f + 5 * slots;
+  int *iv = reinterpret_cast<int *>(f + 6 * slots);
+  t.xb = iv;
+  t.jpos = iv + slots;
+  short *sv = reinterpret_cast<short *>(iv + 2 * slots);
+  t.ylo = sv;
+  t.yhi = sv + slots;
+  t.bin = sv + 2 * slots;
+  return t;
+}
+
 // Reflection-side constants of the chain rule, staged in LDS once per block so
-// the kernel does not hold them in scalar registers throughout.
+// the kernels do not hold them in scalar registers throughout.
 struct ChainConsts {
   double rec[9];
   double k, k2, me, width, sigma, fac;
@@ -111,40 +149,61 @@ struct ChainConsts {
   double phi, Pz, Pt;
 };
 
+__device__ __forceinline__ void stage_chain_consts(const AdjArgs &a, ChainConsts &c) {
+  for (int i = 0; i < 9; ++i) c.rec[i] = a.rec[i];
+  c.k = a.k;
+  c.k2 = a.k2;
+  c.me = a.me;
+  c.width = a.width;
+  c.sigma = a.sigma;
+  c.fac = a.fac;
+  c.model = a.model;
+  c.precession = a.precession;
+  c.phi = a.phi;
+  c.Pz = a.Pz;
+  c.Pt = a.Pt;
+}
+
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
   return v;
 }
 
+// Fixed-point value of one Gaussian term amp exp(ep r^2) at offset (dx, dy),
+// written once so the splat and the maximum search round it identically.
+__device__ __forceinline__ int gauss_term(float amp, float dx, float dy, float ep, float scale) {
+  return __float2int_rn(amp * expf(ep * fmaf(dy, dy, dx * dx)) * scale);
+}
+
 // Splat (kGather = false) or gather (true) of every binned spot footprint
 // inside strip rows [y0, y1).  Work item = (spot, footprint column) so a wave
 // touches consecutive LDS columns.
 template <bool kGather>
-__device__ __forceinline__ void strip_pass(const AdjArgs &a, AdjShared &sh, aacc_t *acc,
-                                           const float *G, int b0, int nb, bool binned, int y0,
-                                           int y1, float scale, int tid) {
+__device__ __forceinline__ void strip_pass(const AdjArgs &a, const SpotTab &t, int cols_max,
+                                           aacc_t *acc, const float *G, int b0, int nb,
+                                           bool binned, int y0, int y1, float scale, int tid) {
   const int W = a.W;
-  const int ncols = sh.cols_max;
+  const int ncols = cols_max;
   const int n_items = nb * ncols;
   for (int it = tid; it < n_items; it += kAdjThreads) {
     const int bi = it / ncols;
     const int c = it - bi * ncols;
-    const int q = binned ? sh.bin[b0 + bi] : bi;
-    const int ya = max(static_cast<int>(sh.ylo[q]), y0);
-    const int yb = min(static_cast<int>(sh.yhi[q]), y1 - 1);
+    const int q = binned ? t.bin[b0 + bi] : bi;
+    const int ya = max(static_cast<int>(t.ylo[q]), y0);
+    const int yb = min(static_cast<int>(t.yhi[q]), y1 - 1);
     if (ya > yb) continue;
-    const int xlo = sh.xb[q] >> 16, xhi = sh.xb[q] & 0xffff;
+    const int xlo = t.xb[q] >> 16, xhi = t.xb[q] & 0xffff;
     const int x = xlo + c;
     if (x > xhi) continue;
-    const float dx = static_cast<float>(x) - sh.X[q];
-    const float cy = sh.Y[q];
+    const float dx = static_cast<float>(x) - t.X[q];
+    const float cy = t.Y[q];
     if (!kGather) {
-      const float amp = sh.I[q] * a.pref;
+      const float amp = t.I[q] * a.pref;
       aacc_t *col = acc + x;
       for (int y = ya; y <= yb; ++y) {
         const float dy = static_cast<float>(y) - cy;
-        atomicAdd(col + (y - y0) * W, __float2int_rn(amp * expf(a.ep * (dy * dy + dx * dx)) * scale));
+        atomicAdd(col + (y - y0) * W, gauss_term(amp, dx, dy, a.ep, scale));
       }
     } else {
       float sa = 0.0f, sy = 0.0f;
@@ -155,9 +214,9 @@ __device__ __forceinline__ void strip_pass(const AdjArgs &a, AdjShared &sh, aacc
         sy = fmaf(kg, dy, sy);
       }
       if (sa != 0.0f || sy != 0.0f) {
-        atomicAdd(&sh.A[q], sa);
-        atomicAdd(&sh.BX[q], sa * dx);
-        atomicAdd(&sh.BY[q], sy);
+        atomicAdd(&t.A[q], sa);
+        atomicAdd(&t.BX[q], sa * dx);
+        atomicAdd(&t.BY[q], sy);
       }
     }
   }
@@ -165,7 +224,7 @@ __device__ __forceinline__ void strip_pass(const AdjArgs &a, AdjShared &sh, aacc
 
 // Load entries [base, base + m) of a strip's global bin into LDS slots 0..m-1
 // (large patterns: more in-frame spots than the LDS holds).
-__device__ __forceinline__ void load_chunk(const AdjArgs &a, AdjShared &sh, const float4 *spots,
+__device__ __forceinline__ void load_chunk(const AdjArgs &a, const SpotTab &t, const float4 *spots,
                                            const int *bins, int base, int m, int tid) {
   for (int i = tid; i < m; i += kAdjThreads) {
     const int j = bins[base + i];
@@ -180,16 +239,16 @@ __device__ __forceinline__ void load_chunk(const AdjArgs &a, AdjShared &sh, cons
       xlo = 0;
       xhi = 0;
     }
-    sh.I[i] = sp.w;
-    sh.X[i] = X;
-    sh.Y[i] = Y;
-    sh.A[i] = 0.0f;
-    sh.BX[i] = 0.0f;
-    sh.BY[i] = 0.0f;
-    sh.xb[i] = (xlo << 16) | xhi;
-    sh.jpos[i] = j;
-    sh.ylo[i] = static_cast<short>(lo);
-    sh.yhi[i] = static_cast<short>(hi);
+    t.I[i] = sp.w;
+    t.X[i] = X;
+    t.Y[i] = Y;
+    t.A[i] = 0.0f;
+    t.BX[i] = 0.0f;
+    t.BY[i] = 0.0f;
+    t.xb[i] = (xlo << 16) | xhi;
+    t.jpos[i] = j;
+    t.ylo[i] = static_cast<short>(lo);
+    t.yhi[i] = static_cast<short>(hi);
   }
 }
 
@@ -261,32 +320,130 @@ __device__ __forceinline__ void max_term(const AdjArgs &a, float corr, int px, i
   BY -= kc * dy;
 }
 
-__global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) {
-  extern __shared__ __align__(16) aacc_t s_acc[];  // 16-byte aligned for the 16-byte read-out
-  __shared__ AdjShared sh;
-  __shared__ ChainConsts s_cc;
+// Fixed-point value at pixel (x, y) of LDS spot q (0 outside its clip box).
+__device__ __forceinline__ int spot_at(const AdjArgs &a, const SpotTab &t, int q, int x, int y,
+                                       float scale) {
+  if (y < t.ylo[q] || y > t.yhi[q] || x < (t.xb[q] >> 16) || x > (t.xb[q] & 0xffff)) return 0;
+  return gauss_term(t.I[q] * a.pref, static_cast<float>(x) - t.X[q],
+                    static_cast<float>(y) - t.Y[q], a.ep, scale);
+}
+
+// Pattern maximum and its first pixel, as the key value << kPixBits |
+// (2^kPixBits - 1 - pixel), without rendering the whole pattern: a tile whose
+// bound (the per-tile sums tile_u of the spots' peak values, in units of
+// `unit`) stays below the exact value at the brightest spot's nearest pixel
+// cannot hold the maximum, and the few remaining tiles are evaluated pixel by
+// pixel with the splat's own fixed-point terms (integer sums: the result equals
+// a full pass bit for bit).  Every pixel holding the maximum lies in a
+// remaining tile, so the first of them is found.  Returns false (uniformly)
+// when too many tiles or spots remain; the caller then runs the full pass.
+// tile_u is overwritten (candidate slots); cand_q borrows the gradient buffer.
+__device__ bool bounded_max_key(const AdjArgs &a, AdjShared &sh, const SpotTab &t, int *tile_u,
+                                short (*cand_q)[kAdjCandSpots], int ntx, int n_tiles, int ns,
+                                float scale, float unit, int tid, int lane, int wid,
+                                long long &key_out) {
+  const int H = a.H, W = a.W;
+  if (tid == 0) {
+    sh.n_cand = 0;
+    sh.fail = 0;
+  }
+  if (wid == 0) {
+    const int qs = sh.qstar;
+    const int xs = min(max(static_cast<int>(rintf(t.X[qs])), t.xb[qs] >> 16), t.xb[qs] & 0xffff);
+    const int ys = min(max(static_cast<int>(rintf(t.Y[qs])), static_cast<int>(t.ylo[qs])),
+                       static_cast<int>(t.yhi[qs]));
+    long long acc = 0;
+    for (int q = lane; q < ns; q += kWave) acc += spot_at(a, t, q, xs, ys, scale);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+    if (lane == 0)
+      sh.lkey = (max(acc, 0ll) << kPixBits) |
+                static_cast<long long>((1 << kPixBits) - 1 - (ys * W + xs));
+  }
+  lds_sync();
+  const long long lkey = sh.lkey;
+  const float lfix = static_cast<float>(lkey >> kPixBits);
+  // per term: the float product is within 1e-5 of the bound's share and the
+  // rounding adds at most one half
+  const float margin = 0.5f * static_cast<float>(ns) + 1.0f;
+  int *slot_of = tile_u;
+  for (int ti = tid; ti < n_tiles; ti += kAdjThreads) {
+    const float bound = static_cast<float>(tile_u[ti]) * unit * scale * 1.00001f + margin;
+    int slot = -1;
+    if (bound >= lfix) {
+      slot = atomicAdd(&sh.n_cand, 1);
+      if (slot < kAdjMaxCand) {
+        sh.cand_tile[slot] = ti;
+        sh.cand_n[slot] = 0;
+      } else {
+        slot = -1;
+        sh.fail = 1;
+      }
+    }
+    slot_of[ti] = slot;
+  }
+  lds_sync();
+  if (sh.fail) return false;
+  const int n_cand = sh.n_cand;
+  for (int q = tid; q < ns; q += kAdjThreads) {
+    const int tx0 = (t.xb[q] >> 16) / kAdjTile, tx1 = (t.xb[q] & 0xffff) / kAdjTile;
+    const int ty0 = t.ylo[q] / kAdjTile, ty1 = t.yhi[q] / kAdjTile;
+    for (int ty = ty0; ty <= ty1; ++ty)
+      for (int tx = tx0; tx <= tx1; ++tx) {
+        const int sl = slot_of[ty * ntx + tx];
+        if (sl < 0) continue;
+        const int pos = atomicAdd(&sh.cand_n[sl], 1);
+        if (pos < kAdjCandSpots)
+          cand_q[sl][pos] = static_cast<short>(q);
+        else
+          sh.fail = 1;
+      }
+  }
+  lds_sync();
+  if (sh.fail) return false;
+  long long key = lkey;
+  for (int i = tid; i < n_cand * kAdjTile * kAdjTile; i += kAdjThreads) {
+    const int sl = i / (kAdjTile * kAdjTile);
+    const int ti = sh.cand_tile[sl];
+    const int x = (ti % ntx) * kAdjTile + (i % kAdjTile);
+    const int y = (ti / ntx) * kAdjTile + (i / kAdjTile) % kAdjTile;
+    if (x >= W || y >= H) continue;
+    long long acc = 0;
+    const int nq = sh.cand_n[sl];
+    for (int k = 0; k < nq; ++k) acc += spot_at(a, t, cand_q[sl][k], x, y, scale);
+    const long long kv =
+        (max(acc, 0ll) << kPixBits) | static_cast<long long>((1 << kPixBits) - 1 - (y * W + x));
+    key = kv > key ? kv : key;
+  }
+  key = wave_max_acc(key);
+  if (lane == 0) sh.wkey[wid] = key;
+  lds_sync();
+  key = 0;
+  for (int w = 0; w < kAdjWaves; ++w) key = sh.wkey[w] > key ? sh.wkey[w] : key;
+  key_out = key;
+  lds_sync();  // wkey and the candidate state are reused
+  return true;
+}
+
+// One pattern per block iteration.  kLarge = false: every in-frame spot fits
+// the LDS slots (the host guarantees slots >= cap) and the per-spot sums go to
+// gsum for spot_chain_kernel, which keeps the FP64 chain rule (and its
+// registers) out of this kernel.  kLarge = true: patterns may hold more
+// in-frame spots than the slots (their bins and sums live in the per-block
+// workspace) and the chain rule runs here, per pattern.
+template <bool kLarge>
+__device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, AdjShared &sh,
+                                             ChainConsts &s_cc) {
   float *s_G = reinterpret_cast<float *>(s_acc + a.acc_words);
+  const SpotTab t = spot_tab(s_acc, a.acc_words, a.slots);
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wid = tid / kWave;
   const int H = a.H, W = a.W;
   const size_t HW = static_cast<size_t>(H) * W;
-  if (tid == 0) {
-    for (int i = 0; i < 9; ++i) s_cc.rec[i] = a.rec[i];
-    s_cc.k = a.k;
-    s_cc.k2 = a.k2;
-    s_cc.me = a.me;
-    s_cc.width = a.width;
-    s_cc.sigma = a.sigma;
-    s_cc.fac = a.fac;
-    s_cc.model = a.model;
-    s_cc.precession = a.precession;
-    s_cc.phi = a.phi;
-    s_cc.Pz = a.Pz;
-    s_cc.Pt = a.Pt;
-  }
-  int *wbins = a.work ? a.work + (size_t)blockIdx.x * a.work_per_block : nullptr;
-  float *wacc = a.work ? reinterpret_cast<float *>(wbins + a.work_entries) : nullptr;
+  if (kLarge && tid == 0) stage_chain_consts(a, s_cc);
+  int *wbins = kLarge && a.work ? a.work + (size_t)blockIdx.x * a.work_per_block : nullptr;
+  float *wacc = kLarge && a.work ? reinterpret_cast<float *>(wbins + a.work_entries) : nullptr;
   for (int i = tid; i < a.acc_words; i += kAdjThreads) s_acc[i] = 0;
   lds_sync();
 
@@ -295,16 +452,16 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
     const float4 *spots = a.spots + (size_t)p * a.cap;
     if (tid == 0) {
       sh.ns = 0;
-      sh.overflow = 0;
       sh.cols_max = 1;
+      sh.qstar = 0;
     }
-    if (tid < 9 && a.Rt) sh.R[tid] = a.Rt[(size_t)p * 9 + tid];
+    if (kLarge && tid < 9 && a.want_grad) sh.R[tid] = a.Rt[(size_t)p * 9 + tid];
     for (int s = tid; s <= kAdjMaxStrips; s += kAdjThreads) sh.bin_start[s] = 0;
     lds_sync();
 
     // ---- 1. spots in frame with their clip boxes --------------------------
     // every such spot counts in the strip bins and the maximum; the first
-    // kAdjMaxSpots also go to LDS (all of them for the usual pattern)
+    // `slots` also go to LDS (all of them unless the pattern is large)
     float tmax = 0.0f;
     for (int j = tid; j < n; j += kAdjThreads) {
       const float4 sp = spots[j];
@@ -316,17 +473,17 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
       atomicMax(&sh.cols_max, xhi - xlo + 1);
       for (int s = lo / a.strip_rows; s <= hi / a.strip_rows; ++s) atomicAdd(&sh.bin_start[s + 1], 1);
       const int q = atomicAdd(&sh.ns, 1);
-      if (q >= kAdjMaxSpots) continue;
-      sh.I[q] = sp.w;
-      sh.X[q] = X;
-      sh.Y[q] = Y;
-      sh.A[q] = 0.0f;
-      sh.BX[q] = 0.0f;
-      sh.BY[q] = 0.0f;
-      sh.xb[q] = (xlo << 16) | xhi;
-      sh.jpos[q] = j;
-      sh.ylo[q] = static_cast<short>(lo);
-      sh.yhi[q] = static_cast<short>(hi);
+      if (kLarge && q >= a.slots) continue;
+      t.I[q] = sp.w;
+      t.X[q] = X;
+      t.Y[q] = Y;
+      t.A[q] = 0.0f;
+      t.BX[q] = 0.0f;
+      t.BY[q] = 0.0f;
+      t.xb[q] = (xlo << 16) | xhi;
+      t.jpos[q] = j;
+      t.ylo[q] = static_cast<short>(lo);
+      t.yhi[q] = static_cast<short>(hi);
     }
     tmax = wave_max(tmax);
     if (lane == 0) sh.wmaxf[wid] = tmax;
@@ -348,14 +505,14 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
     const int entries = sh.bin_start[a.n_strips];
     // large: more in-frame spots than LDS slots; their bins live in the
     // caller's workspace and the spots pass through LDS in chunks
-    const bool large = ns_all > kAdjMaxSpots;
+    const bool large = kLarge && ns_all > a.slots;
     const bool fits = !large || (wbins != nullptr && entries <= a.work_entries);
     if (tid == 0 && a.status) a.status[p] = fits ? 0 : 1;
     const int ns = large ? 0 : ns_all;  // spots held in LDS for the whole pattern
-    const bool binned = !large && entries <= kAdjBinCap;
+    const bool binned = !large && entries <= a.bin_cap;
     for (int s = tid; s < a.n_strips; s += kAdjThreads) sh.bin_fill[s] = sh.bin_start[s];
     lds_sync();
-    if (!fits) {
+    if (kLarge && !fits) {
       // no room: the pattern is reported in status and left as zeros
       if (a.pattern)
         for (size_t i = tid; i < HW; i += kAdjThreads) a.pattern[(size_t)p * HW + i] = 0.0f;
@@ -382,20 +539,22 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
       __syncthreads();  // bins and accumulators in global memory, block-visible
     } else if (binned) {
       for (int q = tid; q < ns; q += kAdjThreads) {
-        for (int s = sh.ylo[q] / a.strip_rows; s <= sh.yhi[q] / a.strip_rows; ++s) {
+        for (int s = t.ylo[q] / a.strip_rows; s <= t.yhi[q] / a.strip_rows; ++s) {
           const int pos = atomicAdd(&sh.bin_fill[s], 1);
-          sh.bin[pos] = static_cast<short>(q);
+          t.bin[pos] = static_cast<short>(q);
         }
       }
     }
     lds_sync();
+    const int cols_max = sh.cols_max;
 
     // ---- 2b. fixed-point scale: per-tile sums of the spots' peak values ------
+    const int ntx = (W + kAdjTile - 1) / kAdjTile;
+    const int n_tiles = ntx * ((H + kAdjTile - 1) / kAdjTile);
+    const bool tiles = n_tiles <= kAdjMaxTiles && n_tiles <= a.acc_words;
+    int *tile_u = s_acc;  // the strip buffer is zero here and zeroed again below
+    const float unit = pmax / kAdjBoundUnits;  // value of one bound unit
     {
-      const int ntx = (W + kAdjTile - 1) / kAdjTile;
-      const int n_tiles = ntx * ((H + kAdjTile - 1) / kAdjTile);
-      const bool tiles = n_tiles <= kAdjMaxTiles && n_tiles <= a.acc_words;
-      int *tile_u = s_acc;  // the strip buffer is zero here and zeroed again below
       const float inv_p = pmax > 0.0f ? kAdjBoundUnits / pmax : 0.0f;
       int usum = 0;
       auto add_bound = [&](float I, int lo, int hi, int xlo, int xhi) {
@@ -407,8 +566,10 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
           for (int tx = xlo / kAdjTile; tx <= xhi / kAdjTile; ++tx) atomicAdd(&tile_u[ty * ntx + tx], u);
       };
       if (!large) {
-        for (int q = tid; q < ns; q += kAdjThreads)
-          add_bound(sh.I[q], sh.ylo[q], sh.yhi[q], sh.xb[q] >> 16, sh.xb[q] & 0xffff);
+        for (int q = tid; q < ns; q += kAdjThreads) {
+          if (fabsf(t.I[q]) * a.pref == pmax) sh.qstar = q;  // the brightest spot (any of a tie)
+          add_bound(t.I[q], t.ylo[q], t.yhi[q], t.xb[q] >> 16, t.xb[q] & 0xffff);
+        }
       } else {
         for (int j = tid; j < n; j += kAdjThreads) {
           const float4 sp = spots[j];
@@ -422,7 +583,7 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
       int lu = 0;
       if (tiles) {
         lds_sync();
-        for (int t = tid; t < n_tiles; t += kAdjThreads) lu = max(lu, tile_u[t]);
+        for (int ti = tid; ti < n_tiles; ti += kAdjThreads) lu = max(lu, tile_u[ti]);
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) lu = max(lu, __shfl_xor(lu, off, kWave));
       } else {
@@ -432,11 +593,8 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
       lds_sync();
       int Ui = 0;
       for (int w = 0; w < kAdjWaves; ++w) Ui = tiles ? max(Ui, sh.wu[w]) : Ui + sh.wu[w];
-      if (tiles)
-        for (int t = tid; t < n_tiles; t += kAdjThreads) tile_u[t] = 0;
-      const float U = static_cast<float>(Ui) * (pmax / kAdjBoundUnits);
+      const float U = static_cast<float>(Ui) * unit;
       scale = U > 0.0f ? kAdjRange / (U * 1.00001f) : 1.0f;
-      lds_sync();
     }
 
     // splat (or gather) strip s: every LDS-resident spot of its bin, or, for a
@@ -447,30 +605,30 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
         const int nb = binned ? sh.bin_start[s + 1] - b0 : ns;
         if (nb == 0) return;
         if (gather)
-          strip_pass<true>(a, sh, s_acc, s_G, b0, nb, binned, y0, y1, scale, tid);
+          strip_pass<true>(a, t, cols_max, s_acc, s_G, b0, nb, binned, y0, y1, scale, tid);
         else
-          strip_pass<false>(a, sh, s_acc, nullptr, b0, nb, binned, y0, y1, scale, tid);
+          strip_pass<false>(a, t, cols_max, s_acc, nullptr, b0, nb, binned, y0, y1, scale, tid);
         return;
       }
       const int e0 = sh.bin_start[s], e1 = sh.bin_start[s + 1];
-      for (int c0 = e0; c0 < e1; c0 += kAdjMaxSpots) {
-        const int m = min(kAdjMaxSpots, e1 - c0);
-        load_chunk(a, sh, spots, wbins, c0, m, tid);
+      for (int c0 = e0; c0 < e1; c0 += a.slots) {
+        const int m = min(a.slots, e1 - c0);
+        load_chunk(a, t, spots, wbins, c0, m, tid);
         lds_sync();
         if (gather) {
-          strip_pass<true>(a, sh, s_acc, s_G, 0, m, false, y0, y1, scale, tid);
+          strip_pass<true>(a, t, cols_max, s_acc, s_G, 0, m, false, y0, y1, scale, tid);
           lds_sync();
           for (int i = tid; i < m; i += kAdjThreads) {
             // one entry per (spot, strip) and strips run in order: a plain
             // read-modify-write, made visible by the barrier below
-            float *w = wacc + 3 * sh.jpos[i];
-            w[0] += sh.A[i];
-            w[1] += sh.BX[i];
-            w[2] += sh.BY[i];
+            float *w = wacc + 3 * t.jpos[i];
+            w[0] += t.A[i];
+            w[1] += t.BX[i];
+            w[2] += t.BY[i];
           }
           __syncthreads();
         } else {
-          strip_pass<false>(a, sh, s_acc, nullptr, 0, m, false, y0, y1, scale, tid);
+          strip_pass<false>(a, t, cols_max, s_acc, nullptr, 0, m, false, y0, y1, scale, tid);
           lds_sync();  // the next chunk overwrites the slots
         }
       }
@@ -478,7 +636,17 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
 
     // ---- 3. pass A: maximum and its first pixel -----------------------------
     long long key = 0;
-    if (a.normalize && ns_all > 0) {
+    bool have_key = false;
+    if (a.normalize && ns_all > 0 && tiles && !large &&
+        a.acc_words * sizeof(float) >= sizeof(short) * kAdjMaxCand * kAdjCandSpots) {
+      have_key = bounded_max_key(a, sh, t, tile_u, reinterpret_cast<short(*)[kAdjCandSpots]>(s_G),
+                                 ntx, n_tiles, ns, scale, unit, tid, lane, wid, key);
+    }
+    if (tiles) {  // hand the borrowed words back zeroed
+      for (int ti = tid; ti < n_tiles; ti += kAdjThreads) tile_u[ti] = 0;
+      lds_sync();
+    }
+    if (a.normalize && ns_all > 0 && !have_key) {
       for (int s = 0; s < a.n_strips; ++s) {
         const int y0 = s * a.strip_rows;
         const int y1 = min(H, y0 + a.strip_rows);
@@ -516,6 +684,22 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
     float *pat = a.pattern ? a.pattern + (size_t)p * HW : nullptr;
     const bool l2 = a.mode == DSX_ADJ_L2;
     const bool fwd_only = a.mode == DSX_ADJ_FORWARD;
+    // the target of strip s + 1 is loaded into registers while strip s is
+    // gathered and strip s + 1 splatted, so its HBM latency is hidden
+    constexpr int kPre = kAdjPre;
+    native_float4 tp[kPre];
+    auto prefetch = [&](int s) {
+      if (kLarge || fwd_only || (W & 3) != 0 || s >= a.n_strips) return;
+      const int y0 = s * a.strip_rows;
+      const int cnt4 = (min(H, y0 + a.strip_rows) - y0) * W / 4;
+      const native_float4 *src = reinterpret_cast<const native_float4 *>(tgt + (size_t)y0 * W);
+#pragma unroll
+      for (int k = 0; k < kPre; ++k) {
+        const int i = tid + k * kAdjThreads;
+        if (i < cnt4) tp[k] = __builtin_nontemporal_load(src + i);
+      }
+    };
+    prefetch(0);
     for (int s = 0; s < a.n_strips; ++s) {
       const int y0 = s * a.strip_rows;
       const int y1 = min(H, y0 + a.strip_rows);
@@ -527,27 +711,40 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
       }
       const size_t off = (size_t)y0 * W;
       if ((W & 3) == 0) {
-        for (int i = tid; i < cnt / 4; i += kAdjThreads) {
+        // four pixels i: pattern, loss and G / m from the target values tv
+        auto readout4 = [&](int i, const native_float4 &tv) {
           int4 *pv = reinterpret_cast<int4 *>(s_acc) + i;
           const int4 u = *pv;
           if (any) *pv = make_int4(0, 0, 0, 0);
           const float4 P = make_float4(static_cast<float>(u.x) * out_scale, static_cast<float>(u.y) * out_scale,
                                        static_cast<float>(u.z) * out_scale, static_cast<float>(u.w) * out_scale);
           if (pat) store_nt(P, reinterpret_cast<float4 *>(pat + off) + i);
-          if (fwd_only) continue;
-          const native_float4 t =
-              __builtin_nontemporal_load(reinterpret_cast<const native_float4 *>(tgt + off) + i);
+          if (fwd_only) return;
           float4 g;
           if (l2) {
-            const float4 d = make_float4(P.x - t.x, P.y - t.y, P.z - t.z, P.w - t.w);
+            const float4 d = make_float4(P.x - tv.x, P.y - tv.y, P.z - tv.z, P.w - tv.w);
             lsum += static_cast<double>(d.x * d.x + d.y * d.y + d.z * d.z + d.w * d.w);
             g = make_float4(2.0f * d.x, 2.0f * d.y, 2.0f * d.z, 2.0f * d.w);
           } else {
-            g = make_float4(t.x, t.y, t.z, t.w);
+            g = make_float4(tv.x, tv.y, tv.z, tv.w);
           }
           csum += static_cast<double>(g.x * P.x + g.y * P.y + g.z * P.z + g.w * P.w);
           reinterpret_cast<float4 *>(s_G)[i] =
               make_float4(g.x * gscale, g.y * gscale, g.z * gscale, g.w * gscale);
+        };
+        const int n4 = cnt / 4;
+        if (kLarge || fwd_only) {
+          const native_float4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+          for (int i = tid; i < n4; i += kAdjThreads)
+            readout4(i, fwd_only ? zero
+                                 : __builtin_nontemporal_load(
+                                       reinterpret_cast<const native_float4 *>(tgt + off) + i));
+        } else {
+#pragma unroll
+          for (int k = 0; k < kPre; ++k) {
+            const int i = tid + k * kAdjThreads;
+            if (i < n4) readout4(i, tp[k]);
+          }
         }
       } else {
         for (int i = tid; i < cnt; i += kAdjThreads) {
@@ -556,19 +753,20 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
           const float P = static_cast<float>(v) * out_scale;
           if (pat) __builtin_nontemporal_store(P, pat + off + i);
           if (fwd_only) continue;
-          const float t = tgt[off + i];
+          const float tv = tgt[off + i];
           float g;
           if (l2) {
-            const float d = P - t;
+            const float d = P - tv;
             lsum += static_cast<double>(d * d);
             g = 2.0f * d;
           } else {
-            g = t;
+            g = tv;
           }
           csum += static_cast<double>(g * P);
           s_G[i] = g * gscale;
         }
       }
+      prefetch(s + 1);
       lds_sync();
       if (a.want_grad && any) strip_spots(s, true, y0, y1);
       lds_sync();
@@ -590,45 +788,112 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_kernel(AdjArgs a) 
       // ---- 5. per spot: the normalisation's max term, then the chain rule ----
       const float corr = norm ? static_cast<float>(ctot) * gscale : 0.0f;
       const int px = pstar % W, py = pstar / W;
-      double dR[9];
-#pragma unroll
-      for (int i = 0; i < 9; ++i) dR[i] = 0.0;
-      const int *idx = a.idx + (size_t)p * a.cap;
-      if (!large) {
+      if (!kLarge) {
+        // sums with the max term to gsum; spot_chain_kernel does the rest
+        float4 *gs = a.gsum + (size_t)p * a.slots;
         for (int q = tid; q < ns; q += kAdjThreads) {
-          float A = sh.A[q], BX = sh.BX[q], BY = sh.BY[q];
+          float A = t.A[q], BX = t.BX[q], BY = t.BY[q];
           if (norm)
-            max_term(a, corr, px, py, sh.X[q], sh.Y[q], sh.xb[q] >> 16, sh.xb[q] & 0xffff, sh.ylo[q],
-                     sh.yhi[q], A, BX, BY);
-          spot_chain(a, s_cc, sh.R, idx[sh.jpos[q]], sh.I[q], A, BX, BY, dR);
+            max_term(a, corr, px, py, t.X[q], t.Y[q], t.xb[q] >> 16, t.xb[q] & 0xffff, t.ylo[q],
+                     t.yhi[q], A, BX, BY);
+          gs[q] = make_float4(A, BX, BY, __int_as_float(t.jpos[q]));
         }
+        if (tid == 0) a.gcount[p] = ns;
       } else {
-        for (int j = tid; j < n; j += kAdjThreads) {
-          const float4 sp = spots[j];
-          float X, Y;
-          if (!to_pixel(sp, a, X, Y)) continue;
-          int lo, hi, xlo, xhi;
-          if (!subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, H, W, lo, hi, xlo, xhi)) continue;
-          float A = wacc[3 * j], BX = wacc[3 * j + 1], BY = wacc[3 * j + 2];
-          if (norm) max_term(a, corr, px, py, X, Y, xlo, xhi, lo, hi, A, BX, BY);
-          spot_chain(a, s_cc, sh.R, idx[j], sp.w, A, BX, BY, dR);
+        double dR[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) dR[i] = 0.0;
+        const int *idx = a.idx + (size_t)p * a.cap;
+        if (!large) {
+          for (int q = tid; q < ns; q += kAdjThreads) {
+            float A = t.A[q], BX = t.BX[q], BY = t.BY[q];
+            if (norm)
+              max_term(a, corr, px, py, t.X[q], t.Y[q], t.xb[q] >> 16, t.xb[q] & 0xffff, t.ylo[q],
+                       t.yhi[q], A, BX, BY);
+            spot_chain(a, s_cc, sh.R, idx[t.jpos[q]], t.I[q], A, BX, BY, dR);
+          }
+        } else {
+          for (int j = tid; j < n; j += kAdjThreads) {
+            const float4 sp = spots[j];
+            float X, Y;
+            if (!to_pixel(sp, a, X, Y)) continue;
+            int lo, hi, xlo, xhi;
+            if (!subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, H, W, lo, hi, xlo, xhi)) continue;
+            float A = wacc[3 * j], BX = wacc[3 * j + 1], BY = wacc[3 * j + 2];
+            if (norm) max_term(a, corr, px, py, X, Y, xlo, xhi, lo, hi, A, BX, BY);
+            spot_chain(a, s_cc, sh.R, idx[j], sp.w, A, BX, BY, dR);
+          }
         }
-      }
 #pragma unroll
-      for (int i = 0; i < 9; ++i) dR[i] = wave_sum_d(dR[i]);
-      if (lane == 0) {
+        for (int i = 0; i < 9; ++i) dR[i] = wave_sum_d(dR[i]);
+        if (lane == 0) {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) sh.red[wid][2 + i] = dR[i];
-      }
-      lds_sync();
-      if (tid < 9) {
-        double v = 0.0;
-        for (int w = 0; w < kAdjWaves; ++w) v += sh.red[w][2 + tid];
-        a.dRt[(size_t)p * 9 + tid] = v;
+          for (int i = 0; i < 9; ++i) sh.red[wid][2 + i] = dR[i];
+        }
+        lds_sync();
+        if (tid < 9) {
+          double v = 0.0;
+          for (int w = 0; w < kAdjWaves; ++w) v += sh.red[w][2 + tid];
+          a.dRt[(size_t)p * 9 + tid] = v;
+        }
       }
     }
     if (tid == 0 && a.loss) a.loss[p] = static_cast<float>(ltot);
-    __syncthreads();  // LDS and workspace state of this pattern is dead before the next one
+    if (kLarge)
+      __syncthreads();  // LDS and workspace state of this pattern is dead before the next one
+    else
+      lds_sync();  // LDS state of this pattern is dead before the next one
+  }
+}
+
+// Two blocks per CU: at most 128 VGPRs (four waves per SIMD).
+__global__ __launch_bounds__(kAdjThreads, 4) void render_adjoint_kernel(AdjArgs a) {
+  extern __shared__ __align__(16) aacc_t s_acc[];  // 16-byte aligned for the 16-byte read-out
+  __shared__ AdjShared sh;
+  __shared__ ChainConsts s_cc;  // unused here
+  adjoint_body<false>(a, s_acc, sh, s_cc);
+}
+
+__global__ __launch_bounds__(kAdjThreads) void render_adjoint_large_kernel(AdjArgs a) {
+  extern __shared__ __align__(16) aacc_t s_acc[];
+  __shared__ AdjShared sh;
+  __shared__ ChainConsts s_cc;
+  adjoint_body<true>(a, s_acc, sh, s_cc);
+}
+
+// Chain rule of every spot gathered by render_adjoint_kernel: one wave per
+// pattern; dL/dRt of the pattern (a wave sum) and dL/dg, dL/d|F|^2 of the
+// reflections (global atomics).
+constexpr int kChainThreads = 256;
+
+__global__ __launch_bounds__(kChainThreads) void spot_chain_kernel(AdjArgs a) {
+  __shared__ ChainConsts s_cc;
+  if (threadIdx.x == 0) stage_chain_consts(a, s_cc);
+  __syncthreads();
+  const int lane = lane_id();
+  const int waves = gridDim.x * (kChainThreads / kWave);
+  for (int p = blockIdx.x * (kChainThreads / kWave) + threadIdx.x / kWave; p < a.n; p += waves) {
+    double R[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) R[i] = a.Rt[(size_t)p * 9 + i];
+    double dR[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dR[i] = 0.0;
+    const int ns = a.gcount[p];
+    const float4 *gs = a.gsum + (size_t)p * a.slots;
+    const int *idx = a.idx + (size_t)p * a.cap;
+    const float4 *spots = a.spots + (size_t)p * a.cap;
+    for (int q = lane; q < ns; q += kWave) {
+      const float4 v = gs[q];
+      const int j = __float_as_int(v.w);
+      spot_chain(a, s_cc, R, idx[j], spots[j].w, v.x, v.y, v.z, dR);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dR[i] = wave_sum_d(dR[i]);
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < 9; ++i) a.dRt[(size_t)p * 9 + i] = dR[i];
+    }
   }
 }
 
@@ -801,12 +1066,41 @@ __global__ __launch_bounds__(kSfaThreads) void structure_factor_adjoint_kernel(S
 
 using namespace dsx;
 
-// Workspace layout per block: int bins[kWorkEntriesPerSpot * cap] | float sums[3 * cap].
+// Workspace of the large-pattern kernel, per block: int bins[kWorkEntriesPerSpot
+// * cap] | float sums[3 * cap].  Of the small-pattern kernel with gradients:
+// float4 gsum[n * cap] | int gcount[n].
 constexpr int kWorkEntriesPerSpot = 4;
+constexpr int kAdjLdsBytes = 160 * 1024;
 
-extern "C" long long dsx_render_adjoint_workspace_ints(int cap) {
-  if (cap < 1) return 0;
-  return static_cast<long long>(cu_count()) * (kWorkEntriesPerSpot + 3) * static_cast<long long>(cap);
+static bool adjoint_large(int cap) { return cap > kAdjMaxSpots; }
+
+extern "C" long long dsx_render_adjoint_workspace_ints(int cap, int n) {
+  if (cap < 1 || n < 0) return 0;
+  if (adjoint_large(cap))
+    return static_cast<long long>(cu_count()) * (kWorkEntriesPerSpot + 3) * static_cast<long long>(cap);
+  return static_cast<long long>(n) * (4ll * cap + 1);
+}
+
+// Static LDS of a kernel (queried once) and its dynamic LDS limit raised to
+// what is left of the CU's 160 KB.
+static int adjoint_static_lds(const void *kernel, int *cache, int *raised) {
+  if (*cache < 0) {
+    hipFuncAttributes attr;
+    if (hipFuncGetAttributes(&attr, kernel) != hipSuccess) {
+      (void)hipGetLastError();
+      return -1;
+    }
+    *cache = static_cast<int>(attr.sharedSizeBytes);
+  }
+  if (!*raised) {
+    if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            kAdjLdsBytes - *cache) != hipSuccess) {
+      (void)hipGetLastError();
+      return -1;
+    }
+    *raised = 1;
+  }
+  return *cache;
 }
 
 extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const int *count_d, int cap,
@@ -875,21 +1169,6 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   a.ep = static_cast<float>(-1.0 / (2.0 * sg * sg));
   a.inv_s2 = static_cast<float>(1.0 / (sg * sg));
   a.clip = rp->clip_threshold;
-  const int budget = kAdjStripBytes / static_cast<int>(sizeof(aacc_t) + sizeof(float));
-  if (a.W > budget) {
-    set_error("dsx_render_adjoint: pattern width %d exceeds the LDS strip budget", a.W);
-    return DSX_EINVAL;
-  }
-  const int rows = std::max(1, std::min(a.H, budget / a.W));
-  const int n_strips = (a.H + rows - 1) / rows;
-  if (n_strips > kAdjMaxStrips) {
-    set_error("dsx_render_adjoint: pattern height %d needs %d strips (> %d)", a.H, n_strips,
-              kAdjMaxStrips);
-    return DSX_EINVAL;
-  }
-  a.strip_rows = rows;
-  a.n_strips = n_strips;
-  a.acc_words = rows * a.W;
   a.mode = mode;
   a.want_grad = want_grad ? 1 : 0;
   a.target = target_d;
@@ -899,34 +1178,92 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   a.dRt = dRt_d;
   a.dtab = dtab_d;
   a.status = status_d;
-  const int blocks = std::min(n, cu_count());
-  const long long per_block = (kWorkEntriesPerSpot + 3) * static_cast<long long>(cap);
-  if (work_d && work_ints >= per_block * blocks && kWorkEntriesPerSpot * static_cast<long long>(cap) < (1ll << 31)) {
-    a.work = work_d;
-    a.work_per_block = per_block;
-    a.work_entries = kWorkEntriesPerSpot * cap;
-  } else if (work_d) {
-    set_error("dsx_render_adjoint: workspace of %lld ints is smaller than %lld "
-              "(dsx_render_adjoint_workspace_ints)", work_ints, per_block * blocks);
+  a.work = nullptr;
+  a.work_per_block = 0;
+  a.work_entries = 0;
+  a.gsum = nullptr;
+  a.gcount = nullptr;
+
+  // LDS per block: the static part, the spot table (slots entries; bins) and
+  // two strip buffers of rows x W words.  Patterns up to kAdjMaxSpots spots
+  // use two blocks per CU when their strips hold at least 8 rows (or the whole
+  // pattern); larger lists use the large-pattern kernel, one block per CU.
+  const bool large = adjoint_large(cap);
+  static int static_small = -1, static_large = -1, raised_small = 0, raised_large = 0;
+  const void *kern = large ? reinterpret_cast<const void *>(&render_adjoint_large_kernel)
+                           : reinterpret_cast<const void *>(&render_adjoint_kernel);
+  const int st = large ? adjoint_static_lds(kern, &static_large, &raised_large)
+                       : adjoint_static_lds(kern, &static_small, &raised_small);
+  if (st < 0) {
+    set_error("dsx_render_adjoint: cannot query or reserve the kernel's LDS");
+    return DSX_ELAUNCH;
+  }
+  a.slots = large ? kAdjMaxSpots : cap;
+  a.bin_cap = large ? kAdjBinCap : std::min(kAdjBinCap, 4 * cap + 256);
+  const long long table = static_cast<long long>(kSpotTabBytes) * a.slots + 2ll * a.bin_cap + 16;
+  auto rows_for = [&](int per_block) {
+    const long long strip =
+        std::min<long long>(large ? kAdjStripBytes : kAdjSmallStripBytes, per_block - st - table);
+    return strip < 8ll * a.W ? 0 : static_cast<int>(std::min<long long>(a.H, strip / (8ll * a.W)));
+  };
+  int bpc = large ? 1 : 2;
+  int rows = rows_for(kAdjLdsBytes / bpc);
+  if (bpc == 2 && rows < std::min(a.H, 8)) {
+    bpc = 1;
+    rows = rows_for(kAdjLdsBytes);
+  }
+  if (rows < 1) {
+    set_error("dsx_render_adjoint: pattern width %d exceeds the LDS strip budget", a.W);
     return DSX_EINVAL;
-  } else {
-    a.work = nullptr;
-    a.work_per_block = 0;
-    a.work_entries = 0;
   }
-  const size_t lds = static_cast<size_t>(a.acc_words) * (sizeof(aacc_t) + sizeof(float));
-  static int reserved = -1;  // raised once, to the largest strip this build uses
-  if (reserved < 0) {
-    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&render_adjoint_kernel),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, kAdjStripBytes) != hipSuccess) {
-      (void)hipGetLastError();
-      set_error("dsx_render_adjoint: cannot reserve %zu bytes of LDS", lds);
-      return DSX_ELAUNCH;
+  const int n_strips = (a.H + rows - 1) / rows;
+  if (n_strips > kAdjMaxStrips) {
+    set_error("dsx_render_adjoint: pattern height %d needs %d strips (> %d)", a.H, n_strips,
+              kAdjMaxStrips);
+    return DSX_EINVAL;
+  }
+  a.strip_rows = rows;
+  a.n_strips = n_strips;
+  a.acc_words = rows * a.W;
+  const int blocks = std::min(n, cu_count() * bpc);
+  if (large) {
+    const long long per_block = (kWorkEntriesPerSpot + 3) * static_cast<long long>(cap);
+    if (work_d && work_ints >= per_block * blocks &&
+        kWorkEntriesPerSpot * static_cast<long long>(cap) < (1ll << 31)) {
+      a.work = work_d;
+      a.work_per_block = per_block;
+      a.work_entries = kWorkEntriesPerSpot * cap;
+    } else if (work_d) {
+      set_error("dsx_render_adjoint: workspace of %lld ints is smaller than %lld "
+                "(dsx_render_adjoint_workspace_ints)", work_ints, per_block * blocks);
+      return DSX_EINVAL;
     }
-    reserved = 1;
+  } else if (want_grad) {
+    const long long need = static_cast<long long>(n) * (4ll * cap + 1);
+    if (!work_d || work_ints < need) {
+      set_error("dsx_render_adjoint: gradients need a workspace of %lld ints, got %lld "
+                "(dsx_render_adjoint_workspace_ints)", need, work_d ? work_ints : 0ll);
+      return DSX_EINVAL;
+    }
+    if (reinterpret_cast<uintptr_t>(work_d) % 16 != 0) {
+      set_error("dsx_render_adjoint: workspace must be 16-byte aligned");
+      return DSX_EINVAL;
+    }
+    a.gsum = reinterpret_cast<float4 *>(work_d);
+    a.gcount = work_d + static_cast<size_t>(n) * 4 * cap;
   }
-  hipLaunchKernelGGL(render_adjoint_kernel, dim3(blocks), dim3(kAdjThreads), lds, as_stream(stream), a);
-  return check_launch("dsx_render_adjoint");
+  const size_t lds = ((2ull * a.acc_words * 4 + static_cast<size_t>(table)) + 15) & ~static_cast<size_t>(15);
+  hipStream_t s = as_stream(stream);
+  if (large) {
+    hipLaunchKernelGGL(render_adjoint_large_kernel, dim3(blocks), dim3(kAdjThreads), lds, s, a);
+    return check_launch("dsx_render_adjoint");
+  }
+  hipLaunchKernelGGL(render_adjoint_kernel, dim3(blocks), dim3(kAdjThreads), lds, s, a);
+  int rc = check_launch("dsx_render_adjoint");
+  if (rc != DSX_OK || !want_grad) return rc;
+  const int cblocks = std::min((n + kChainThreads / kWave - 1) / (kChainThreads / kWave), cu_count() * 8);
+  hipLaunchKernelGGL(spot_chain_kernel, dim3(cblocks), dim3(kChainThreads), 0, s, a);
+  return check_launch("dsx_render_adjoint (chain)");
 }
 
 extern "C" int dsx_structure_factor_adjoint(const int *hkl_d, int n, const double rec[9],

@@ -335,12 +335,11 @@ class RefinementModel:
     def _sim_params(self, t: _Tables) -> _lib.SimParams:
         return self.settings.params(t.g_max, t.rec)
 
-    def workspace(self, cap: int) -> Optional[torch.Tensor]:
-        """Scratch for patterns with more in-frame spots than the render LDS
-        holds (None when the spot capacity cannot exceed it)."""
-        if cap <= int(self.L.dsx_render_max_spots()):
-            return None
-        need = int(self.L.dsx_render_adjoint_workspace_ints(int(cap)))
+    def workspace(self, cap: int, n: int) -> torch.Tensor:
+        """Scratch of dsx_render_adjoint: per-spot sums between its render and
+        chain-rule passes, or, for spot lists longer than the render LDS
+        holds, per-block strip bins."""
+        need = max(1, int(self.L.dsx_render_adjoint_workspace_ints(int(cap), int(n))))
         if getattr(self, "_work", None) is None or self._work.numel() < need:
             self._work = torch.empty(need, dtype=torch.int32, device=self.dev)
         return self._work
@@ -369,7 +368,7 @@ class RefinementModel:
         sync_status = status is None
         if status is None:
             status = torch.zeros(n, dtype=torch.int32, device=self.dev)
-        work = self.workspace(batch.cap)
+        work = self.workspace(batch.cap, n)
         sp = self._sim_params(t)
         stream = _lib.stream_handle(self.dev)
         _lib.check(self.L.dsx_render_adjoint(

@@ -289,17 +289,20 @@ int dsx_render_fast_ragged(const float *spot_d, const long long *spot_off_d, con
  *                                ACCUMULATED (zero it first)
  * dRt_d and dtab_d may both be NULL (loss / pattern only).  pattern_d
  * (float [n][H][W], may be NULL) receives P; loss_d (float [n], may be NULL).
- * Patterns with more than dsx_render_max_spots() in-frame spots need the
- * workspace work_d of dsx_render_adjoint_workspace_ints(cap) ints (may be NULL
- * otherwise); without it, or when it is too small for a pattern, that
- * pattern's status_d entry is 1.  At most 2^22 pixels per pattern.
+ * work_d is a 16-byte aligned workspace of dsx_render_adjoint_workspace_ints(
+ * cap, n) ints.  With cap <= dsx_render_max_spots() it is needed only for
+ * gradients (per-spot sums handed from the render pass to the chain-rule pass;
+ * EINVAL without it) and every status_d entry is 0.  With a larger cap it
+ * holds the strip bins of patterns with more in-frame spots than the LDS
+ * holds; without it, or when it is too small for a pattern, that pattern's
+ * status_d entry is 1.  At most 2^22 pixels per pattern.
  * This entry point also serves as the sub-pixel renderer for such large
  * patterns (mode DSX_ADJ_FORWARD).
  */
 #define DSX_ADJ_L2 0
 #define DSX_ADJ_PATTERN_GRAD 1
 #define DSX_ADJ_FORWARD 2
-long long dsx_render_adjoint_workspace_ints(int cap);
+long long dsx_render_adjoint_workspace_ints(int cap, int n);
 int dsx_render_adjoint(const int *idx_d, const float *spot_d, const int *count_d, int cap,
                        int n, const int *hkl_d, const float *F2_d, int n_refl,
                        const double *Rt_d, const dsx_sim_params *sim,

@@ -235,9 +235,9 @@ def test_large_pattern_subpixel_render_and_gradients():
 
 
 def test_forward_mode_equals_subpixel_renderer():
-    """dsx_render_adjoint's forward mode (64-bit fixed point) and
-    dsx_render_subpixel (32-bit fixed point) agree to 1e-6 of the pattern
-    maximum on patterns both can hold."""
+    """dsx_render_adjoint's forward mode and dsx_render_subpixel (each with
+    its own fixed-point scale) agree to 1e-6 of the pattern maximum on
+    patterns both can hold."""
     import ctypes
 
     from diffsims_amd import _lib, engine
