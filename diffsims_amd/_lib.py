@@ -50,6 +50,7 @@ EXPORTED = (
     "dsx_hkl_workspace_ints",
     "dsx_hkl_enumerate",
     "dsx_structure_factor",
+    "dsx_stage_atoms",
     "dsx_simulate",
     "dsx_render_max_spots",
     "dsx_render_fast",
@@ -134,6 +135,7 @@ _SIGS = {
                                ctypes.c_double, _P, _I, _P, _P, _P]),
     "dsx_structure_factor": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _P, _I,
                                   _P, _I, _I, _P, _P, _P, _P]),
+    "dsx_stage_atoms": (_I, [_P, _P, _P, _I, _P, _P, _P]),
     "dsx_simulate": (_I, [_P, _P, _P, _I, _P, _I, ctypes.POINTER(SimParams), _I, _P, _P, _P,
                           _P, _P, _P]),
     "dsx_render_max_spots": (_I, []),

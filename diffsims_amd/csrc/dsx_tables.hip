@@ -271,6 +271,42 @@ extern "C" int dsx_hkl_enumerate(const int highest[3], const double metric[6], d
   return check_launch("dsx_hkl_enumerate");
 }
 
+// Atom staging for dsx_structure_factor from device-resident parameters:
+// row i takes atom order[i]; fractional position -> uint32 turns
+// round((f - floor(f)) * 2^32) mod 2^32 (rint: half to even), B -> float.
+__global__ void stage_atoms_kernel(const double *xyz, const double *B, const long long *order, int n,
+                                   uint4 *fx, float *Bs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long j = order[i];
+  uint32_t u[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double f = xyz[3 * j + c];
+    const double r = rint((f - floor(f)) * 4294967296.0);
+    u[c] = static_cast<uint32_t>(static_cast<unsigned long long>(r) & 0xFFFFFFFFull);
+  }
+  fx[i] = make_uint4(u[0], u[1], u[2], 0u);
+  Bs[i] = static_cast<float>(B[j]);
+}
+
+extern "C" int dsx_stage_atoms(const double *xyz_d, const double *B_d, const long long *order_d,
+                               int n_atoms, uint32_t *xyz_fx_d, float *B_out_d, void *stream) {
+  clear_error();
+  if (n_atoms < 0) {
+    set_error("dsx_stage_atoms: bad atom count %d", n_atoms);
+    return DSX_EINVAL;
+  }
+  if (n_atoms == 0) return DSX_OK;
+  if (!xyz_d || !B_d || !order_d || !xyz_fx_d || !B_out_d) {
+    set_error("dsx_stage_atoms: null device pointer");
+    return DSX_EINVAL;
+  }
+  hipLaunchKernelGGL(stage_atoms_kernel, dim3((n_atoms + 255) / 256), dim3(256), 0, as_stream(stream),
+                     xyz_d, B_d, order_d, n_atoms, reinterpret_cast<uint4 *>(xyz_fx_d), B_out_d);
+  return check_launch("dsx_stage_atoms");
+}
+
 extern "C" int dsx_structure_factor(const int *hkl_d, int n, const double rec[9],
                                     const uint32_t *xyz_fx_d, const int *elem_d,
                                     const float *occ_d, const float *B_d, int n_atoms,

@@ -226,15 +226,6 @@ class _Tables:
     g_max: float
 
 
-def _fixed_point_t(xyz: torch.Tensor) -> torch.Tensor:
-    """float64 fractional positions -> uint32 turns (as int32 bits), on device."""
-    f = xyz.detach().to(torch.float64)
-    f = f - torch.floor(f)
-    u = torch.round(f * 4294967296.0).to(torch.int64) & 0xFFFFFFFF
-    u = torch.where(u >= 2 ** 31, u - 2 ** 32, u)
-    return u.to(torch.int32)
-
-
 class RefinementModel:
     """Fixed part of a differentiable simulation: one phase's elements and
     occupancies, the hkl set (from the initial lattice and
@@ -294,11 +285,20 @@ class RefinementModel:
     def tables(self, rec: torch.Tensor, xyz: torch.Tensor, B: torch.Tensor) -> _Tables:
         """g vectors and |F|^2 of the fixed hkl set for the given parameters."""
         rec_np = np.ascontiguousarray(rec.detach().cpu().numpy(), dtype=np.float64).reshape(3, 3)
-        xs = xyz.detach().to(self.dev, torch.float64)[self.order_dev]
-        fx = torch.zeros((max(self.n_atoms, 1), 4), dtype=torch.int32, device=self.dev)
-        if self.n_atoms:
-            fx[:self.n_atoms, :3] = _fixed_point_t(xs)
-        Bs = B.detach().to(self.dev, torch.float32)[self.order_dev].contiguous()
+        stream = _lib.stream_handle(self.dev)
+        na = self.n_atoms
+        if na:
+            # element order, uint32 turns and float B in one launch
+            fx = torch.empty((na, 4), dtype=torch.int32, device=self.dev)
+            Bs = torch.empty(na, dtype=torch.float32, device=self.dev)
+            xs = xyz.detach().to(self.dev, torch.float64).contiguous()
+            Bd = B.detach().to(self.dev, torch.float64).contiguous()
+            _lib.check(self.L.dsx_stage_atoms(_lib.ptr(xs), _lib.ptr(Bd), _lib.ptr(self.order_dev),
+                                              na, _lib.ptr(fx), _lib.ptr(Bs), stream),
+                       "dsx_stage_atoms")
+        else:
+            fx = torch.zeros((1, 4), dtype=torch.int32, device=self.dev)
+            Bs = torch.zeros(1, dtype=torch.float32, device=self.dev)
         N = self.n_refl
         g4 = torch.empty((N, 4), dtype=torch.float32, device=self.dev)
         F2 = torch.empty(N, dtype=torch.float32, device=self.dev)
@@ -308,7 +308,7 @@ class RefinementModel:
             _lib.ptr(self.hkl), N, rec9, _lib.ptr(fx), _lib.ptr(self.elem), _lib.ptr(self.occ),
             _lib.ptr(Bs), self.n_atoms, _lib.ptr(self.coeff), self.n_elem,
             engine.SCATTER_IDS[self.scatter], _lib.ptr(g4), _lib.ptr(F2), _lib.ptr(F),
-            _lib.stream_handle(self.dev)), "dsx_structure_factor")
+            stream), "dsx_structure_factor")
         g_max = float(np.sqrt(np.max(np.sum((self.hkl_np @ rec_np.T) ** 2, axis=1)))) if N else 0.0
         return _Tables(self.hkl, g4, F2, F, fx, Bs, rec_np, g_max)
 
@@ -362,9 +362,14 @@ class RefinementModel:
                 raise ValueError(f"target must be [n, {H}, {W}] or [{H}, {W}]")
         loss = torch.empty(n, dtype=torch.float32, device=self.dev)
         dRt = dtab = None
+        na = max(self.n_atoms, 1)
         if want_grad:
             dRt = torch.empty((n, 9), dtype=torch.float64, device=self.dev)
-            dtab = torch.zeros((self.n_refl, 4), dtype=torch.float32, device=self.dev)
+            # the accumulated gradients, zeroed by one fill:
+            # dtab [n_refl, 4] | dxyz [na, 3] | dB [na] (float32) ; drec [9] (float64)
+            acc = torch.zeros(4 * self.n_refl + 4 * na, dtype=torch.float32, device=self.dev)
+            dtab = acc[:4 * self.n_refl].view(self.n_refl, 4)
+            drec = torch.zeros(9, dtype=torch.float64, device=self.dev)
         sync_status = status is None
         if status is None:
             status = torch.zeros(n, dtype=torch.int32, device=self.dev)
@@ -379,11 +384,11 @@ class RefinementModel:
             0 if work is None else work.numel(), _lib.ptr(status), stream), "dsx_render_adjoint")
         if sync_status:
             engine.check_render_status(status)
-        dxyz = dB = drec = None
+        dxyz = dB = None
         if want_grad:
-            dxyz_s = torch.zeros((max(self.n_atoms, 1), 3), dtype=torch.float32, device=self.dev)
-            dB_s = torch.zeros(max(self.n_atoms, 1), dtype=torch.float32, device=self.dev)
-            drec = torch.zeros(9, dtype=torch.float64, device=self.dev)
+            o = 4 * self.n_refl
+            dxyz_s = acc[o:o + 3 * na].view(na, 3)
+            dB_s = acc[o + 3 * na:o + 4 * na]
             rec9 = (ctypes.c_double * 9)(*t.rec.reshape(-1).tolist())
             _lib.check(self.L.dsx_structure_factor_adjoint(
                 _lib.ptr(t.hkl), self.n_refl, rec9, _lib.ptr(t.xyz_fx), _lib.ptr(self.elem),
@@ -392,13 +397,16 @@ class RefinementModel:
                 _lib.ptr(drec), stream), "dsx_structure_factor_adjoint")
             dxyz = dxyz_s[:self.n_atoms][self.inv_order_dev]
             dB = dB_s[:self.n_atoms][self.inv_order_dev]
+        else:
+            drec = None
         return loss, dRt, dtab, dxyz, dB, drec
 
     # -- public differentiable functions --------------------------------------
 
     def l2_loss(self, Rt: torch.Tensor, rec: torch.Tensor, xyz: torch.Tensor, B: torch.Tensor,
                 target: torch.Tensor) -> torch.Tensor:
-        """sum_patterns sum_pixels (P - T)^2 as a float64 scalar on the device."""
+        """sum_patterns sum_pixels (P - T)^2 as a float64 CPU scalar (fetched
+        with the step's one synchronisation; the backward then needs none)."""
         return _L2Loss.apply(Rt, rec, xyz, B, self, target)
 
     def patterns(self, Rt: torch.Tensor, rec: torch.Tensor, xyz: torch.Tensor,
@@ -430,28 +438,56 @@ def _grow_if_needed(model: "RefinementModel", batch: engine.SpotBatch, status: t
     return False
 
 
+def _fetch_step(batch: engine.SpotBatch, status: torch.Tensor, loss: torch.Tensor,
+                drec: Optional[torch.Tensor]) -> np.ndarray:
+    """The step's one device->host copy: [sum of the losses, max ncand, max
+    status, dL/drec (9, when given)] as float64."""
+    parts = [loss.sum(dtype=torch.float64).view(1),
+             torch.stack([batch.ncand.max(), status.max()]).to(torch.float64)]
+    if drec is not None:
+        parts.append(drec)
+    return torch.cat(parts).cpu().numpy()
+
+
 class _L2Loss(torch.autograd.Function):
+    """Forward and backward in one pass (the adjoint runs with the loss); the
+    loss is returned as a CPU float64 scalar fetched with the step's single
+    synchronisation, so the backward needs none."""
+
     @staticmethod
     def forward(ctx, Rt, rec, xyz, B, model: RefinementModel, target):
         want = any(ctx.needs_input_grad[:4])
         t = model.tables(rec, xyz, B)
         while True:
             batch = model.spots(t, Rt, check=False)
-            status = torch.zeros(batch.n, dtype=torch.int32, device=model.dev)
+            if batch.n == 0:
+                host, dRt, dxyz, dB = np.zeros(12), Rt.detach().new_zeros(0, 9), None, None
+                if want:
+                    dxyz = torch.zeros_like(xyz.detach(), dtype=torch.float32)
+                    dB = torch.zeros_like(B.detach(), dtype=torch.float32)
+                break
+            status = torch.empty(batch.n, dtype=torch.int32, device=model.dev)  # every entry written
             loss, dRt, _, dxyz, dB, drec = model.adjoint(t, Rt.detach(), batch, _lib.ADJ_L2, target,
                                                          want, status=status)
-            if not _grow_if_needed(model, batch, status):
+            host = _fetch_step(batch, status, loss, drec if want else None)
+            if int(host[2]) != 0:
+                engine.check_render_status(status)
+            if int(host[1]) <= batch.cap:
                 break
+            model.cap = int(host[1]) + 64
         if want:
-            g = _grads_like((Rt, rec, xyz, B), dRt, drec, dxyz, dB)
+            drec_h = torch.from_numpy(np.array(host[3:12]))
+            g = _grads_like((Rt, rec, xyz, B), dRt, drec_h, dxyz, dB)
             ctx.save_for_backward(*g)
-        return loss.to(torch.float64).sum()
+        return torch.tensor(float(host[0]), dtype=torch.float64)
 
     @staticmethod
     def backward(ctx, gl):
         dRt, drec, dxyz, dB = ctx.saved_tensors
-        return (gl.to(dRt.device) * dRt, gl.to(drec.device) * drec, gl.to(dxyz.device) * dxyz,
-                gl.to(dB.device) * dB, None, None)
+        s = float(gl)  # a CPU scalar: no synchronisation
+        if s == 1.0:
+            return dRt, drec, dxyz, dB, None, None
+        return dRt * s, drec * s, dxyz * s, dB * s, None, None
 
 
 class _Patterns(torch.autograd.Function):

@@ -101,6 +101,19 @@ int dsx_structure_factor(const int *hkl_d, int n, const double rec[9],
                          float *g4_d, float *F2_d, float *F_d, void *stream);
 
 /*
+ * Atom staging for dsx_structure_factor when positions and Debye-Waller
+ * factors live on the device (gradient-based refinement; no counterpart in
+ * the reference, whose Phase keeps them on the host).  Row i of the outputs
+ * takes atom order_d[i] (int64 [n_atoms]):
+ *   xyz_fx_d[4 i + c] = round((f - floor(f)) * 2^32) mod 2^32 (half to even)
+ *                       of xyz_d[3 order + c] (double fractional), c < 3; the
+ *                       fourth word is 0
+ *   B_out_d[i]        = (float) B_d[order] (A^2)
+ */
+int dsx_stage_atoms(const double *xyz_d, const double *B_d, const long long *order_d,
+                    int n_atoms, uint32_t *xyz_fx_d, float *B_out_d, void *stream);
+
+/*
  * Simulation parameters (SimulationGenerator.__init__ / calculate_diffraction2d
  * arguments, diffsims/generators/simulation_generator.py:74-146).
  */
