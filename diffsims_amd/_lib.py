@@ -143,7 +143,7 @@ _SIGS = {
     "dsx_render_subpixel": (_I, [_P, _P, _I, _I, ctypes.POINTER(RenderParams), _P, _P, _P]),
     "dsx_render_adjoint": (_I, [_P, _P, _P, _I, _I, _P, _P, _I, _P, ctypes.POINTER(SimParams),
                                 ctypes.POINTER(RenderParams), _I, _P, ctypes.c_longlong, _P, _P,
-                                _P, _P, _P, ctypes.c_longlong, _P, _P]),
+                                _P, _P, _P, _P, ctypes.c_longlong, _P, _P]),
     "dsx_render_adjoint_workspace_ints": (ctypes.c_longlong, [_I, _I]),
     "dsx_structure_factor_adjoint": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _P,
                                           _I, _P, _I, _I, _P, _P, _P, _P, _P]),

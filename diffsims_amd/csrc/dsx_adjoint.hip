@@ -5,22 +5,29 @@
 // of dsx_simulate + dsx_render_subpixel + dsx_structure_factor.
 //
 // render_adjoint_kernel (one block per pattern, persistent over patterns)
-//   recomputes the pattern exactly as render_kernel<subpixel> does (same
-//   spots, clip boxes, 64-bit fixed-point LDS strips), so it needs no stored
-//   pattern:
-//     pass A  pattern maximum m and its first pixel p* (normalize);
+//   recomputes the pattern as render_kernel<subpixel> does (same spots and
+//   clip boxes; 32-bit fixed-point LDS strips), so it needs no stored pattern:
+//     pass A  pattern maximum m and its first pixel p* (normalize): exact
+//             evaluation of the few 16x16 tiles whose bound can reach the
+//             value at the brightest spot (a full pass when too many remain);
 //     pass B  per strip: P = Q/m, then the upstream gradient G = dL/dP
 //             (2 (P - T) for the L2 loss, or given), kept in LDS as G/m;
 //             each spot gathers A = sum G/m K, B = sum G/m K (x - X) over its
 //             box (the transpose of the splat: no atomics on pixels);
-//     then    the max term of the normalisation, -(sum_q G P)/m at p*;
-//             per spot the chain through pixel position, intensity
-//             I = |F|^2 f(s, r) and the excitation error (recomputed from the
-//             integer hkl, the orientation and the reciprocal basis), giving
-//             dL/dRt for the pattern and dL/dg, dL/d|F|^2 per reflection.
+//     then    the max term of the normalisation, -(sum_q G P)/m at p*, and
+//             the per-spot sums go to a workspace.
+// spot_chain_kernel (one wave per pattern)
+//   per spot the chain through pixel position, intensity I = |F|^2 f(s, r)
+//   and the excitation error (recomputed in FP64 from the integer hkl, the
+//   orientation and the reciprocal basis): dL/dRt of the pattern, dL/d|F|^2
+//   per reflection (atomics) and dL/drec through g (block partials, summed by
+//   sum_partials_kernel in a fixed order).
+// render_adjoint_large_kernel: the same for spot lists longer than the LDS
+//   spot table (their bins live in a per-block workspace and the spots pass
+//   through LDS in chunks), with the chain rule in the kernel.
 // structure_factor_adjoint_kernel (one thread per reflection)
-//   recomputes F(hkl) and reduces dL/d|F|^2 into dL/dxyz and dL/dB per atom and,
-//   with dL/dg, into dL/d(rec).
+//   recomputes F(hkl) and reduces dL/d|F|^2 into dL/dxyz and dL/dB per atom
+//   and into dL/d(rec) through |g|.
 #include <algorithm>
 
 #include "dsx_internal.h"
@@ -39,7 +46,7 @@ constexpr int kAdjBinCap = 4096;
 #endif
 constexpr int kAdjStripBytes = DSX_ADJ_STRIP_KB * 1024;
 constexpr int kPixBits = 22;  // argmax key: value << 22 | (2^22 - 1 - pixel)
-constexpr int kNRed = 11;     // block sums: loss, sum G P, dRt[9]
+constexpr int kNRed = 20;     // block sums: loss, sum G P, dRt[9], dL/drec[9]
 constexpr int kAdjTile = 16;      // tile edge of the fixed-point bound
 constexpr int kAdjMaxTiles = 1024;
 constexpr int kAdjMaxCand = 16;    // candidate tiles of the maximum search
@@ -91,7 +98,9 @@ struct AdjArgs {
   float *pattern;
   float *loss;
   double *dRt;
-  float *dtab;
+  float *dF2;    // [n_refl] dL/d|F|^2, accumulated (may be NULL)
+  double *drec;  // [9] sum dL/dg (x) hkl, accumulated (may be NULL)
+  double *dpart; // chain kernel: per-block partial sums of drec
   int *status;
 };
 
@@ -107,6 +116,7 @@ struct AdjShared {
   int cand_tile[kAdjMaxCand], cand_n[kAdjMaxCand];
   double red[kAdjWaves][kNRed];
   double R[9];
+  double drec[9];  // large-pattern kernel: this block's dL/drec over its patterns
 };
 
 // The spot table lives in dynamic LDS after the two strip buffers (acc_words
@@ -253,10 +263,11 @@ __device__ __forceinline__ void load_chunk(const AdjArgs &a, const SpotTab &t, c
 }
 
 // Chain rule of one spot: (dL/dI, dL/dX, dL/dY) -> dL/dRt of the pattern (dR,
-// accumulated) and dL/dg, dL/d|F|^2 of its reflection (global atomics).
+// accumulated), dL/dg (x) hkl of its reflection (dG, accumulated when a.drec)
+// and dL/d|F|^2 of its reflection (a global atomic when a.dF2).
 __device__ __forceinline__ void spot_chain(const AdjArgs &a, const ChainConsts &c, const double *R,
                                            int row, double I, double dI, double BX, double BY,
-                                           double *dR) {
+                                           double *dR, double *dG) {
   const double dX = I * BX * a.inv_s2;
   const double dY = I * BY * a.inv_s2;
   if (dI == 0.0 && dX == 0.0 && dY == 0.0) return;
@@ -300,11 +311,16 @@ __device__ __forceinline__ void spot_chain(const AdjArgs &a, const ChainConsts &
   dR[0] += gx * g0; dR[1] += gx * g1; dR[2] += gx * g2;
   dR[3] += gy * g0; dR[4] += gy * g1; dR[5] += gy * g2;
   dR[6] += gz * g0; dR[7] += gz * g1; dR[8] += gz * g2;
-  float *dt = a.dtab + (size_t)row * 4;
-  unsafeAtomicAdd(dt + 0, static_cast<float>(R[0] * gx + R[3] * gy + R[6] * gz));
-  unsafeAtomicAdd(dt + 1, static_cast<float>(R[1] * gx + R[4] * gy + R[7] * gz));
-  unsafeAtomicAdd(dt + 2, static_cast<float>(R[2] * gx + R[5] * gy + R[8] * gz));
-  unsafeAtomicAdd(dt + 3, static_cast<float>(dF2));
+  if (a.drec) {
+    // dL/dg (crystal frame) = Rt^T dL/dg_lab; g = rec hkl: dL/drec += dL/dg hkl^T
+    const double d0 = R[0] * gx + R[3] * gy + R[6] * gz;
+    const double d1 = R[1] * gx + R[4] * gy + R[7] * gz;
+    const double d2 = R[2] * gx + R[5] * gy + R[8] * gz;
+    dG[0] += d0 * hx; dG[1] += d0 * hy; dG[2] += d0 * hz;
+    dG[3] += d1 * hx; dG[4] += d1 * hy; dG[5] += d1 * hz;
+    dG[6] += d2 * hx; dG[7] += d2 * hy; dG[8] += d2 * hz;
+  }
+  if (a.dF2) unsafeAtomicAdd(a.dF2 + row, static_cast<float>(dF2));
 }
 
 // The max term of the normalisation (-(sum G P)/m at the maximum's pixel p*)
@@ -442,6 +458,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
   const int H = a.H, W = a.W;
   const size_t HW = static_cast<size_t>(H) * W;
   if (kLarge && tid == 0) stage_chain_consts(a, s_cc);
+  if (kLarge && tid < 9) sh.drec[tid] = 0.0;
   int *wbins = kLarge && a.work ? a.work + (size_t)blockIdx.x * a.work_per_block : nullptr;
   float *wacc = kLarge && a.work ? reinterpret_cast<float *>(wbins + a.work_entries) : nullptr;
   for (int i = tid; i < a.acc_words; i += kAdjThreads) s_acc[i] = 0;
@@ -800,9 +817,12 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
         }
         if (tid == 0) a.gcount[p] = ns;
       } else {
-        double dR[9];
+        double dR[9], dG[9];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) dR[i] = 0.0;
+        for (int i = 0; i < 9; ++i) {
+          dR[i] = 0.0;
+          dG[i] = 0.0;
+        }
         const int *idx = a.idx + (size_t)p * a.cap;
         if (!large) {
           for (int q = tid; q < ns; q += kAdjThreads) {
@@ -810,7 +830,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
             if (norm)
               max_term(a, corr, px, py, t.X[q], t.Y[q], t.xb[q] >> 16, t.xb[q] & 0xffff, t.ylo[q],
                        t.yhi[q], A, BX, BY);
-            spot_chain(a, s_cc, sh.R, idx[t.jpos[q]], t.I[q], A, BX, BY, dR);
+            spot_chain(a, s_cc, sh.R, idx[t.jpos[q]], t.I[q], A, BX, BY, dR, dG);
           }
         } else {
           for (int j = tid; j < n; j += kAdjThreads) {
@@ -821,20 +841,31 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
             if (!subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, H, W, lo, hi, xlo, xhi)) continue;
             float A = wacc[3 * j], BX = wacc[3 * j + 1], BY = wacc[3 * j + 2];
             if (norm) max_term(a, corr, px, py, X, Y, xlo, xhi, lo, hi, A, BX, BY);
-            spot_chain(a, s_cc, sh.R, idx[j], sp.w, A, BX, BY, dR);
+            spot_chain(a, s_cc, sh.R, idx[j], sp.w, A, BX, BY, dR, dG);
           }
         }
 #pragma unroll
         for (int i = 0; i < 9; ++i) dR[i] = wave_sum_d(dR[i]);
+        if (a.drec) {
+#pragma unroll
+          for (int i = 0; i < 9; ++i) dG[i] = wave_sum_d(dG[i]);
+        }
         if (lane == 0) {
 #pragma unroll
-          for (int i = 0; i < 9; ++i) sh.red[wid][2 + i] = dR[i];
+          for (int i = 0; i < 9; ++i) {
+            sh.red[wid][2 + i] = dR[i];
+            sh.red[wid][11 + i] = dG[i];
+          }
         }
         lds_sync();
         if (tid < 9) {
-          double v = 0.0;
-          for (int w = 0; w < kAdjWaves; ++w) v += sh.red[w][2 + tid];
+          double v = 0.0, u = 0.0;
+          for (int w = 0; w < kAdjWaves; ++w) {
+            v += sh.red[w][2 + tid];
+            u += sh.red[w][11 + tid];
+          }
           a.dRt[(size_t)p * 9 + tid] = v;
+          sh.drec[tid] += u;
         }
       }
     }
@@ -844,6 +875,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
     else
       lds_sync();  // LDS state of this pattern is dead before the next one
   }
+  if (kLarge && a.drec && tid < 9 && sh.drec[tid] != 0.0) unsafeAtomicAdd(a.drec + tid, sh.drec[tid]);
 }
 
 // Two blocks per CU: at most 128 VGPRs (four waves per SIMD).
@@ -868,11 +900,16 @@ constexpr int kChainThreads = 256;
 
 __global__ __launch_bounds__(kChainThreads) void spot_chain_kernel(AdjArgs a) {
   __shared__ ChainConsts s_cc;
+  __shared__ double s_part[kChainThreads / kWave][9];
   if (threadIdx.x == 0) stage_chain_consts(a, s_cc);
   __syncthreads();
   const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
   const int waves = gridDim.x * (kChainThreads / kWave);
-  for (int p = blockIdx.x * (kChainThreads / kWave) + threadIdx.x / kWave; p < a.n; p += waves) {
+  double dG[9];  // dL/drec over this wave's patterns
+#pragma unroll
+  for (int i = 0; i < 9; ++i) dG[i] = 0.0;
+  for (int p = blockIdx.x * (kChainThreads / kWave) + wid; p < a.n; p += waves) {
     double R[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) R[i] = a.Rt[(size_t)p * 9 + i];
@@ -886,7 +923,7 @@ __global__ __launch_bounds__(kChainThreads) void spot_chain_kernel(AdjArgs a) {
     for (int q = lane; q < ns; q += kWave) {
       const float4 v = gs[q];
       const int j = __float_as_int(v.w);
-      spot_chain(a, s_cc, R, idx[j], spots[j].w, v.x, v.y, v.z, dR);
+      spot_chain(a, s_cc, R, idx[j], spots[j].w, v.x, v.y, v.z, dR, dG);
     }
 #pragma unroll
     for (int i = 0; i < 9; ++i) dR[i] = wave_sum_d(dR[i]);
@@ -895,6 +932,30 @@ __global__ __launch_bounds__(kChainThreads) void spot_chain_kernel(AdjArgs a) {
       for (int i = 0; i < 9; ++i) a.dRt[(size_t)p * 9 + i] = dR[i];
     }
   }
+  if (!a.drec) return;  // uniform
+  // this block's partial dL/drec (summed in a fixed order by sum_partials_kernel)
+#pragma unroll
+  for (int i = 0; i < 9; ++i) dG[i] = wave_sum_d(dG[i]);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) s_part[wid][i] = dG[i];
+  }
+  __syncthreads();
+  if (threadIdx.x < 9) {
+    double v = 0.0;
+    for (int w = 0; w < kChainThreads / kWave; ++w) v += s_part[w][threadIdx.x];
+    a.dpart[(size_t)blockIdx.x * 9 + threadIdx.x] = v;
+  }
+}
+
+// drec[i] += sum over blocks of part[b][i]; one wave per entry.
+__global__ __launch_bounds__(9 * kWave) void sum_partials_kernel(const double *part, int nb,
+                                                                 double *drec) {
+  const int i = threadIdx.x / kWave, lane = lane_id();
+  double v = 0.0;
+  for (int b = lane; b < nb; b += kWave) v += part[(size_t)b * 9 + i];
+  v = wave_sum_d(v);
+  if (lane == 0) drec[i] += v;
 }
 
 // ---------------------------------------------------------------------------
@@ -916,7 +977,7 @@ struct SfAdjArgs {
   const double *coeff;
   int n_elem;
   int scatter;
-  const float4 *dtab;
+  const float *dF2;
   float *dxyz;
   float *dB;
   double *drec;
@@ -941,8 +1002,7 @@ __global__ __launch_bounds__(kSfaThreads) void structure_factor_adjoint_kernel(S
   const int i = blockIdx.x * kSfaThreads + tid;
   const bool valid = i < a.n;
   const int4 hkl = valid ? a.hkl[i] : make_int4(0, 0, 0, 0);
-  const float4 dt = valid ? a.dtab[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  const double w = dt.w;
+  const double w = valid ? static_cast<double>(a.dF2[i]) : 0.0;
   const double hx = hkl.x, hy = hkl.y, hz = hkl.z;
   const double gx = a.rec[0] * hx + a.rec[1] * hy + a.rec[2] * hz;
   const double gy = a.rec[3] * hx + a.rec[4] * hy + a.rec[5] * hz;
@@ -985,11 +1045,11 @@ __global__ __launch_bounds__(kSfaThreads) void structure_factor_adjoint_kernel(S
       Qi = fma(damp, static_cast<double>(sn), Qi);
     }
   }
-  // dL/dg = dtab.xyz + 2 g dL/d(g^2), dL/d(g^2) = w 2 Re(conj(F) dF/dq)
+  // through |F|^2: dL/dg = 2 g dL/d(g^2), dL/d(g^2) = w 2 Re(conj(F) dF/dq)
   const double dq = w * 2.0 * (Fr * Qr + Fi * Qi);
-  const double dgx = static_cast<double>(dt.x) + 2.0 * gx * dq;
-  const double dgy = static_cast<double>(dt.y) + 2.0 * gy * dq;
-  const double dgz = static_cast<double>(dt.z) + 2.0 * gz * dq;
+  const double dgx = 2.0 * gx * dq;
+  const double dgy = 2.0 * gy * dq;
+  const double dgz = 2.0 * gz * dq;
   double dr[9] = {dgx * hx, dgx * hy, dgx * hz, dgy * hx, dgy * hy, dgy * hz,
                   dgz * hx, dgz * hy, dgz * hz};
 #pragma unroll
@@ -1068,17 +1128,22 @@ using namespace dsx;
 
 // Workspace of the large-pattern kernel, per block: int bins[kWorkEntriesPerSpot
 // * cap] | float sums[3 * cap].  Of the small-pattern kernel with gradients:
-// float4 gsum[n * cap] | int gcount[n].
+// float4 gsum[n * cap] | int gcount[n] | (8-byte aligned) double
+// dpart[chain blocks][9].
 constexpr int kWorkEntriesPerSpot = 4;
 constexpr int kAdjLdsBytes = 160 * 1024;
 
 static bool adjoint_large(int cap) { return cap > kAdjMaxSpots; }
+static int chain_blocks_max() { return cu_count() * 8; }
+static long long small_work_ints(int cap, int n) {
+  return 4ll * n * cap + n + 1 + 18ll * chain_blocks_max();
+}
 
 extern "C" long long dsx_render_adjoint_workspace_ints(int cap, int n) {
   if (cap < 1 || n < 0) return 0;
   if (adjoint_large(cap))
     return static_cast<long long>(cu_count()) * (kWorkEntriesPerSpot + 3) * static_cast<long long>(cap);
-  return static_cast<long long>(n) * (4ll * cap + 1);
+  return small_work_ints(cap, n);
 }
 
 // Static LDS of a kernel (queried once) and its dynamic LDS limit raised to
@@ -1108,7 +1173,8 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
                                   const double *Rt_d, const dsx_sim_params *sim,
                                   const dsx_render_params *rp, int mode, const float *target_d,
                                   long long target_stride, float *pattern_d, float *loss_d,
-                                  double *dRt_d, float *dtab_d, int *work_d, long long work_ints,
+                                  double *dRt_d, float *dF2_d, double *drec_d, int *work_d,
+                                  long long work_ints,
                                   int *status_d, void *stream) {
   clear_error();
   if (!sim || !rp || n < 0 || cap < 1 || n_refl < 0 || work_ints < 0 || target_stride < 0) {
@@ -1120,14 +1186,17 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
     return DSX_EINVAL;
   }
   if (n == 0) return DSX_OK;
-  const bool want_grad = dRt_d != nullptr || dtab_d != nullptr;
+  const bool want_grad = dRt_d != nullptr;
+  if (!want_grad && (dF2_d || drec_d)) {
+    set_error("dsx_render_adjoint: dF2_d and drec_d need dRt_d");
+    return DSX_EINVAL;
+  }
   if (mode == DSX_ADJ_FORWARD && (want_grad || !pattern_d)) {
     set_error("dsx_render_adjoint: forward mode writes patterns and no gradients");
     return DSX_EINVAL;
   }
   if (!spot_d || !count_d || (mode != DSX_ADJ_FORWARD && !target_d) ||
-      (want_grad && (!dRt_d || !dtab_d || !idx_d || !Rt_d)) ||
-      (want_grad && n_refl > 0 && (!hkl_d || !F2_d))) {
+      (want_grad && (!idx_d || !Rt_d)) || (want_grad && n_refl > 0 && (!hkl_d || !F2_d))) {
     set_error("dsx_render_adjoint: null device pointer");
     return DSX_EINVAL;
   }
@@ -1176,7 +1245,9 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   a.pattern = pattern_d;
   a.loss = loss_d;
   a.dRt = dRt_d;
-  a.dtab = dtab_d;
+  a.dF2 = dF2_d;
+  a.drec = drec_d;
+  a.dpart = nullptr;
   a.status = status_d;
   a.work = nullptr;
   a.work_per_block = 0;
@@ -1239,7 +1310,7 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
       return DSX_EINVAL;
     }
   } else if (want_grad) {
-    const long long need = static_cast<long long>(n) * (4ll * cap + 1);
+    const long long need = small_work_ints(cap, n);
     if (!work_d || work_ints < need) {
       set_error("dsx_render_adjoint: gradients need a workspace of %lld ints, got %lld "
                 "(dsx_render_adjoint_workspace_ints)", need, work_d ? work_ints : 0ll);
@@ -1251,6 +1322,8 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
     }
     a.gsum = reinterpret_cast<float4 *>(work_d);
     a.gcount = work_d + static_cast<size_t>(n) * 4 * cap;
+    const size_t off = (static_cast<size_t>(n) * (4ull * cap + 1) + 1) & ~static_cast<size_t>(1);
+    a.dpart = reinterpret_cast<double *>(work_d + off);
   }
   const size_t lds = ((2ull * a.acc_words * 4 + static_cast<size_t>(table)) + 15) & ~static_cast<size_t>(15);
   hipStream_t s = as_stream(stream);
@@ -1261,8 +1334,11 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   hipLaunchKernelGGL(render_adjoint_kernel, dim3(blocks), dim3(kAdjThreads), lds, s, a);
   int rc = check_launch("dsx_render_adjoint");
   if (rc != DSX_OK || !want_grad) return rc;
-  const int cblocks = std::min((n + kChainThreads / kWave - 1) / (kChainThreads / kWave), cu_count() * 8);
+  const int cblocks =
+      std::min((n + kChainThreads / kWave - 1) / (kChainThreads / kWave), chain_blocks_max());
   hipLaunchKernelGGL(spot_chain_kernel, dim3(cblocks), dim3(kChainThreads), 0, s, a);
+  if (drec_d)
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(9 * kWave), 0, s, a.dpart, cblocks, drec_d);
   return check_launch("dsx_render_adjoint (chain)");
 }
 
@@ -1270,7 +1346,7 @@ extern "C" int dsx_structure_factor_adjoint(const int *hkl_d, int n, const doubl
                                             const uint32_t *xyz_fx_d, const int *elem_d,
                                             const float *occ_d, const float *B_d, int n_atoms,
                                             const double *coeff_d, int n_elem, int scatter,
-                                            const float *dtab_d, float *dxyz_d, float *dB_d,
+                                            const float *dF2_d, float *dxyz_d, float *dB_d,
                                             double *drec_d, void *stream) {
   clear_error();
   if (n < 0 || n_atoms < 0 || n_elem < 0 || !rec) {
@@ -1286,7 +1362,7 @@ extern "C" int dsx_structure_factor_adjoint(const int *hkl_d, int n, const doubl
     set_error("dsx_structure_factor_adjoint: atoms given but no element coefficients");
     return DSX_EINVAL;
   }
-  if (!hkl_d || !dtab_d || !drec_d ||
+  if (!hkl_d || !dF2_d || !drec_d ||
       (n_atoms > 0 && (!xyz_fx_d || !elem_d || !occ_d || !B_d || !dxyz_d || !dB_d)) ||
       (n_elem > 0 && !coeff_d)) {
     set_error("dsx_structure_factor_adjoint: null device pointer");
@@ -1304,7 +1380,7 @@ extern "C" int dsx_structure_factor_adjoint(const int *hkl_d, int n, const doubl
   a.coeff = coeff_d;
   a.n_elem = n_elem;
   a.scatter = scatter;
-  a.dtab = reinterpret_cast<const float4 *>(dtab_d);
+  a.dF2 = dF2_d;
   a.dxyz = dxyz_d;
   a.dB = dB_d;
   a.drec = drec_d;

@@ -346,10 +346,14 @@ class RefinementModel:
 
     def adjoint(self, t: _Tables, Rt: torch.Tensor, batch: engine.SpotBatch, mode: int,
                 target: Optional[torch.Tensor], want_grad: bool,
-                pattern: Optional[torch.Tensor] = None, status: Optional[torch.Tensor] = None):
-        """One launch of the render adjoint (+ the structure-factor adjoint when
-        gradients are wanted).  ``target`` is [n, H, W] or one [H, W] pattern
-        shared by every orientation.  Returns (loss[n], dRt, dtab, dxyz, dB, drec)."""
+                pattern: Optional[torch.Tensor] = None, status: Optional[torch.Tensor] = None,
+                atoms: bool = True, lattice: bool = True):
+        """The render adjoint (+ the structure-factor adjoint when gradients of
+        the atoms or the lattice are wanted).  ``target`` is [n, H, W] or one
+        [H, W] pattern shared by every orientation.  With ``want_grad`` dL/dRt
+        is always computed; ``atoms`` adds dL/dxyz and dL/dB, ``lattice``
+        dL/drec.  Returns (loss[n], dRt, dF2, dxyz, dB, drec), None where not
+        computed."""
         n = batch.n
         H, W = self.shape
         stride = 0
@@ -361,14 +365,16 @@ class RefinementModel:
             elif tuple(target.shape) != (H, W):
                 raise ValueError(f"target must be [n, {H}, {W}] or [{H}, {W}]")
         loss = torch.empty(n, dtype=torch.float32, device=self.dev)
-        dRt = dtab = None
+        dRt = dF2 = drec = None
         na = max(self.n_atoms, 1)
+        table = want_grad and (atoms or lattice)  # gradients through |F|^2
         if want_grad:
             dRt = torch.empty((n, 9), dtype=torch.float64, device=self.dev)
-            # the accumulated gradients, zeroed by one fill:
-            # dtab [n_refl, 4] | dxyz [na, 3] | dB [na] (float32) ; drec [9] (float64)
-            acc = torch.zeros(4 * self.n_refl + 4 * na, dtype=torch.float32, device=self.dev)
-            dtab = acc[:4 * self.n_refl].view(self.n_refl, 4)
+        if table:
+            # the accumulated gradients, zeroed by one fill each:
+            # dF2 [n_refl] | dxyz [na, 3] | dB [na] (float32) ; drec [9] (float64)
+            acc = torch.zeros(self.n_refl + 4 * na, dtype=torch.float32, device=self.dev)
+            dF2 = acc[:self.n_refl]
             drec = torch.zeros(9, dtype=torch.float64, device=self.dev)
         sync_status = status is None
         if status is None:
@@ -380,26 +386,27 @@ class RefinementModel:
             _lib.ptr(batch.idx), _lib.ptr(batch.spots), _lib.ptr(batch.count), batch.cap, n,
             _lib.ptr(t.hkl), _lib.ptr(t.F2), self.n_refl, _lib.ptr(Rt), ctypes.byref(sp),
             ctypes.byref(self.render), mode, _lib.ptr(target), stride, _lib.ptr(pattern),
-            _lib.ptr(loss), _lib.ptr(dRt), _lib.ptr(dtab), _lib.ptr(work),
-            0 if work is None else work.numel(), _lib.ptr(status), stream), "dsx_render_adjoint")
+            _lib.ptr(loss), _lib.ptr(dRt), _lib.ptr(dF2), _lib.ptr(drec if lattice else None),
+            _lib.ptr(work), work.numel(), _lib.ptr(status), stream), "dsx_render_adjoint")
         if sync_status:
             engine.check_render_status(status)
         dxyz = dB = None
-        if want_grad:
-            o = 4 * self.n_refl
+        if table:
+            o = self.n_refl
             dxyz_s = acc[o:o + 3 * na].view(na, 3)
             dB_s = acc[o + 3 * na:o + 4 * na]
             rec9 = (ctypes.c_double * 9)(*t.rec.reshape(-1).tolist())
             _lib.check(self.L.dsx_structure_factor_adjoint(
                 _lib.ptr(t.hkl), self.n_refl, rec9, _lib.ptr(t.xyz_fx), _lib.ptr(self.elem),
                 _lib.ptr(self.occ), _lib.ptr(t.B), self.n_atoms, _lib.ptr(self.coeff), self.n_elem,
-                engine.SCATTER_IDS[self.scatter], _lib.ptr(dtab), _lib.ptr(dxyz_s), _lib.ptr(dB_s),
+                engine.SCATTER_IDS[self.scatter], _lib.ptr(dF2), _lib.ptr(dxyz_s), _lib.ptr(dB_s),
                 _lib.ptr(drec), stream), "dsx_structure_factor_adjoint")
-            dxyz = dxyz_s[:self.n_atoms][self.inv_order_dev]
-            dB = dB_s[:self.n_atoms][self.inv_order_dev]
-        else:
-            drec = None
-        return loss, dRt, dtab, dxyz, dB, drec
+            if atoms:
+                dxyz = dxyz_s[:self.n_atoms][self.inv_order_dev]
+                dB = dB_s[:self.n_atoms][self.inv_order_dev]
+            if not lattice:
+                drec = None
+        return loss, dRt, dF2, dxyz, dB, drec
 
     # -- public differentiable functions --------------------------------------
 
@@ -456,7 +463,9 @@ class _L2Loss(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, Rt, rec, xyz, B, model: RefinementModel, target):
-        want = any(ctx.needs_input_grad[:4])
+        need = ctx.needs_input_grad
+        want = any(need[:4])
+        atoms, lattice = need[2] or need[3], need[1]
         t = model.tables(rec, xyz, B)
         while True:
             batch = model.spots(t, Rt, check=False)
@@ -468,26 +477,29 @@ class _L2Loss(torch.autograd.Function):
                 break
             status = torch.empty(batch.n, dtype=torch.int32, device=model.dev)  # every entry written
             loss, dRt, _, dxyz, dB, drec = model.adjoint(t, Rt.detach(), batch, _lib.ADJ_L2, target,
-                                                         want, status=status)
-            host = _fetch_step(batch, status, loss, drec if want else None)
+                                                         want, status=status, atoms=atoms,
+                                                         lattice=lattice)
+            host = _fetch_step(batch, status, loss, drec)
             if int(host[2]) != 0:
                 engine.check_render_status(status)
             if int(host[1]) <= batch.cap:
                 break
             model.cap = int(host[1]) + 64
         if want:
-            drec_h = torch.from_numpy(np.array(host[3:12]))
+            drec_h = torch.from_numpy(np.array(host[3:12])) if lattice else None
             g = _grads_like((Rt, rec, xyz, B), dRt, drec_h, dxyz, dB)
-            ctx.save_for_backward(*g)
+            ctx.nones = [x is None for x in g]
+            ctx.save_for_backward(*[x for x in g if x is not None])
         return torch.tensor(float(host[0]), dtype=torch.float64)
 
     @staticmethod
     def backward(ctx, gl):
-        dRt, drec, dxyz, dB = ctx.saved_tensors
+        saved = iter(ctx.saved_tensors)
+        g = [None if none else next(saved) for none in ctx.nones]
         s = float(gl)  # a CPU scalar: no synchronisation
-        if s == 1.0:
-            return dRt, drec, dxyz, dB, None, None
-        return dRt * s, drec * s, dxyz * s, dB * s, None, None
+        if s != 1.0:
+            g = [None if x is None else x * s for x in g]
+        return (*g, None, None)
 
 
 class _Patterns(torch.autograd.Function):

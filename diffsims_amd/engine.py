@@ -373,7 +373,7 @@ def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigm
         sim.k, sim.shape_width, sim.shape_model = 1.0, 1.0, _lib.SF_LINEAR
         _lib.check(L.dsx_render_adjoint(
             None, _lib.ptr(spots), _lib.ptr(count), cap, n, None, None, 0, None, ctypes.byref(sim),
-            ctypes.byref(p), _lib.ADJ_FORWARD, None, 0, _lib.ptr(out), None, None, None,
+            ctypes.byref(p), _lib.ADJ_FORWARD, None, 0, _lib.ptr(out), None, None, None, None,
             _lib.ptr(work), work.numel(), _lib.ptr(status), stream), "dsx_render_adjoint")
     elif fast and cap > int(L.dsx_render_max_spots()):
         # more spots than the splat renderer holds: whole pattern in LDS when

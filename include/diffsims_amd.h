@@ -297,10 +297,14 @@ int dsx_render_fast_ragged(const float *spot_d, const long long *spot_off_d, con
  * every pattern with one target).  G is pulled back through the render, the
  * intensity |F|^2 f(s, r) and the excitation error, which are recomputed from
  * idx_d, hkl_d, F2_d, Rt_d and sim->rec (the inputs dsx_simulate used):
- *   dRt_d   double [n][9]        dL/dRt of each pattern (overwritten)
- *   dtab_d  float  [n_refl][4]   {dL/dg (crystal frame, 3), dL/d|F(hkl)|^2},
- *                                ACCUMULATED (zero it first)
- * dRt_d and dtab_d may both be NULL (loss / pattern only).  pattern_d
+ *   dRt_d   double [n][9]    dL/dRt of each pattern (overwritten); NULL: no
+ *                            gradients (loss / pattern only)
+ *   dF2_d   float  [n_refl]  dL/d|F(hkl)|^2, ACCUMULATED (zero it first); may
+ *                            be NULL (skipped)
+ *   drec_d  double [9]       sum over spots of dL/dg (crystal frame) hkl^T,
+ *                            i.e. dL/drec through the spot positions and
+ *                            excitation errors, ACCUMULATED; may be NULL
+ * dF2_d and drec_d need dRt_d.  pattern_d
  * (float [n][H][W], may be NULL) receives P; loss_d (float [n], may be NULL).
  * work_d is a 16-byte aligned workspace of dsx_render_adjoint_workspace_ints(
  * cap, n) ints.  With cap <= dsx_render_max_spots() it is needed only for
@@ -321,25 +325,27 @@ int dsx_render_adjoint(const int *idx_d, const float *spot_d, const int *count_d
                        const double *Rt_d, const dsx_sim_params *sim,
                        const dsx_render_params *params, int mode, const float *target_d,
                        long long target_stride, float *pattern_d, float *loss_d,
-                       double *dRt_d, float *dtab_d, int *work_d, long long work_ints,
-                       int *status_d, void *stream);
+                       double *dRt_d, float *dF2_d, double *drec_d, int *work_d,
+                       long long work_ints, int *status_d, void *stream);
 
 /*
  * Adjoint of dsx_structure_factor.  No counterpart in the reference.  Given
- * dtab_d = {dL/dg, dL/d|F|^2} per reflection (as accumulated by
+ * dF2_d = dL/d|F|^2 per reflection (float [n], as accumulated by
  * dsx_render_adjoint), recomputes F(hkl) and ACCUMULATES (zero first):
  *   dxyz_d  float  [n_atoms][3]  dL/d(fractional position), atoms in the order
  *                                given (the order dsx_structure_factor took)
  *   dB_d    float  [n_atoms]     dL/dB (A^2)
- *   drec_d  double [9]           dL/d rec (g = rec hkl, row-major), including
- *                                the |g|-dependence of f and of exp(-B g^2/4)
+ *   drec_d  double [9]           dL/d rec (g = rec hkl, row-major) through
+ *                                |F|^2: the |g|-dependence of f and of
+ *                                exp(-B g^2/4) (the geometric part comes from
+ *                                dsx_render_adjoint's drec_d)
  * Other arguments as for dsx_structure_factor.
  */
 int dsx_structure_factor_adjoint(const int *hkl_d, int n, const double rec[9],
                                  const uint32_t *xyz_fx_d, const int *elem_d,
                                  const float *occ_d, const float *B_d, int n_atoms,
                                  const double *coeff_d, int n_elem, int scatter,
-                                 const float *dtab_d, float *dxyz_d, float *dB_d,
+                                 const float *dF2_d, float *dxyz_d, float *dB_d,
                                  double *drec_d, void *stream);
 
 /*

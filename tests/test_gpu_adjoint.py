@@ -257,7 +257,7 @@ def test_forward_mode_equals_subpixel_renderer():
     _lib.check(L.dsx_render_adjoint(
         None, _lib.ptr(b.spots), _lib.ptr(b.count), b.cap, b.n, None, None, 0, None,
         ctypes.byref(sim), ctypes.byref(model.render), _lib.ADJ_FORWARD, None, 0, _lib.ptr(out),
-        None, None, None, None, 0, _lib.ptr(status), _lib.stream_handle()), "forward")
+        None, None, None, None, None, 0, _lib.ptr(status), _lib.stream_handle()), "forward")
     torch.cuda.synchronize()
     assert int(status.max()) == 0
     assert float((out - ref).abs().max()) <= 1e-6 * float(ref.abs().max())
