@@ -44,13 +44,14 @@ namespace dsx {
 #ifndef DSX_RENDER_THREADS
 #define DSX_RENDER_THREADS 512
 #endif
-#ifndef DSX_STRIP_KB
-#define DSX_STRIP_KB 48
-#endif
-#ifndef DSX_RENDER_BLOCKS_PER_CU
-#define DSX_RENDER_BLOCKS_PER_CU 2
-#endif
 constexpr int kRenderThreads = DSX_RENDER_THREADS;
+// Launch shapes of the splat renderer: at most 80 VGPRs (six waves per SIMD),
+// so three 512-thread blocks fit a CU when each holds a 28 KB strip; with two
+// blocks per CU the strips are 48 KB (fewer strips, fewer barriers).  Three
+// blocks pay off when the normalised fast mode's maximum search leaves gaps
+// in a block's HBM writes for the other blocks to fill.
+constexpr int kRenderMinWaves = 6;
+constexpr int kMaxRenderBlocksPerCU = 3;
 constexpr int kMaxSpots = 1024;  // distinct in-frame spots per pattern
 constexpr int kHash = 2048;      // LDS hash slots (load factor <= 1/2)
 constexpr int kMaxStrips = 256;
@@ -64,9 +65,20 @@ constexpr int kCandSpots = 64;    // spots touching one candidate tile
 constexpr float kScaleRange = 2040109465.6f;  // 1.9 * 2^30: fixed-point units per tile bound
 
 typedef int racc_t;  // strip accumulator word
-constexpr int kStripBytes = DSX_STRIP_KB * 1024;
+constexpr int kStripBytes = 48 * 1024;       // one or two blocks per CU
+constexpr int kStripBytes3 = 28 * 1024;      // three blocks per CU
+
 
 enum { kModeFast = 0, kModeSubpixel = 1 };
+
+static int splat_blocks_per_cu(int mode, bool normalize, int requested) {
+  if (requested > 0) return std::min(requested, kMaxRenderBlocksPerCU);
+  return mode == kModeFast && normalize ? 3 : 2;
+}
+
+static int strip_bytes_for(int blocks_per_cu) {
+  return blocks_per_cu >= 3 ? kStripBytes3 : kStripBytes;
+}
 
 struct RenderArgs {
   const float4 *spots;
@@ -344,7 +356,7 @@ __device__ bool bounded_max(RenderShared<kMode> &sh, int *tile_u, short (*cand_q
 // kR > 0: the fast-mode tap radius as a compile-time constant (the footprint
 // loops unroll and the taps stay in scalar registers); 0: taken from the args.
 template <int kMode, int kR>
-__global__ __launch_bounds__(kRenderThreads) void render_kernel(RenderArgs a) {
+__global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel(RenderArgs a) {
   // 16-byte aligned: the read-outs use 16-byte LDS accesses, which run at a
   // fraction of their rate on an 8-byte-aligned base
   extern __shared__ __align__(16) racc_t s_acc[];  // strip accumulator; hash table before the passes
@@ -947,7 +959,8 @@ __global__ __launch_bounds__(kLargeThreads) void render_fast_large_kernel(LargeA
   }
 }
 
-static int fill_render_args(int mode, const dsx_render_params *p, RenderArgs &a, const char *name) {
+static int fill_render_args(int mode, const dsx_render_params *p, RenderArgs &a, const char *name,
+                            int strip_bytes = kStripBytes) {
   if (p->height < 1 || p->width < 1 || p->height > 32767 || p->width > 32767 ||
       !(p->calibration > 0.0f) || !(p->sigma > 0.0f)) {
     set_error("%s: bad pattern shape %dx%d, calibration or sigma", name, p->height, p->width);
@@ -979,7 +992,7 @@ static int fill_render_args(int mode, const dsx_render_params *p, RenderArgs &a,
   a.ep = static_cast<float>(-1.0 / (2.0 * sg * sg));
   a.clip = p->clip_threshold;
   for (int i = 0; i <= kMaxR; ++i) a.taps[i] = (mode == kModeFast && i <= p->radius) ? p->taps[i] : 0.0f;
-  const int budget = kStripBytes / static_cast<int>(sizeof(racc_t));
+  const int budget = strip_bytes / static_cast<int>(sizeof(racc_t));
   if (a.W > budget) {
     set_error("%s: pattern width %d exceeds the LDS strip budget", name, a.W);
     return DSX_EINVAL;
@@ -998,10 +1011,10 @@ static int fill_render_args(int mode, const dsx_render_params *p, RenderArgs &a,
   return DSX_OK;
 }
 
-static int launch_splat(int mode, const RenderArgs &a, int blocks_per_cu, hipStream_t st,
-                        const char *name) {
+// bpc: blocks per CU as resolved by splat_blocks_per_cu (the strips of `a`
+// were sized for it by fill_render_args)
+static int launch_splat(int mode, const RenderArgs &a, int bpc, hipStream_t st, const char *name) {
   const size_t lds = static_cast<size_t>(a.lds_words) * sizeof(racc_t);
-  const int bpc = blocks_per_cu > 0 ? blocks_per_cu : DSX_RENDER_BLOCKS_PER_CU;
   const int blocks = std::min(a.n, cu_count() * bpc);
   if (mode == kModeSubpixel) {
     hipLaunchKernelGGL((render_kernel<kModeSubpixel, 0>), dim3(blocks), dim3(kRenderThreads), lds, st, a);
@@ -1086,7 +1099,8 @@ static int render_common(int mode, const float *spot_d, const int *count_d, int 
     return DSX_EINVAL;
   }
   RenderArgs a;
-  const int st = fill_render_args(mode, p, a, name);
+  const int bpc = splat_blocks_per_cu(mode, p->normalize != 0, 0);
+  const int st = fill_render_args(mode, p, a, name, strip_bytes_for(bpc));
   if (st != DSX_OK) return st;
   a.spots = reinterpret_cast<const float4 *>(spot_d);
   a.count = count_d;
@@ -1094,7 +1108,7 @@ static int render_common(int mode, const float *spot_d, const int *count_d, int 
   a.n = n;
   a.out = out_d;
   a.status = status_d;
-  return launch_splat(mode, a, 0, as_stream(stream), name);
+  return launch_splat(mode, a, bpc, as_stream(stream), name);
 }
 
 }  // namespace dsx
@@ -1127,7 +1141,9 @@ extern "C" int dsx_render_fast_ragged(const float *spot_d, const long long *spot
     return DSX_EINVAL;
   }
   RenderArgs a;
-  const int st = fill_render_args(kModeFast, params, a, name);
+  const int bpc = splat_blocks_per_cu(kModeFast, params->normalize != 0, blocks_per_cu);
+  const int st = fill_render_args(kModeFast, params, a, name,
+                                  method == DSX_RENDER_SPLAT ? strip_bytes_for(bpc) : kStripBytes);
   if (st != DSX_OK) return st;
   a.spots = reinterpret_cast<const float4 *>(spot_d);
   a.count = count_d;
@@ -1139,7 +1155,7 @@ extern "C" int dsx_render_fast_ragged(const float *spot_d, const long long *spot
   a.out = out_d;
   a.status = status_d;
   return method == DSX_RENDER_DENSE ? launch_dense(a, blocks_per_cu, as_stream(stream), name)
-                                    : launch_splat(kModeFast, a, blocks_per_cu, as_stream(stream), name);
+                                    : launch_splat(kModeFast, a, bpc, as_stream(stream), name);
 }
 
 extern "C" int dsx_render_dense_max_pixels(void) { return kDenseMaxPix; }
