@@ -290,8 +290,21 @@ def main():
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
+        # the same buffers timed with HIP events: what a plain copy and a plain
+        # store stream reach on this GPU (ceilings for the HBM-bound kernels)
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record()
+        for _ in range(3):
+            dst.copy_(src)
+        e[1].record()
+        for _ in range(3):
+            dst.fill_(2.0)
+        e[2].record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
+        copy_ms, fill_ms = e[0].elapsed_time(e[1]) / 3, e[1].elapsed_time(e[2]) / 3
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n,
+                          "copy_GBps": round(8 * n / copy_ms / 1e6, 1),
+                          "fill_GBps": round(4 * n / fill_ms / 1e6, 1)}), flush=True)
         return
     wl = workloads.template_library(args.orientations)
     pa = wl.phases[0]
