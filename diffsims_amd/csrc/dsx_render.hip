@@ -373,11 +373,26 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
   for (int i = tid; i <= kMaxR; i += kRenderThreads) sh.taps[i] = a.taps[i];
   // the strip buffer is kept zeroed between uses (each read-out clears it)
   for (int i = tid; i < a.lds_words; i += kRenderThreads) s_acc[i] = 0;
+  // spot tid and the count of the block's next pattern, loaded while the
+  // current pattern renders so the scans below start without a memory wait
+  float4 next_head = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  int next_count = 0;
+  auto prefetch = [&](int pi) {
+    if (pi >= a.n) return;
+    const int p = a.list ? a.list[pi] : pi;
+    const int c = a.capv ? a.capv[p] : a.cap;
+    const float4 *sp = a.spots + (a.spot_off ? a.spot_off[p] : (long long)p * a.cap);
+    next_count = min(a.count[p], c);
+    if (tid < next_count) next_head = sp[tid];
+  };
+  prefetch(blockIdx.x);
   lds_sync();
 
   for (int pi = blockIdx.x; pi < a.n; pi += gridDim.x) {
     const int p = a.list ? a.list[pi] : pi;
-    const int n = min(a.count[p], a.capv ? a.capv[p] : a.cap);
+    const int n = next_count;
+    const float4 head = next_head;  // spots[tid] when tid < n
+    next_head = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // dead until the prefetch below
     const float4 *spots = a.spots + (a.spot_off ? a.spot_off[p] : (long long)p * a.cap);
     if (kMode == kModeFast) {
       for (int i = tid; i < kHash; i += kRenderThreads) {
@@ -397,7 +412,7 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
     if (kMode == kModeFast) {
       for (int j = tid; j < n; j += kRenderThreads) {
         float X, Y;
-        if (!to_pixel(spots[j], a, X, Y)) continue;
+        if (!to_pixel(j == tid ? head : spots[j], a, X, Y)) continue;
         const int key = min(static_cast<int>(Y), H - 1) * W + min(static_cast<int>(X), W - 1);
         unsigned slot = hash_slot(key);
         bool placed = false;
@@ -417,7 +432,7 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
     // ---- 1b. survivors -> LDS list with their footprint -----------------
     float tmax = 0.0f;  // largest spot weight: bound on any single contribution
     for (int j = tid; j < n; j += kRenderThreads) {
-      const float4 sp = spots[j];
+      const float4 sp = j == tid ? head : spots[j];
       float X, Y;
       if (!to_pixel(sp, a, X, Y)) continue;
       int lo, hi, packed, width;
@@ -486,6 +501,7 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
       } else {
         for (int i = tid; i < H * W; i += kRenderThreads) __builtin_nontemporal_store(0.0f, out + i);
       }
+      prefetch(pi + gridDim.x);
       lds_sync();
       continue;
     }
@@ -556,7 +572,11 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
           nb = sh.bin_start[s + 1] - b0;
         }
         float *dst = out + (size_t)y0 * W;
+        // the next pattern's head loads while the last strip is read out (one
+        // call per pattern on every path)
+        const bool last = pass == 1 && s == a.n_strips - 1;
         if (nb == 0) {
+          if (last) prefetch(pi + gridDim.x);
           if (pass == 1) {
             if (vec4) {
               const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -627,6 +647,7 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
           }
         }
         lds_sync();
+        if (last) prefetch(pi + gridDim.x);
         // read out and clear the strip (16-byte LDS accesses: 4 words)
         if (pass == 0) {
           if ((cnt & 3) == 0) {
