@@ -169,15 +169,19 @@ __device__ __forceinline__ float fast_weight(int y, int c, int n, int R, bool sh
   return short_taps ? reflect_weight3(y, c, n, R, taps) : reflect_weight(y, c, n, R, taps);
 }
 
+// Fast-mode fixed-point term: ((I wx) scale) wy truncated (the values are
+// non-negative, so this is the floor; one conversion instruction).  The splat
+// and the maximum search both use it, so they agree bit for bit.
+__device__ __forceinline__ int fast_fixed(float iws, float wy) { return static_cast<int>(iws * wy); }
+
 // Fixed-point contribution of a survivor (I at pixel xi, yi; rows lo..hi) to
 // pixel (x, y), rounded exactly as the splat rounds it; 0 outside its footprint.
 __device__ __forceinline__ int fast_term(float I, int xi, int yi, int lo, int hi, int x, int y,
                                          int H, int W, int R, bool short_taps, const float *taps,
                                          float scale) {
   if (x < xi - R || x > xi + R || y < lo || y > hi) return 0;
-  const float iw = I * fast_weight(x, xi, W, R, short_taps, taps);
-  const float v = iw * fast_weight(y, yi, H, R, short_taps, taps);
-  return __float2int_rn(v * scale);
+  const float iws = I * fast_weight(x, xi, W, R, short_taps, taps) * scale;
+  return fast_fixed(iws, fast_weight(y, yi, H, R, short_taps, taps));
 }
 
 __device__ __forceinline__ void racc_add(racc_t *p, float v, float scale) {
@@ -214,9 +218,13 @@ __device__ int tile_bounds(RenderShared<kMode> &sh, int *tile_u, bool tiles, int
     const float I = sh.I[q];
     int tx0, tx1;
     float uf;
+    bool tight = false;  // interior fast-mode spot: per-tile bound from the nearest taps
+    int xi = 0, yi = 0;
     if (kMode == kModeFast) {
       if (fabsf(I) == pmax) sh.qstar = q;
-      const int yi = sh.xy[q] >> 16, xi = sh.xy[q] & 0xffff;
+      yi = sh.xy[q] >> 16;
+      xi = sh.xy[q] & 0xffff;
+      tight = short_taps && xi >= R && xi + R < W && yi >= R && yi + R < H;
       const float bx = !short_taps ? 1.0f : ((xi >= R && xi + R < W) ? w0 : fminf(1.0f, 3.0f * w0));
       const float by = !short_taps ? 1.0f : ((yi >= R && yi + R < H) ? w0 : fminf(1.0f, 3.0f * w0));
       uf = fabsf(I) * bx * by;
@@ -233,7 +241,18 @@ __device__ int tile_bounds(RenderShared<kMode> &sh, int *tile_u, bool tiles, int
     if (tiles) {
       const int ty0 = sh.ylo[q] / kTile, ty1 = sh.yhi[q] / kTile;
       for (int ty = ty0; ty <= ty1; ++ty)
-        for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&tile_u[ty * ntx + tx], u);
+        for (int tx = tx0; tx <= tx1; ++tx) {
+          int ut = u;
+          if (tight) {
+            // the taps fall off from the centre: the tile's largest weight is
+            // at its pixel nearest (xi, yi), at most R away along each axis
+            const int dx = max(0, max(tx * kTile - xi, xi - (tx * kTile + kTile - 1)));
+            const int dy = max(0, max(ty * kTile - yi, yi - (ty * kTile + kTile - 1)));
+            const float uft = fabsf(I) * sh.taps[dx] * sh.taps[dy];
+            ut = min(static_cast<int>(ceilf(uft * inv_p)) + 1, static_cast<int>(kBoundUnits) + 1);
+          }
+          atomicAdd(&tile_u[ty * ntx + tx], ut);
+        }
     }
   }
   if (!tiles) {
@@ -286,10 +305,12 @@ __device__ bool bounded_max(RenderShared<kMode> &sh, int *tile_u, short (*cand_q
   }
   lds_sync();
   const long long lfix = sh.lfix;
-  // candidate tiles: bound (with a rounding margin per term) >= lower bound;
-  // tile_u is reused to hold each tile's candidate slot (-1: pruned)
+  // candidate tiles: bound >= lower bound.  Truncated terms never exceed
+  // their float products, which the 1e-5 factor covers (the subpixel terms
+  // are rounded to nearest: half a unit each); tile_u is reused to hold each
+  // tile's candidate slot (-1: pruned)
   int *slot_of = tile_u;
-  const float margin = 0.5f * static_cast<float>(ns) + 1.0f;
+  const float margin = (kMode == kModeFast ? 0.0f : 0.5f * static_cast<float>(ns)) + 1.0f;
   for (int t = tid; t < n_tiles; t += kRenderThreads) {
     const float bound = static_cast<float>(tile_u[t]) * unit * scale * 1.00001f + margin;
     int slot = -1;
@@ -615,21 +636,29 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
               wx = sh.taps[c > R ? c - R : R - c];
             else
               wx = reflect_weight3(x, xi, W, R, sh.taps);
-            const float iw = I * wx;
+            const float iws = I * wx * scale;
             racc_t *col = s_acc + x;
             if (short_taps && yi >= R && yi + R < H) {
               // the tap index is uniform across the wave: taps come from
               // scalar registers, not from a dependent LDS load per row
+              racc_t *top = col + (yi - R - y0) * W;  // row yi - R (dereferenced only in the strip)
+              if (ya == yi - R && yb == yi + R) {
+                // the whole footprint lies in this strip: no per-row tests
 #pragma unroll
-              for (int dd = -R; dd <= R; ++dd) {
-                const int y = yi + dd;
-                if (y >= ya && y <= yb) racc_add(col + (y - y0) * W, iw * a.taps[dd < 0 ? -dd : dd], scale);
+                for (int dd = -R; dd <= R; ++dd)
+                  atomicAdd(top + (dd + R) * W, fast_fixed(iws, a.taps[dd < 0 ? -dd : dd]));
+              } else {
+#pragma unroll
+                for (int dd = -R; dd <= R; ++dd) {
+                  const int y = yi + dd;
+                  if (y >= ya && y <= yb) atomicAdd(top + (dd + R) * W, fast_fixed(iws, a.taps[dd < 0 ? -dd : dd]));
+                }
               }
             } else {
               for (int y = ya; y <= yb; ++y) {
                 const float wy = short_taps ? reflect_weight3(y, yi, H, R, sh.taps)
                                             : reflect_weight(y, yi, H, R, sh.taps);
-                racc_add(col + (y - y0) * W, iw * wy, scale);
+                atomicAdd(col + (y - y0) * W, fast_fixed(iws, wy));
               }
             }
           } else {
