@@ -38,13 +38,14 @@ namespace dsx {
 
 constexpr int kAdjThreads = 512;
 constexpr int kAdjWaves = kAdjThreads / kWave;
-constexpr int kAdjMaxSpots = 1024;
+constexpr int kAdjMaxSpots = 1024;   // longest spot list of the two-blocks-per-CU kernel
 constexpr int kAdjMaxStrips = 256;
 constexpr int kAdjBinCap = 4096;
-#ifndef DSX_ADJ_STRIP_KB
-#define DSX_ADJ_STRIP_KB 96
-#endif
-constexpr int kAdjStripBytes = DSX_ADJ_STRIP_KB * 1024;
+// large-pattern kernel: 2,048 LDS spot slots and 64 KB strips, so a 512-wide
+// strip's bin usually fits one chunk (loaded once for the splat and reused
+// by the gather)
+constexpr int kAdjLargeSlots = 2048;
+constexpr int kAdjStripBytes = 64 * 1024;
 constexpr int kPixBits = 22;  // argmax key: value << 22 | (2^22 - 1 - pixel)
 constexpr int kNRed = 20;     // block sums: loss, sum G P, dRt[9], dL/drec[9]
 constexpr int kAdjTile = 16;      // tile edge of the fixed-point bound
@@ -84,12 +85,13 @@ struct AdjArgs {
   float cal, cx, cy, ca, sa;
   int mirrored, normalize;
   float pref, ep, inv_s2, clip;
+  float e2ep;  // exp(2 ep): ratio of consecutive row ratios of the Gaussian
   int strip_rows, n_strips, acc_words;
   int slots, bin_cap;       // LDS spot slots and strip-bin entries
   int mode;
   int want_grad;
   long long target_stride;  // elements between targets (0: one target for every pattern)
-  int *work;                // large patterns: per-block bins + per-spot sums (may be NULL)
+  int *work;                // large patterns: per-block records, bins and boxes (may be NULL)
   long long work_per_block; // ints
   int work_entries;         // bin entries per block
   float4 *gsum;             // per pattern and slot: A, BX, BY, list position (chain kernel input)
@@ -196,69 +198,110 @@ __device__ __forceinline__ void strip_pass(const AdjArgs &a, const SpotTab &t, i
   const int W = a.W;
   const int ncols = cols_max;
   const int n_items = nb * ncols;
-  for (int it = tid; it < n_items; it += kAdjThreads) {
-    const int bi = it / ncols;
-    const int c = it - bi * ncols;
-    const int q = binned ? t.bin[b0 + bi] : bi;
-    const int ya = max(static_cast<int>(t.ylo[q]), y0);
-    const int yb = min(static_cast<int>(t.yhi[q]), y1 - 1);
-    if (ya > yb) continue;
-    const int xlo = t.xb[q] >> 16, xhi = t.xb[q] & 0xffff;
-    const int x = xlo + c;
-    if (x > xhi) continue;
-    const float dx = static_cast<float>(x) - t.X[q];
-    const float cy = t.Y[q];
-    if (!kGather) {
+  if (!kGather) {
+    for (int it = tid; it < n_items; it += kAdjThreads) {
+      const int bi = it / ncols;
+      const int c = it - bi * ncols;
+      const int q = binned ? t.bin[b0 + bi] : bi;
+      const int ya = max(static_cast<int>(t.ylo[q]), y0);
+      const int yb = min(static_cast<int>(t.yhi[q]), y1 - 1);
+      if (ya > yb) continue;
+      const int xlo = t.xb[q] >> 16, xhi = t.xb[q] & 0xffff;
+      const int x = xlo + c;
+      if (x > xhi) continue;
+      const float dx = static_cast<float>(x) - t.X[q];
+      const float cy = t.Y[q];
       const float amp = t.I[q] * a.pref;
       aacc_t *col = acc + x;
       for (int y = ya; y <= yb; ++y) {
         const float dy = static_cast<float>(y) - cy;
         atomicAdd(col + (y - y0) * W, gauss_term(amp, dx, dy, a.ep, scale));
       }
-    } else {
-      float sa = 0.0f, sy = 0.0f;
-      for (int y = ya; y <= yb; ++y) {
-        const float dy = static_cast<float>(y) - cy;
-        const float kg = a.pref * expf(a.ep * (dy * dy + dx * dx)) * G[(y - y0) * W + x];
-        sa += kg;
-        sy = fmaf(kg, dy, sy);
+    }
+    return;
+  }
+  // Gather: the ncols consecutive items of a spot sit in consecutive lanes,
+  // so their column sums are combined across lanes first (segmented shuffle
+  // reduction) and only the first lane of each spot's run adds them to LDS:
+  // LDS float atomics to one address serialise, and a spot's columns would
+  // otherwise all hit the same three words.
+  const int lane = lane_id();
+  for (int base = tid - lane; base < n_items; base += kAdjThreads) {
+    const int it = base + lane;
+    const int bi = it / ncols;
+    const int c = it - bi * ncols;
+    int q = -1;
+    float sa = 0.0f, sx = 0.0f, sy = 0.0f;
+    if (it < n_items) {
+      q = binned ? t.bin[b0 + bi] : bi;
+      const int ya = max(static_cast<int>(t.ylo[q]), y0);
+      const int yb = min(static_cast<int>(t.yhi[q]), y1 - 1);
+      const int xlo = t.xb[q] >> 16, xhi = t.xb[q] & 0xffff;
+      const int x = xlo + c;
+      if (ya <= yb && x <= xhi) {
+        // down the column the Gaussian follows K(y + 1) = K(y) r(y) with
+        // r(y + 1) = r(y) exp(2 ep): two products per row instead of an exp
+        // (a few ulps over a box; the splat keeps its exact per-pixel terms)
+        const float dx = static_cast<float>(x) - t.X[q];
+        float dy = static_cast<float>(ya) - t.Y[q];
+        float k = a.pref * expf(a.ep * (dy * dy + dx * dx));
+        float r = expf(a.ep * (2.0f * dy + 1.0f));
+        const float *g = G + (ya - y0) * W + x;
+        for (int y = ya; y <= yb; ++y) {
+          const float kg = k * *g;
+          sa += kg;
+          sy = fmaf(kg, dy, sy);
+          k *= r;
+          r *= a.e2ep;
+          dy += 1.0f;
+          g += W;
+        }
+        sx = sa * dx;
       }
-      if (sa != 0.0f || sy != 0.0f) {
-        atomicAdd(&t.A[q], sa);
-        atomicAdd(&t.BX[q], sa * dx);
-        atomicAdd(&t.BY[q], sy);
+    }
+    // lane L takes lane L + d while both lie in the same spot's run; after
+    // the steps the first lane of each run (in this wave) holds its sums
+    for (int d = 1; d < ncols && d < kWave; d <<= 1) {
+      const float oa = __shfl_down(sa, d, kWave);
+      const float ox = __shfl_down(sx, d, kWave);
+      const float oy = __shfl_down(sy, d, kWave);
+      if (c + d < ncols && lane + d < kWave) {
+        sa += oa;
+        sx += ox;
+        sy += oy;
       }
+    }
+    if (q >= 0 && (c == 0 || lane == 0) && (sa != 0.0f || sx != 0.0f || sy != 0.0f)) {
+      atomicAdd(&t.A[q], sa);
+      atomicAdd(&t.BX[q], sx);
+      atomicAdd(&t.BY[q], sy);
     }
   }
 }
 
 // Load entries [base, base + m) of a strip's global bin into LDS slots 0..m-1
-// (large patterns: more in-frame spots than the LDS holds).
+// (large patterns: more in-frame spots than the LDS holds).  The clip boxes
+// come from the per-spot cache the first scan wrote (boxes[j] = {(xlo << 16) |
+// xhi, (lo << 16) | hi}); only the pixel position is recomputed.
 __device__ __forceinline__ void load_chunk(const AdjArgs &a, const SpotTab &t, const float4 *spots,
-                                           const int *bins, int base, int m, int tid) {
+                                           const int *bins, const int2 *boxes, int base, int m,
+                                           int tid) {
   for (int i = tid; i < m; i += kAdjThreads) {
     const int j = bins[base + i];
     const float4 sp = spots[j];
+    const int2 bx = boxes[j];
     float X, Y;
-    int lo = 0, hi = -1, xlo = 0, xhi = -1;
-    // the same tests as in the scan that binned j (cannot fail; kept for safety)
-    if (!to_pixel(sp, a, X, Y) ||
-        !subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, a.H, a.W, lo, hi, xlo, xhi)) {
-      lo = 1;
-      hi = 0;  // empty rows: the slot contributes nothing
-      xlo = 0;
-      xhi = 0;
-    }
+    to_pixel(sp, a, X, Y);  // in frame: the scan that binned j tested it
     t.I[i] = sp.w;
     t.X[i] = X;
     t.Y[i] = Y;
     t.A[i] = 0.0f;
     t.BX[i] = 0.0f;
     t.BY[i] = 0.0f;
-    t.xb[i] = (xlo << 16) | xhi;
+    t.xb[i] = bx.x;
     t.jpos[i] = j;
-    t.ylo[i] = static_cast<short>(lo);
-    t.yhi[i] = static_cast<short>(hi);
+    t.ylo[i] = static_cast<short>(bx.y >> 16);
+    t.yhi[i] = static_cast<short>(bx.y & 0xffff);
   }
 }
 
@@ -459,8 +502,13 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
   const size_t HW = static_cast<size_t>(H) * W;
   if (kLarge && tid == 0) stage_chain_consts(a, s_cc);
   if (kLarge && tid < 9) sh.drec[tid] = 0.0;
-  int *wbins = kLarge && a.work ? a.work + (size_t)blockIdx.x * a.work_per_block : nullptr;
-  float *wacc = kLarge && a.work ? reinterpret_cast<float *>(wbins + a.work_entries) : nullptr;
+  // large-pattern workspace of this block: float4 records[entries] (gathered
+  // sums of each (spot, strip) bin entry and its spot) | int bins[entries] |
+  // int2 boxes[cap] (clip boxes cached by the first scan)
+  int *wblock = kLarge && a.work ? a.work + (size_t)blockIdx.x * a.work_per_block : nullptr;
+  float4 *wrec = reinterpret_cast<float4 *>(wblock);
+  int *wbins = wblock ? wblock + 4 * static_cast<size_t>(a.work_entries) : nullptr;
+  int2 *wbox = wblock ? reinterpret_cast<int2 *>(wbins + a.work_entries) : nullptr;
   for (int i = tid; i < a.acc_words; i += kAdjThreads) s_acc[i] = 0;
   lds_sync();
 
@@ -482,10 +530,13 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
     float tmax = 0.0f;
     for (int j = tid; j < n; j += kAdjThreads) {
       const float4 sp = spots[j];
-      float X, Y;
-      if (!to_pixel(sp, a, X, Y)) continue;
-      int lo, hi, xlo, xhi;
-      if (!subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, H, W, lo, hi, xlo, xhi)) continue;
+      float X = 0.0f, Y = 0.0f;
+      int lo = 1, hi = 0, xlo = 0, xhi = 0;
+      const bool in = to_pixel(sp, a, X, Y) &&
+                      subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, H, W, lo, hi, xlo, xhi);
+      // the large-pattern scans below read the cached box ({0, 1 << 16}: none)
+      if (kLarge && wbox) wbox[j] = in ? make_int2((xlo << 16) | xhi, (lo << 16) | hi) : make_int2(0, 1 << 16);
+      if (!in) continue;
       tmax = fmaxf(tmax, fabsf(sp.w) * a.pref);
       atomicMax(&sh.cols_max, xhi - xlo + 1);
       for (int s = lo / a.strip_rows; s <= hi / a.strip_rows; ++s) atomicAdd(&sh.bin_start[s + 1], 1);
@@ -504,7 +555,10 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
     }
     tmax = wave_max(tmax);
     if (lane == 0) sh.wmaxf[wid] = tmax;
-    lds_sync();
+    if (kLarge)
+      __syncthreads();  // the cached boxes in global memory, block-visible
+    else
+      lds_sync();
     const int ns_all = sh.ns;
     float pmax = 0.0f;
     for (int w = 0; w < kAdjWaves; ++w) pmax = fmaxf(pmax, sh.wmaxf[w]);
@@ -540,20 +594,13 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
     }
     if (large) {
       for (int j = tid; j < n; j += kAdjThreads) {
-        const float4 sp = spots[j];
-        float X, Y;
-        if (!to_pixel(sp, a, X, Y)) continue;
-        int lo, hi, xlo, xhi;
-        if (!subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, H, W, lo, hi, xlo, xhi)) continue;
+        const int yb = wbox[j].y;
+        const int lo = yb >> 16, hi = yb & 0xffff;
+        if (lo > hi) continue;  // not in frame
         for (int s = lo / a.strip_rows; s <= hi / a.strip_rows; ++s)
           wbins[atomicAdd(&sh.bin_fill[s], 1)] = j;
-        if (a.want_grad) {
-          wacc[3 * j + 0] = 0.0f;
-          wacc[3 * j + 1] = 0.0f;
-          wacc[3 * j + 2] = 0.0f;
-        }
       }
-      __syncthreads();  // bins and accumulators in global memory, block-visible
+      __syncthreads();  // bins in global memory, block-visible
     } else if (binned) {
       for (int q = tid; q < ns; q += kAdjThreads) {
         for (int s = t.ylo[q] / a.strip_rows; s <= t.yhi[q] / a.strip_rows; ++s) {
@@ -589,12 +636,10 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
         }
       } else {
         for (int j = tid; j < n; j += kAdjThreads) {
-          const float4 sp = spots[j];
-          float X, Y;
-          if (!to_pixel(sp, a, X, Y)) continue;
-          int lo, hi, xlo, xhi;
-          if (!subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, H, W, lo, hi, xlo, xhi)) continue;
-          add_bound(sp.w, lo, hi, xlo, xhi);
+          const int2 bx = wbox[j];
+          const int lo = bx.y >> 16, hi = bx.y & 0xffff;
+          if (lo > hi) continue;  // not in frame
+          add_bound(spots[j].w, lo, hi, bx.x >> 16, bx.x & 0xffff);
         }
       }
       int lu = 0;
@@ -615,7 +660,10 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
     }
 
     // splat (or gather) strip s: every LDS-resident spot of its bin, or, for a
-    // large pattern, its global bin chunk by chunk
+    // large pattern, its global bin chunk by chunk.  A strip whose bin fits
+    // one chunk is gathered from the slots its splat just loaded.  Large
+    // patterns store each entry's gathered sums in the workspace record of
+    // the entry (no read-modify-write; every entry belongs to one strip).
     auto strip_spots = [&](int s, bool gather, int y0, int y1) {
       if (!large) {
         const int b0 = binned ? sh.bin_start[s] : 0;
@@ -628,22 +676,19 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
         return;
       }
       const int e0 = sh.bin_start[s], e1 = sh.bin_start[s + 1];
+      const bool single = e1 - e0 <= a.slots;
       for (int c0 = e0; c0 < e1; c0 += a.slots) {
         const int m = min(a.slots, e1 - c0);
-        load_chunk(a, t, spots, wbins, c0, m, tid);
-        lds_sync();
+        if (!(gather && single)) {
+          load_chunk(a, t, spots, wbins, wbox, c0, m, tid);
+          lds_sync();
+        }
         if (gather) {
           strip_pass<true>(a, t, cols_max, s_acc, s_G, 0, m, false, y0, y1, scale, tid);
           lds_sync();
-          for (int i = tid; i < m; i += kAdjThreads) {
-            // one entry per (spot, strip) and strips run in order: a plain
-            // read-modify-write, made visible by the barrier below
-            float *w = wacc + 3 * t.jpos[i];
-            w[0] += t.A[i];
-            w[1] += t.BX[i];
-            w[2] += t.BY[i];
-          }
-          __syncthreads();
+          for (int i = tid; i < m; i += kAdjThreads)
+            wrec[c0 + i] = make_float4(t.A[i], t.BX[i], t.BY[i], __int_as_float(t.jpos[i]));
+          lds_sync();  // the next chunk overwrites the slots
         } else {
           strip_pass<false>(a, t, cols_max, s_acc, nullptr, 0, m, false, y0, y1, scale, tid);
           lds_sync();  // the next chunk overwrites the slots
@@ -833,15 +878,27 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
             spot_chain(a, s_cc, sh.R, idx[t.jpos[q]], t.I[q], A, BX, BY, dR, dG);
           }
         } else {
-          for (int j = tid; j < n; j += kAdjThreads) {
-            const float4 sp = spots[j];
-            float X, Y;
-            if (!to_pixel(sp, a, X, Y)) continue;
-            int lo, hi, xlo, xhi;
-            if (!subpixel_box(sp.w, X, Y, a.pref, a.ep, a.clip, H, W, lo, hi, xlo, xhi)) continue;
-            float A = wacc[3 * j], BX = wacc[3 * j + 1], BY = wacc[3 * j + 2];
-            if (norm) max_term(a, corr, px, py, X, Y, xlo, xhi, lo, hi, A, BX, BY);
-            spot_chain(a, s_cc, sh.R, idx[j], sp.w, A, BX, BY, dR, dG);
+          // the chain rule is linear in the gathered sums: apply it per bin
+          // entry (a spot's strips in turn), then once more for the max term
+          // of each spot whose box holds p*
+          __syncthreads();  // the entry records in global memory, block-visible
+          for (int e = tid; e < entries; e += kAdjThreads) {
+            const float4 r = wrec[e];
+            const int j = __float_as_int(r.w);
+            spot_chain(a, s_cc, sh.R, idx[j], spots[j].w, r.x, r.y, r.z, dR, dG);
+          }
+          if (norm) {
+            for (int j = tid; j < n; j += kAdjThreads) {
+              const int2 bx = wbox[j];
+              const int lo = bx.y >> 16, hi = bx.y & 0xffff, xlo = bx.x >> 16, xhi = bx.x & 0xffff;
+              if (py < lo || py > hi || px < xlo || px > xhi) continue;
+              const float4 sp = spots[j];
+              float X, Y;
+              to_pixel(sp, a, X, Y);
+              float A = 0.0f, BX = 0.0f, BY = 0.0f;
+              max_term(a, corr, px, py, X, Y, xlo, xhi, lo, hi, A, BX, BY);
+              spot_chain(a, s_cc, sh.R, idx[j], sp.w, A, BX, BY, dR, dG);
+            }
           }
         }
 #pragma unroll
@@ -1126,10 +1183,10 @@ __global__ __launch_bounds__(kSfaThreads) void structure_factor_adjoint_kernel(S
 
 using namespace dsx;
 
-// Workspace of the large-pattern kernel, per block: int bins[kWorkEntriesPerSpot
-// * cap] | float sums[3 * cap].  Of the small-pattern kernel with gradients:
-// float4 gsum[n * cap] | int gcount[n] | (8-byte aligned) double
-// dpart[chain blocks][9].
+// Workspace of the large-pattern kernel, per block (E = kWorkEntriesPerSpot *
+// cap bin entries): float4 records[E] | int bins[E] | int2 boxes[cap], padded
+// to 16 bytes.  Of the small-pattern kernel with gradients: float4 gsum[n *
+// cap] | int gcount[n] | (8-byte aligned) double dpart[chain blocks][9].
 constexpr int kWorkEntriesPerSpot = 4;
 constexpr int kAdjLdsBytes = 160 * 1024;
 
@@ -1138,11 +1195,14 @@ static int chain_blocks_max() { return cu_count() * 8; }
 static long long small_work_ints(int cap, int n) {
   return 4ll * n * cap + n + 1 + 18ll * chain_blocks_max();
 }
+static long long large_block_ints(int cap) {
+  const long long e = kWorkEntriesPerSpot * static_cast<long long>(cap);
+  return (5 * e + 2ll * cap + 3) & ~3ll;
+}
 
 extern "C" long long dsx_render_adjoint_workspace_ints(int cap, int n) {
   if (cap < 1 || n < 0) return 0;
-  if (adjoint_large(cap))
-    return static_cast<long long>(cu_count()) * (kWorkEntriesPerSpot + 3) * static_cast<long long>(cap);
+  if (adjoint_large(cap)) return static_cast<long long>(cu_count()) * large_block_ints(cap);
   return small_work_ints(cap, n);
 }
 
@@ -1237,6 +1297,7 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   a.pref = static_cast<float>(1.0 / (2.0 * M_PI * sg * sg));
   a.ep = static_cast<float>(-1.0 / (2.0 * sg * sg));
   a.inv_s2 = static_cast<float>(1.0 / (sg * sg));
+  a.e2ep = static_cast<float>(exp(-1.0 / (sg * sg)));
   a.clip = rp->clip_threshold;
   a.mode = mode;
   a.want_grad = want_grad ? 1 : 0;
@@ -1269,7 +1330,7 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
     set_error("dsx_render_adjoint: cannot query or reserve the kernel's LDS");
     return DSX_ELAUNCH;
   }
-  a.slots = large ? kAdjMaxSpots : cap;
+  a.slots = large ? kAdjLargeSlots : cap;
   a.bin_cap = large ? kAdjBinCap : std::min(kAdjBinCap, 4 * cap + 256);
   const long long table = static_cast<long long>(kSpotTabBytes) * a.slots + 2ll * a.bin_cap + 16;
   auto rows_for = [&](int per_block) {
@@ -1298,7 +1359,11 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   a.acc_words = rows * a.W;
   const int blocks = std::min(n, cu_count() * bpc);
   if (large) {
-    const long long per_block = (kWorkEntriesPerSpot + 3) * static_cast<long long>(cap);
+    const long long per_block = large_block_ints(cap);
+    if (work_d && reinterpret_cast<uintptr_t>(work_d) % 16 != 0) {
+      set_error("dsx_render_adjoint: workspace must be 16-byte aligned");
+      return DSX_EINVAL;
+    }
     if (work_d && work_ints >= per_block * blocks &&
         kWorkEntriesPerSpot * static_cast<long long>(cap) < (1ll << 31)) {
       a.work = work_d;
