@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--orientations", type=int, default=200_000)
-    ap.add_argument("--cpu-sample", type=int, default=2000,
+    ap.add_argument("--cpu-sample", type=int, default=8000,
                     help="orientations in the CPU-baseline sample (0 disables it)")
     ap.add_argument("--refine-steps", type=int, default=10,
                     help="timed steps of the fused forward+backward leg (0 disables it)")
