@@ -20,7 +20,12 @@
 
 namespace dsx {
 
-constexpr int kOT = 4;            // orientations per wave
+#ifndef DSX_SIM_OT
+#define DSX_SIM_OT 4
+#endif
+constexpr int kOT = DSX_SIM_OT;   // orientations per wave
+constexpr int kSlotBits = kOT <= 2 ? 1 : kOT <= 4 ? 2 : kOT <= 8 ? 3 : 4;
+static_assert(kOT <= 16, "orientation slot bits");
 constexpr int kSimThreads = 256;  // 4 waves per block
 
 struct SimArgs {
@@ -50,7 +55,7 @@ __host__ __device__ __forceinline__ float pretest_threshold(double me, double k,
                             1e-6 * (2.0 * k * gmax + gmax * gmax));
 }
 
-// Candidate queue per wave: (reflection row << 2) | orientation slot, in the
+// Candidate queue per wave: (reflection row << kSlotBits) | orientation slot, in the
 // order the sweep produced them (row-major, so per orientation ascending rows).
 constexpr int kQueue = 64 + kOT * kWave;  // one batch plus one worst-case chunk
 constexpr int kWavesPerBlock = kSimThreads / kWave;
@@ -91,8 +96,8 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
     float R[kOT][3];  // third rows (beam direction) for the FP32 pretest
     int cnt[kOT];
     float lmax[kOT], lmin[kOT];
-    if (lane < kOT * 9) {
-      const int o = lane / 9, j = lane - o * 9;
+    for (int e = lane; e < kOT * 9; e += kWave) {
+      const int o = e / 9, j = e - o * 9;
       s_R[wib][o][j] = o < n_here ? Rt[(size_t)(o0 + o) * 9 + j] : 0.0;
     }
 #pragma unroll
@@ -114,8 +119,8 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
       float I = 0.0f;
       if (lane < nb) {
         const int e = queue[(head + lane) % kQueue];
-        row = e >> 2;
-        slot = e & 3;
+        row = e >> kSlotBits;
+        slot = e & (kOT - 1);
         const int4 h = hkl[row];
         const double hx = h.x, hy = h.y, hz = h.z;
         const double gx = rec[0] * hx + rec[1] * hy + rec[2] * hz;
@@ -164,7 +169,7 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
           const bool pre = v && fabsf(fmaf(-two_k, zf, g.w)) < pre_thr;
           const uint64_t pm = __ballot(pre);
           if (pm == 0) continue;
-          if (pre) queue[(head + qn + mask_rank(pm)) % kQueue] = (i << 2) | o;
+          if (pre) queue[(head + qn + mask_rank(pm)) % kQueue] = (i << kSlotBits) | o;
           qn += __popcll(pm);
         }
       }
