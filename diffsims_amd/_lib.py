@@ -43,6 +43,11 @@ ADJ_FORWARD = 2
 RENDER_SPLAT = 0
 RENDER_DENSE = 1
 
+MESH_UV_SPHERE = 0
+MESH_NORMALIZED_CUBE = 1
+MESH_SPHERIFIED_EDGE = 2
+MESH_SPHERIFIED_CORNER = 3
+
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTED = (
     "dsx_version",
@@ -51,6 +56,9 @@ EXPORTED = (
     "dsx_hkl_enumerate",
     "dsx_structure_factor",
     "dsx_stage_atoms",
+    "dsx_beam_directions_candidates",
+    "dsx_beam_directions_workspace_ints",
+    "dsx_beam_directions",
     "dsx_simulate",
     "dsx_render_max_spots",
     "dsx_render_fast",
@@ -136,6 +144,10 @@ _SIGS = {
     "dsx_structure_factor": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _P, _I,
                                   _P, _I, _I, _P, _P, _P, _P]),
     "dsx_stage_atoms": (_I, [_P, _P, _P, _I, _P, _P, _P]),
+    "dsx_beam_directions_candidates": (ctypes.c_longlong, [_I, ctypes.c_double]),
+    "dsx_beam_directions_workspace_ints": (ctypes.c_longlong, [_I, ctypes.c_double]),
+    "dsx_beam_directions": (_I, [_I, ctypes.c_double, _I, ctypes.POINTER(ctypes.c_int), _P,
+                                 ctypes.c_longlong, _P, _P, ctypes.c_longlong, _P]),
     "dsx_simulate": (_I, [_P, _P, _P, _I, _P, _I, ctypes.POINTER(SimParams), _I, _P, _P, _P,
                           _P, _P, _P]),
     "dsx_render_max_spots": (_I, []),

@@ -349,6 +349,33 @@ int dsx_structure_factor_adjoint(const int *hkl_d, int n, const double rec[9],
                                  double *drec_d, void *stream);
 
 /*
+ * Beam-direction grid of a template library — replaces get_beam_directions_grid
+ * (diffsims/generators/rotation_list_generators.py:176-267) for the meshes of
+ * diffsims/generators/sphere_mesh_generators.py: the UV sphere (:42-93) and the
+ * normalized, spherified-edge and spherified-corner cube meshes (:96-197).
+ * Writes, in the reference's row order, the Bunge angles in degrees
+ * {0, Phi, phi2} (beam_directions_grid_to_euler, :486-526) of every mesh
+ * vertex v with planes[4 i + 3] * (planes[4 i .. 4 i + 2] . v) >= -1e-13 for
+ * each of the n_planes (<= 3) crop planes (the stereographic triangle: normal
+ * = cross of two corners, sign = that normal . the third corner; the caller
+ * applies the reference's hexagonal resolution / sqrt(2)).
+ *   euler_d  double [capacity][3]; rows past capacity are not written
+ *   count_d  int64 [1]: rows of the full result
+ *   work_d   dsx_beam_directions_workspace_ints(mesh, resolution) ints
+ * dsx_beam_directions_candidates() is the vertex count (an upper bound of the
+ * rows); both return -1 for an unknown mesh or a bad resolution.
+ */
+#define DSX_MESH_UV_SPHERE 0
+#define DSX_MESH_NORMALIZED_CUBE 1
+#define DSX_MESH_SPHERIFIED_EDGE 2
+#define DSX_MESH_SPHERIFIED_CORNER 3
+long long dsx_beam_directions_candidates(int mesh, double resolution);
+long long dsx_beam_directions_workspace_ints(int mesh, double resolution);
+int dsx_beam_directions(int mesh, double resolution, int n_planes, const int *planes,
+                        double *euler_d, long long capacity, long long *count_d, int *work_d,
+                        long long work_ints, void *stream);
+
+/*
  * Polar templates from spot lists — replaces Simulation2D.polar_flatten_simulations
  * (simulation2d.py:313-355) with DiffractingVector.to_flat_polar
  * (_diffracting_vector.py:186-194) and get_closest (simulation2d.py:767-781).

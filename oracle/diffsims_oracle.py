@@ -29,6 +29,10 @@ Parity pins (what fixes each function):
                              reference does), total-intensity KAT
   render_subpixel            detector KATs (test_detector_functions.py:160-306),
                              fast-vs-slow equality KAT (test_simulations2d.py:133-180)
+  beam_directions_grid       mesh sizes and contained directions
+                             (test_sphere_mesh_generators.py:48-78), grid sizes per
+                             crystal system (test_rotation_list_generator.py:80-94),
+                             vector -> Euler KAT (test_sphere_mesh_generators.py:124-143)
 Parity unpinned (no reference test and no closed form fixes it):
   * the order of the Cartesian sort for oblique cells when rounding could tie
     (exact arithmetic gives descending lexicographic (h,k,l); see hkl_box);
@@ -782,3 +786,117 @@ def polar_flatten(spot_lists, radial_axes=None, azimuthal_axes=None):
     if radial_axes is not None and azimuthal_axes is not None:
         r_t, t_t = r_t.astype(int), t_t.astype(int)
     return r_t, t_t, i_t
+
+
+# ---------------------------------------------------------------------------
+# beam-direction grids (rotation lists for template libraries)
+# ---------------------------------------------------------------------------
+
+def uv_sphere_vertices(resolution: float) -> np.ndarray:
+    """UV (spherical-coordinate) mesh: get_uv_sphere_mesh_vertices
+    (diffsims/generators/sphere_mesh_generators.py:42-93) with
+    vectorised_spherical_polars_to_cartesians (diffsims/utils/vector_utils.py:82-105):
+    psi outer, theta inner, one pole point each (psi == 0)."""
+    n_theta = int(np.ceil(180 / resolution)) + 1
+    n_psi = int(np.ceil(360 / resolution))
+    psi = np.linspace(0, 2 * np.pi, num=n_psi, endpoint=False)
+    theta = np.linspace(0, np.pi, num=n_theta, endpoint=True)
+    P = np.repeat(psi, n_theta)
+    T = np.tile(theta, n_psi)
+    keep = (np.abs(T) > 0) | (P == 0)
+    P, T = P[keep], T[keep]
+    keep = (np.abs(T) < np.deg2rad(180)) | (P == 0)
+    P, T = P[keep], T[keep]
+    return np.stack([np.cos(P) * np.sin(T), np.sin(P) * np.sin(T), np.cos(T)], axis=1)
+
+
+def cube_mesh_vertices(resolution: float, grid_type: str = "spherified_corner") -> np.ndarray:
+    """Cube mesh projected on the sphere: get_cube_mesh_vertices
+    (sphere_mesh_generators.py:96-197): per-face grid coordinates i (linear,
+    equal angles towards the edges, or equal angles towards the corners), the
+    six faces in the order bottom, top, east, west, south, north, then the two
+    missing corners; rows normalised."""
+    if grid_type == "normalized":
+        steps = np.ceil(1 / np.tan(np.deg2rad(resolution)))
+        i = np.arange(-steps, steps) / steps
+    elif grid_type == "spherified_edge":
+        steps = np.ceil(np.rad2deg(np.deg2rad(45)) / resolution)
+        i = np.tan(np.arange(-steps, steps) * (np.arctan(1.0) / steps))
+    elif grid_type == "spherified_corner":
+        steps = np.ceil(np.arccos(1 / np.sqrt(3)) / np.deg2rad(resolution))
+        i = np.tan(np.arange(-steps, steps) * (np.arctan(np.sqrt(2)) / steps)) / np.sqrt(2)
+    else:
+        raise ValueError(f"grid type {grid_type} not a valid grid type")
+    x, y = np.meshgrid(i, i)
+    x, y = x.ravel(), y.ravel()
+    z = np.ones(x.shape[0])
+    faces = [(-x, -y, -z), (x, y, z), (z, x, -y), (-z, -x, y), (x, -z, y), (-x, z, -y)]
+    v = np.vstack([np.stack(f, axis=1) for f in faces] + [np.array([[-1.0, 1, 1], [1, -1, -1]])])
+    return v / np.linalg.norm(v, axis=1)[:, None]
+
+
+def directions_to_euler(v: np.ndarray) -> np.ndarray:
+    """beam_directions_grid_to_euler (sphere_mesh_generators.py:486-526): Bunge
+    angles in degrees with phi1 = 0, Phi = acos(z/|v|), phi2 = 90 deg minus
+    the signed azimuth (sign of y, of x where y == 0; 0 on the pole)."""
+    v = np.asarray(v, dtype=np.float64)
+    norm = np.linalg.norm(v, axis=1)
+    Phi = np.arccos(v[:, 2] / norm)
+    nproj = np.linalg.norm(v[:, :2], axis=1)
+    sign = np.sign(v[:, 1])
+    sign[v[:, 1] == 0] = np.sign(v[v[:, 1] == 0, 0])
+    with np.errstate(invalid="ignore", divide="ignore"):
+        phi2 = sign * np.nan_to_num(np.arccos(v[:, 0] / nproj))
+    return np.rad2deg(np.stack([np.zeros(len(v)), Phi, np.pi / 2 - phi2], axis=1))
+
+
+# stereographic triangles, rotation_list_generators.py:48-55
+CRYSTAL_TRIANGLES = {
+    "cubic": [(0, 0, 1), (1, 1, 1), (1, 0, 1)],
+    "hexagonal": [(0, 0, 0, 1), (9, 1, -10, 0), (2, -1, -1, 0)],
+    "trigonal": [(0, 0, 0, 1), (-2, 1, 1, 0), (-1, 2, -1, 0)],
+    "tetragonal": [(0, 0, 1), (1, 0, 0), (1, 1, 0)],
+    "orthorhombic": [(0, 0, 1), (-1, 0, 0), (0, 1, 0)],
+    "monoclinic": [(0, -1, 0), (0, 0, 1), (0, 1, 0)],
+}
+
+
+def uvtw_to_uvw(d):
+    """Four-index direction -> three-index (diffsims/utils/sim_utils.py:496-510)."""
+    u, v, t, w = d
+    u, v, w = 2 * u + v, 2 * v + u, w
+    g = math.gcd(math.gcd(u, v), w)
+    return tuple(int(x / g) for x in (u, v, w))
+
+
+def beam_directions_grid(crystal_system: str, resolution: float,
+                         mesh: str = "spherified_cube_edge") -> np.ndarray:
+    """get_beam_directions_grid (diffsims/generators/rotation_list_generators.py:
+    176-267) for the deterministic meshes: the mesh, then the crop to the
+    crystal system's stereographic triangle (each side: the sign of the
+    triangle's opposite corner, tolerance -1e-13), then directions_to_euler.
+    The monoclinic branch keeps only its second test (x >= -1e-13): its first
+    result is overwritten (rotation_list_generators.py:236-241)."""
+    if mesh == "uv_sphere":
+        v = uv_sphere_vertices(resolution)
+    elif mesh == "spherified_cube_corner":
+        v = cube_mesh_vertices(resolution, "spherified_corner")
+    elif mesh in ("normalized_cube", "spherified_cube_edge"):
+        if crystal_system == "hexagonal":
+            resolution = resolution / np.sqrt(2)
+        v = cube_mesh_vertices(resolution, "normalized" if mesh == "normalized_cube"
+                               else "spherified_edge")
+    else:
+        raise NotImplementedError(f"mesh {mesh}")
+    eps = -1e-13
+    if crystal_system == "triclinic":
+        return directions_to_euler(v)
+    if crystal_system == "monoclinic":
+        return directions_to_euler(v[np.dot(np.array([1, 0, 0]), v.T) >= eps])
+    a, b, c = CRYSTAL_TRIANGLES[crystal_system]
+    if len(a) == 4:
+        a, b, c = uvtw_to_uvw(a), uvtw_to_uvw(b), uvtw_to_uvw(c)
+    for p, q, r in ((a, b, c), (b, c, a), (c, a, b)):
+        n = np.cross(p, q)
+        v = v[np.dot(n, r) * np.dot(n, v.T) >= eps]
+    return directions_to_euler(v)

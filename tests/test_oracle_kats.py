@@ -418,3 +418,55 @@ def test_precession_integral_closed_forms():
     brute = [np.trapezoid(np.maximum(1 - np.abs(si + ai * np.cos(t)) / w, 0), t) / np.pi
              for si, ai in zip(s, A)]
     assert np.allclose(lin, brute, rtol=1e-7, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# beam-direction grids: the reference's own known answers
+# ---------------------------------------------------------------------------
+
+def test_sphere_mesh_kats():
+    """test_sphere_mesh_generators.py:48-78: mesh sizes at 10 deg, zero sum,
+    no duplicates, and the four directions every cube mesh contains."""
+    uv = O.uv_sphere_vertices(10)
+    assert uv.shape == (614, 3)
+    assert abs(uv.sum()) < 1e-7
+    assert np.unique(uv, axis=0).shape[0] == 614
+    want = np.array([[1, 1, 1], [1, 0, 0], [1, 1, 0], [-1, 1, -1]], dtype=float)
+    want = np.round(want / np.linalg.norm(want, axis=1)[:, None], 13).tolist()
+    for grid_type, n in (("normalized", 866), ("spherified_edge", 602), ("spherified_corner", 866)):
+        g = O.cube_mesh_vertices(10, grid_type)
+        assert g.shape == (n, 3)
+        assert abs(g.sum()) < 1e-7
+        assert np.unique(g, axis=0).shape[0] == n
+        rounded = np.round(g, 13).tolist()
+        for w in want:
+            assert w in rounded
+    with pytest.raises(ValueError):
+        O.cube_mesh_vertices(10, "non_existant")
+
+
+@pytest.mark.parametrize("crystal_system, size", [
+    ("cubic", 300), ("hexagonal", 1050), ("trigonal", 1657), ("tetragonal", 852),
+    ("orthorhombic", 1657), ("monoclinic", 6441), ("triclinic", 12698)])
+def test_beam_directions_grid_size_kat(crystal_system, size):
+    """test_rotation_list_generator.py:80-94 (default mesh, 2 deg)."""
+    assert O.beam_directions_grid(crystal_system, 2).shape == (size, 3)
+
+
+def test_directions_to_euler_kat():
+    """test_sphere_mesh_generators.py:124-143."""
+    v = np.array([[1, 0, 0], [0, 1, 0], [0, 1, 1], [1, 0, 1]], dtype=float)
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    assert np.allclose(O.directions_to_euler(v), [[0, 90, 90], [0, 90, 0], [0, 45, 0], [0, 45, 90]])
+
+
+def test_beam_directions_euler_is_the_direction():
+    """Closed form: with g_lab = M^T g, the crystal direction along the beam
+    (lab z) of each grid orientation is M e_z, and every such direction lies
+    in the cubic standard triangle 001-101-111."""
+    e = O.beam_directions_grid("cubic", 3)
+    M = O.bunge_matrix(np.deg2rad(e))
+    d = M[:, :, 2]
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0)
+    assert np.all(d[:, 2] >= d[:, 0] - 1e-12) and np.all(d[:, 0] >= d[:, 1] - 1e-12)
+    assert np.all(d[:, 1] >= -1e-12)

@@ -93,3 +93,45 @@ def test_struct_layouts_match_the_header(tmp_path):
                                            check=True).stdout.split()]
     assert vals == [ctypes.sizeof(_lib.SimParams), _lib.SimParams.rec.offset,
                     ctypes.sizeof(_lib.RenderParams), _lib.RenderParams.taps.offset]
+
+
+def test_beam_direction_mesh_sizes_on_the_host():
+    """dsx_beam_directions_candidates (host arithmetic) equals the oracle's
+    mesh sizes: every cube-mesh vertex is a candidate; the UV sphere drops
+    all but one point per pole from its psi x theta product."""
+    from oracle import diffsims_oracle as O
+
+    lib = _lib.load()
+    for res in (10.0, 3.0, 1.3):
+        for mesh, grid_type in ((_lib.MESH_NORMALIZED_CUBE, "normalized"),
+                                (_lib.MESH_SPHERIFIED_EDGE, "spherified_edge"),
+                                (_lib.MESH_SPHERIFIED_CORNER, "spherified_corner")):
+            assert lib.dsx_beam_directions_candidates(mesh, res) == O.cube_mesh_vertices(res, grid_type).shape[0]
+        n_theta, n_psi = int(np.ceil(180 / res)) + 1, int(np.ceil(360 / res))
+        assert lib.dsx_beam_directions_candidates(_lib.MESH_UV_SPHERE, res) == n_theta * n_psi
+        assert O.uv_sphere_vertices(res).shape[0] == n_psi * (n_theta - 2) + 2
+    assert lib.dsx_beam_directions_candidates(7, 1.0) == -1
+    assert lib.dsx_beam_directions_candidates(_lib.MESH_UV_SPHERE, 0.0) == -1
+    text = open(HEADER).read()
+    for name in ("UV_SPHERE", "NORMALIZED_CUBE", "SPHERIFIED_EDGE", "SPHERIFIED_CORNER"):
+        v = int(re.search(rf"#define DSX_MESH_{name} (\d+)", text).group(1))
+        assert v == getattr(_lib, f"MESH_{name}")
+
+
+def test_crop_planes_match_the_oracle_triangles():
+    """The integer crop rows equal the oracle's triangle normals and signs."""
+    from oracle import diffsims_oracle as O
+    from diffsims_amd import generators
+
+    for cs, corners in O.CRYSTAL_TRIANGLES.items():
+        rows = generators.crop_planes(cs)
+        if cs == "monoclinic":
+            assert rows == [(1, 0, 0, 1)]
+            continue
+        a, b, c = corners
+        if len(a) == 4:
+            a, b, c = O.uvtw_to_uvw(a), O.uvtw_to_uvw(b), O.uvtw_to_uvw(c)
+        for row, (p, q, r) in zip(rows, ((a, b, c), (b, c, a), (c, a, b))):
+            n = np.cross(p, q)
+            assert row == tuple(int(x) for x in n) + (int(np.dot(n, r)),)
+    assert generators.crop_planes("triclinic") == []
