@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--large-chunks", type=int, default=1,
                     help="65,536-orientation chunks of the 500-atom precession case to run "
                          "fused forward+backward (16 = the full 10^6 orientations; 0 disables)")
+    ap.add_argument("--grid-resolution", type=float, default=0.1,
+                    help="degrees of the triclinic beam-direction grid leg (0 disables it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (the reference "
                          "kernel for scaling the HBM byte counters of a profiler) and exit")
@@ -213,6 +215,30 @@ def ragged_leg(dev, steps, scale):
     del lib
     torch.cuda.empty_cache()
     return res
+
+
+def grid_leg(dev, resolution):
+    """Beam-direction grid of a template library on the device (triclinic,
+    spherified-cube-edge mesh): directions/s of dsx_beam_directions (its
+    parity with the oracle is a GPU test)."""
+    import torch
+
+    from diffsims_amd import generators
+
+    g = generators.beam_directions_grid_device("triclinic", resolution, device=dev)
+    torch.cuda.synchronize()
+    reps = 5
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        generators.beam_directions_grid_device("triclinic", resolution, device=dev)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    n = int(g.shape[0])
+    return {"workload": f"triclinic beam-direction grid, spherified_cube_edge mesh, {resolution} deg",
+            "directions": n, "ms_per_grid": round(ms, 3),
+            "directions_per_s": round(n / (ms * 1e-3), 1)}
 
 
 def large_precession_leg(dev, n_chunks):
@@ -405,6 +431,8 @@ def main():
         torch.cuda.empty_cache()
     if args.large_chunks > 0:
         res["large_precession"] = large_precession_leg(dev, args.large_chunks)
+    if args.grid_resolution > 0:
+        res["beam_grid"] = grid_leg(dev, args.grid_resolution)
     print(json.dumps(res), flush=True)
 
 
