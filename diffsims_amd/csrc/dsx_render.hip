@@ -749,6 +749,13 @@ __device__ __forceinline__ int reflect_index(int x, int n) {
   return m < n ? m : period - 1 - m;
 }
 
+// reflect_index for |x - clamp(x)| <= R: one reflection suffices when R < n
+// (selects instead of an integer modulo)
+__device__ __forceinline__ int reflect_near(int x, int n, int R) {
+  if (R >= n) return reflect_index(x, n);
+  return x < 0 ? -x - 1 : (x >= n ? 2 * n - 1 - x : x);
+}
+
 template <int kR>
 __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs a) {
   extern __shared__ __align__(16) float s_buf[];
@@ -801,22 +808,32 @@ __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs 
       lds_sync();
       continue;
     }
-    // rows: Af = B (*) taps along x
-    for (int i = tid; i < HW; i += kDenseThreads) {
-      const int y = i / W, x = i - y * W;
-      const float *row = B + y * W;
-      float v = s_taps[0] * row[x];
-      if (x >= R && x + R < W) {
+    // rows: Af = B (*) taps along x.  The interior columns run without
+    // boundary tests; the 2R edge columns (every column when 2R >= W) go
+    // through a separate loop, so no wave mixes the two paths.
+    const int Wi = W - 2 * R;
+    if (Wi > 0) {
+      for (int i = tid; i < H * Wi; i += kDenseThreads) {
+        const int y = i / Wi, x = R + (i - y * Wi);
+        const float *row = B + y * W;
+        float v = s_taps[0] * row[x];
 #pragma unroll
         for (int d = 1; d <= (kR > 0 ? kR : kMaxR); ++d) {
           if (kR == 0 && d > R) break;
           v = fmaf(s_taps[d], row[x - d] + row[x + d], v);
         }
-      } else {
-        for (int d = 1; d <= R; ++d)
-          v = fmaf(s_taps[d], row[reflect_index(x - d, W)] + row[reflect_index(x + d, W)], v);
+        Af[y * W + x] = v;
       }
-      Af[i] = v;
+    }
+    const int ne = Wi > 0 ? 2 * R : W;
+    for (int i = tid; i < H * ne; i += kDenseThreads) {
+      const int y = i / ne, e = i - y * ne;
+      const int x = Wi > 0 && e >= R ? W - 2 * R + e : e;
+      const float *row = B + y * W;
+      float v = s_taps[0] * row[x];
+      for (int d = 1; d <= R; ++d)
+        v = fmaf(s_taps[d], row[reflect_near(x - d, W, R)] + row[reflect_near(x + d, W, R)], v);
+      Af[y * W + x] = v;
     }
     lds_sync();
     // columns: B = Af (*) taps along y, and the maximum
@@ -833,7 +850,7 @@ __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs 
         }
       } else {
         for (int d = 1; d <= R; ++d)
-          v = fmaf(s_taps[d], col[reflect_index(y - d, H) * W] + col[reflect_index(y + d, H) * W], v);
+          v = fmaf(s_taps[d], col[reflect_near(y - d, H, R) * W] + col[reflect_near(y + d, H, R) * W], v);
       }
       B[i] = v;
       lmax = fmaxf(lmax, v);
