@@ -740,6 +740,18 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
 
 constexpr int kDenseThreads = 1024;
 constexpr int kDenseMaxPix = 16384;  // two FP32 H x W buffers in 128 KB of LDS
+// register-blocked separable filter: two padded buffers (transposed with row
+// H + 1, then row-major with row W + 1) of at most this many words each
+constexpr int kDenseRegionWords = 20352;
+constexpr int kDenseSeg = 16;  // outputs per thread and item of the blocked filter
+
+__host__ __device__ __forceinline__ bool dense_blocked(int R_tmpl, int H, int W) {
+  return R_tmpl > 0 && H * W + (H > W ? H : W) <= kDenseRegionWords;
+}
+
+__host__ __device__ __forceinline__ int dense_region_words(int R_tmpl, int H, int W) {
+  return dense_blocked(R_tmpl, H, W) ? ((H * W + (H > W ? H : W) + 3) & ~3) : ((H * W + 3) & ~3);
+}
 
 // scipy.ndimage 'reflect' (half-sample symmetric) index for any offset
 __device__ __forceinline__ int reflect_index(int x, int n) {
@@ -767,10 +779,10 @@ __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs 
   const int wid = tid / kWave;
   const int H = a.H, W = a.W, HW = a.H * a.W;
   const int R = kR > 0 ? kR : a.R;
-  const int HWp = (HW + 3) & ~3;
-  int *A = reinterpret_cast<int *>(s_buf);  // owner map, then the row-filtered image
+  const bool blocked = dense_blocked(kR, H, W);
+  int *A = reinterpret_cast<int *>(s_buf);  // owner map, then the first filter pass
   float *Af = s_buf;
-  float *B = s_buf + HWp;                   // spot image, then the filtered pattern
+  float *B = s_buf + dense_region_words(kR, H, W);  // spot image, then the filtered pattern
   for (int i = tid; i <= kMaxR; i += kDenseThreads) s_taps[i] = a.taps[i];
 
   for (int pi = blockIdx.x; pi < a.n; pi += gridDim.x) {
@@ -808,52 +820,105 @@ __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs 
       lds_sync();
       continue;
     }
-    // rows: Af = B (*) taps along x.  The interior columns run without
-    // boundary tests; the 2R edge columns (every column when 2R >= W) go
-    // through a separate loop, so no wave mixes the two paths.
-    const int Wi = W - 2 * R;
-    if (Wi > 0) {
-      for (int i = tid; i < H * Wi; i += kDenseThreads) {
-        const int y = i / Wi, x = R + (i - y * Wi);
+    float lmax = 0.0f;
+    int os = W;  // row stride of the filtered pattern in B
+    if (blocked) {
+      // scipy's order (axis 0, then axis 1), each pass register-blocked: an
+      // item loads kDenseSeg + 2R inputs along its line once and forms
+      // kDenseSeg outputs.  Pass 1 reads columns of B (lanes = consecutive x)
+      // and writes them transposed (Af[x (H + 1) + y]); pass 2 reads those
+      // (lanes = consecutive y) and writes rows of stride W + 1: the odd
+      // strides keep both LDS write patterns free of bank conflicts.
+      constexpr int kS = kDenseSeg, kN = kDenseSeg + 2 * (kR > 0 ? kR : 1);
+      const int Hp = H + 1, Wp = W + 1;
+      const int nsy = (H + kS - 1) / kS;
+      for (int it = tid; it < W * nsy; it += kDenseThreads) {
+        const int x = it % W, y0 = (it / W) * kS;
+        float v[kN];
+#pragma unroll
+        for (int k = 0; k < kN; ++k) {
+          const int y = y0 - kR + k;
+          v[k] = y < H + kR ? B[reflect_near(y, H, kR) * W + x] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kS; ++k) {
+          if (y0 + k < H) {
+            float o = s_taps[0] * v[k + kR];
+#pragma unroll
+            for (int d = 1; d <= kR; ++d) o = fmaf(s_taps[d], v[k + kR - d] + v[k + kR + d], o);
+            Af[x * Hp + y0 + k] = o;
+          }
+        }
+      }
+      lds_sync();
+      const int nsx = (W + kS - 1) / kS;
+      for (int it = tid; it < H * nsx; it += kDenseThreads) {
+        const int y = it % H, x0 = (it / H) * kS;
+        float v[kN];
+#pragma unroll
+        for (int k = 0; k < kN; ++k) {
+          const int x = x0 - kR + k;
+          v[k] = x < W + kR ? Af[reflect_near(x, W, kR) * Hp + y] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kS; ++k) {
+          if (x0 + k < W) {
+            float o = s_taps[0] * v[k + kR];
+#pragma unroll
+            for (int d = 1; d <= kR; ++d) o = fmaf(s_taps[d], v[k + kR - d] + v[k + kR + d], o);
+            B[y * Wp + x0 + k] = o;
+            lmax = fmaxf(lmax, o);
+          }
+        }
+      }
+      os = Wp;
+    } else {
+      // rows: Af = B (*) taps along x.  The interior columns run without
+      // boundary tests; the 2R edge columns (every column when 2R >= W) go
+      // through a separate loop, so no wave mixes the two paths.
+      const int Wi = W - 2 * R;
+      if (Wi > 0) {
+        for (int i = tid; i < H * Wi; i += kDenseThreads) {
+          const int y = i / Wi, x = R + (i - y * Wi);
+          const float *row = B + y * W;
+          float v = s_taps[0] * row[x];
+#pragma unroll
+          for (int d = 1; d <= (kR > 0 ? kR : kMaxR); ++d) {
+            if (kR == 0 && d > R) break;
+            v = fmaf(s_taps[d], row[x - d] + row[x + d], v);
+          }
+          Af[y * W + x] = v;
+        }
+      }
+      const int ne = Wi > 0 ? 2 * R : W;
+      for (int i = tid; i < H * ne; i += kDenseThreads) {
+        const int y = i / ne, e = i - y * ne;
+        const int x = Wi > 0 && e >= R ? W - 2 * R + e : e;
         const float *row = B + y * W;
         float v = s_taps[0] * row[x];
-#pragma unroll
-        for (int d = 1; d <= (kR > 0 ? kR : kMaxR); ++d) {
-          if (kR == 0 && d > R) break;
-          v = fmaf(s_taps[d], row[x - d] + row[x + d], v);
-        }
+        for (int d = 1; d <= R; ++d)
+          v = fmaf(s_taps[d], row[reflect_near(x - d, W, R)] + row[reflect_near(x + d, W, R)], v);
         Af[y * W + x] = v;
       }
-    }
-    const int ne = Wi > 0 ? 2 * R : W;
-    for (int i = tid; i < H * ne; i += kDenseThreads) {
-      const int y = i / ne, e = i - y * ne;
-      const int x = Wi > 0 && e >= R ? W - 2 * R + e : e;
-      const float *row = B + y * W;
-      float v = s_taps[0] * row[x];
-      for (int d = 1; d <= R; ++d)
-        v = fmaf(s_taps[d], row[reflect_near(x - d, W, R)] + row[reflect_near(x + d, W, R)], v);
-      Af[y * W + x] = v;
-    }
-    lds_sync();
-    // columns: B = Af (*) taps along y, and the maximum
-    float lmax = 0.0f;
-    for (int i = tid; i < HW; i += kDenseThreads) {
-      const int y = i / W, x = i - y * W;
-      const float *col = Af + x;
-      float v = s_taps[0] * col[y * W];
-      if (y >= R && y + R < H) {
+      lds_sync();
+      // columns: B = Af (*) taps along y, and the maximum
+      for (int i = tid; i < HW; i += kDenseThreads) {
+        const int y = i / W, x = i - y * W;
+        const float *col = Af + x;
+        float v = s_taps[0] * col[y * W];
+        if (y >= R && y + R < H) {
 #pragma unroll
-        for (int d = 1; d <= (kR > 0 ? kR : kMaxR); ++d) {
-          if (kR == 0 && d > R) break;
-          v = fmaf(s_taps[d], col[(y - d) * W] + col[(y + d) * W], v);
+          for (int d = 1; d <= (kR > 0 ? kR : kMaxR); ++d) {
+            if (kR == 0 && d > R) break;
+            v = fmaf(s_taps[d], col[(y - d) * W] + col[(y + d) * W], v);
+          }
+        } else {
+          for (int d = 1; d <= R; ++d)
+            v = fmaf(s_taps[d], col[reflect_near(y - d, H, R) * W] + col[reflect_near(y + d, H, R) * W], v);
         }
-      } else {
-        for (int d = 1; d <= R; ++d)
-          v = fmaf(s_taps[d], col[reflect_near(y - d, H, R) * W] + col[reflect_near(y + d, H, R) * W], v);
+        B[i] = v;
+        lmax = fmaxf(lmax, v);
       }
-      B[i] = v;
-      lmax = fmaxf(lmax, v);
     }
     lmax = wave_max(lmax);
     if (lane == 0) s_wmax[wid] = lmax;
@@ -862,13 +927,22 @@ __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs 
     for (int w = 0; w < kDenseThreads / kWave; ++w) m = fmaxf(m, s_wmax[w]);
     // a zero maximum leaves the zeros as they are instead of producing 0/0
     const float sc = (a.normalize && m > 0.0f) ? 1.0f / m : 1.0f;
-    if ((HW & 3) == 0) {
+    if (os == W && (HW & 3) == 0) {
       for (int i = tid; i < HW / 4; i += kDenseThreads) {
         const float4 v = reinterpret_cast<const float4 *>(B)[i];
         store_nt(make_float4(v.x * sc, v.y * sc, v.z * sc, v.w * sc), reinterpret_cast<float4 *>(out) + i);
       }
+    } else if ((W & 3) == 0) {
+      for (int i = tid; i < HW / 4; i += kDenseThreads) {
+        const int y = (4 * i) / W, x = 4 * i - y * W;
+        const float *r = B + y * os + x;
+        store_nt(make_float4(r[0] * sc, r[1] * sc, r[2] * sc, r[3] * sc), reinterpret_cast<float4 *>(out) + i);
+      }
     } else {
-      for (int i = tid; i < HW; i += kDenseThreads) __builtin_nontemporal_store(B[i] * sc, out + i);
+      for (int i = tid; i < HW; i += kDenseThreads) {
+        const int y = i / W, x = i - y * W;
+        __builtin_nontemporal_store(B[y * os + x] * sc, out + i);
+      }
     }
     lds_sync();
   }
@@ -1115,7 +1189,7 @@ static int launch_dense_r(const RenderArgs &a, int blocks, size_t lds, hipStream
   if (reserved < 0) {
     if (hipFuncSetAttribute(reinterpret_cast<const void *>(&render_dense_kernel<kR>),
                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                            static_cast<int>(2 * kDenseMaxPix * sizeof(float))) != hipSuccess) {
+                            static_cast<int>(2 * kDenseRegionWords * sizeof(float))) != hipSuccess) {
       (void)hipGetLastError();
       set_error("dsx_render_fast_ragged: cannot reserve %zu bytes of LDS", lds);
       return DSX_ELAUNCH;
@@ -1132,7 +1206,7 @@ static int launch_dense(const RenderArgs &a, int blocks_per_cu, hipStream_t st, 
     set_error("%s: the dense renderer holds at most %d pixels (got %dx%d)", name, kDenseMaxPix, a.H, a.W);
     return DSX_EINVAL;
   }
-  const size_t lds = 2 * static_cast<size_t>((HW + 3) & ~3) * sizeof(float);
+  const size_t lds = 2 * static_cast<size_t>(dense_region_words(a.R <= 8 ? a.R : 0, a.H, a.W)) * sizeof(float);
   const int per_cu = std::max(1, std::min(blocks_per_cu > 0 ? blocks_per_cu : 1,
                                           static_cast<int>((160 * 1024) / (lds + 1024))));
   const int blocks = std::min(a.n, cu_count() * per_cu);
