@@ -56,6 +56,8 @@ EXPORTED = (
     "dsx_hkl_enumerate",
     "dsx_structure_factor",
     "dsx_stage_atoms",
+    "dsx_hkl_lines_workspace_ints",
+    "dsx_hkl_lines",
     "dsx_beam_directions_candidates",
     "dsx_beam_directions_workspace_ints",
     "dsx_beam_directions",
@@ -105,12 +107,13 @@ class PhaseDesc(ctypes.Structure):
         ("g_max", ctypes.c_double),
         ("Rt", ctypes.c_void_p),
         ("n_orient", ctypes.c_int),
-        ("reserved", ctypes.c_int),
+        ("n_lines", ctypes.c_int),
         ("idx", ctypes.c_void_p),
         ("spots", ctypes.c_void_p),
         ("count", ctypes.c_void_p),
         ("ncand", ctypes.c_void_p),
         ("maxI", ctypes.c_void_p),
+        ("lines", ctypes.c_void_p),
     ]
 
 
@@ -148,8 +151,10 @@ _SIGS = {
     "dsx_beam_directions_workspace_ints": (ctypes.c_longlong, [_I, ctypes.c_double]),
     "dsx_beam_directions": (_I, [_I, ctypes.c_double, _I, ctypes.POINTER(ctypes.c_int), _P,
                                  ctypes.c_longlong, _P, _P, ctypes.c_longlong, _P]),
-    "dsx_simulate": (_I, [_P, _P, _P, _I, _P, _I, ctypes.POINTER(SimParams), _I, _P, _P, _P,
-                          _P, _P, _P]),
+    "dsx_hkl_lines_workspace_ints": (ctypes.c_longlong, [_I]),
+    "dsx_hkl_lines": (_I, [_P, _P, _I, _P, _I, _P, _P, ctypes.c_longlong, _P]),
+    "dsx_simulate": (_I, [_P, _P, _P, _I, _P, _I, _P, _I, ctypes.POINTER(SimParams), _I, _P, _P,
+                          _P, _P, _P, _P]),
     "dsx_render_max_spots": (_I, []),
     "dsx_render_fast": (_I, [_P, _P, _I, _I, ctypes.POINTER(RenderParams), _P, _P, _P]),
     "dsx_render_subpixel": (_I, [_P, _P, _I, _I, ctypes.POINTER(RenderParams), _P, _P, _P]),

@@ -13,6 +13,15 @@
 // appended in reflection order with a wave ballot + mbcnt prefix.  The relative intensity cut needs the maximum over the whole
 // intersecting set, so survivors are written first and, only if some fall
 // below the cut, re-compacted in place by the same wave.
+//
+// With a line index (dsx_hkl_lines) the sweep visits only rows that can pass
+// the pretest: a row passes only if its beam coordinate z lies in a slab
+// [(wmin - thr) / 2k, (wmax + thr) / 2k] (w = |g|^2 over the row's line), and
+// along a line z is linear in l, so each line contributes one interval of
+// rows per orientation.  The intervals of 64 lines are flattened across the
+// wave (prefix sum, per-lane binary search) in row order, so the pretest sees
+// exactly the rows it would pass in a full sweep, in the same order, and the
+// outputs are identical.
 #include <algorithm>
 
 #include "dsx_internal.h"
@@ -27,6 +36,13 @@ constexpr int kOT = DSX_SIM_OT;   // orientations per wave
 constexpr int kSlotBits = kOT <= 2 ? 1 : kOT <= 4 ? 2 : kOT <= 8 ? 3 : 4;
 static_assert(kOT <= 16, "orientation slot bits");
 constexpr int kSimThreads = 256;  // 4 waves per block
+// The line sweep pays once lines average this many rows (measured break-even
+// about 15: a line costs about as much as a 64-row step of the full sweep).
+constexpr int kMinLineRows = 16;
+
+__host__ __device__ __forceinline__ bool lines_pay(int n_refl, int n_lines) {
+  return n_lines > 0 && static_cast<long long>(n_refl) >= static_cast<long long>(kMinLineRows) * n_lines;
+}
 
 struct SimArgs {
   double k, k2;
@@ -60,10 +76,11 @@ __host__ __device__ __forceinline__ float pretest_threshold(double me, double k,
 constexpr int kQueue = 64 + kOT * kWave;  // one batch plus one worst-case chunk
 constexpr int kWavesPerBlock = kSimThreads / kWave;
 
-template <bool kRagged, bool kAvg>
+template <bool kRagged, bool kAvg, bool kLines>
 __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
   __shared__ int s_queue[kWavesPerBlock][kQueue];
   __shared__ double s_R[kWavesPerBlock][kOT][9];
+  __shared__ double s_abc[kWavesPerBlock][kOT][4];  // beam . (a*, b*, c*), 1 / (beam . c*)
   const int lane = lane_id();
   const int wib = threadIdx.x / kWave;
   int *queue = s_queue[wib];
@@ -156,32 +173,153 @@ __global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
       qn -= nb;
     };
 
+    // Queue the rows of this lane that pass the pretest of orientation o
+    // (beam direction r0, r1, r2 in FP32).
+    auto pretest = [&](bool v, int row, const float4 &g, float r0, float r1, float r2, int o) {
+      const float zf = fmaf(r0, g.x, fmaf(r1, g.y, r2 * g.z));
+      const bool pre = v && fabsf(fmaf(-two_k, zf, g.w)) < pre_thr;
+      const uint64_t pm = __ballot(pre);
+      if (pm == 0) return;
+      if (pre) queue[(head + qn + mask_rank(pm)) % kQueue] = (row << kSlotBits) | o;
+      qn += __popcll(pm);
+    };
+
+    // Line sweep state: current group of 64 lines (lb), orientation (lo),
+    // window (win) of the group's total candidates for lo; this lane's line.
+    const bool use_lines = kLines && d->lines != nullptr && lines_pay(n_refl, d->n_lines);
+    const int4 *lines = reinterpret_cast<const int4 *>(d->lines);
+    const int n_lines = use_lines ? d->n_lines : 0;
+    int lb = -kWave, lo = kOT - 1, win = 0, total = 0;
+    int first = 0, nrow = 0, lh = 0, lk = 0, l0 = 0, P = 0, rb = 0;
+    double zlo = 0.0, zhi = -1.0;
+    float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
+    // pretest pass => |w - 2k zf| < thr (1 + 2^-24); zf is within 3e-7 |g|
+    // of the FP64 beam coordinate, covered by the margin zm below
+    const double two_kd = static_cast<double>(two_k);
+    const double thr = static_cast<double>(pre_thr) * (1.0 + 1e-6);
+    if (use_lines) {
+      // beam direction (third row of R) against the reciprocal axes, FP64
+      if (lane < kOT * 4) {
+        const int o = lane / 4, c = lane - 4 * o, cc = c < 3 ? c : 2;
+        double v = 0.0;
+        if (o < n_here) {
+          const double *b = Rt + (size_t)(o0 + o) * 9 + 6;
+          v = b[0] * rec[cc] + b[1] * rec[3 + cc] + b[2] * rec[6 + cc];
+          // 1 / (beam . c*); 0 marks a line direction perpendicular to the beam
+          if (c == 3) v = fabs(v) < 1e-200 ? 0.0 : 1.0 / v;
+        }
+        s_abc[wib][o][c] = v;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    }
+
+    // One step per iteration: a window of 64 line candidates of one
+    // orientation, or 64 table rows for every orientation; full batches are
+    // drained as they fill, everything after the last step.
     for (int base = 0;; base += kWave) {
-      const bool more = base < n_refl;
-      if (more) {
-        const int i = base + lane;
-        const bool v = i < n_refl;
-        const float4 g = v ? g4[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      bool done;
+      if (use_lines) {
+        done = false;
+        win += kWave;
+        while (win >= total) {
+          if (++lo >= n_here) {  // next group of lines
+            lo = 0;
+            lb += kWave;
+            if (lb >= n_lines) {
+              done = true;
+              break;
+            }
+            const int j = lb + lane;
+            nrow = 0;
+            if (j < n_lines) {
+              const int4 e0 = lines[2 * (size_t)j], e1 = lines[2 * (size_t)j + 1];
+              first = e0.x;
+              nrow = e0.y;
+              lh = e0.z;
+              lk = e0.w;
+              l0 = e1.x;
+              const double wmin = __int_as_float(e1.y), wmax = __int_as_float(e1.z);
+              const double zm = 1e-5 * (1.0 + sqrt(wmax));
+              zlo = (wmin - thr) / two_kd - zm;
+              zhi = (wmax + thr) / two_kd + zm;
+            }
+          }
+          // z of row first + t is zt - t ga (l = l0 - t): the rows with z in [zlo, zhi]
+          const double al = s_abc[wib][lo][0], be = s_abc[wib][lo][1], ga = s_abc[wib][lo][2];
+          const double inv = s_abc[wib][lo][3];
+          const double zt = static_cast<double>(lh) * al + static_cast<double>(lk) * be +
+                            static_cast<double>(l0) * ga;
+          int t0 = 0, len = 0;
+          if (inv == 0.0) {  // line perpendicular to the beam: all rows or none
+            if (nrow > 0 && zt >= zlo && zt <= zhi) len = nrow;
+          } else if (nrow > 0) {
+            const double q1 = (zt - zhi) * inv, q2 = (zt - zlo) * inv;
+            const double tlo = fmax(ceil(fmin(q1, q2)), 0.0);
+            const double thi = fmin(floor(fmax(q1, q2)), static_cast<double>(nrow - 1));
+            if (thi >= tlo) {
+              t0 = static_cast<int>(tlo);
+              len = static_cast<int>(thi - tlo) + 1;
+            }
+          }
+          int incl = len;  // inclusive prefix over the lanes (lines in row order)
 #pragma unroll
-        for (int o = 0; o < kOT; ++o) {
-          if (o >= n_here) break;  // uniform
-          const float zf = fmaf(R[o][0], g.x, fmaf(R[o][1], g.y, R[o][2] * g.z));
-          const bool pre = v && fabsf(fmaf(-two_k, zf, g.w)) < pre_thr;
-          const uint64_t pm = __ballot(pre);
-          if (pm == 0) continue;
-          if (pre) queue[(head + qn + mask_rank(pm)) % kQueue] = (i << kSlotBits) | o;
-          qn += __popcll(pm);
+          for (int off = 1; off < kWave; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+          }
+          total = __builtin_amdgcn_readfirstlane(__shfl(incl, kWave - 1));
+          P = incl - len;  // first candidate of this lane's line
+          rb = first + t0;
+          win = 0;
+          r0 = R[0][0];
+          r1 = R[0][1];
+          r2 = R[0][2];
+#pragma unroll
+          for (int o = 1; o < kOT; ++o) {
+            if (lo == o) {
+              r0 = R[o][0];
+              r1 = R[o][1];
+              r2 = R[o][2];
+            }
+          }
+        }
+        if (!done) {
+          const int c = win + lane;
+          // the last lane whose line starts at or before candidate c holds it
+          int src = 0;
+#pragma unroll
+          for (int step = kWave / 2; step > 0; step >>= 1) {
+            const int p = __shfl(P, src + step);
+            if (p <= c) src += step;
+          }
+          const int row = __shfl(rb, src) + (c - __shfl(P, src));
+          const bool v = c < total && static_cast<unsigned>(row) < static_cast<unsigned>(n_refl);
+          const float4 g = v ? g4[row] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          pretest(v, row, g, r0, r1, r2, lo);
+        }
+      } else {
+        done = base >= n_refl;
+        if (!done) {
+          const int i = base + lane;
+          const bool v = i < n_refl;
+          const float4 g = v ? g4[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+          for (int o = 0; o < kOT; ++o) {
+            if (o >= n_here) break;  // uniform
+            pretest(v, i, g, R[o][0], R[o][1], R[o][2], o);
+          }
         }
       }
-      // drain full batches during the sweep, everything at its end
-      const int need = more ? kWave : 1;
+      const int need = done ? 1 : kWave;
       if (qn >= need) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
         while (qn >= need) drain(min(qn, kWave));
       }
-      if (!more) break;
+      if (done) break;
     }
 
 #pragma unroll
@@ -277,11 +415,11 @@ static int launch_blocks(int n_items, int blocks_per_cu) {
 using namespace dsx;
 
 extern "C" int dsx_simulate(const int *hkl_d, const float *g4_d, const float *F2_d, int n_refl,
-                            const double *Rt_d, int n_orient, const dsx_sim_params *p, int cap,
-                            int *idx_d, float *spot_d, int *count_d, int *ncand_d, float *maxI_d,
-                            void *stream) {
+                            const int *lines_d, int n_lines, const double *Rt_d, int n_orient,
+                            const dsx_sim_params *p, int cap, int *idx_d, float *spot_d,
+                            int *count_d, int *ncand_d, float *maxI_d, void *stream) {
   clear_error();
-  if (!p || n_refl < 0 || n_orient < 0 || cap < 1) {
+  if (!p || n_refl < 0 || n_orient < 0 || cap < 1 || n_lines < 0 || (n_lines > 0 && !lines_d)) {
     set_error("dsx_simulate: bad arguments (n_refl=%d n_orient=%d cap=%d)", n_refl, n_orient, cap);
     return DSX_EINVAL;
   }
@@ -309,19 +447,25 @@ extern "C" int dsx_simulate(const int *hkl_d, const float *g4_d, const float *F2
   d.g_max = p->g_max;
   d.Rt = Rt_d;
   d.n_orient = n_orient;
-  d.reserved = 0;
+  d.n_lines = lines_d ? n_lines : 0;
   d.idx = idx_d;
   d.spots = spot_d;
   d.count = count_d;
   d.ncand = ncand_d;
   d.maxI = maxI_d;
+  d.lines = lines_d;
   a.n_items = (n_orient + kOT - 1) / kOT;
-  if (p->shape_model >= DSX_SF_PRECESSION_AVG)
-    hipLaunchKernelGGL((simulate_kernel<false, true>), dim3(launch_blocks(a.n_items, 0)),
-                       dim3(kSimThreads), 0, as_stream(stream), a);
+  const dim3 grid(launch_blocks(a.n_items, 0));
+  hipStream_t s = as_stream(stream);
+  // the averaged-precession model keeps the full sweep: its FP64 evaluation
+  // dominates, and the line state would cost it occupancy
+  const bool avg = p->shape_model >= DSX_SF_PRECESSION_AVG;
+  if (lines_d && !avg && lines_pay(n_refl, n_lines))
+    hipLaunchKernelGGL((simulate_kernel<false, false, true>), grid, dim3(kSimThreads), 0, s, a);
+  else if (avg)
+    hipLaunchKernelGGL((simulate_kernel<false, true, false>), grid, dim3(kSimThreads), 0, s, a);
   else
-    hipLaunchKernelGGL((simulate_kernel<false, false>), dim3(launch_blocks(a.n_items, 0)),
-                       dim3(kSimThreads), 0, as_stream(stream), a);
+    hipLaunchKernelGGL((simulate_kernel<false, false, false>), grid, dim3(kSimThreads), 0, s, a);
   return check_launch("dsx_simulate");
 }
 
@@ -347,11 +491,13 @@ extern "C" int dsx_simulate_ragged(const dsx_phase_desc *phases_d, int n_phases,
   a.phases = phases_d;
   a.items = reinterpret_cast<const int2 *>(items_d);
   a.n_items = n_items;
+  // each phase uses its line index when it has one (lines != NULL) and its
+  // lines are long enough, except under the averaged-precession model
   if (p->shape_model >= DSX_SF_PRECESSION_AVG)
-    hipLaunchKernelGGL((simulate_kernel<true, true>), dim3(launch_blocks(n_items, blocks_per_cu)),
+    hipLaunchKernelGGL((simulate_kernel<true, true, false>), dim3(launch_blocks(n_items, blocks_per_cu)),
                        dim3(kSimThreads), 0, as_stream(stream), a);
   else
-    hipLaunchKernelGGL((simulate_kernel<true, false>), dim3(launch_blocks(n_items, blocks_per_cu)),
+    hipLaunchKernelGGL((simulate_kernel<true, false, true>), dim3(launch_blocks(n_items, blocks_per_cu)),
                        dim3(kSimThreads), 0, as_stream(stream), a);
   return check_launch("dsx_simulate_ragged");
 }

@@ -142,6 +142,61 @@ __global__ __launch_bounds__(1024) void hkl_scan_kernel(int *__restrict__ work, 
 }
 
 // ---------------------------------------------------------------------------
+// Line index of a reflection table
+// ---------------------------------------------------------------------------
+//
+// A line is a maximal run of consecutive rows with equal (h, k) and l falling
+// by one per row -- the rows of one (h, k) column of the hkl box, which the
+// enumeration's descending order keeps contiguous.  Per line: first row, row
+// count, h, k, l of the first row and the smallest / largest |g|^2 (g4.w) of
+// its rows, which bound where along the beam its rows can pass the simulate
+// pretest.
+
+constexpr int kLineThreads = 256;
+
+__device__ __forceinline__ bool line_continues(const int4 *hkl, int i) {
+  const int4 a = hkl[i - 1], b = hkl[i];
+  return a.x == b.x && a.y == b.y && a.z == b.z + 1;
+}
+
+template <int kPass>  // 0: count line starts per block; 1: write lines
+__global__ __launch_bounds__(kLineThreads) void hkl_lines_kernel(const int4 *hkl, const float4 *g4,
+                                                                 int n, int *lines, int capacity,
+                                                                 int *work) {
+  __shared__ int s_wave[kLineThreads / kWave];
+  const int i = blockIdx.x * kLineThreads + threadIdx.x;
+  const bool start = i < n && (i == 0 || !line_continues(hkl, i));
+  const uint64_t m = __ballot(start);
+  const int lane = lane_id(), wid = threadIdx.x / kWave;
+  if (lane == 0) s_wave[wid] = __popcll(m);
+  __syncthreads();
+  if (kPass == 0) {
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int w = 0; w < kLineThreads / kWave; ++w) t += s_wave[w];
+      work[blockIdx.x] = t;
+    }
+    return;
+  }
+  if (!start) return;
+  int q = work[gridDim.x + blockIdx.x] + mask_rank(m);
+  for (int w = 0; w < wid; ++w) q += s_wave[w];
+  if (q >= capacity) return;
+  float wmin = g4[i].w, wmax = wmin;
+  int e = i + 1;
+  while (e < n && line_continues(hkl, e)) {
+    const float w = g4[e].w;
+    wmin = fminf(wmin, w);
+    wmax = fmaxf(wmax, w);
+    ++e;
+  }
+  const int4 h = hkl[i];
+  int4 *out = reinterpret_cast<int4 *>(lines) + 2 * static_cast<size_t>(q);
+  out[0] = make_int4(i, e - i, h.x, h.y);
+  out[1] = make_int4(h.z, __float_as_int(wmin), __float_as_int(wmax), 0);
+}
+
+// ---------------------------------------------------------------------------
 // Structure factor
 // ---------------------------------------------------------------------------
 
@@ -349,4 +404,39 @@ extern "C" int dsx_structure_factor(const int *hkl_d, int n, const double rec[9]
   hipLaunchKernelGGL(structure_factor_kernel, dim3((n + kSfThreads - 1) / kSfThreads),
                      dim3(kSfThreads), 0, as_stream(stream), a);
   return check_launch("dsx_structure_factor");
+}
+
+extern "C" long long dsx_hkl_lines_workspace_ints(int n_refl) {
+  if (n_refl < 0) return -1;
+  return 2ll * ((n_refl + kLineThreads - 1) / kLineThreads);
+}
+
+extern "C" int dsx_hkl_lines(const int *hkl_d, const float *g4_d, int n_refl, int *lines_d,
+                             int capacity, int *count_d, int *work_d, long long work_ints,
+                             void *stream) {
+  clear_error();
+  if (n_refl < 0 || capacity < 0 || !count_d || (capacity > 0 && !lines_d) ||
+      (n_refl > 0 && (!hkl_d || !g4_d || !work_d))) {
+    set_error("dsx_hkl_lines: bad arguments (n_refl=%d capacity=%d) or null pointer", n_refl,
+              capacity);
+    return DSX_EINVAL;
+  }
+  const int nb = (n_refl + kLineThreads - 1) / kLineThreads;
+  if (n_refl > 0 && work_ints < 2ll * nb) {
+    set_error("dsx_hkl_lines: workspace of %lld ints < %lld", work_ints, 2ll * nb);
+    return DSX_EINVAL;
+  }
+  hipStream_t s = as_stream(stream);
+  if (n_refl == 0) {  // the scan of no blocks writes a zero count
+    hipLaunchKernelGGL(hkl_scan_kernel, dim3(1), dim3(1024), 0, s, work_d, 0, count_d);
+    return check_launch("dsx_hkl_lines");
+  }
+  const int4 *hkl = reinterpret_cast<const int4 *>(hkl_d);
+  const float4 *g4 = reinterpret_cast<const float4 *>(g4_d);
+  hipLaunchKernelGGL(hkl_lines_kernel<0>, dim3(nb), dim3(kLineThreads), 0, s, hkl, g4, n_refl,
+                     lines_d, capacity, work_d);
+  hipLaunchKernelGGL(hkl_scan_kernel, dim3(1), dim3(1024), 0, s, work_d, nb, count_d);
+  hipLaunchKernelGGL(hkl_lines_kernel<1>, dim3(nb), dim3(kLineThreads), 0, s, hkl, g4, n_refl,
+                     lines_d, capacity, work_d);
+  return check_launch("dsx_hkl_lines");
 }

@@ -224,6 +224,7 @@ class _Tables:
     B: torch.Tensor
     rec: np.ndarray
     g_max: float
+    lines: Optional[torch.Tensor] = None  # line index for the spot kernel
 
 
 class RefinementModel:
@@ -258,6 +259,10 @@ class RefinementModel:
         self.hkl = init.hkl
         self.hkl_np = init.hkl[:, :3].cpu().numpy().astype(np.float64)
         self.n_refl = init.n
+        # the line index's rows are fixed by the hkl set, its |g|^2 bounds follow
+        # the lattice; built per step only where the spot kernel uses it
+        self.n_lines = init.n_lines
+        self.use_lines = init.n_lines > 0 and init.n >= 16 * init.n_lines
         # atoms sorted by element (the structure-factor kernel's order)
         elements = [element_of(s) for s in phase.symbols]
         uniq = sorted(set(elements))
@@ -310,11 +315,12 @@ class RefinementModel:
             engine.SCATTER_IDS[self.scatter], _lib.ptr(g4), _lib.ptr(F2), _lib.ptr(F),
             stream), "dsx_structure_factor")
         g_max = float(np.sqrt(np.max(np.sum((self.hkl_np @ rec_np.T) ** 2, axis=1)))) if N else 0.0
-        return _Tables(self.hkl, g4, F2, F, fx, Bs, rec_np, g_max)
+        lines = engine.build_lines(self.hkl, g4, self.n_lines, stream) if self.use_lines else None
+        return _Tables(self.hkl, g4, F2, F, fx, Bs, rec_np, g_max, lines)
 
     def _phase_table(self, t: _Tables) -> engine.PhaseTable:
         return engine.PhaseTable(t.hkl, t.g4, t.F2, t.F, self.n_refl, False, t.g_max, t.rec,
-                                 None, "")
+                                 None, "", t.lines)
 
     def spots(self, t: _Tables, Rt: torch.Tensor, batch: Optional[engine.SpotBatch] = None,
               check: bool = True) -> engine.SpotBatch:

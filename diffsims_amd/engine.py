@@ -68,10 +68,15 @@ class PhaseTable:
     rec: np.ndarray         # FP64 reciprocal basis (columns a*, b*, c*)
     base: np.ndarray
     name: str = ""
+    lines: Optional[torch.Tensor] = None  # line index (dsx_hkl_lines), int32 [n_lines, 8]
 
     @property
     def n(self) -> int:
         return int(self.hkl.shape[0])
+
+    @property
+    def n_lines(self) -> int:
+        return 0 if self.lines is None else int(self.lines.shape[0])
 
     def hkl_host(self) -> np.ndarray:
         return self.hkl[:, :3].cpu().numpy()
@@ -82,6 +87,28 @@ def _fixed_point(frac: np.ndarray) -> np.ndarray:
     f = np.asarray(frac, dtype=np.float64)
     f = f - np.floor(f)
     return (np.round(f * 4294967296.0).astype(np.uint64) & 0xFFFFFFFF).astype(np.uint32)
+
+
+def build_lines(hkl: torch.Tensor, g4: torch.Tensor, n_lines: Optional[int] = None,
+                stream=None) -> torch.Tensor:
+    """Line index of a reflection table (dsx_hkl_lines): int32 [n_lines, 8].
+    With ``n_lines`` known (same hkl rows as an earlier call) nothing is read
+    back; otherwise the count is read once."""
+    L = _lib.load(require_gpu=True)
+    dev = hkl.device
+    n = int(hkl.shape[0])
+    stream = _lib.stream_handle(dev) if stream is None else stream
+    wn = int(L.dsx_hkl_lines_workspace_ints(n))
+    work = torch.empty(max(wn, 1), dtype=torch.int32, device=dev)
+    count = torch.empty(1, dtype=torch.int32, device=dev)
+    if n_lines is None:
+        _lib.check(L.dsx_hkl_lines(_lib.ptr(hkl), _lib.ptr(g4), n, None, 0, _lib.ptr(count),
+                                   _lib.ptr(work), wn, stream), "dsx_hkl_lines")
+        n_lines = int(count.item())
+    lines = torch.empty((n_lines, 8), dtype=torch.int32, device=dev)
+    _lib.check(L.dsx_hkl_lines(_lib.ptr(hkl), _lib.ptr(g4), n, _lib.ptr(lines), n_lines,
+                               _lib.ptr(count), _lib.ptr(work), wn, stream), "dsx_hkl_lines")
+    return lines
 
 
 def build_phase_table(pa: PhaseArrays, reciprocal_radius: float, with_direct_beam: bool,
@@ -158,7 +185,8 @@ def build_phase_table(pa: PhaseArrays, reciprocal_radius: float, with_direct_bea
                                       len(uniq), SCATTER_IDS[scattering_params], _lib.ptr(g4),
                                       _lib.ptr(F2), _lib.ptr(F), stream), "dsx_structure_factor")
     g_max = float(torch.sqrt(g4[:, 3].max()).item()) if N else 0.0
-    return PhaseTable(hkl, g4, F2, F, n_box, with_direct_beam, g_max, rec, base, pa.name)
+    return PhaseTable(hkl, g4, F2, F, n_box, with_direct_beam, g_max, rec, base, pa.name,
+                      build_lines(hkl, g4, stream=stream))
 
 
 def table_from_hkl(hkl: np.ndarray, abc, angles, F2: Optional[np.ndarray] = None,
@@ -184,7 +212,7 @@ def table_from_hkl(hkl: np.ndarray, abc, angles, F2: Optional[np.ndarray] = None
     else:
         F2t.copy_(torch.as_tensor(np.asarray(F2, dtype=np.float32)))
     g_max = float(torch.sqrt(g4[:, 3].max()).item()) if N else 0.0
-    return PhaseTable(t_hkl, g4, F2t, None, N, False, g_max, rec, base)
+    return PhaseTable(t_hkl, g4, F2t, None, N, False, g_max, rec, base, "", build_lines(t_hkl, g4))
 
 
 @dataclass
@@ -295,7 +323,7 @@ def simulate(tab: PhaseTable, Rt: torch.Tensor, settings: SimSettings, reciproca
         ncand = torch.empty(n, dtype=torch.int32, device=dev)
         maxI = torch.empty(n, dtype=torch.float32, device=dev)
         _lib.check(L.dsx_simulate(_lib.ptr(tab.hkl), _lib.ptr(tab.g4), _lib.ptr(tab.F2), tab.n,
-                                  _lib.ptr(Rt), n,
+                                  _lib.ptr(tab.lines), tab.n_lines, _lib.ptr(Rt), n,
                                   ctypes.byref(params), cap, _lib.ptr(idx), _lib.ptr(spots),
                                   _lib.ptr(count), _lib.ptr(ncand), _lib.ptr(maxI), stream),
                    "dsx_simulate")
@@ -428,7 +456,7 @@ def simulate_into(tab: PhaseTable, Rt: torch.Tensor, settings: SimSettings, out:
     params = settings.params(tab.g_max, tab.rec)
     stream = _lib.stream_handle(Rt.device) if stream is None else stream
     _lib.check(L.dsx_simulate(_lib.ptr(tab.hkl), _lib.ptr(tab.g4), _lib.ptr(tab.F2), tab.n,
-                              _lib.ptr(Rt), int(Rt.shape[0]), ctypes.byref(params), out.cap,
+                              _lib.ptr(tab.lines), tab.n_lines, _lib.ptr(Rt), int(Rt.shape[0]), ctypes.byref(params), out.cap,
                               _lib.ptr(out.idx), _lib.ptr(out.spots), _lib.ptr(out.count),
                               _lib.ptr(out.ncand), _lib.ptr(out.maxI), stream), "dsx_simulate")
 

@@ -134,6 +134,8 @@ class RaggedLibrary:
             d.count = self.count.data_ptr() + 4 * s.first
             d.ncand = self.ncand.data_ptr() + 4 * s.first
             d.maxI = self.maxI.data_ptr() + 4 * s.first
+            d.n_lines = t.n_lines
+            d.lines = t.lines.data_ptr() if t.lines is not None else None
         raw = np.frombuffer(bytes(descs), dtype=np.uint8)
         self.descs = torch.from_numpy(raw.copy()).to(self.dev)
         self.graph = None

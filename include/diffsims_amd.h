@@ -101,6 +101,28 @@ int dsx_structure_factor(const int *hkl_d, int n, const double rec[9],
                          float *g4_d, float *F2_d, float *F_d, void *stream);
 
 /*
+ * Line index of a reflection table, for dsx_simulate / dsx_simulate_ragged.
+ * No counterpart in the reference: it lets the spot kernel skip rows that
+ * get_intersecting_reflections (simulation_generator.py:319-412) would
+ * reject, without changing its output.
+ *
+ * A line is a maximal run of consecutive rows with equal (h, k) and l falling
+ * by one per row (the rows of one (h, k) column of dsx_hkl_enumerate's box;
+ * any table works, irregular ones just have short lines).  Entry j is int32
+ * [8] {first row, row count, h, k, l of the first row, min |g|^2, max |g|^2
+ * (float bits of g4.w over the line's rows), 0}, lines in row order.
+ *   hkl_d, g4_d : the table (n_refl rows; g4 from dsx_structure_factor)
+ *   lines_d     : device int32 [capacity][8]
+ *   count_d     : device int32 [1], receives the number of lines (all of them,
+ *                 also when more than capacity; only capacity are written)
+ *   work_d      : device int32 scratch of dsx_hkl_lines_workspace_ints ints
+ */
+long long dsx_hkl_lines_workspace_ints(int n_refl);
+int dsx_hkl_lines(const int *hkl_d, const float *g4_d, int n_refl, int *lines_d,
+                  int capacity, int *count_d, int *work_d, long long work_ints,
+                  void *stream);
+
+/*
  * Atom staging for dsx_structure_factor when positions and Debye-Waller
  * factors live on the device (gradient-based refinement; no counterpart in
  * the reference, whose Phase keeps them on the host).  Row i of the outputs
@@ -137,6 +159,13 @@ typedef struct dsx_sim_params {
  *   hkl_d, g4_d, F2_d : tables (n_refl rows; int4 hkl as enumerated, with
  *                the direct-beam rows appended by the caller when
  *                with_direct_beam; g4/F2 from dsx_structure_factor)
+ *   lines_d    : optional line index of the table (dsx_hkl_lines, n_lines
+ *                entries) or NULL.  With it the kernel visits only the rows
+ *                of each line that can pass the pretest for an orientation
+ *                instead of every row; the outputs are identical.  It is
+ *                used when the lines average at least 16 rows (shorter ones
+ *                cost more than the rows they skip) and not under the
+ *                averaged-precession models.
  *   Rt_d       : double [n_orient][9] row-major R = M^T (g_lab = R g), M the
  *                orix to_matrix() of each orientation
  * The FP32 g4 table only drives a conservative pretest; every candidate is
@@ -152,9 +181,9 @@ typedef struct dsx_sim_params {
  * ncand_d[o] > cap; the caller re-runs it with a larger cap.
  */
 int dsx_simulate(const int *hkl_d, const float *g4_d, const float *F2_d, int n_refl,
-                 const double *Rt_d, int n_orient, const dsx_sim_params *params,
-                 int cap, int *idx_d, float *spot_d, int *count_d, int *ncand_d,
-                 float *maxI_d, void *stream);
+                 const int *lines_d, int n_lines, const double *Rt_d, int n_orient,
+                 const dsx_sim_params *params, int cap, int *idx_d, float *spot_d,
+                 int *count_d, int *ncand_d, float *maxI_d, void *stream);
 
 /*
  * One phase of a ragged multi-phase launch (device-resident array of these).
@@ -171,12 +200,13 @@ typedef struct dsx_phase_desc {
   double g_max;
   const double *Rt;        /* double [n_orient][9]                             */
   int n_orient;
-  int reserved;
+  int n_lines;             /* entries of lines (0 when lines is NULL)          */
   int *idx;                /* int32 [n_orient][cap]                            */
   float *spots;            /* float [n_orient][cap][4]                         */
   int *count;              /* int32 [n_orient]                                 */
   int *ncand;              /* int32 [n_orient]                                 */
   float *maxI;             /* float [n_orient]                                 */
+  const int *lines;        /* optional line index (dsx_hkl_lines) or NULL      */
 } dsx_phase_desc;
 
 /* Orientations per work item (tile) of dsx_simulate_ragged. */

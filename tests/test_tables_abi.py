@@ -68,8 +68,15 @@ def test_entry_points_reject_bad_arguments_without_a_gpu():
     assert lib.dsx_hkl_enumerate(h, m, 1.0, None, 0, 1, 1, None) == _lib.DSX_EINVAL
     assert "bad box" in _lib.last_error()
     p = _lib.SimParams()
-    assert lib.dsx_simulate(None, None, None, 0, None, 1, ctypes.byref(p), 0, None, None, None,
-                            None, None, None) == _lib.DSX_EINVAL
+    assert lib.dsx_simulate(None, None, None, 0, None, 0, None, 1, ctypes.byref(p), 0, None, None,
+                            None, None, None, None) == _lib.DSX_EINVAL
+    assert lib.dsx_simulate(None, None, None, 0, None, 3, None, 1, ctypes.byref(p), 1, 1, 1, 1, 1,
+                            1, None) == _lib.DSX_EINVAL
+    assert lib.dsx_hkl_lines(None, None, -1, None, 0, 1, None, 0, None) == _lib.DSX_EINVAL
+    assert lib.dsx_hkl_lines(1, 1, 1000, None, 0, 1, 1, 1, None) == _lib.DSX_EINVAL
+    assert "workspace" in _lib.last_error()
+    assert lib.dsx_hkl_lines_workspace_ints(1000) == 2 * 4
+    assert lib.dsx_hkl_lines_workspace_ints(-1) == -1
     rp = _lib.RenderParams()
     rp.height = 0
     assert lib.dsx_render_fast(1, 1, 1, 1, ctypes.byref(rp), 1, None, None) == _lib.DSX_EINVAL
@@ -83,8 +90,9 @@ def test_struct_layouts_match_the_header(tmp_path):
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "diffsims_amd.h"\n'
         "int main(void){\n"
-        'printf("%zu %zu %zu %zu\\n", sizeof(dsx_sim_params), offsetof(dsx_sim_params, rec),'
-        " sizeof(dsx_render_params), offsetof(dsx_render_params, taps));\n"
+        'printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(dsx_sim_params), offsetof(dsx_sim_params, rec),'
+        " sizeof(dsx_render_params), offsetof(dsx_render_params, taps), sizeof(dsx_phase_desc),"
+        " offsetof(dsx_phase_desc, n_lines), offsetof(dsx_phase_desc, lines));\n"
         "return 0;}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
@@ -92,7 +100,9 @@ def test_struct_layouts_match_the_header(tmp_path):
     vals = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
                                            check=True).stdout.split()]
     assert vals == [ctypes.sizeof(_lib.SimParams), _lib.SimParams.rec.offset,
-                    ctypes.sizeof(_lib.RenderParams), _lib.RenderParams.taps.offset]
+                    ctypes.sizeof(_lib.RenderParams), _lib.RenderParams.taps.offset,
+                    ctypes.sizeof(_lib.PhaseDesc), _lib.PhaseDesc.n_lines.offset,
+                    _lib.PhaseDesc.lines.offset]
 
 
 def test_beam_direction_mesh_sizes_on_the_host():
