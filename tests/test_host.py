@@ -225,3 +225,20 @@ def test_host_shape_models_match_oracle():
                                    rtol=1e-12, atol=1e-15)
     np.testing.assert_allclose(sfm.lorentzian_precession(s, 0.02, 1.3, 0.01),
                                O.sf_lorentzian_precession(s, 0.02, 1.3, 0.01), rtol=1e-12)
+
+
+# -- bench contract ---------------------------------------------------------------
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message before any
+    device work."""
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
