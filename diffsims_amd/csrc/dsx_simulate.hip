@@ -76,8 +76,14 @@ __host__ __device__ __forceinline__ float pretest_threshold(double me, double k,
 constexpr int kQueue = 64 + kOT * kWave;  // one batch plus one worst-case chunk
 constexpr int kWavesPerBlock = kSimThreads / kWave;
 
+// Register budgets (waves per SIMD), measured: the full sweep at 5 (96 VGPRs)
+// and the line sweep at 4 (128 VGPRs) are 5-6 % faster than the compiler's own
+// choice (111 and 151-165 VGPRs) despite a few spilled registers; the
+// averaged-precession models keep theirs.
+constexpr int sim_min_waves(bool avg, bool lines) { return avg ? 1 : (lines ? 4 : 5); }
+
 template <bool kRagged, bool kAvg, bool kLines>
-__global__ __launch_bounds__(kSimThreads) void simulate_kernel(SimArgs a) {
+__global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simulate_kernel(SimArgs a) {
   __shared__ int s_queue[kWavesPerBlock][kQueue];
   __shared__ double s_R[kWavesPerBlock][kOT][9];
   __shared__ double s_abc[kWavesPerBlock][kOT][4];  // beam . (a*, b*, c*), 1 / (beam . c*)
