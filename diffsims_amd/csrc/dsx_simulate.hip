@@ -199,8 +199,9 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
     int first = 0, nrow = 0, lh = 0, lk = 0, l0 = 0, P = 0, rb = 0;
     double zlo = 0.0, zhi = -1.0;
     float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
-    // pretest pass => |w - 2k zf| < thr (1 + 2^-24); zf is within 3e-7 |g|
-    // of the FP64 beam coordinate, covered by the margin zm below
+    // pretest pass => |w - 2k zf| < thr (1 + 2^-24); zf (FP32 R and g, two
+    // fmaf) is within 1e-6 |g| of the FP64 beam coordinate, ten times inside
+    // the margin zm = 1e-5 (1 + max |g| of the line) below
     const double two_kd = static_cast<double>(two_k);
     const double thr = static_cast<double>(pre_thr) * (1.0 + 1e-6);
     if (use_lines) {
