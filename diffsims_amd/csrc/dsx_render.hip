@@ -744,6 +744,7 @@ constexpr int kDenseMaxPix = 16384;  // two FP32 H x W buffers in 128 KB of LDS
 // H + 1, then row-major with row W + 1) of at most this many words each
 constexpr int kDenseRegionWords = 20352;
 constexpr int kDenseSeg = 16;  // outputs per thread and item of the blocked filter
+constexpr int kDensePre = 6;   // spots per thread of the next pattern held in registers
 
 __host__ __device__ __forceinline__ bool dense_blocked(int R_tmpl, int H, int W) {
   return R_tmpl > 0 && H * W + (H > W ? H : W) <= kDenseRegionWords;
@@ -785,10 +786,45 @@ __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs 
   float *B = s_buf + dense_region_words(kR, H, W);  // spot image, then the filtered pattern
   for (int i = tid; i <= kMaxR; i += kDenseThreads) s_taps[i] = a.taps[i];
 
-  for (int pi = blockIdx.x; pi < a.n; pi += gridDim.x) {
+  // The block's next pattern (list entry, count, spot pointer) and the pixel
+  // and intensity of its first kDensePre * kDenseThreads spots are loaded
+  // while the current pattern is filtered, so a pattern starts without the
+  // chain of dependent memory waits; longer lists load the rest in place.
+  int nx_p = 0, nx_n = 0;
+  long long nx_off = 0;
+  int pk[kDensePre];    // pixel of spot tid + k kDenseThreads, -1 when out of frame
+  float pw[kDensePre];  // its intensity
+  auto prefetch = [&](int pi) {
+#pragma unroll
+    for (int k = 0; k < kDensePre; ++k) {
+      pk[k] = -1;
+      pw[k] = 0.0f;
+    }
+    if (pi >= a.n) return;
     const int p = a.list ? a.list[pi] : pi;
-    const int n = min(a.count[p], a.capv ? a.capv[p] : a.cap);
-    const float4 *spots = a.spots + (a.spot_off ? a.spot_off[p] : (long long)p * a.cap);
+    nx_p = p;
+    nx_n = min(a.count[p], a.capv ? a.capv[p] : a.cap);
+    nx_off = a.spot_off ? a.spot_off[p] : (long long)p * a.cap;
+    const float4 *sp = a.spots + nx_off;
+#pragma unroll
+    for (int k = 0; k < kDensePre; ++k) {
+      const int j = tid + k * kDenseThreads;
+      if (j < nx_n) {
+        const float4 v = sp[j];
+        float X, Y;
+        if (to_pixel(v, a, X, Y)) {
+          pk[k] = min(static_cast<int>(Y), H - 1) * W + min(static_cast<int>(X), W - 1);
+          pw[k] = v.w;
+        }
+      }
+    }
+  };
+  prefetch(blockIdx.x);
+
+  for (int pi = blockIdx.x; pi < a.n; pi += gridDim.x) {
+    const int p = nx_p;
+    const int n = nx_n;
+    const float4 *spots = a.spots + nx_off;
     for (int i = tid; i < HW; i += kDenseThreads) {
       A[i] = -1;
       B[i] = 0.0f;
@@ -797,7 +833,14 @@ __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs 
     lds_sync();
     // owner of each pixel = largest list position (last write wins)
     bool any = false;
-    for (int j = tid; j < n; j += kDenseThreads) {
+#pragma unroll
+    for (int k = 0; k < kDensePre; ++k) {
+      if (pk[k] >= 0) {
+        any = true;
+        atomicMax(&A[pk[k]], tid + k * kDenseThreads);
+      }
+    }
+    for (int j = tid + kDensePre * kDenseThreads; j < n; j += kDenseThreads) {
       float X, Y;
       if (!to_pixel(spots[j], a, X, Y)) continue;
       any = true;
@@ -805,7 +848,10 @@ __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs 
     }
     if (__ballot(any) != 0 && lane == 0) s_any = 1;
     lds_sync();
-    for (int j = tid; j < n; j += kDenseThreads) {
+#pragma unroll
+    for (int k = 0; k < kDensePre; ++k)
+      if (pk[k] >= 0 && A[pk[k]] == tid + k * kDenseThreads) B[pk[k]] = pw[k];
+    for (int j = tid + kDensePre * kDenseThreads; j < n; j += kDenseThreads) {
       const float4 sp = spots[j];
       float X, Y;
       if (!to_pixel(sp, a, X, Y)) continue;
@@ -813,6 +859,7 @@ __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs 
       if (A[key] == j) B[key] = sp.w;
     }
     lds_sync();
+    prefetch(pi + gridDim.x);  // overlaps the filter passes below
     float *out = a.out + (size_t)p * HW;
     if (tid == 0 && a.status) a.status[p] = 0;
     if (s_any == 0) {  // no spot in frame: np.zeros(shape), not normalised
