@@ -218,3 +218,27 @@ def test_ragged_line_sweep_equals_full_sweep():
         for o in range(s.n):
             assert np.array_equal(idx[o, :c[o]].cpu().numpy(), b.idx[o, :c[o]].cpu().numpy())
             assert torch.equal(sp[o, :c[o]], b.spots[o, :c[o]])
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_line_sweep_random_cells(seed):
+    """Random triclinic cells, radii, voltages, excitation errors, shape
+    models and precession: the line sweep equals the full sweep bit for bit."""
+    from diffsims_amd import engine
+
+    rng = np.random.default_rng(900 + seed)
+    edges = (float(rng.uniform(3, 10)), float(rng.uniform(10, 24)))
+    spec = P.triclinic(int(rng.integers(1, 30)), 1000 + seed, edges=edges)
+    model = ["lorentzian", "linear", "sinc", "sin2c", "atanc", "binary"][seed % 6]
+    kw = {"accelerating_voltage": float(rng.choice([30.0, 100.0, 200.0, 300.0])),
+          "max_excitation_error": float(rng.uniform(0.005, 0.05))}
+    if seed % 3 == 0:
+        kw["precession_angle"] = float(rng.uniform(0.3, 2.0))
+    st = engine.SimSettings(shape_model=model, **kw)
+    rr = float(rng.uniform(1.6, 2.6))
+    print(f"random cell {seed}: {spec.abc} {model} {kw} r={rr:.2f}", flush=True)
+    tab = engine.build_phase_table(P.to_arrays(spec), rr, bool(seed % 2))
+    if not uses_lines(tab):
+        pytest.skip(f"lines average {tab.n / tab.n_lines:.1f} rows: the kernel keeps the full sweep")
+    a, b = sweep_and_lines(tab, rotations(384, 2000 + seed), st, rr)
+    assert_same_batches(a, b)
