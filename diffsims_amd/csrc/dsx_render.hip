@@ -882,10 +882,16 @@ __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs 
       for (int it = tid; it < W * nsy; it += kDenseThreads) {
         const int x = it % W, y0 = (it / W) * kS;
         float v[kN];
+        if (y0 >= kR && y0 + kS + kR <= H) {  // interior segment: no reflection
+          const float *col = B + (y0 - kR) * W + x;
 #pragma unroll
-        for (int k = 0; k < kN; ++k) {
-          const int y = y0 - kR + k;
-          v[k] = y < H + kR ? B[reflect_near(y, H, kR) * W + x] : 0.0f;
+          for (int k = 0; k < kN; ++k) v[k] = col[k * W];
+        } else {
+#pragma unroll
+          for (int k = 0; k < kN; ++k) {
+            const int y = y0 - kR + k;
+            v[k] = y < H + kR ? B[reflect_near(y, H, kR) * W + x] : 0.0f;
+          }
         }
 #pragma unroll
         for (int k = 0; k < kS; ++k) {
@@ -902,10 +908,16 @@ __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs 
       for (int it = tid; it < H * nsx; it += kDenseThreads) {
         const int y = it % H, x0 = (it / H) * kS;
         float v[kN];
+        if (x0 >= kR && x0 + kS + kR <= W) {  // interior segment: no reflection
+          const float *row = Af + (x0 - kR) * Hp + y;
 #pragma unroll
-        for (int k = 0; k < kN; ++k) {
-          const int x = x0 - kR + k;
-          v[k] = x < W + kR ? Af[reflect_near(x, W, kR) * Hp + y] : 0.0f;
+          for (int k = 0; k < kN; ++k) v[k] = row[k * Hp];
+        } else {
+#pragma unroll
+          for (int k = 0; k < kN; ++k) {
+            const int x = x0 - kR + k;
+            v[k] = x < W + kR ? Af[reflect_near(x, W, kR) * Hp + y] : 0.0f;
+          }
         }
 #pragma unroll
         for (int k = 0; k < kS; ++k) {
