@@ -16,7 +16,13 @@ hold those checks.
 Parity pins (what fixes each function):
   electron_wavelength        λ KATs (tests/utils/test_sim_utils.py:49-60)
   lattice_basis / rnorm      ferrite g-spacings (test_reciprocal_lattice_vector.py:157-187),
-                             golden old_simulation.npy (hexagonal frame)
+                             golden old_simulation.npy (hexagonal frame); oblique
+                             (monoclinic / triclinic) frame: the cell recovered from
+                             the basis rows, x || a, b in the xy-plane, c* || +z,
+                             right-handed, base @ rec = I (the conditions stated at
+                             test_simulation_generator.py:311 fix the frame), with
+                             planted mistakes in a31/a32; rnorm = the textbook
+                             triclinic d-spacing from the inverse direct metric
   highest_hkl / hkl_box      ferrite 18/92/750, SiC box, Al 798/256 + order KATs
   bunge_matrix / quat*       golden old_simulation.npy; closed-form orthogonality;
                              quaternion<->matrix round trip
@@ -28,7 +34,9 @@ Parity pins (what fixes each function):
   render_fast                golden old_simulation.npy (uses scipy.ndimage, as the
                              reference does), total-intensity KAT
   render_subpixel            detector KATs (test_detector_functions.py:160-306),
-                             fast-vs-slow equality KAT (test_simulations2d.py:133-180)
+                             fast-vs-slow equality KAT (test_simulations2d.py:133-180),
+                             normalisation at sigma = 1.7, 2.5, 3.2 (pixel sum = I,
+                             peak = I / (2 pi sigma^2), the clip box edge)
   beam_directions_grid       mesh sizes and contained directions
                              (test_sphere_mesh_generators.py:48-78), grid sizes per
                              crystal system (test_rotation_list_generator.py:80-94),
