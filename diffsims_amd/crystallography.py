@@ -25,6 +25,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 __all__ = ["Lattice", "Atom", "Structure", "Phase", "PhaseArrays", "phase_arrays",
+           "reference_frame_positions",
            "lattice_setup", "reciprocal_metric", "highest_hkl", "element_of"]
 
 
@@ -234,6 +235,33 @@ class PhaseArrays:
     name: str = ""
 
 
+def reference_frame_positions(lattice, xyz: np.ndarray) -> np.ndarray:
+    """Fractional atom positions in the frame of the Miller indices.
+
+    orix re-aligns every structure it holds to x || a, z || c* and keeps the
+    atoms' Cartesian positions, which rewrites their fractional coordinates;
+    the reference undoes that with xyz @ inv(stdbase @ recbase) before the
+    structure factor (diffsims/utils/sim_utils.py:288-291).  The same matrix
+    is applied here whenever the lattice carries diffpy's ``stdbase`` and
+    ``recbase`` (diffpy / orix objects; it is the identity for a lattice that
+    was never re-aligned).  This module's own :class:`Lattice` keeps the
+    positions as given and has no ``stdbase``, so they pass unchanged.  A
+    lattice with a ``baserot`` other than the identity but no ``stdbase``
+    cannot be undone and raises ``ValueError``.
+    """
+    std = getattr(lattice, "stdbase", None)
+    rb = getattr(lattice, "recbase", None)
+    if std is not None and rb is not None:
+        mat = np.linalg.inv(np.asarray(std, dtype=np.float64) @ np.asarray(rb, dtype=np.float64))
+        return np.asarray(xyz, dtype=np.float64) @ mat
+    br = getattr(lattice, "baserot", None)
+    if br is not None and not np.allclose(np.asarray(br, dtype=np.float64), np.eye(3),
+                                          rtol=0, atol=1e-12):
+        raise ValueError("the lattice has a base rotation but no stdbase/recbase, so the atom "
+                         "positions cannot be expressed in the frame of the Miller indices")
+    return np.asarray(xyz, dtype=np.float64)
+
+
 def phase_arrays(phase, debye_waller_factors=None) -> PhaseArrays:
     """Read a phase (this module's, or orix/diffpy) into :class:`PhaseArrays`.
 
@@ -246,7 +274,8 @@ def phase_arrays(phase, debye_waller_factors=None) -> PhaseArrays:
     lat = st.lattice
     atoms = list(st)
     symbols = [str(a.element) for a in atoms]
-    xyz = np.array([np.asarray(a.xyz, dtype=np.float64) for a in atoms]).reshape(-1, 3)
+    xyz = reference_frame_positions(
+        lat, np.array([np.asarray(a.xyz, dtype=np.float64) for a in atoms]).reshape(-1, 3))
     occ = np.array([float(a.occupancy) for a in atoms], dtype=np.float64)
     if debye_waller_factors is None:
         debye_waller_factors = {}

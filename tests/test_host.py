@@ -43,6 +43,58 @@ def test_phase_arrays_debye_waller():
         phase_arrays(p, [0.1])
 
 
+class _RealignedLattice(Lattice):
+    """A diffpy-style lattice after orix's re-alignment: ``stdbase`` is the
+    basis the atoms were first placed in (the cell in another orientation),
+    ``base`` the aligned basis (x || a, z || c*), ``recbase = inv(base)``."""
+
+    def __init__(self, abc, ang, Q):
+        super().__init__(*abc, *ang)
+        self._aligned = lattice_setup(abc, ang)[0]
+        self.stdbase = self._aligned @ Q
+        self.baserot = np.linalg.inv(self.stdbase) @ self._aligned
+
+    @property
+    def base(self):
+        return self._aligned
+
+    @property
+    def recbase(self):
+        return np.linalg.inv(self._aligned)
+
+
+def test_phase_arrays_undo_orix_realignment():
+    # sim_utils.py:288-291: positions of a re-aligned structure are mapped back
+    # with inv(stdbase @ recbase); orix keeps the Cartesian positions, so the
+    # re-aligned fractional coordinates are xyz0 @ stdbase @ inv(base)
+    abc, ang = (5.1, 6.3, 7.7), (80.0, 95.0, 105.0)
+    Q = O.bunge_matrix(np.array([0.3, 1.1, -0.7]))
+    lat = _RealignedLattice(abc, ang, Q)
+    xyz0 = np.random.default_rng(4).uniform(0, 1, (5, 3))
+    moved = xyz0 @ lat.stdbase @ np.linalg.inv(lat.base)
+    assert np.abs(moved - xyz0).max() > 0.05                 # the re-alignment moved them
+    p = Phase("x", structure=Structure([Atom("Fe", v) for v in moved], lat))
+    pa = phase_arrays(p)
+    assert np.abs(pa.xyz - xyz0).max() < 1e-12
+    # a lattice never re-aligned (stdbase = base) keeps the positions
+    lat2 = _RealignedLattice(abc, ang, np.eye(3))
+    pa = phase_arrays(Phase("y", structure=Structure([Atom("Fe", v) for v in xyz0], lat2)))
+    assert np.abs(pa.xyz - xyz0).max() < 1e-12
+    # this package's own lattice has no stdbase: positions as given
+    pa = phase_arrays(Phase("z", structure=Structure([Atom("Fe", v) for v in xyz0],
+                                                     Lattice(*abc, *ang))))
+    assert np.array_equal(pa.xyz, xyz0)
+
+
+def test_phase_arrays_reject_unrecoverable_base_rotation():
+    class Rotated(Lattice):
+        baserot = O.bunge_matrix(np.array([0.2, 0.4, 0.1]))
+
+    p = Phase("x", structure=Structure([Atom("Fe", [0.1, 0.2, 0.3])], Rotated(3, 4, 5)))
+    with pytest.raises(ValueError, match="base rotation"):
+        phase_arrays(p)
+
+
 # -- rotations --------------------------------------------------------------------
 
 def test_rotation_conventions_match_oracle():
