@@ -33,13 +33,9 @@ __device__ __forceinline__ void store_nt(float4 v, float4 *p) {
 // for every outstanding global store of the calling wave (s_waitcnt vmcnt(0)),
 // which stalls kernels that stream their output between barriers.
 __device__ __forceinline__ void lds_sync() {
-#ifdef DSX_FULL_BARRIER
-  __syncthreads();
-#else
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-#endif
 }
 
 // ---- wave-level helpers -------------------------------------------------

@@ -396,7 +396,7 @@ def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigm
     if not fast and cap > int(L.dsx_render_max_spots()):
         # more spots than the render LDS holds: the adjoint kernel's forward
         # mode renders them chunk by chunk (same pixels as dsx_render_subpixel)
-        work = _workspace(dev, int(L.dsx_render_adjoint_workspace_ints(int(cap), int(n))))
+        work = _workspace(dev, int(L.dsx_render_adjoint_workspace_ints(int(cap), int(n))), stream)
         sim = _lib.SimParams()
         sim.k, sim.shape_width, sim.shape_model = 1.0, 1.0, _lib.SF_LINEAR
         _lib.check(L.dsx_render_adjoint(
@@ -415,7 +415,7 @@ def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigm
                 ctypes.byref(p), _lib.RENDER_DENSE, 0, _lib.ptr(out), _lib.ptr(status), stream),
                 "dsx_render_fast_ragged")
         else:
-            work = _workspace(dev, int(L.dsx_render_fast_large_workspace_ints(int(cap))))
+            work = _workspace(dev, int(L.dsx_render_fast_large_workspace_ints(int(cap))), stream)
             _lib.check(L.dsx_render_fast_large(
                 _lib.ptr(spots), _lib.ptr(count), cap, n, ctypes.byref(p), _lib.ptr(work),
                 work.numel(), _lib.ptr(out), _lib.ptr(status), stream), "dsx_render_fast_large")
@@ -428,15 +428,17 @@ def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigm
     return out
 
 
-_WORK = {}
+def _workspace(dev, ints: int, stream) -> torch.Tensor:
+    """Device scratch of ``ints`` int32 words for one launch on ``stream``.
 
-
-def _workspace(dev, ints: int) -> torch.Tensor:
-    """Reusable device scratch of at least ``ints`` int32 words per device."""
-    w = _WORK.get(dev)
-    if w is None or w.numel() < ints:
-        w = torch.empty(ints, dtype=torch.int32, device=dev)
-        _WORK[dev] = w
+    Taken from torch's caching allocator per call (no hipMalloc once its pool
+    holds a block of the size), so launches on different streams never share
+    scratch.  When the launch stream is not torch's current stream the block
+    is recorded on it, so the allocator hands it out again only after the
+    kernel is done."""
+    w = torch.empty(max(1, int(ints)), dtype=torch.int32, device=dev)
+    if int(stream) != torch.cuda.current_stream(dev).cuda_stream:
+        w.record_stream(torch.cuda.ExternalStream(int(stream), device=dev))
     return w
 
 

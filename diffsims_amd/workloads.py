@@ -50,6 +50,13 @@ def random_euler(n: int, rng) -> np.ndarray:
     return np.stack([p1, P, p2], axis=1)
 
 
+def haar_euler_rows(n: int, rng) -> np.ndarray:
+    """Haar-uniform Bunge angles drawn row by row (three uniforms per row), so
+    the first m rows do not depend on n."""
+    u = rng.uniform(0, 1, (n, 3))
+    return np.stack([2 * np.pi * u[:, 0], np.arccos(2 * u[:, 1] - 1), 2 * np.pi * u[:, 2]], axis=1)
+
+
 def small_fcc_al(n_orient: int = 500) -> Workload:
     """FCC Al a = 4.05 A, 4 atoms, B = 0.5 A^2; 200 kV, r = 1.0 1/A,
     max_excitation_error 0.01, linear, no direct beam; 144x144 at 0.01 1/A/px,
@@ -142,10 +149,12 @@ def ragged_library(n_phases: int = 64, orient_scale: float = 1.0, seed: int = SE
     """Ragged multi-phase batch: per phase, atom count log-uniform in [1, 200],
     triclinic cell with edges U[3, 25] A and angles U[80, 110] deg, Z uniform
     in [3, 42], B U[0.2, 1.5] A^2, occupancy 1, a log-uniform 2,000-50,000
-    Haar-random orientations (times ``orient_scale``) and reciprocal_radius
-    U[1.0, 2.5] 1/A.  200 kV, max_excitation_error 0.02, lorentzian, no direct
-    beam; 128x128 f32 patterns at 0.04 1/A/px (the frame holds r <= 2.56),
-    sigma 1 px, fast render, normalised."""
+    Haar-random orientations (times ``orient_scale``; the phases do not depend
+    on it and a reduced batch holds the first orientations of each phase's
+    full set) and reciprocal_radius U[1.0, 2.5] 1/A.  200 kV,
+    max_excitation_error 0.02, lorentzian, no direct beam; 128x128 f32
+    patterns at 0.04 1/A/px (the frame holds r <= 2.56), sigma 1 px, fast
+    render, normalised."""
     rng = np.random.default_rng(seed)
     phases, rots, radii = [], [], []
     for i in range(n_phases):
@@ -158,8 +167,12 @@ def ragged_library(n_phases: int = 64, orient_scale: float = 1.0, seed: int = SE
         phases.append(PhaseArrays(abc, angles, [symbol(int(z)) for z in Z], xyz, np.ones(n_atoms),
                                   B, f"ragged{i:02d}"))
         n_o = max(1, int(round(np.exp(rng.uniform(np.log(2000), np.log(50000))) * orient_scale)))
-        rots.append(Rotation.from_euler(random_euler(n_o, rng)).to_matrix())
         radii.append(float(rng.uniform(1.0, 2.5)))
+        # orientations from a stream of their own, one row at a time, so the
+        # phases do not depend on orient_scale and a reduced batch holds the
+        # first orientations of the full one
+        rots.append(Rotation.from_euler(haar_euler_rows(n_o, np.random.default_rng([seed, i])))
+                    .to_matrix())
     st = SimSettings(accelerating_voltage=200, max_excitation_error=0.02, shape_model="lorentzian",
                      minimum_intensity=1e-20)
     return Workload("ragged-64", phases, rots, radii, False, st, (128, 128), 0.04, 1.0,

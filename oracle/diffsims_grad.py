@@ -146,30 +146,40 @@ def numpy_patterns(pr: Problem, M: np.ndarray, xyz=None, B=None, abc=None, angle
     tab = O.phase_tables(ph, pr.p)
     k = 1 / O.electron_wavelength(pr.p.kv)
     pats, frozen = [], []
-    H, W = pr.shape
     for o in range(M.shape[0]):
         sp = O.simulate_one(tab, M[o], k, pr.p)
-        pats.append(O.render_subpixel(sp.xyz[:, :2], sp.intensity, pr.shape, pr.sigma,
-                                      pr.calibration, pr.in_plane_angle, pr.mirrored,
-                                      pr.normalize, pr.clip))
-        X, Y = O.pixel_coords(sp.xyz[:, :2], pr.shape, pr.calibration, ((W - 1) / 2, (H - 1) / 2),
-                              pr.in_plane_angle, pr.mirrored)
-        boxes = np.full((len(sp.idx), 4), -1, dtype=np.int64)
-        pref = 1 / (2 * np.pi * pr.sigma ** 2)
-        ep = -1 / (2 * pr.sigma ** 2)
-        for j in range(len(sp.idx)):
-            if not (0 <= X[j] < W and 0 <= Y[j] < H):
-                continue
-            with np.errstate(invalid="ignore", divide="ignore"):
-                rad = np.sqrt(np.log(pr.clip / (pref * sp.intensity[j])) / ep)
-            if np.isnan(rad):
-                continue
-            y0, y1 = max(0, int(np.ceil(Y[j] - rad))), min(H, int(np.floor(Y[j] + rad + 1)))
-            x0, x1 = max(0, int(np.ceil(X[j] - rad))), min(W, int(np.floor(X[j] + rad + 1)))
-            if y1 > y0 and x1 > x0:
-                boxes[j] = (y0, y1, x0, x1)
-        frozen.append(FrozenPattern(sp.idx, boxes, int(np.argmax(pats[-1]))))
+        pat, fz = freeze(pr, sp)
+        pats.append(pat)
+        frozen.append(fz)
     return np.array(pats), frozen, tab
+
+
+def freeze(pr: Problem, sp: O.Spots, rows: Optional[np.ndarray] = None):
+    """The NumPy oracle's sub-pixel pattern of one spot list and its frozen
+    structure: each in-frame spot's clip box (detector_functions.py:333-358)
+    and the first pixel of the maximum.  ``rows`` (default ``sp.idx``) are the
+    table rows the frozen pattern refers to."""
+    H, W = pr.shape
+    pat = O.render_subpixel(sp.xyz[:, :2], sp.intensity, pr.shape, pr.sigma, pr.calibration,
+                            pr.in_plane_angle, pr.mirrored, pr.normalize, pr.clip)
+    X, Y = O.pixel_coords(sp.xyz[:, :2], pr.shape, pr.calibration, ((W - 1) / 2, (H - 1) / 2),
+                          pr.in_plane_angle, pr.mirrored)
+    boxes = np.full((len(sp.idx), 4), -1, dtype=np.int64)
+    pref = 1 / (2 * np.pi * pr.sigma ** 2)
+    ep = -1 / (2 * pr.sigma ** 2)
+    for j in range(len(sp.idx)):
+        if not (0 <= X[j] < W and 0 <= Y[j] < H):
+            continue
+        with np.errstate(invalid="ignore", divide="ignore"):
+            rad = np.sqrt(np.log(pr.clip / (pref * sp.intensity[j])) / ep)
+        if np.isnan(rad):
+            continue
+        y0, y1 = max(0, int(np.ceil(Y[j] - rad))), min(H, int(np.floor(Y[j] + rad + 1)))
+        x0, x1 = max(0, int(np.ceil(X[j] - rad))), min(W, int(np.floor(X[j] + rad + 1)))
+        if y1 > y0 and x1 > x0:
+            boxes[j] = (y0, y1, x0, x1)
+    idx = sp.idx if rows is None else np.asarray(rows)
+    return pat, FrozenPattern(idx, boxes, int(np.argmax(pat)))
 
 
 def numpy_loss(pr: Problem, M, target, **kw) -> float:

@@ -74,16 +74,22 @@ def excitation_errors(otab: O.PhaseTables, M: np.ndarray, k: float):
     return (-np.sqrt(k ** 2 - r ** 2) + k) - g_lab[:, 2]
 
 
+def _intensity_error(gpu_I, ref: O.Spots) -> float:
+    if ref.idx.size == 0:
+        return 0.0
+    if ref.max_intensity == 0.0 or float(np.max(ref.intensity)) == 0.0:
+        return float(np.max(np.abs(np.asarray(gpu_I, np.float64))))
+    scale = float(np.max(ref.intensity))
+    return float(np.max(np.abs(np.asarray(gpu_I, np.float64) - ref.intensity))) / scale
+
+
 def compare_spot_lists(gpu_idx, gpu_I, ref: O.Spots, s_all, me, stats=None):
     """Check one orientation; returns the number of carved-out reflections."""
     g = np.asarray(gpu_idx)
     r = np.asarray(ref.idx)
     if np.array_equal(g, r):
         if r.size:
-            scale = max(float(np.max(ref.intensity)), 1e-300)
-            err = float(np.max(np.abs(np.asarray(gpu_I, np.float64) - ref.intensity))) / scale
-            if ref.max_intensity == 0.0:
-                err = float(np.max(np.abs(gpu_I)))
+            err = _intensity_error(gpu_I, ref)
             assert err <= TOL_I, f"intensity error {err:.3g} > {TOL_I}"
             if stats is not None:
                 stats["max_rel_err"] = max(stats.get("max_rel_err", 0.0), err)
@@ -92,6 +98,84 @@ def compare_spot_lists(gpu_idx, gpu_I, ref: O.Spots, s_all, me, stats=None):
     near = np.abs(np.abs(s_all[diff]) - me) < CUT_EPS
     assert near.all(), f"index sets differ away from the cut: {diff[~near][:10]}"
     return int(diff.size)
+
+
+def cut_distance(otab: O.PhaseTables, M: np.ndarray, k: float, p: O.SimParams, rows=None):
+    """FP64 distance of each row from the boundary of the intersection test
+    (simulation_generator.py:364-375): ||s| - max_exc| without precession; with
+    precession the distance of z -+ max_exc from the band edges (the nearer of
+    the two)."""
+    g = otab.g if rows is None else otab.g[np.asarray(rows)]
+    g_lab = g @ M
+    r = np.sqrt(np.sum(g_lab[:, :2] ** 2, axis=1))
+    z = g_lab[:, 2]
+    me = p.max_excitation_error
+    if p.precession_angle == 0:
+        s = (-np.sqrt(k ** 2 - r ** 2) + k) - z
+        return np.abs(np.abs(s) - me)
+    phi = np.deg2rad(abs(p.precession_angle))
+    Pz, Pt = k * np.cos(phi), k * np.sin(phi)
+    with np.errstate(invalid="ignore"):
+        up = Pz - np.sqrt(k ** 2 - (r + Pt) ** 2)
+        do = Pz - np.sqrt(k ** 2 - (r - Pt) ** 2)
+    d = np.minimum(np.abs(z - me - up), np.abs(z + me - do))
+    return np.nan_to_num(d, nan=np.inf)
+
+
+def reference_for_rows(otab: O.PhaseTables, M: np.ndarray, k: float, p: O.SimParams,
+                       rows) -> O.Spots:
+    """The FP64 spots of a given survivor set: lab-frame g of those rows and
+    shape factor x |F|^2 evaluated in FP64, with no re-selection.  Used when the
+    GPU chose differently at the cut: its survivor set is then the one the
+    pattern is compared on."""
+    rows = np.asarray(rows, dtype=np.int64)
+    g_lab = otab.g[rows] @ M
+    if rows.size == 0:
+        return O.Spots(rows, g_lab, np.zeros(0), 0, 0.0)
+    r = np.sqrt(np.sum(g_lab[:, :2] ** 2, axis=1))
+    s = (-np.sqrt(k ** 2 - r ** 2) + k) - g_lab[:, 2]
+    I = O.shape_factor(s, r, p) * otab.F2[rows]
+    return O.Spots(rows, g_lab, I, int(rows.size), float(np.max(I)))
+
+
+def check_orientation(otab: O.PhaseTables, M: np.ndarray, k: float, p: O.SimParams, gpu_idx,
+                      gpu_I, stats=None):
+    """One orientation's GPU spot list against the oracle.
+
+    Equal survivor sets: intensities within TOL_I of the maximum.  Different
+    sets: every differing row must lie within CUT_EPS of the intersection
+    boundary (FP64), and the GPU's intensities are then compared with the FP64
+    values of the GPU's own set (:func:`reference_for_rows`).  Returns the
+    reference spots the pattern is to be compared with and the number of
+    carved-out rows; ``stats`` collects ``max_rel_err``, ``carved_rows`` and
+    ``carved_orientations``."""
+    g = np.asarray(gpu_idx)
+    ref = O.simulate_one(otab, M, k, p)
+    carved = 0
+    if not np.array_equal(g, ref.idx):
+        diff = np.setxor1d(g, ref.idx)
+        d = cut_distance(otab, M, k, p, diff)
+        assert (d < CUT_EPS).all(), \
+            f"survivor sets differ away from the cut: rows {diff[d >= CUT_EPS][:10]}"
+        carved = int(diff.size)
+        ref = reference_for_rows(otab, M, k, p, g)
+    err = _intensity_error(gpu_I, ref)
+    assert err <= TOL_I, f"intensity error {err:.3g} > {TOL_I}"
+    if stats is not None:
+        stats["max_rel_err"] = max(stats.get("max_rel_err", 0.0), err)
+        stats["carved_rows"] = stats.get("carved_rows", 0) + carved
+        stats["carved_orientations"] = stats.get("carved_orientations", 0) + int(carved > 0)
+        stats["orientations"] = stats.get("orientations", 0) + 1
+    return ref, carved
+
+
+def pattern_error(pat, rp) -> float:
+    """max |P - P64| relative to max(P64) (absolute when P64 is all zero)."""
+    pat = np.asarray(pat, np.float64)
+    mx = float(np.max(rp)) if rp.size else 0.0
+    if mx > 0:
+        return float(np.max(np.abs(pat - rp))) / mx
+    return float(np.max(np.abs(pat))) if pat.size else 0.0
 
 
 def flipped_reference_pattern(ref: O.Spots, gpu_xy, shape, sigma, cal):
@@ -127,3 +211,115 @@ def render_from_pixels(X, Y, I, shape, sigma, normalize=True):
     if normalize and out.max() > 0:
         out = out / out.max()
     return out
+
+
+def oracle_params(settings, rr, wdb, scattering="lobato") -> O.SimParams:
+    """The oracle's SimParams for a device ``SimSettings``."""
+    return O.SimParams(kv=settings.accelerating_voltage, reciprocal_radius=rr,
+                       max_excitation_error=settings.max_excitation_error,
+                       shape_factor_width=settings.shape_factor_width,
+                       shape_model=settings.shape_model,
+                       shape_kwargs=({"minima_number": settings.minima_number}
+                                     if settings.shape_model in ("sinc", "sin2c", "atanc") else {}),
+                       precession_angle=settings.precession_angle,
+                       approximate_precession=settings.approximate_precession,
+                       minimum_intensity=settings.minimum_intensity, with_direct_beam=wdb,
+                       scattering_params=scattering)
+
+
+def phase_spec(pa) -> O.PhaseSpec:
+    """The oracle's PhaseSpec of a device PhaseArrays."""
+    return O.PhaseSpec(tuple(pa.abc), tuple(pa.angles), list(pa.symbols), np.asarray(pa.xyz),
+                       np.asarray(pa.occ), np.asarray(pa.B))
+
+
+def sparse_oracle_table(ph: O.PhaseSpec, p: O.SimParams, rotations, hkl=None) -> O.PhaseTables:
+    """The oracle's table of a phase with FP64 |F|^2 evaluated only for the
+    rows that intersect one of ``rotations`` (orix to_matrix(), (n, 3, 3)) or
+    lie within 1e-5 of the cut (the rows a GPU survivor set can hold); the
+    other rows carry NaN.  For large cells this is what makes an FP64 check of
+    a few orientations cheap.  ``hkl`` defaults to the oracle's box."""
+    _, rec = ph.basis()
+    if hkl is None:
+        hkl = O.hkl_box(rec, 1.0 / p.reciprocal_radius, include_zero=p.with_direct_beam)
+        if p.with_direct_beam:
+            hkl = np.vstack([hkl, np.zeros((1, 3), dtype=hkl.dtype)])
+    hkl = np.asarray(hkl, dtype=np.int64)
+    g = O.g_cartesian(hkl, rec)
+    gn = np.linalg.norm(g, axis=1)
+    k = 1 / O.electron_wavelength(p.kv)
+    otab0 = O.PhaseTables(hkl, g, gn, np.zeros(len(hkl)), len(hkl))
+    rows = np.zeros(0, dtype=np.int64)
+    for M in np.asarray(rotations, dtype=np.float64).reshape(-1, 3, 3):
+        rows = np.union1d(rows, np.nonzero(O.intersect(g @ M, k, p)[2])[0])
+        rows = np.union1d(rows, np.nonzero(cut_distance(otab0, M, k, p) < 1e-5)[0])
+    coeffs, _ = O.atom_coeffs(ph.symbols, p.scattering_params)
+    F = O.structure_factor(hkl[rows], gn[rows], ph.xyz, ph.occ, ph.B, coeffs,
+                           p.scattering_params, "exact")
+    F2 = np.full(len(hkl), np.nan)
+    F2[rows] = (F * F.conjugate()).real
+    return O.PhaseTables(hkl, g, gn, F2, len(hkl) - (1 if p.with_direct_beam else 0))
+
+
+def subpixel_envelope(xy, intensity, shape, sigma, calibration, clip_threshold=1.0,
+                      normalize=True, in_plane_angle=0.0, mirrored=False):
+    """The oracle's sub-pixel pattern (detector_functions.py:314-359 via
+    simulation2d.py:405-442) with the clip-box carve-out: a box edge
+    ceil(c - rad) / floor(c + rad) whose argument lies within PIX_EPS of an
+    integer may fall on either side under FP32 coordinates, which adds or
+    drops one row or column of pixels worth about clip_threshold.  Returns
+    (nominal, lo, hi): the FP64 pattern and the patterns with every such
+    borderline row/column dropped and kept, each normalised by the nominal
+    maximum when ``normalize``."""
+    H, W = shape
+    X, Y = O.pixel_coords(np.asarray(xy)[:, :2], shape, calibration, ((W - 1) / 2, (H - 1) / 2),
+                          in_plane_angle, mirrored)
+    inf = (X >= 0) & (X < W) & (Y >= 0) & (Y < H)
+    I = np.asarray(intensity, dtype=np.float64)[inf]
+    X, Y = X[inf], Y[inf]
+    nominal = np.zeros(shape)
+    lo = np.zeros(shape)
+    hi = np.zeros(shape)
+    pref = 1 / (2 * np.pi * sigma ** 2)
+    ep = -1 / (2 * sigma ** 2)
+    for cx, cy, a in zip(X, Y, I):
+        with np.errstate(invalid="ignore", divide="ignore"):
+            rad = np.sqrt(np.log(clip_threshold / (pref * a)) / ep)
+        if np.isnan(rad):
+            continue
+        edges = []
+        for v, up in ((cy - rad, True), (cy + rad, False), (cx - rad, True), (cx + rad, False)):
+            r = np.round(v)
+            amb = abs(v - r) < PIX_EPS
+            nom = int(np.ceil(v)) if up else int(np.floor(v))
+            if amb:
+                inc = int(r) if up else int(r)           # the edge pixel kept
+                exc = int(r) + 1 if up else int(r) - 1   # the edge pixel dropped
+            else:
+                inc = exc = nom
+            edges.append((nom, inc, exc))
+        for out, pick in ((nominal, 0), (hi, 1), (lo, 2)):
+            y0 = max(0, edges[0][pick])
+            y1 = min(H, edges[1][pick] + 1)
+            x0 = max(0, edges[2][pick])
+            x1 = min(W, edges[3][pick] + 1)
+            if y1 <= y0 or x1 <= x0:
+                continue
+            yy, xx = np.mgrid[y0:y1, x0:x1]
+            out[y0:y1, x0:x1] += a * pref * np.exp(ep * ((yy - cy) ** 2 + (xx - cx) ** 2))
+    if normalize:
+        mx = nominal.max()
+        if mx > 0:
+            nominal, lo, hi = nominal / mx, lo / mx, hi / mx
+    return nominal, lo, hi
+
+
+def envelope_error(pat, nominal, lo, hi) -> float:
+    """Error of a GPU sub-pixel pattern against the FP64 envelope, relative to
+    the nominal maximum: pixels between lo and hi count as exact."""
+    pat = np.asarray(pat, np.float64)
+    a = np.minimum(lo, hi)
+    b = np.maximum(lo, hi)
+    d = np.maximum(np.maximum(a - pat, pat - b), 0.0)
+    mx = float(nominal.max())
+    return float(d.max()) / mx if mx > 0 else float(np.abs(pat).max())
