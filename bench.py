@@ -409,7 +409,8 @@ def ragged_leg(dev, steps, scale):
         ev[2 + j].record(stream)
     torch.cuda.synchronize()
     kernel_ms["simulate"] = ev[0].elapsed_time(ev[1])
-    names = {_lib.RENDER_SPLAT: "render_splat", _lib.RENDER_DENSE: "render_dense"}
+    names = {_lib.RENDER_SPLAT: "render_splat", _lib.RENDER_DENSE: "render_dense",
+             _lib.RENDER_DENSE_INPLACE: "render_dense_inplace"}
     for j, m in enumerate(lib.render_lists):
         kernel_ms[names[m]] = ev[1 + j].elapsed_time(ev[2 + j])
     n = lib.n_total
@@ -462,7 +463,9 @@ def ragged_leg(dev, steps, scale):
         "kernel_ms": {k_: round(v, 3) for k_, v in kernel_ms.items()},
         "setup_s_incl_tables_and_tuning": round(setup, 2),
         "render_phases": {"splat": sum(s.method == _lib.RENDER_SPLAT for s in lib.slots),
-                          "dense": sum(s.method == _lib.RENDER_DENSE for s in lib.slots)},
+                          "dense": sum(s.method == _lib.RENDER_DENSE for s in lib.slots),
+                          "dense_inplace": sum(s.method == _lib.RENDER_DENSE_INPLACE
+                                               for s in lib.slots)},
         "tuned": {"simulate_blocks_per_cu": lib.sim_bpc,
                   "render_blocks_per_cu": {str(k_): v for k_, v in lib.render_bpc.items()}},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1),

@@ -42,6 +42,7 @@ ADJ_FORWARD = 2
 
 RENDER_SPLAT = 0
 RENDER_DENSE = 1
+RENDER_DENSE_INPLACE = 2
 
 MESH_UV_SPHERE = 0
 MESH_NORMALIZED_CUBE = 1
@@ -61,6 +62,11 @@ EXPORTED = (
     "dsx_beam_directions_candidates",
     "dsx_beam_directions_workspace_ints",
     "dsx_beam_directions",
+    "dsx_grid_workspace_ints",
+    "dsx_icosahedral_vertices",
+    "dsx_icosahedral_nn_dots",
+    "dsx_beam_directions_icosahedral",
+    "dsx_beam_directions_from_vertices",
     "dsx_simulate",
     "dsx_render_max_spots",
     "dsx_render_fast",
@@ -72,6 +78,7 @@ EXPORTED = (
     "dsx_simulate_ragged",
     "dsx_render_dense_max_pixels",
     "dsx_render_fast_ragged",
+    "dsx_render_dense_inplace_fits",
     "dsx_polar_templates",
     "dsx_render_fast_large_workspace_ints",
     "dsx_render_fast_large",
@@ -151,6 +158,16 @@ _SIGS = {
     "dsx_beam_directions_workspace_ints": (ctypes.c_longlong, [_I, ctypes.c_double]),
     "dsx_beam_directions": (_I, [_I, ctypes.c_double, _I, ctypes.POINTER(ctypes.c_int), _P,
                                  ctypes.c_longlong, _P, _P, ctypes.c_longlong, _P]),
+    "dsx_grid_workspace_ints": (ctypes.c_longlong, [ctypes.c_longlong]),
+    "dsx_icosahedral_vertices": (ctypes.c_longlong, [_I]),
+    "dsx_icosahedral_nn_dots": (_I, [_I, _I, ctypes.POINTER(ctypes.c_int), _P, _P]),
+    "dsx_beam_directions_icosahedral": (_I, [_I, ctypes.POINTER(ctypes.c_int), _I,
+                                             ctypes.POINTER(ctypes.c_int), _I, _P,
+                                             ctypes.c_longlong, _P, _P, ctypes.c_longlong, _P]),
+    "dsx_beam_directions_from_vertices": (_I, [_P, ctypes.c_longlong, _I,
+                                               ctypes.POINTER(ctypes.c_int), _P,
+                                               ctypes.c_longlong, _P, _P, ctypes.c_longlong,
+                                               _P]),
     "dsx_hkl_lines_workspace_ints": (ctypes.c_longlong, [_I]),
     "dsx_hkl_lines": (_I, [_P, _P, _I, _P, _I, _P, _P, ctypes.c_longlong, _P]),
     "dsx_simulate": (_I, [_P, _P, _P, _I, _P, _I, _P, _I, ctypes.POINTER(SimParams), _I, _P, _P,
@@ -167,6 +184,7 @@ _SIGS = {
     "dsx_simulate_tile_size": (_I, []),
     "dsx_simulate_ragged": (_I, [_P, _I, _P, _I, ctypes.POINTER(SimParams), _I, _P]),
     "dsx_render_dense_max_pixels": (_I, []),
+    "dsx_render_dense_inplace_fits": (_I, [_I, _I, _I]),
     "dsx_render_fast_large_workspace_ints": (ctypes.c_longlong, [_I]),
     "dsx_euler_to_rt": (_I, [_P, _I, _P, _P]),
     "dsx_euler_to_rt_backward": (_I, [_P, _P, _I, _P, _P]),

@@ -10,14 +10,27 @@
 // reference's operation order (contraction off), the triangle test, the
 // angles.  Rows stay in candidate order through a count / scan / write
 // compaction (block counts, one scan block, block-local ballot ranks).
+#include <algorithm>
+
 #include "dsx_internal.h"
 
 namespace dsx {
 
 constexpr int kGridThreads = 256;
 
+// Icosahedron of get_icosahedral_mesh_vertices (sphere_mesh_generators.py:394-431)
+// with the caller's edge order (the order the reference's edge set iterates in).
+struct IcosaTopo {
+  double corner[12][3];
+  int face[20][3];
+  int edge[30][2];              // sorted corner pairs, in the caller's order
+  signed char edge_id[12][12];  // edge holding a corner pair, -1 if none
+  signed char edge_face[30][2]; // the two faces of each edge
+  signed char corner_face[12][5];
+};
+
 struct GridArgs {
-  int mesh;
+  int mesh;  // DSX_MESH_*, or kMeshIcosahedral / kMeshVertices
   long long n_cand;
   // UV sphere: psi outer (n_psi steps of dpsi), theta inner (n_theta steps of
   // dtheta, the last exactly pi); pi_ref = deg2rad(180) as numpy forms it
@@ -31,7 +44,230 @@ struct GridArgs {
   double plane[3][4];
   double *euler;
   long long capacity;
+  int xyz_out;  // write the (kept) vertices instead of Bunge angles
+  // icosahedral mesh at refinement level `level` (10 level^2 + 2 vertices)
+  int level;
+  IcosaTopo ico;
+  // explicit vertices (double [n_cand][3])
+  const double *verts;
 };
+
+constexpr int kMeshIcosahedral = 100;
+constexpr int kMeshVertices = 101;
+
+// ---- icosahedral mesh ------------------------------------------------------
+//
+// Vertex ids at level n (_compose_from_faces, sphere_mesh_generators.py:200-325):
+// the 12 corners; per edge e (caller order) n-1 points (1-t) c0 + t c1 with
+// t = m step + 1/n, step = (1 - 1/n) / (n-1) (numpy's linspace); per face
+// (f0, f1, f2) the interior points of rows i = 1..n-2 (outer) and
+// j = 1..n-i-1 (inner) with weights (1 - i/n - j/n, j/n, i/n).  In face-local
+// coordinates (a, b), a + b <= n, the point is f0 (1 - a/n - b/n) + f1 b/n +
+// f2 a/n, so interior (i, j) is (a, b) = (i, j).
+
+__device__ __forceinline__ long long icosa_count(long long n) { return 10 * n * n + 2; }
+
+__host__ __device__ __forceinline__ void icosa_vertex(const IcosaTopo &t, int n, long long g, double &x,
+                                             double &y, double &z) {
+#pragma clang fp contract(off)
+  if (g < 12) {
+    x = t.corner[g][0];
+    y = t.corner[g][1];
+    z = t.corner[g][2];
+    return;
+  }
+  const long long ne = 30ll * (n - 1);
+  if (g < 12 + ne) {
+    const int e = static_cast<int>((g - 12) / (n - 1));
+    const int m = static_cast<int>((g - 12) - static_cast<long long>(e) * (n - 1));
+    const double start = 1.0 / n;
+    const double step = (1.0 - start) / (n - 1);
+    const double tt = static_cast<double>(m) * step + start;
+    const double u = 1.0 - tt;
+    const int i0 = t.edge[e][0], i1 = t.edge[e][1];
+    x = u * t.corner[i0][0] + tt * t.corner[i1][0];
+    y = u * t.corner[i0][1] + tt * t.corner[i1][1];
+    z = u * t.corner[i0][2] + tt * t.corner[i1][2];
+    return;
+  }
+  const long long per = static_cast<long long>(n - 1) * (n - 2) / 2;
+  const long long r = g - 12 - ne;
+  const int f = static_cast<int>(r / per);
+  const long long q = r - f * per;
+  // row u = i - 1 starts at u (2L - u + 1) / 2, L = n - 2
+  const double L2 = 2.0 * (n - 2) + 1.0;
+  long long u = static_cast<long long>((L2 - sqrt(L2 * L2 - 8.0 * static_cast<double>(q))) * 0.5);
+  auto off = [&](long long uu) { return uu * (2ll * (n - 2) - uu + 1) / 2; };
+  while (u > 0 && off(u) > q) --u;
+  while (off(u + 1) <= q) ++u;
+  const int i = static_cast<int>(u) + 1;
+  const int j = static_cast<int>(q - off(u)) + 1;
+  const double b0 = static_cast<double>(i) / n, b1 = static_cast<double>(j) / n;
+  const double w0 = (1.0 - b0) - b1;
+  const int f0 = t.face[f][0], f1 = t.face[f][1], f2 = t.face[f][2];
+  x = (t.corner[f0][0] * w0 + t.corner[f1][0] * b1) + t.corner[f2][0] * b0;
+  y = (t.corner[f0][1] * w0 + t.corner[f1][1] * b1) + t.corner[f2][1] * b0;
+  z = (t.corner[f0][2] * w0 + t.corner[f1][2] * b1) + t.corner[f2][2] * b0;
+}
+
+// Vertex id of face-local point (a, b) of face f at level n.
+__host__ __device__ __forceinline__ long long icosa_local(const IcosaTopo &t, int n, int f, int a, int b) {
+  const int f0 = t.face[f][0], f1 = t.face[f][1], f2 = t.face[f][2];
+  if (a == 0 && b == 0) return f0;
+  if (a == 0 && b == n) return f1;
+  if (a == n && b == 0) return f2;
+  int p, q, d;  // edge p-q, distance d/n from p
+  if (a == 0) {
+    p = f0; q = f1; d = b;
+  } else if (b == 0) {
+    p = f0; q = f2; d = a;
+  } else if (a + b == n) {
+    p = f1; q = f2; d = a;
+  } else {
+    const long long per = static_cast<long long>(n - 1) * (n - 2) / 2;
+    const long long u = a - 1;
+    return 12 + 30ll * (n - 1) + f * per + u * (2ll * (n - 2) - u + 1) / 2 + (b - 1);
+  }
+  const int e = t.edge_id[p][q];
+  const int m = p < q ? d - 1 : n - d - 1;
+  return 12 + static_cast<long long>(e) * (n - 1) + m;
+}
+
+__host__ __device__ __forceinline__ void unit(double &x, double &y, double &z) {
+#pragma clang fp contract(off)
+  const double nrm = sqrt((x * x + y * y) + z * z);
+  x = x / nrm;
+  y = y / nrm;
+  z = z / nrm;
+}
+
+// Largest dot product between unit vertex g and its neighbours in face f,
+// where g sits at face-local (a, b).  The nearest neighbour of a vertex of
+// this mesh is one of its triangulation neighbours: radial projection
+// distorts a face's triangles by at most a factor 1/cos(37.4 deg) = 1.26,
+// while a vertex two steps away is at least sqrt(3) steps from it.
+__host__ __device__ __forceinline__ double icosa_face_best(const IcosaTopo &t, int n, int f, int a, int b,
+                                                  double ux, double uy, double uz, double best) {
+#pragma clang fp contract(off)
+  const int da[6] = {1, -1, 0, 0, 1, -1};
+  const int db[6] = {0, 0, 1, -1, -1, 1};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    const int a2 = a + da[k], b2 = b + db[k];
+    if (a2 < 0 || b2 < 0 || a2 + b2 > n) continue;
+    double x, y, z;
+    icosa_vertex(t, n, icosa_local(t, n, f, a2, b2), x, y, z);
+    unit(x, y, z);
+    const double d = (ux * x + uy * y) + uz * z;
+    best = d > best ? d : best;
+  }
+  return best;
+}
+
+// Dot product of vertex g (level n) with its nearest neighbour.
+__host__ __device__ double icosa_nn_dot(const IcosaTopo &t, int n, long long g) {
+  double ux, uy, uz;
+  icosa_vertex(t, n, g, ux, uy, uz);
+  unit(ux, uy, uz);
+  double best = -2.0;
+  if (g < 12) {
+    const int c = static_cast<int>(g);
+    for (int k = 0; k < 5; ++k) {
+      const int f = t.corner_face[c][k];
+      const int a = t.face[f][2] == c ? n : 0;
+      const int b = t.face[f][1] == c ? n : 0;
+      best = icosa_face_best(t, n, f, a, b, ux, uy, uz, best);
+    }
+    return best;
+  }
+  const long long ne = 30ll * (n - 1);
+  if (g < 12 + ne) {
+    const int e = static_cast<int>((g - 12) / (n - 1));
+    const int m = static_cast<int>((g - 12) - static_cast<long long>(e) * (n - 1));
+    const int i0 = t.edge[e][0], i1 = t.edge[e][1];
+    for (int k = 0; k < 2; ++k) {
+      const int f = t.edge_face[e][k];
+      const int f0 = t.face[f][0], f1 = t.face[f][1], f2 = t.face[f][2];
+      int p, q, a = 0, b = 0;  // the face side holding the edge, from p to q
+      if ((f0 == i0 && f1 == i1) || (f0 == i1 && f1 == i0)) {
+        p = f0; q = f1;
+      } else if ((f0 == i0 && f2 == i1) || (f0 == i1 && f2 == i0)) {
+        p = f0; q = f2;
+      } else {
+        p = f1; q = f2;
+      }
+      const int d = p < q ? m + 1 : n - m - 1;  // distance from p in steps
+      if (p == f0 && q == f1) {
+        b = d;
+      } else if (p == f0 && q == f2) {
+        a = d;
+      } else {
+        a = d;
+        b = n - d;
+      }
+      best = icosa_face_best(t, n, f, a, b, ux, uy, uz, best);
+    }
+    return best;
+  }
+  const long long per = static_cast<long long>(n - 1) * (n - 2) / 2;
+  const long long r = g - 12 - ne;
+  const int f = static_cast<int>(r / per);
+  const long long q = r - f * per;
+  const double L2 = 2.0 * (n - 2) + 1.0;
+  long long u = static_cast<long long>((L2 - sqrt(L2 * L2 - 8.0 * static_cast<double>(q))) * 0.5);
+  auto off = [&](long long uu) { return uu * (2ll * (n - 2) - uu + 1) / 2; };
+  while (u > 0 && off(u) > q) --u;
+  while (off(u + 1) <= q) ++u;
+  return icosa_face_best(t, n, f, static_cast<int>(u) + 1, static_cast<int>(q - off(u)) + 1, ux,
+                         uy, uz, best);
+}
+
+// Vertices of levels first..: S(L) = sum_{n=1..L} (10 n^2 + 2).
+__host__ __device__ __forceinline__ long long icosa_cum(long long L) {
+  return 10 * L * (L + 1) * (2 * L + 1) / 6 + 2 * L;
+}
+
+constexpr int kIcoThreads = 256;
+
+__global__ __launch_bounds__(kIcoThreads) void icosa_init_kernel(unsigned long long *bits, int n) {
+  const int i = blockIdx.x * kIcoThreads + threadIdx.x;
+  if (i < n) bits[i] = 0x7FF0000000000000ull;  // +inf
+}
+
+// Smallest nearest-neighbour dot product of each level in [first, last]
+// (stored as the bits of a positive double, so unsigned order is value order).
+__global__ __launch_bounds__(kIcoThreads) void icosa_nn_kernel(IcosaTopo t, int first, int last,
+                                                               unsigned long long *bits) {
+  const long long base = icosa_cum(first - 1);
+  const long long total = icosa_cum(last) - base;
+  for (long long w0 = static_cast<long long>(blockIdx.x) * kIcoThreads; w0 < total;
+       w0 += static_cast<long long>(gridDim.x) * kIcoThreads) {
+    const long long flat = w0 + threadIdx.x;
+    double d = 2.0;
+    int lv = 0;
+    if (flat < total) {
+      const long long gf = base + flat;
+      // level: largest L with icosa_cum(L - 1) <= gf
+      long long L = static_cast<long long>(cbrt(static_cast<double>(gf) * 0.3));
+      if (L < 1) L = 1;
+      while (L > 1 && icosa_cum(L - 1) > gf) --L;
+      while (icosa_cum(L) <= gf) ++L;
+      lv = static_cast<int>(L);
+      d = icosa_nn_dot(t, lv, gf - icosa_cum(L - 1));
+    }
+    // one atomic per level present in the wave
+    const int lo = __shfl(lv, 0, kWave);
+    for (int L = lo;; ++L) {
+      double m = lv == L ? d : 2.0;
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) m = fmin(m, __shfl_xor(m, o, kWave));
+      if (lane_id() == 0 && m < 2.0)
+        atomicMin(bits + (L - first), static_cast<unsigned long long>(__double_as_longlong(m)));
+      const uint64_t more = __ballot(lv > L);
+      if (!more) break;
+    }
+  }
+}
 
 // Per-face grid coordinate j in [0, m) of the cube meshes.
 __device__ __forceinline__ double cube_coord(const GridArgs &a, int j) {
@@ -46,6 +282,17 @@ __device__ __forceinline__ double cube_coord(const GridArgs &a, int j) {
 __device__ __forceinline__ bool mesh_vertex(const GridArgs &a, long long c, double &x, double &y,
                                             double &z) {
 #pragma clang fp contract(off)
+  if (a.mesh == kMeshVertices) {
+    x = a.verts[3 * c];
+    y = a.verts[3 * c + 1];
+    z = a.verts[3 * c + 2];
+    return true;
+  }
+  if (a.mesh == kMeshIcosahedral) {  // rows divided by sqrt(einsum('ij,ij->i'))
+    icosa_vertex(a.ico, a.level, c, x, y, z);
+    unit(x, y, z);
+    return true;
+  }
   if (a.mesh == DSX_MESH_UV_SPHERE) {
     const int ip = static_cast<int>(c / a.n_theta), it = static_cast<int>(c % a.n_theta);
     const double psi = static_cast<double>(ip) * a.dpsi;
@@ -119,6 +366,12 @@ __global__ __launch_bounds__(kGridThreads) void grid_kernel(GridArgs a, int *wor
   long long row = work[gridDim.x + blockIdx.x] + mask_rank(m);
   for (int w = 0; w < wid; ++w) row += s_wave[w];
   if (row >= a.capacity) return;
+  if (a.xyz_out) {
+    a.euler[3 * row + 0] = x;
+    a.euler[3 * row + 1] = y;
+    a.euler[3 * row + 2] = z;
+    return;
+  }
   // beam_directions_grid_to_euler: Phi = acos(z/|v|), phi2 = sign * acos(x/|v_xy|)
   const double nrm = sqrt((x * x + y * y) + z * z);
   const double Phi = acos(z / nrm);
@@ -191,6 +444,86 @@ static bool grid_params(int mesh, double res, GridArgs &a) {
   return true;
 }
 
+// Count / scan / write launch shared by the grid entry points.
+static int launch_grid(GridArgs &a, int n_planes, const int *planes, double *out_d,
+                       long long capacity, long long *count_d, int *work_d, long long work_ints,
+                       void *stream, const char *name) {
+  if (n_planes < 0 || n_planes > 3 || (n_planes > 0 && !planes) || capacity < 0) {
+    set_error("%s: bad crop planes (%d) or capacity", name, n_planes);
+    return DSX_EINVAL;
+  }
+  const long long nb = (a.n_cand + kGridThreads - 1) / kGridThreads;
+  if (nb > (1ll << 30)) {
+    set_error("%s: %lld candidates is too many", name, a.n_cand);
+    return DSX_EINVAL;
+  }
+  if (!out_d || !count_d || !work_d || work_ints < 2 * nb) {
+    set_error("%s: null pointer or workspace of %lld ints < %lld", name, work_ints, 2 * nb);
+    return DSX_EINVAL;
+  }
+  a.n_planes = n_planes;
+  for (int i = 0; i < n_planes; ++i)
+    for (int j = 0; j < 4; ++j) a.plane[i][j] = static_cast<double>(planes[4 * i + j]);
+  a.euler = out_d;
+  a.capacity = capacity;
+  hipStream_t s = as_stream(stream);
+  if (nb == 0) {
+    hipLaunchKernelGGL(grid_scan_kernel, dim3(1), dim3(1024), 0, s, work_d, 0, count_d);
+    return check_launch(name);
+  }
+  hipLaunchKernelGGL(grid_kernel<0>, dim3(static_cast<unsigned>(nb)), dim3(kGridThreads), 0, s, a, work_d);
+  hipLaunchKernelGGL(grid_scan_kernel, dim3(1), dim3(1024), 0, s, work_d, static_cast<int>(nb), count_d);
+  hipLaunchKernelGGL(grid_kernel<1>, dim3(static_cast<unsigned>(nb)), dim3(kGridThreads), 0, s, a, work_d);
+  return check_launch(name);
+}
+
+// Icosahedron topology from the caller's edge order (30 sorted corner pairs);
+// false unless they are exactly the 30 edges of the faces.
+bool icosa_topology(const int *edge_order, IcosaTopo &t) {
+  static const int faces[20][3] = {
+      {0, 11, 5}, {0, 5, 1},  {0, 1, 7},   {0, 7, 10}, {0, 10, 11}, {1, 5, 9},  {5, 11, 4},
+      {11, 10, 2}, {10, 7, 6}, {7, 1, 8},  {3, 9, 4},  {3, 4, 2},   {3, 2, 6},  {3, 6, 8},
+      {3, 8, 9},  {4, 9, 5},  {2, 4, 11}, {6, 2, 10}, {8, 6, 7},   {9, 8, 1}};
+  const double g = (1.0 + sqrt(5.0)) / 2.0;
+  const double c[12][3] = {{-1, g, 0}, {1, g, 0},  {-1, -g, 0}, {1, -g, 0},
+                           {0, -1, g}, {0, 1, g},  {0, -1, -g}, {0, 1, -g},
+                           {g, 0, -1}, {g, 0, 1},  {-g, 0, -1}, {-g, 0, 1}};
+  for (int i = 0; i < 12; ++i)
+    for (int k = 0; k < 3; ++k) t.corner[i][k] = c[i][k];
+  for (int f = 0; f < 20; ++f)
+    for (int k = 0; k < 3; ++k) t.face[f][k] = faces[f][k];
+  for (int i = 0; i < 12; ++i)
+    for (int j = 0; j < 12; ++j) t.edge_id[i][j] = -1;
+  if (!edge_order) return false;
+  for (int e = 0; e < 30; ++e) {
+    const int p = edge_order[2 * e], q = edge_order[2 * e + 1];
+    if (p < 0 || q > 11 || p >= q || t.edge_id[p][q] >= 0) return false;
+    t.edge[e][0] = p;
+    t.edge[e][1] = q;
+    t.edge_id[p][q] = t.edge_id[q][p] = static_cast<signed char>(e);
+    t.edge_face[e][0] = t.edge_face[e][1] = -1;
+  }
+  int n_corner_faces[12] = {0};
+  for (int f = 0; f < 20; ++f) {
+    for (int k = 0; k < 3; ++k) {
+      const int p = faces[f][k], q = faces[f][(k + 1) % 3];
+      const int e = t.edge_id[p][q];
+      if (e < 0) return false;
+      signed char *ef = t.edge_face[e];
+      if (ef[0] < 0) ef[0] = static_cast<signed char>(f);
+      else if (ef[1] < 0) ef[1] = static_cast<signed char>(f);
+      else return false;
+      if (n_corner_faces[p] >= 5) return false;
+      t.corner_face[p][n_corner_faces[p]++] = static_cast<signed char>(f);
+    }
+  }
+  for (int e = 0; e < 30; ++e)
+    if (t.edge_face[e][1] < 0) return false;
+  return true;
+}
+
+constexpr int kIcoMaxLevel = 1 << 14;
+
 }  // namespace dsx
 
 using namespace dsx;
@@ -206,36 +539,90 @@ extern "C" long long dsx_beam_directions_workspace_ints(int mesh, double resolut
   return n < 0 ? -1 : 2 * ((n + kGridThreads - 1) / kGridThreads);
 }
 
+
 extern "C" int dsx_beam_directions(int mesh, double resolution, int n_planes, const int *planes,
                                    double *euler_d, long long capacity, long long *count_d,
                                    int *work_d, long long work_ints, void *stream) {
   clear_error();
   GridArgs a;
+  a.xyz_out = 0;
   if (!(resolution > 0.0) || !grid_params(mesh, resolution, a)) {
     set_error("dsx_beam_directions: unknown mesh %d or bad resolution %g", mesh, resolution);
     return DSX_EINVAL;
   }
-  if (n_planes < 0 || n_planes > 3 || (n_planes > 0 && !planes) || capacity < 0) {
-    set_error("dsx_beam_directions: bad crop planes (%d) or capacity", n_planes);
+  return launch_grid(a, n_planes, planes, euler_d, capacity, count_d, work_d, work_ints, stream,
+                     "dsx_beam_directions");
+}
+
+extern "C" long long dsx_grid_workspace_ints(long long n_vertices) {
+  return n_vertices < 0 ? -1 : 2 * ((n_vertices + kGridThreads - 1) / kGridThreads);
+}
+
+extern "C" long long dsx_icosahedral_vertices(int level) {
+  return level < 1 || level > kIcoMaxLevel ? -1 : 10ll * level * level + 2;
+}
+
+extern "C" int dsx_icosahedral_nn_dots(int level_first, int level_last, const int *edge_order,
+                                       double *min_dot_d, void *stream) {
+  clear_error();
+  IcosaTopo t;
+  if (level_first < 1 || level_last < level_first || level_last > kIcoMaxLevel) {
+    set_error("dsx_icosahedral_nn_dots: bad level range [%d, %d] (1..%d)", level_first,
+              level_last, kIcoMaxLevel);
     return DSX_EINVAL;
   }
-  const long long nb = (a.n_cand + kGridThreads - 1) / kGridThreads;
-  if (nb > (1ll << 30)) {
-    set_error("dsx_beam_directions: %lld candidates is too many", a.n_cand);
+  if (!min_dot_d || !icosa_topology(edge_order, t)) {
+    set_error("dsx_icosahedral_nn_dots: null output or the edge order is not the 30 edges of "
+              "the icosahedron as sorted corner pairs");
     return DSX_EINVAL;
   }
-  if (!euler_d || !count_d || !work_d || work_ints < 2 * nb) {
-    set_error("dsx_beam_directions: null pointer or workspace of %lld ints < %lld", work_ints, 2 * nb);
-    return DSX_EINVAL;
-  }
-  a.n_planes = n_planes;
-  for (int i = 0; i < n_planes; ++i)
-    for (int j = 0; j < 4; ++j) a.plane[i][j] = static_cast<double>(planes[4 * i + j]);
-  a.euler = euler_d;
-  a.capacity = capacity;
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(grid_kernel<0>, dim3(static_cast<unsigned>(nb)), dim3(kGridThreads), 0, s, a, work_d);
-  hipLaunchKernelGGL(grid_scan_kernel, dim3(1), dim3(1024), 0, s, work_d, static_cast<int>(nb), count_d);
-  hipLaunchKernelGGL(grid_kernel<1>, dim3(static_cast<unsigned>(nb)), dim3(kGridThreads), 0, s, a, work_d);
-  return check_launch("dsx_beam_directions");
+  const int nl = level_last - level_first + 1;
+  auto *bits = reinterpret_cast<unsigned long long *>(min_dot_d);
+  hipLaunchKernelGGL(icosa_init_kernel, dim3((nl + kIcoThreads - 1) / kIcoThreads),
+                     dim3(kIcoThreads), 0, s, bits, nl);
+  const long long L1 = level_first - 1, L2 = level_last;
+  const long long total = (10 * L2 * (L2 + 1) * (2 * L2 + 1) / 6 + 2 * L2) -
+                          (10 * L1 * (L1 + 1) * (2 * L1 + 1) / 6 + 2 * L1);
+  const long long want = (total + kIcoThreads - 1) / kIcoThreads;
+  const int blocks = static_cast<int>(std::min<long long>(want, 64ll * cu_count()));
+  hipLaunchKernelGGL(icosa_nn_kernel, dim3(blocks), dim3(kIcoThreads), 0, s, t, level_first,
+                     level_last, bits);
+  return check_launch("dsx_icosahedral_nn_dots");
+}
+
+extern "C" int dsx_beam_directions_icosahedral(int level, const int *edge_order, int n_planes,
+                                               const int *planes, int xyz_out, double *out_d,
+                                               long long capacity, long long *count_d,
+                                               int *work_d, long long work_ints, void *stream) {
+  clear_error();
+  GridArgs a;
+  if (level < 1 || level > kIcoMaxLevel || !icosa_topology(edge_order, a.ico)) {
+    set_error("dsx_beam_directions_icosahedral: bad level %d or edge order", level);
+    return DSX_EINVAL;
+  }
+  a.mesh = kMeshIcosahedral;
+  a.level = level;
+  a.n_cand = 10ll * level * level + 2;
+  a.xyz_out = xyz_out ? 1 : 0;
+  return launch_grid(a, n_planes, planes, out_d, capacity, count_d, work_d, work_ints, stream,
+                     "dsx_beam_directions_icosahedral");
+}
+
+extern "C" int dsx_beam_directions_from_vertices(const double *xyz_d, long long n, int n_planes,
+                                                 const int *planes, double *euler_d,
+                                                 long long capacity, long long *count_d,
+                                                 int *work_d, long long work_ints, void *stream) {
+  clear_error();
+  GridArgs a;
+  if (n < 0 || (n > 0 && !xyz_d)) {
+    set_error("dsx_beam_directions_from_vertices: bad vertex array (n = %lld)", n);
+    return DSX_EINVAL;
+  }
+  a.mesh = kMeshVertices;
+  a.verts = xyz_d;
+  a.n_cand = n;
+  a.xyz_out = 0;
+  return launch_grid(a, n_planes, planes, euler_d, capacity, count_d, work_d, work_ints, stream,
+                     "dsx_beam_directions_from_vertices");
 }

@@ -1008,6 +1008,247 @@ __global__ __launch_bounds__(kDenseThreads) void render_dense_kernel(RenderArgs 
 }
 
 // ---------------------------------------------------------------------------
+// In-place dense fast-mode renderer: the same result as render_dense_kernel
+// from one FP32 buffer per pattern, so two 512-thread blocks (two patterns)
+// share a CU.  The buffer holds H rows of stride P = W + 1 (odd: lanes along
+// x and lanes along y both hit distinct banks) and goes through:
+//   owner map   int words, empty = INT_MIN; atomicMax of the spot's list
+//               position (last write wins);
+//   spot image  the owning spot writes its intensity with the sign bit
+//               flipped (I >= 0 gives a negative int, so a later compare of
+//               another spot's position never matches it; empty decodes to
+//               -0 = 0);
+//   column pass scipy's axis 0: each lane filters a segment of one column in
+//               place; the 2R rows of halo it needs from the neighbouring
+//               segments (or their reflections) are read into registers
+//               before a barrier, its own rows as it goes;
+//   row pass    axis 1, the same along a segment of one row, and the maximum;
+//   output      scaled by 1/max, streamed to HBM with non-temporal stores,
+//               every word reset to empty for the next pattern.
+// Every output is formed in scipy's order, t0 v0 + sum_d t_d (v_-d + v_d), as
+// in render_dense_kernel, so both kernels give the same bits.  The shape is a
+// template parameter (every offset an immediate; with runtime strides the
+// compiler hoists the halo addresses out of the pattern loop and spills):
+// 128 x 128 patterns with R <= 5 (wider taps spill at four waves per SIMD),
+// H and W multiples of kInplaceSeg, R < min(H, W), W <= kInplaceThreads.
+// ---------------------------------------------------------------------------
+
+constexpr int kInplaceThreads = 512;
+constexpr int kInplaceSeg = 16;   // outputs per register block along a line
+constexpr int kInplacePre = 4;    // spots per thread of the next pattern held in registers
+constexpr int kEmpty = static_cast<int>(0x80000000u);
+
+__host__ __device__ constexpr int inplace_segments(int lanes_per_seg, int len) {
+  // segments of a line of `len` samples: as many as the block's lanes allow,
+  // each a whole number of register blocks
+  const int chunks = len / kInplaceSeg;
+  int nseg = kInplaceThreads / lanes_per_seg;
+  if (nseg > chunks) nseg = chunks;
+  while (nseg > 1 && chunks % nseg != 0) --nseg;
+  return nseg < 1 ? 1 : nseg;
+}
+
+__host__ __device__ __forceinline__ bool inplace_fits(int R, int H, int W) {
+  return R >= 1 && R <= 5 && H % kInplaceSeg == 0 && W % kInplaceSeg == 0 && W <= kInplaceThreads &&
+         H <= kInplaceThreads && R < H && R < W;
+}
+
+__device__ __forceinline__ float dec_px(int bits) { return __int_as_float(bits ^ kEmpty); }
+
+// Filter the line segment [r0, r0 + len) of one column (stride = P) or one
+// row (stride = 1) in place.  top/bot: the kR samples before and after the
+// segment (reflected at the pattern edges), read before the caller's barrier.
+template <int kR, bool kDecode, int stride, int len>
+__device__ __forceinline__ float inplace_line(int *buf, int base, int r0,
+                                              const float (&top)[kR], const float (&bot)[kR],
+                                              const float *taps, float lmax) {
+  constexpr int kS = kInplaceSeg, kN = kS + 2 * kR;
+  float v[kN];
+#pragma unroll
+  for (int m = 0; m < kR; ++m) v[m] = top[m];
+#pragma unroll
+  for (int m = 0; m < kR; ++m) {
+    const int w = buf[base + (r0 + m) * stride];
+    v[kR + m] = kDecode ? dec_px(w) : __int_as_float(w);
+  }
+  constexpr int nc = len / kS;
+#pragma unroll
+  for (int c = 0; c < nc; ++c) {
+    const int y0 = r0 + c * kS;
+    if (c + 1 < nc) {
+#pragma unroll
+      for (int k = 0; k < kS; ++k) {
+        const int w = buf[base + (y0 + kR + k) * stride];
+        v[2 * kR + k] = kDecode ? dec_px(w) : __int_as_float(w);
+      }
+    } else {  // last block: the samples past the segment are the halo
+#pragma unroll
+      for (int k = 0; k < kS - kR; ++k) {
+        const int w = buf[base + (y0 + kR + k) * stride];
+        v[2 * kR + k] = kDecode ? dec_px(w) : __int_as_float(w);
+      }
+#pragma unroll
+      for (int m = 0; m < kR; ++m) v[kS + kR + m] = bot[m];
+    }
+#pragma unroll
+    for (int k = 0; k < kS; ++k) {
+      float o = taps[0] * v[k + kR];
+#pragma unroll
+      for (int d = 1; d <= kR; ++d) o = fmaf(taps[d], v[k + kR - d] + v[k + kR + d], o);
+      buf[base + (y0 + k) * stride] = __float_as_int(o);
+      lmax = fmaxf(lmax, o);
+    }
+#pragma unroll
+    for (int m = 0; m < 2 * kR; ++m) v[m] = v[kS + m];
+  }
+  return lmax;
+}
+
+template <int kR, int kH, int kW>
+__global__ __launch_bounds__(kInplaceThreads, 4) void render_inplace_kernel(RenderArgs a) {
+  extern __shared__ __align__(16) int s_img[];
+  __shared__ float s_taps[kR + 1];
+  __shared__ float s_wmax[kInplaceThreads / kWave];
+  __shared__ int s_any;
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int wid = tid / kWave;
+  constexpr int H = kH, W = kW, P = kW + 1;
+  const int words = H * P;
+  if (tid <= kR) s_taps[tid] = a.taps[tid];
+  for (int i = tid; i < words; i += kInplaceThreads) s_img[i] = kEmpty;
+  if (tid == 0) s_any = 0;
+  // lane roles (fixed for the launch): column pass lane (x1, segment s1),
+  // row pass lane (y2, segment s2), output lane (column xo, first row yo)
+  constexpr int ns1 = inplace_segments(W, H), L1 = H / ns1;
+  const int x1 = tid % W, s1 = tid / W;
+  constexpr int ns2 = inplace_segments(H, W), L2 = W / ns2;
+  const int y2 = tid % H, s2 = tid / H;
+  const int xo = tid % W, yo = tid / W;
+  constexpr int ystep = kInplaceThreads / W;
+
+  int nx_p = 0, nx_n = 0;
+  long long nx_off = 0;
+  int pk[kInplacePre];
+  float pw[kInplacePre];
+  auto prefetch = [&](int pi) {
+#pragma unroll
+    for (int k = 0; k < kInplacePre; ++k) {
+      pk[k] = -1;
+      pw[k] = 0.0f;
+    }
+    if (pi >= a.n) return;
+    const int p = a.list ? a.list[pi] : pi;
+    nx_p = p;
+    nx_n = min(a.count[p], a.capv ? a.capv[p] : a.cap);
+    nx_off = a.spot_off ? a.spot_off[p] : (long long)p * a.cap;
+    const float4 *sp = a.spots + nx_off;
+#pragma unroll
+    for (int k = 0; k < kInplacePre; ++k) {
+      const int j = tid + k * kInplaceThreads;
+      if (j < nx_n) {
+        const float4 v = sp[j];
+        float X, Y;
+        if (to_pixel(v, a, X, Y)) {
+          pk[k] = min(static_cast<int>(Y), H - 1) * P + min(static_cast<int>(X), W - 1);
+          pw[k] = v.w;
+        }
+      }
+    }
+  };
+  prefetch(blockIdx.x);
+  lds_sync();
+
+  for (int pi = blockIdx.x; pi < a.n; pi += gridDim.x) {
+    const int p = nx_p;
+    const int n = nx_n;
+    const float4 *spots = a.spots + nx_off;
+    // 1. owner map
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < kInplacePre; ++k) {
+      if (pk[k] >= 0) {
+        any = true;
+        atomicMax(&s_img[pk[k]], tid + k * kInplaceThreads);
+      }
+    }
+    for (int j = tid + kInplacePre * kInplaceThreads; j < n; j += kInplaceThreads) {
+      float X, Y;
+      if (!to_pixel(spots[j], a, X, Y)) continue;
+      any = true;
+      atomicMax(&s_img[min(static_cast<int>(Y), H - 1) * P + min(static_cast<int>(X), W - 1)], j);
+    }
+    if (__ballot(any) != 0 && lane == 0) s_any = 1;
+    lds_sync();
+    // 2. spot image: the owner writes its intensity, sign bit flipped
+#pragma unroll
+    for (int k = 0; k < kInplacePre; ++k)
+      if (pk[k] >= 0 && s_img[pk[k]] == tid + k * kInplaceThreads)
+        s_img[pk[k]] = __float_as_int(pw[k]) ^ kEmpty;
+    for (int j = tid + kInplacePre * kInplaceThreads; j < n; j += kInplaceThreads) {
+      const float4 sp = spots[j];
+      float X, Y;
+      if (!to_pixel(sp, a, X, Y)) continue;
+      const int key = min(static_cast<int>(Y), H - 1) * P + min(static_cast<int>(X), W - 1);
+      if (s_img[key] == j) s_img[key] = __float_as_int(sp.w) ^ kEmpty;
+    }
+    lds_sync();
+    prefetch(pi + gridDim.x);  // overlaps the filter passes below
+    float *out = a.out + (size_t)p * H * W;
+    if (tid == 0 && a.status) a.status[p] = 0;
+    if (s_any == 0) {  // no spot in frame: np.zeros(shape), not normalised
+      for (int i = tid; i < H * W; i += kInplaceThreads) __builtin_nontemporal_store(0.0f, out + i);
+      lds_sync();
+      continue;
+    }
+    // 3. column pass (axis 0)
+    float top[kR], bot[kR];
+    const bool act1 = s1 < ns1;
+    if (act1) {
+      const int r0 = s1 * L1;
+#pragma unroll
+      for (int m = 0; m < kR; ++m) {
+        top[m] = dec_px(s_img[reflect_near(r0 - kR + m, H, kR) * P + x1]);
+        bot[m] = dec_px(s_img[reflect_near(r0 + L1 + m, H, kR) * P + x1]);
+      }
+    }
+    lds_sync();
+    if (act1) inplace_line<kR, true, P, L1>(s_img, x1, s1 * L1, top, bot, s_taps, 0.0f);
+    lds_sync();
+    // 4. row pass (axis 1) and the maximum
+    const bool act2 = s2 < ns2;
+    if (act2) {
+      const int c0 = s2 * L2;
+#pragma unroll
+      for (int m = 0; m < kR; ++m) {
+        top[m] = __int_as_float(s_img[y2 * P + reflect_near(c0 - kR + m, W, kR)]);
+        bot[m] = __int_as_float(s_img[y2 * P + reflect_near(c0 + L2 + m, W, kR)]);
+      }
+    }
+    lds_sync();
+    float lmax = 0.0f;
+    if (act2) lmax = inplace_line<kR, false, 1, L2>(s_img, y2 * P, s2 * L2, top, bot, s_taps, 0.0f);
+    lmax = wave_max(lmax);
+    if (lane == 0) s_wmax[wid] = lmax;
+    lds_sync();
+    float m = 0.0f;
+    for (int w = 0; w < kInplaceThreads / kWave; ++w) m = fmaxf(m, s_wmax[w]);
+    // a zero maximum leaves the zeros as they are instead of producing 0/0
+    const float sc = (a.normalize && m > 0.0f) ? 1.0f / m : 1.0f;
+    // 5. output, and the buffer back to empty
+    if (yo < ystep) {
+      for (int y = yo; y < H; y += ystep) {
+        const int i = y * P + xo;
+        __builtin_nontemporal_store(__int_as_float(s_img[i]) * sc, out + y * W + xo);
+        s_img[i] = kEmpty;
+      }
+    }
+    if (tid == 0) s_any = 0;
+    lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Windowed dense fast-mode renderer for large patterns with many spots (more
 // pixels than the dense kernel holds and more spots than the splat kernel
 // holds).  In-frame spots are binned by their source row strip into a
@@ -1259,6 +1500,45 @@ static int launch_dense_r(const RenderArgs &a, int blocks, size_t lds, hipStream
   return DSX_OK;
 }
 
+template <int kR, int kH, int kW>
+static int launch_inplace_r(const RenderArgs &a, int blocks, size_t lds, hipStream_t st) {
+  static int reserved = -1;  // raised once, so graph-recorded launches issue no attribute calls
+  constexpr int bytes = kH * (kW + 1) * static_cast<int>(sizeof(int));
+  if (reserved < 0) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&render_inplace_kernel<kR, kH, kW>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("dsx_render_fast_ragged: cannot reserve %d bytes of LDS", bytes);
+      return DSX_ELAUNCH;
+    }
+    reserved = 1;
+  }
+  hipLaunchKernelGGL((render_inplace_kernel<kR, kH, kW>), dim3(blocks), dim3(kInplaceThreads), lds, st, a);
+  return DSX_OK;
+}
+
+static int launch_inplace(const RenderArgs &a, int blocks_per_cu, hipStream_t st, const char *name) {
+  if (!(inplace_fits(a.R, a.H, a.W) && a.H == 128 && a.W == 128)) {
+    set_error("%s: the in-place dense renderer takes 128x128 patterns with tap radius 1..5 "
+              "(got %dx%d, radius %d)", name, a.H, a.W, a.R);
+    return DSX_EINVAL;
+  }
+  const size_t lds = static_cast<size_t>(a.H) * (a.W + 1) * sizeof(int);
+  const int per_cu = std::max(1, std::min(blocks_per_cu > 0 ? blocks_per_cu : 2,
+                                          static_cast<int>((160 * 1024) / (lds + 1024))));
+  const int blocks = std::min(a.n, cu_count() * per_cu);
+  int st_ = DSX_OK;
+  switch (a.R) {
+    case 1: st_ = launch_inplace_r<1, 128, 128>(a, blocks, lds, st); break;
+    case 2: st_ = launch_inplace_r<2, 128, 128>(a, blocks, lds, st); break;
+    case 3: st_ = launch_inplace_r<3, 128, 128>(a, blocks, lds, st); break;
+    case 4: st_ = launch_inplace_r<4, 128, 128>(a, blocks, lds, st); break;
+    default: st_ = launch_inplace_r<5, 128, 128>(a, blocks, lds, st);
+  }
+  if (st_ != DSX_OK) return st_;
+  return check_launch(name);
+}
+
 static int launch_dense(const RenderArgs &a, int blocks_per_cu, hipStream_t st, const char *name) {
   const int HW = a.H * a.W;
   if (HW > kDenseMaxPix) {
@@ -1331,7 +1611,8 @@ extern "C" int dsx_render_fast_ragged(const float *spot_d, const long long *spot
   const char *name = "dsx_render_fast_ragged";
   clear_error();
   if (!params || n < 0 || blocks_per_cu < 0 ||
-      (method != DSX_RENDER_SPLAT && method != DSX_RENDER_DENSE)) {
+      (method != DSX_RENDER_SPLAT && method != DSX_RENDER_DENSE &&
+       method != DSX_RENDER_DENSE_INPLACE)) {
     set_error("%s: bad arguments (n=%d method=%d)", name, n, method);
     return DSX_EINVAL;
   }
@@ -1354,11 +1635,17 @@ extern "C" int dsx_render_fast_ragged(const float *spot_d, const long long *spot
   a.list = list_d;
   a.out = out_d;
   a.status = status_d;
+  if (method == DSX_RENDER_DENSE_INPLACE)
+    return launch_inplace(a, blocks_per_cu, as_stream(stream), name);
   return method == DSX_RENDER_DENSE ? launch_dense(a, blocks_per_cu, as_stream(stream), name)
                                     : launch_splat(kModeFast, a, bpc, as_stream(stream), name);
 }
 
 extern "C" int dsx_render_dense_max_pixels(void) { return kDenseMaxPix; }
+
+extern "C" int dsx_render_dense_inplace_fits(int height, int width, int radius) {
+  return inplace_fits(radius, height, width) && height == 128 && width == 128 ? 1 : 0;
+}
 
 extern "C" long long dsx_render_fast_large_workspace_ints(int cap) {
   return cap < 1 ? 0 : static_cast<long long>(cu_count()) * cap;

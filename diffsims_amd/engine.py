@@ -410,9 +410,11 @@ def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigm
             lst = torch.arange(n, dtype=torch.int32, device=dev)
             off = torch.arange(n, dtype=torch.int64, device=dev) * cap
             capv = torch.full((n,), cap, dtype=torch.int32, device=dev)
+            method = (_lib.RENDER_DENSE_INPLACE if L.dsx_render_dense_inplace_fits(H, W, p.radius)
+                      else _lib.RENDER_DENSE)
             _lib.check(L.dsx_render_fast_ragged(
                 _lib.ptr(spots), _lib.ptr(off), _lib.ptr(capv), _lib.ptr(count), _lib.ptr(lst), n,
-                ctypes.byref(p), _lib.RENDER_DENSE, 0, _lib.ptr(out), _lib.ptr(status), stream),
+                ctypes.byref(p), method, 0, _lib.ptr(out), _lib.ptr(status), stream),
                 "dsx_render_fast_ragged")
         else:
             work = _workspace(dev, int(L.dsx_render_fast_large_workspace_ints(int(cap))), stream)

@@ -5,12 +5,15 @@ diffsims.generators.rotation_list_generators.get_beam_directions_grid
 (diffsims/generators/rotation_list_generators.py:176-267): a sphere mesh
 (diffsims/generators/sphere_mesh_generators.py), cropped to the crystal
 system's stereographic triangle, as Bunge angles in degrees with phi1 = 0.
-The mesh, the crop and the angles are computed by ``dsx_beam_directions`` in
-the reference's row order; ``beam_directions_grid_device`` keeps the result on
-the GPU (for ``Rotation.from_euler`` / the spot kernel without a host copy).
-The icosahedral and random meshes are not built on the device (the first
-refines until a nearest-neighbour angle test passes, the second draws from
-numpy's generator); they raise NotImplementedError here.
+The mesh, the crop and the angles are computed on the device in the
+reference's row order (``dsx_beam_directions`` for the UV sphere and the cube
+meshes, ``dsx_beam_directions_icosahedral`` for the icosahedral mesh, whose
+refinement level comes from ``dsx_icosahedral_nn_dots``);
+``beam_directions_grid_device`` keeps the result on the GPU (for
+``Rotation.from_euler`` / the spot kernel without a host copy).  The random
+mesh draws its normal deviates from numpy's generator on the host, as the
+reference does (the draws are the definition of that mesh), and is cropped
+and converted on the device (``dsx_beam_directions_from_vertices``).
 """
 from __future__ import annotations
 
@@ -63,6 +66,145 @@ def crop_planes(crystal_system: str):
     return rows
 
 
+# icosahedron faces of get_icosahedral_mesh_vertices (sphere_mesh_generators.py:413-434)
+ICOSA_FACES = ((0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+               (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+               (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1))
+
+
+def icosahedral_edge_order():
+    """The icosahedron's 30 edges as sorted corner pairs in the order the
+    reference refines them: it collects them in a Python set, face by face
+    (sphere_mesh_generators.py:224-232), and iterates that set, so the order
+    is CPython's set order for these tuples (fixed: ints hash to themselves
+    and tuple hashing is not randomised).  The same construction gives the
+    same order here."""
+    edges = set()
+    for f in ICOSA_FACES:
+        for p, q in ((f[0], f[1]), (f[1], f[2]), (f[2], f[0])):
+            edges.add(tuple(sorted((p, q))))
+    return list(edges)
+
+
+def _edge_array():
+    flat = [int(v) for e in icosahedral_edge_order() for v in e]
+    return (ctypes.c_int * 60)(*flat)
+
+
+def icosahedral_level(resolution: float, device=None) -> int:
+    """Refinement level of get_icosahedral_mesh_vertices(resolution)
+    (sphere_mesh_generators.py:432-438): the first level >= 2 whose largest
+    nearest-neighbour angle (degrees(arccos) of dsx_icosahedral_nn_dots) is
+    <= resolution.  Levels are evaluated on the device in batches.  At a
+    resolution the bare icosahedron (63.43 deg) already meets, the reference's
+    refinement loop never runs and it fails; ValueError here."""
+    import torch
+
+    from . import engine
+
+    L = _lib.load(require_gpu=True)
+    dev = engine._device(device)
+    res = float(resolution)
+    if not res > 0:
+        raise ValueError(f"bad resolution {resolution}")
+    edges = _edge_array()
+    stream = _lib.stream_handle(dev)
+    first, last = 1, min(max(8, int(math.ceil(1.3 * 63.5 / res)) + 2), 16384)
+    while True:
+        dots = torch.empty(last - first + 1, dtype=torch.float64, device=dev)
+        _lib.check(L.dsx_icosahedral_nn_dots(first, last, edges, _lib.ptr(dots), stream),
+                   "dsx_icosahedral_nn_dots")
+        ang = np.rad2deg(np.arccos(dots.cpu().numpy()))
+        if first == 1 and not ang[0] > res:
+            raise ValueError(f"resolution {resolution} is not finer than the icosahedron's own "
+                             f"{ang[0]:.4f} degrees; the reference's refinement loop never runs")
+        for i, a in enumerate(ang):
+            if first + i >= 2 and not a > res:
+                return first + i
+        if last >= 16384:
+            raise ValueError(f"resolution {resolution} needs more than 16384 refinement levels")
+        first, last = last + 1, min(2 * last, 16384)
+
+
+def _grid_out(dev, n_cand: int):
+    import torch
+
+    L = _lib.load()
+    work = torch.empty(max(1, int(L.dsx_grid_workspace_ints(int(n_cand)))), dtype=torch.int32,
+                       device=dev)
+    out = torch.empty((max(n_cand, 0), 3), dtype=torch.float64, device=dev)
+    count = torch.zeros(1, dtype=torch.int64, device=dev)
+    return work, out, count
+
+
+def _planes_arg(planes):
+    flat = [int(v) for row in planes for v in row]
+    return (ctypes.c_int * max(1, len(flat)))(*flat)
+
+
+def icosahedral_mesh_device(resolution: float, device=None):
+    """get_icosahedral_mesh_vertices (sphere_mesh_generators.py:376-449) as a
+    float64 torch tensor [10 n^2 + 2, 3] of unit vectors on the device."""
+    from . import engine
+
+    L = _lib.load(require_gpu=True)
+    dev = engine._device(device)
+    level = icosahedral_level(resolution, dev)
+    n = int(L.dsx_icosahedral_vertices(level))
+    work, out, count = _grid_out(dev, n)
+    _lib.check(L.dsx_beam_directions_icosahedral(level, _edge_array(), 0, _planes_arg([]), 1,
+                                                 _lib.ptr(out), n, _lib.ptr(count),
+                                                 _lib.ptr(work), work.numel(),
+                                                 _lib.stream_handle(dev)),
+               "dsx_beam_directions_icosahedral")
+    return out
+
+
+def get_icosahedral_mesh_vertices(resolution: float) -> np.ndarray:
+    """Drop-in for the reference function (device mesh copied to the host)."""
+    return icosahedral_mesh_device(resolution).cpu().numpy()
+
+
+def get_random_sphere_vertices(resolution: float, seed=None) -> np.ndarray:
+    """get_random_sphere_vertices (sphere_mesh_generators.py:452-483):
+    int((360/resolution)^2 / (4 pi)) points from numpy's default_rng(seed)
+    standard normal triples, each divided by its norm.  The draws are numpy's
+    generator by definition (a seeded call must give the reference's points),
+    so this mesh is drawn on the host."""
+    number = int(1 / (4 * np.pi) * (360 / resolution) ** 2)
+    rng = np.random.default_rng(seed=seed) if seed is not None else np.random.default_rng()
+    xyz = rng.normal(size=(number, 3))
+    return (xyz.T / np.linalg.norm(xyz, axis=1)).T
+
+
+def directions_to_euler_device(vectors, planes=(), device=None):
+    """Rows of ``vectors`` (n, 3) kept by the crop ``planes`` (crop_planes
+    rows) as Bunge angles in degrees, on the device (torch float64 [m, 3]);
+    with no planes this is beam_directions_grid_to_euler
+    (sphere_mesh_generators.py:486-526)."""
+    import torch
+
+    from . import engine
+
+    L = _lib.load(require_gpu=True)
+    dev = engine._device(device)
+    v = torch.as_tensor(np.ascontiguousarray(np.asarray(vectors, dtype=np.float64).reshape(-1, 3)))
+    v = v.to(dev)
+    n = int(v.shape[0])
+    work, out, count = _grid_out(dev, n)
+    _lib.check(L.dsx_beam_directions_from_vertices(_lib.ptr(v), n, len(planes), _planes_arg(planes),
+                                                   _lib.ptr(out), n, _lib.ptr(count),
+                                                   _lib.ptr(work), work.numel(),
+                                                   _lib.stream_handle(dev)),
+               "dsx_beam_directions_from_vertices")
+    return out[:int(count.item())]
+
+
+def beam_directions_grid_to_euler(vectors) -> np.ndarray:
+    """Drop-in for the reference function (sphere_mesh_generators.py:486-526)."""
+    return directions_to_euler_device(vectors).cpu().numpy()
+
+
 def beam_directions_grid_device(crystal_system: str, resolution: float,
                                 mesh: str = "spherified_cube_edge", device=None):
     """The grid as a float64 torch tensor [N, 3] (degrees) on the device."""
@@ -70,13 +212,24 @@ def beam_directions_grid_device(crystal_system: str, resolution: float,
 
     from . import engine
 
-    if mesh in ("icosahedral", "random"):
-        raise NotImplementedError(f"the {mesh} mesh is not generated on the device")
-    if mesh not in MESHES:
+    if mesh not in MESHES and mesh not in ("icosahedral", "random"):
         raise NotImplementedError(
             f"The mesh {mesh} is not recognized. Please use: uv_sphere, normalized_cube, "
             f"spherified_cube_edge, spherified_cube_corner, icosahedral, random")
     planes = crop_planes(crystal_system)  # KeyError for an unknown system, as the reference
+    if mesh == "random":
+        return directions_to_euler_device(get_random_sphere_vertices(resolution), planes, device)
+    if mesh == "icosahedral":
+        L = _lib.load(require_gpu=True)
+        dev = engine._device(device)
+        level = icosahedral_level(resolution, dev)
+        n = int(L.dsx_icosahedral_vertices(level))
+        work, out, count = _grid_out(dev, n)
+        _lib.check(L.dsx_beam_directions_icosahedral(
+            level, _edge_array(), len(planes), _planes_arg(planes), 0, _lib.ptr(out), n,
+            _lib.ptr(count), _lib.ptr(work), work.numel(), _lib.stream_handle(dev)),
+            "dsx_beam_directions_icosahedral")
+        return out[:int(count.item())]
     res = float(resolution)
     if mesh in ("normalized_cube", "spherified_cube_edge") and crystal_system == "hexagonal":
         res = res / np.sqrt(2)  # rotation_list_generators.py:212-215

@@ -201,15 +201,24 @@ class RaggedLibrary:
             if H * W > int(self.L.dsx_render_dense_max_pixels()):
                 bad = int(np.nonzero(st)[0][0])
                 raise RuntimeError(f"pattern {bad} overflowed the splat renderer")
+            dense = self._dense_method()
             for s in self.slots:
                 if st[s.first:s.first + s.n].any():
-                    s.method = _lib.RENDER_DENSE
-            self.render_bpc.setdefault(_lib.RENDER_DENSE, 1)
+                    s.method = dense
+            self.render_bpc.setdefault(dense, 2 if dense == _lib.RENDER_DENSE_INPLACE else 1)
             self._lists()
             self.graph = None
             self.status.zero_()
             self.run()
             self.check()
+
+    def _dense_method(self) -> int:
+        """The dense renderer for this shape: the in-place kernel where it
+        applies (same bits, two patterns per CU), else the two-buffer one."""
+        H, W = self.shape
+        return (_lib.RENDER_DENSE_INPLACE
+                if self.L.dsx_render_dense_inplace_fits(H, W, self.render.radius)
+                else _lib.RENDER_DENSE)
 
     # -- hipGraph ---------------------------------------------------------------
 
@@ -254,6 +263,8 @@ class RaggedLibrary:
             cands = [(_lib.RENDER_SPLAT, b) for b in (1, 2, 3)]
             if H * W <= max_pix:
                 cands += [(_lib.RENDER_DENSE, 1)]
+            if self.L.dsx_render_dense_inplace_fits(H, W, self.render.radius):
+                cands += [(_lib.RENDER_DENSE_INPLACE, b) for b in (1, 2)]
 
             def feasible(c, lst=lst):
                 self.status.zero_()

@@ -300,13 +300,20 @@ int dsx_render_subpixel(const float *spot_d, const int *count_d, int cap, int n,
  *               dsx_render_max_spots() distinct in-frame spots per pattern);
  *               DSX_RENDER_DENSE: whole pattern in LDS, owner map + separable
  *               filter, any number of spots, at most
- *               dsx_render_dense_max_pixels() pixels
+ *               dsx_render_dense_max_pixels() pixels;
+ *               DSX_RENDER_DENSE_INPLACE: the same in one buffer (below)
  *   blocks_per_cu  launch width (0 = default)
  * Same results as dsx_render_fast.
  */
 #define DSX_RENDER_SPLAT 0
 #define DSX_RENDER_DENSE 1
+#define DSX_RENDER_DENSE_INPLACE 2
 int dsx_render_dense_max_pixels(void);
+/* DSX_RENDER_DENSE_INPLACE: as DSX_RENDER_DENSE (the same bits) from one LDS
+ * buffer per pattern (owner map, spot image and both filter passes in place),
+ * two patterns per CU; 1 when it takes this shape and tap radius
+ * (128 x 128, radius 1..5), else 0. */
+int dsx_render_dense_inplace_fits(int height, int width, int radius);
 int dsx_render_fast_ragged(const float *spot_d, const long long *spot_off_d, const int *cap_d,
                            const int *count_d, const int *list_d, int n,
                            const dsx_render_params *params, int method, int blocks_per_cu,
@@ -404,6 +411,61 @@ long long dsx_beam_directions_workspace_ints(int mesh, double resolution);
 int dsx_beam_directions(int mesh, double resolution, int n_planes, const int *planes,
                         double *euler_d, long long capacity, long long *count_d, int *work_d,
                         long long work_ints, void *stream);
+
+/*
+ * Workspace of the grid entry points below for n_vertices mesh vertices
+ * (candidates), in ints; -1 for a negative count.
+ */
+long long dsx_grid_workspace_ints(long long n_vertices);
+
+/*
+ * Icosahedral mesh — replaces get_icosahedral_mesh_vertices
+ * (diffsims/generators/sphere_mesh_generators.py:376-449) with
+ * _compose_from_faces (:200-325) and _get_max_grid_angle (:328-372).
+ *
+ * The mesh at refinement level n >= 1 has dsx_icosahedral_vertices(n) =
+ * 10 n^2 + 2 vertices: the 12 corners of the reference's icosahedron, then
+ * for each of its 30 edges in edge_order (60 host ints: sorted corner pairs
+ * {p, q}, p < q, in the order the reference's edge set iterates) the n-1
+ * points (1-t) c_p + t c_q, t = linspace(1/n, 1, n-1, endpoint=False), then
+ * for each face in the reference's face order the interior points
+ * f0 (1 - i/n - j/n) + f1 j/n + f2 i/n for i = 1..n-2, j = 1..n-i-1.
+ *
+ * dsx_icosahedral_nn_dots writes, for every level L in [level_first,
+ * level_last], min over vertices v of max over the other vertices w of
+ * u(v).u(w), u = row / |row| (the cosine of the largest nearest-neighbour
+ * angle), to min_dot_d[L - level_first] (double).  The nearest neighbour is
+ * searched among the vertex's triangulation neighbours (projection distorts
+ * the triangles by at most 1.26, a vertex two steps away is >= sqrt(3)
+ * steps off).  The reference refines level 1, 2, ... until
+ * degrees(arccos(min_dot)) <= resolution; the caller does the same.
+ *
+ * dsx_beam_directions_icosahedral writes the level's vertices divided by
+ * their norms, cropped as dsx_beam_directions crops (n_planes, planes), in
+ * vertex order: Bunge angles in degrees {0, Phi, phi2} per kept vertex, or,
+ * with xyz_out != 0, the unit vertex itself (out_d double [capacity][3];
+ * count_d int64 [1]; work_d dsx_grid_workspace_ints(vertices) ints).
+ */
+long long dsx_icosahedral_vertices(int level);
+int dsx_icosahedral_nn_dots(int level_first, int level_last, const int *edge_order,
+                            double *min_dot_d, void *stream);
+int dsx_beam_directions_icosahedral(int level, const int *edge_order, int n_planes,
+                                    const int *planes, int xyz_out, double *out_d,
+                                    long long capacity, long long *count_d, int *work_d,
+                                    long long work_ints, void *stream);
+
+/*
+ * Crop and Euler conversion of given unit vectors (double [n][3]): the part of
+ * get_beam_directions_grid (rotation_list_generators.py:233-267) after the
+ * mesh, with beam_directions_grid_to_euler (sphere_mesh_generators.py:486-526)
+ * — for the random mesh (get_random_sphere_vertices, :452-483, whose normal
+ * draws are numpy's generator) and for beam_directions_grid_to_euler itself
+ * (n_planes = 0).  Arguments as for dsx_beam_directions.
+ */
+int dsx_beam_directions_from_vertices(const double *xyz_d, long long n, int n_planes,
+                                      const int *planes, double *euler_d, long long capacity,
+                                      long long *count_d, int *work_d, long long work_ints,
+                                      void *stream);
 
 /*
  * Polar templates from spot lists — replaces Simulation2D.polar_flatten_simulations

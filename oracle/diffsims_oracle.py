@@ -41,6 +41,13 @@ Parity pins (what fixes each function):
                              (test_sphere_mesh_generators.py:48-78), grid sizes per
                              crystal system (test_rotation_list_generator.py:80-94),
                              vector -> Euler KAT (test_sphere_mesh_generators.py:124-143)
+  icosahedral mesh           642 unique vertices summing to 0 at 10 deg
+                             (test_sphere_mesh_generators.py:115-121); vertex count
+                             10 n^2 + 2; corners and edge points on the flat
+                             icosahedron; nearest-neighbour angle of the bare
+                             icosahedron = arctan(2) (63.435 deg); level n+1 finer
+  random mesh                10313 points at 1 deg, seeds 7/7/8
+                             (test_sphere_mesh_generators.py:33-45)
 Parity unpinned (no reference test and no closed form fixes it):
   * the order of the Cartesian sort for oblique cells when rounding could tie
     (exact arithmetic gives descending lexicographic (h,k,l); see hkl_box);
@@ -843,6 +850,112 @@ def cube_mesh_vertices(resolution: float, grid_type: str = "spherified_corner") 
     return v / np.linalg.norm(v, axis=1)[:, None]
 
 
+# icosahedron of get_icosahedral_mesh_vertices (sphere_mesh_generators.py:394-431)
+_GOLDEN = (1.0 + np.sqrt(5.0)) / 2.0
+ICOSA_CORNERS = np.array(
+    [[-1, _GOLDEN, 0], [1, _GOLDEN, 0], [-1, -_GOLDEN, 0], [1, -_GOLDEN, 0],
+     [0, -1, _GOLDEN], [0, 1, _GOLDEN], [0, -1, -_GOLDEN], [0, 1, -_GOLDEN],
+     [_GOLDEN, 0, -1], [_GOLDEN, 0, 1], [-_GOLDEN, 0, -1], [-_GOLDEN, 0, 1]])
+ICOSA_FACES = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+               (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+               (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+
+
+def icosa_edge_order(faces=ICOSA_FACES):
+    """The 30 edges as sorted corner pairs, in the order the reference visits
+    them: the iteration order of a Python set filled face by face with the
+    pairs (f0,f1), (f1,f2), (f2,f0) (_compose_from_faces,
+    sphere_mesh_generators.py:224-232).  CPython's set order for tuples of
+    small ints is fixed by their hashes (no hash randomisation for ints)."""
+    edges = set()
+    for f in faces:
+        for p, q in ((f[0], f[1]), (f[1], f[2]), (f[2], f[0])):
+            edges.add((min(p, q), max(p, q)))
+    return list(edges)
+
+
+def icosa_refined_vertices(n: int) -> np.ndarray:
+    """Vertices of the icosahedron with every face split into n^2 triangles
+    (_compose_from_faces, sphere_mesh_generators.py:200-325; only the vertex
+    array is returned there): the 12 corners; then per edge in
+    icosa_edge_order(), the n-1 points (1-t) c0 + t c1 with
+    t = linspace(1/n, 1, n-1, endpoint=False); then per face (f0, f1, f2) the
+    interior points for i = 1..n-1 (outer), j = 1..n-i-1 (inner) with
+    barycentric weights (1 - i/n - j/n, j/n, i/n) on (f0, f1, f2), combined
+    by a matrix product as the reference's np.dot."""
+    C = ICOSA_CORNERS.astype(np.float64)
+    parts = [C]
+    t = np.linspace(1 / n, 1.0, n - 1, endpoint=False)
+    for i0, i1 in icosa_edge_order():
+        parts.append(np.outer(1 - t, C[i0]) + np.outer(t, C[i1]))
+    if n > 1:
+        ii = [i for i in range(1, n) for _ in range(n - i - 1)]
+        jj = [j for i in range(1, n) for j in range(1, n - i)]
+        b = np.array([ii, jj], dtype=np.float64) / n
+        bary = np.array([1.0 - b[0] - b[1], b[1], b[0]])
+        for f in ICOSA_FACES:
+            corner_verts = np.array([C[k] for k in f])
+            parts.append(np.dot(corner_verts.T, bary).T)
+    return np.concatenate(parts)
+
+
+def max_nn_angle(v: np.ndarray) -> float:
+    """Largest angle (degrees) between a mesh point and its nearest neighbour
+    (_get_max_grid_angle / _get_angles_between_nn_gridpoints /
+    _get_first_nearest_neighbors, sphere_mesh_generators.py:328-372): rows
+    normalised, nearest neighbour by a k-d tree (k = 2, leaf size 50)."""
+    from scipy.spatial import cKDTree
+
+    u = (v.T / np.linalg.norm(v, axis=1)).T
+    nn = cKDTree(u, 50).query(u, k=2)[1][:, 1]
+    return float(np.max(np.rad2deg(np.arccos(np.sum(u * u[nn], axis=1)))))
+
+
+def icosahedral_level(resolution: float):
+    """Refinement level n the reference settles on: the corners' angle, then
+    levels 1, 2, ... until the largest nearest-neighbour angle is <= the
+    resolution (get_icosahedral_mesh_vertices, :432-438).  Returns n and the
+    angle of every level visited ([level, angle] rows, the corners first as
+    level 0).  A resolution at or above the corners' angle leaves the
+    reference without vertices (its loop never runs): ValueError here."""
+    angle = max_nn_angle(ICOSA_CORNERS.astype(np.float64))
+    seen = [(0, angle)]
+    n = 1
+    if not angle > resolution:
+        raise ValueError("resolution too coarse for the icosahedral mesh")
+    while angle > resolution:
+        angle = max_nn_angle(icosa_refined_vertices(n))
+        seen.append((n, angle))
+        n += 1
+    return n - 1, seen
+
+
+def icosahedral_vertices_at_level(n: int) -> np.ndarray:
+    """The refined icosahedron of level n with rows divided by the square root
+    of the row-wise dot product (einsum), as get_icosahedral_mesh_vertices
+    returns it (sphere_mesh_generators.py:446-449)."""
+    v = icosa_refined_vertices(n)
+    norms = np.sqrt(np.einsum("ij,ij->i", v, v))
+    return (v.T / norms.T).T
+
+
+def icosahedral_mesh_vertices(resolution: float) -> np.ndarray:
+    """get_icosahedral_mesh_vertices (sphere_mesh_generators.py:376-449) at
+    the level icosahedral_level(resolution) settles on."""
+    n, _ = icosahedral_level(resolution)
+    return icosahedral_vertices_at_level(n)
+
+
+def random_sphere_vertices(resolution: float, seed=None) -> np.ndarray:
+    """get_random_sphere_vertices (sphere_mesh_generators.py:452-483):
+    int((360/res)^2 / (4 pi)) standard-normal triples of
+    numpy.random.default_rng(seed), normalised."""
+    number = int(1 / (4 * np.pi) * (360 / resolution) ** 2)
+    rng = np.random.default_rng(seed=seed) if seed is not None else np.random.default_rng()
+    xyz = rng.normal(size=(number, 3))
+    return (xyz.T / np.linalg.norm(xyz, axis=1)).T
+
+
 def directions_to_euler(v: np.ndarray) -> np.ndarray:
     """beam_directions_grid_to_euler (sphere_mesh_generators.py:486-526): Bunge
     angles in degrees with phi1 = 0, Phi = acos(z/|v|), phi2 = 90 deg minus
@@ -878,14 +991,21 @@ def uvtw_to_uvw(d):
 
 
 def beam_directions_grid(crystal_system: str, resolution: float,
-                         mesh: str = "spherified_cube_edge") -> np.ndarray:
+                         mesh: str = "spherified_cube_edge", vertices=None) -> np.ndarray:
     """get_beam_directions_grid (diffsims/generators/rotation_list_generators.py:
-    176-267) for the deterministic meshes: the mesh, then the crop to the
-    crystal system's stereographic triangle (each side: the sign of the
-    triangle's opposite corner, tolerance -1e-13), then directions_to_euler.
-    The monoclinic branch keeps only its second test (x >= -1e-13): its first
-    result is overwritten (rotation_list_generators.py:236-241)."""
-    if mesh == "uv_sphere":
+    176-267): the mesh, then the crop to the crystal system's stereographic
+    triangle (each side: the sign of the triangle's opposite corner,
+    tolerance -1e-13), then directions_to_euler.  The monoclinic branch keeps
+    only its second test (x >= -1e-13): its first result is overwritten
+    (rotation_list_generators.py:236-241).  mesh="random" takes its points
+    from ``vertices`` when given (the reference draws them unseeded)."""
+    if vertices is not None:
+        v = np.asarray(vertices, dtype=np.float64)
+    elif mesh == "random":
+        v = random_sphere_vertices(resolution)
+    elif mesh == "icosahedral":
+        v = icosahedral_mesh_vertices(resolution)
+    elif mesh == "uv_sphere":
         v = uv_sphere_vertices(resolution)
     elif mesh == "spherified_cube_corner":
         v = cube_mesh_vertices(resolution, "spherified_corner")

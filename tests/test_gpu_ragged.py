@@ -33,7 +33,7 @@ def _phases():
     return specs
 
 
-def _library(shape=(128, 128), cal=0.03, n_orient=(37, 5, 64, 23, 41, 30), radii=None):
+def _library(shape=(128, 128), cal=0.03, n_orient=(37, 5, 64, 23, 41, 30), radii=None, sigma=1.0):
     from diffsims_amd import engine
     from diffsims_amd.ragged import RaggedLibrary
 
@@ -43,7 +43,7 @@ def _library(shape=(128, 128), cal=0.03, n_orient=(37, 5, 64, 23, 41, 30), radii
     st = engine.SimSettings(accelerating_voltage=200, max_excitation_error=0.02,
                             shape_model="lorentzian", minimum_intensity=1e-20)
     lib = RaggedLibrary([P.to_arrays(s, f"p{i}") for i, s in enumerate(specs)], rots, radii, st,
-                        shape, 1.0, cal)
+                        shape, sigma, cal)
     lib.run()
     lib.check()
     torch.cuda.synchronize()
@@ -100,7 +100,7 @@ def test_dense_render_matches_oracle():
             assert err <= P.TOL_P, (i, o, err)
             checked += 1
     assert checked >= 20
-    assert any(s.method == _lib.RENDER_DENSE for s in lib.slots)
+    assert any(s.method in (_lib.RENDER_DENSE, _lib.RENDER_DENSE_INPLACE) for s in lib.slots)
 
 
 def test_dense_kernel_many_spots():
@@ -134,6 +134,31 @@ def test_dense_kernel_many_spots():
                                          lib.calibration)
         assert P.pattern_error(pats[o], rp) <= P.TOL_P, o
     assert stats["carved_orientations"] <= 2, stats
+
+
+@pytest.mark.parametrize("sigma", [0.3, 0.5, 0.7, 1.0, 1.2])
+def test_inplace_dense_equals_two_buffer_dense(sigma):
+    """The in-place dense kernel (one LDS buffer, two patterns per CU) gives
+    the two-buffer dense kernel's bits on every pattern, tap radii 1..5."""
+    from diffsims_amd import _lib
+
+    print(f"in-place dense vs two-buffer dense, sigma {sigma}", flush=True)
+    lib, *_ = _library(cal=0.03, sigma=sigma)
+    assert lib.L.dsx_render_dense_inplace_fits(128, 128, lib.render.radius) == 1
+    outs = []
+    for m in (_lib.RENDER_DENSE, _lib.RENDER_DENSE_INPLACE):
+        for s in lib.slots:
+            s.method = m
+        lib.render_bpc[m] = 2 if m == _lib.RENDER_DENSE_INPLACE else 1
+        lib._lists()
+        lib.out.fill_(-7.0)
+        lib.status.fill_(5)
+        lib.run()
+        torch.cuda.synchronize()
+        assert int(lib.status.abs().max()) == 0
+        outs.append(lib.out.clone())
+    assert int(lib.count.max()) > 1024
+    assert torch.equal(outs[0], outs[1])
 
 
 def test_graph_replay_matches_eager():

@@ -595,3 +595,103 @@ def test_beam_directions_euler_is_the_direction():
     assert np.allclose(np.linalg.norm(d, axis=1), 1.0)
     assert np.all(d[:, 2] >= d[:, 0] - 1e-12) and np.all(d[:, 0] >= d[:, 1] - 1e-12)
     assert np.all(d[:, 1] >= -1e-12)
+
+
+# ---------------------------------------------------------------------------
+# icosahedral and random sphere meshes
+# ---------------------------------------------------------------------------
+
+def test_icosahedral_mesh_kat():
+    """test_sphere_mesh_generators.py:115-121: 642 unique unit vertices summing
+    to zero at 10 degrees (level 8 = 10 n^2 + 2)."""
+    g = O.icosahedral_mesh_vertices(10)
+    assert g.shape == (642, 3)
+    np.testing.assert_almost_equal(np.sum(g), 0)
+    assert np.unique(g, axis=0).shape[0] == 642
+    assert np.allclose(np.linalg.norm(g, axis=1), 1.0, atol=1e-15)
+    n, seen = O.icosahedral_level(10)
+    assert n == 8 and seen[-1][1] <= 10 < seen[-2][1]
+
+
+def test_icosahedral_bare_angle_is_arctan2():
+    """Neighbouring corners of an icosahedron subtend arctan(2); the refinement
+    starts there and its first level is the bare icosahedron again."""
+    _, seen = O.icosahedral_level(40)
+    assert abs(seen[0][1] - np.degrees(np.arctan(2.0))) < 1e-12
+    assert seen[1] == (1, seen[0][1])
+    assert [lv for lv, _ in seen] == [0, 1, 2]
+    with pytest.raises(ValueError):
+        O.icosahedral_level(64)
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 7])
+def test_icosahedral_points_are_the_face_lattice(n):
+    """Every flat vertex is f0 + (f2 - f0) a/n + (f1 - f0) b/n for some face and
+    a + b <= n (brute force over all faces), each lattice point appears once,
+    and the count is 10 n^2 + 2."""
+    v = O.icosa_refined_vertices(n)
+    C = O.ICOSA_CORNERS.astype(np.float64)
+    pts = set()
+    for f in O.ICOSA_FACES:
+        for a in range(n + 1):
+            for b in range(n + 1 - a):
+                p = C[f[0]] + (C[f[2]] - C[f[0]]) * a / n + (C[f[1]] - C[f[0]]) * b / n
+                pts.add(tuple(np.round(p, 9)))
+    got = [tuple(np.round(r, 9)) for r in v]
+    assert len(v) == 10 * n * n + 2 == len(pts) == len(set(got))
+    assert set(got) == pts
+
+
+def test_icosahedral_order_kat():
+    """Row order: corners, then edge points in edge order from the lower
+    corner, then face interiors with i (weight on f2) outer and j (weight on f1)
+    inner.  Spelled out for level 4, face 0 = (0, 11, 5)."""
+    n = 4
+    v = O.icosa_refined_vertices(n)
+    C = O.ICOSA_CORNERS.astype(np.float64)
+    assert np.array_equal(v[:12], C)
+    e0 = O.icosa_edge_order()[0]
+    assert np.allclose(v[12], 0.75 * C[e0[0]] + 0.25 * C[e0[1]], atol=1e-15)
+    assert np.allclose(v[14], 0.25 * C[e0[0]] + 0.75 * C[e0[1]], atol=1e-15)
+    first_face = 12 + 30 * (n - 1)
+    rows = [(1, 1), (1, 2), (2, 1)]  # (i, j) for n = 4: 3 interior points per face
+    for k, (i, j) in enumerate(rows):
+        want = C[0] * (1 - i / n - j / n) + C[11] * (j / n) + C[5] * (i / n)
+        assert np.allclose(v[first_face + k], want, atol=1e-15), (i, j)
+    # planted mistake: weights on f1 and f2 swapped moves the second point
+    wrong = C[0] * (1 - 1 / n - 2 / n) + C[11] * (1 / n) + C[5] * (2 / n)
+    assert not np.allclose(v[first_face + 1], wrong, atol=1e-6)
+
+
+def test_icosahedral_edge_order_is_the_set_order():
+    """The 30 edges are the sorted corner pairs of the faces, each once."""
+    e = O.icosa_edge_order()
+    assert len(e) == 30 == len(set(e)) and all(p < q for p, q in e)
+    want = {tuple(sorted(p)) for f in O.ICOSA_FACES for p in ((f[0], f[1]), (f[1], f[2]), (f[2], f[0]))}
+    assert set(e) == want
+
+
+def test_icosahedral_angles_fall_with_level():
+    _, seen = O.icosahedral_level(3)
+    ang = [a for lv, a in seen[1:]]
+    assert all(b < a for a, b in zip(ang, ang[1:]))
+
+
+def test_random_sphere_mesh_kats():
+    """test_sphere_mesh_generators.py:33-45."""
+    g = O.random_sphere_vertices(1)
+    assert g.shape == (10313, 3)
+    g7 = O.random_sphere_vertices(3, seed=7)
+    assert np.allclose(g7, O.random_sphere_vertices(3, seed=7))
+    assert not np.allclose(g7, O.random_sphere_vertices(3, seed=8))
+    assert np.allclose(np.linalg.norm(g7, axis=1), 1.0)
+
+
+@pytest.mark.parametrize("mesh", ["icosahedral", "random"])
+@pytest.mark.parametrize("crystal_system", ["cubic", "hexagonal", "trigonal", "tetragonal",
+                                            "orthorhombic", "monoclinic", "triclinic"])
+def test_beam_directions_grid_other_meshes_run(crystal_system, mesh):
+    """test_rotation_list_generator.py:53-77 for the icosahedral and random
+    meshes: every crystal system at 5 degrees."""
+    g = O.beam_directions_grid(crystal_system, 5, mesh=mesh)
+    assert g.ndim == 2 and g.shape[1] == 3 and g.shape[0] > 0
