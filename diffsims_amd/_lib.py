@@ -73,6 +73,8 @@ EXPORTED = (
     "dsx_render_subpixel",
     "dsx_render_adjoint",
     "dsx_render_adjoint_workspace_ints",
+    "dsx_render_adjoint_noalloc_max_ints",
+    "dsx_render_adjoint_noalloc",
     "dsx_structure_factor_adjoint",
     "dsx_simulate_tile_size",
     "dsx_simulate_ragged",
@@ -179,6 +181,10 @@ _SIGS = {
                                 ctypes.POINTER(RenderParams), _I, _P, ctypes.c_longlong, _P, _P,
                                 _P, _P, _P, _P, ctypes.c_longlong, _P, _P]),
     "dsx_render_adjoint_workspace_ints": (ctypes.c_longlong, [_I, _I]),
+    "dsx_render_adjoint_noalloc_max_ints": (ctypes.c_longlong, []),
+    "dsx_render_adjoint_noalloc": (_I, [_P, _P, _P, _I, _I, _P, _P, _I, _P,
+                                        ctypes.POINTER(SimParams), ctypes.POINTER(RenderParams),
+                                        _I, _P, ctypes.c_longlong, _P, _P, _P, _P, _P, _P, _P]),
     "dsx_structure_factor_adjoint": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _P,
                                           _I, _P, _I, _I, _P, _P, _P, _P, _P]),
     "dsx_simulate_tile_size": (_I, []),

@@ -29,6 +29,7 @@
 //   recomputes F(hkl) and reduces dL/d|F|^2 into dL/dxyz and dL/dB per atom
 //   and into dL/d(rec) through |g|.
 #include <algorithm>
+#include <mutex>
 
 #include "dsx_internal.h"
 #include "dsx_physics.h"
@@ -1405,6 +1406,74 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   if (drec_d)
     hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(9 * kWave), 0, s, a.dpart, cblocks, drec_d);
   return check_launch("dsx_render_adjoint (chain)");
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free form of dsx_render_adjoint: the workspace is a file-scope
+// device array (one copy per GPU, held until the process ends).  Host code
+// never passes the array's name to a kernel (on the host it is only a
+// stand-in); its device address is looked up once per device with
+// hipGetSymbolAddress and kept here.
+// ---------------------------------------------------------------------------
+
+constexpr long long kNoallocScratchInts = 16ll << 20;  // 64 MiB
+static __device__ __align__(256) int scratch[kNoallocScratchInts];
+
+static std::mutex g_scratch_mu;
+static int *g_scratch_ptr[64] = {};
+
+// Device address of this device's scratch (looked up on the first call).
+static int scratch_address(int **out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+    (void)hipGetLastError();
+    set_error("dsx_render_adjoint_noalloc: no current device (or device id >= 64)");
+    return DSX_ELAUNCH;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      set_error("dsx_render_adjoint_noalloc: hipGetSymbolAddress(scratch) failed: %s",
+                hipGetErrorString(e));
+      return DSX_ELAUNCH;
+    }
+    g_scratch_ptr[dev] = static_cast<int *>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return DSX_OK;
+}
+
+extern "C" long long dsx_render_adjoint_noalloc_max_ints(void) { return kNoallocScratchInts; }
+
+extern "C" int dsx_render_adjoint_noalloc(const int *idx_d, const float *spot_d, const int *count_d,
+                                          int cap, int n, const int *hkl_d, const float *F2_d,
+                                          int n_refl, const double *Rt_d, const dsx_sim_params *sim,
+                                          const dsx_render_params *rp, int mode,
+                                          const float *target_d, long long target_stride,
+                                          float *pattern_d, float *loss_d, double *dRt_d,
+                                          float *dF2_d, double *drec_d, int *status_d,
+                                          void *stream) {
+  clear_error();
+  if (cap < 1 || n < 0) {
+    set_error("dsx_render_adjoint_noalloc: bad arguments (n=%d cap=%d)", n, cap);
+    return DSX_EINVAL;
+  }
+  const long long need = dsx_render_adjoint_workspace_ints(cap, n);
+  if (need > kNoallocScratchInts) {
+    set_error("dsx_render_adjoint_noalloc: %d patterns with capacity %d need %lld ints of "
+              "scratch; the static scratch holds %lld (use dsx_render_adjoint with a workspace, "
+              "or fewer patterns per call)", n, cap, need, kNoallocScratchInts);
+    return DSX_EINVAL;
+  }
+  int *work = nullptr;
+  const int rc = scratch_address(&work);
+  if (rc != DSX_OK) return rc;
+  return dsx_render_adjoint(idx_d, spot_d, count_d, cap, n, hkl_d, F2_d, n_refl, Rt_d, sim, rp, mode,
+                            target_d, target_stride, pattern_d, loss_d, dRt_d, dF2_d, drec_d, work,
+                            kNoallocScratchInts, status_d, stream);
 }
 
 extern "C" int dsx_structure_factor_adjoint(const int *hkl_d, int n, const double rec[9],

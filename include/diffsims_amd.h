@@ -366,6 +366,30 @@ int dsx_render_adjoint(const int *idx_d, const float *spot_d, const int *count_d
                        long long work_ints, int *status_d, void *stream);
 
 /*
+ * Allocation-free dsx_render_adjoint.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight refinement loops, harnesses that forbid hipMalloc).  The
+ * arguments and results are dsx_render_adjoint's without work_d / work_ints:
+ * the workspace is a static device array of
+ * dsx_render_adjoint_noalloc_max_ints() ints (64 MiB), one per GPU, held until
+ * the process ends; its address is looked up on the first call on each
+ * device.  The call allocates and frees nothing.  Sizes whose
+ * dsx_render_adjoint_workspace_ints(cap, n) exceeds the maximum are rejected
+ * with DSX_EINVAL.  Every call on one GPU shares that GPU's array, which
+ * keeps the last call's contents (each call overwrites what it reads, so no
+ * clearing is needed): calls that overlap in time on one GPU (other streams
+ * or threads) race and must be serialised by the caller.
+ */
+long long dsx_render_adjoint_noalloc_max_ints(void);
+int dsx_render_adjoint_noalloc(const int *idx_d, const float *spot_d, const int *count_d, int cap,
+                               int n, const int *hkl_d, const float *F2_d, int n_refl,
+                               const double *Rt_d, const dsx_sim_params *sim,
+                               const dsx_render_params *params, int mode, const float *target_d,
+                               long long target_stride, float *pattern_d, float *loss_d,
+                               double *dRt_d, float *dF2_d, double *drec_d, int *status_d,
+                               void *stream);
+
+/*
  * Adjoint of dsx_structure_factor.  No counterpart in the reference.  Given
  * dF2_d = dL/d|F|^2 per reflection (float [n], as accumulated by
  * dsx_render_adjoint), recomputes F(hkl) and ACCUMULATES (zero first):
