@@ -55,6 +55,7 @@ EXPORTED = (
     "dsx_last_error",
     "dsx_hkl_workspace_ints",
     "dsx_hkl_enumerate",
+    "dsx_hkl_points_in_sphere",
     "dsx_structure_factor",
     "dsx_stage_atoms",
     "dsx_hkl_lines_workspace_ints",
@@ -153,6 +154,8 @@ _SIGS = {
     "dsx_hkl_workspace_ints": (ctypes.c_size_t, [ctypes.POINTER(_I)]),
     "dsx_hkl_enumerate": (_I, [ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double),
                                ctypes.c_double, _P, _I, _P, _P, _P]),
+    "dsx_hkl_points_in_sphere": (_I, [ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.c_double, _P, _I, _P, _P, _P]),
     "dsx_structure_factor": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _P, _I,
                                   _P, _I, _I, _P, _P, _P, _P]),
     "dsx_stage_atoms": (_I, [_P, _P, _P, _I, _P, _P, _P]),

@@ -41,10 +41,7 @@
 
 namespace dsx {
 
-#ifndef DSX_RENDER_THREADS
-#define DSX_RENDER_THREADS 512
-#endif
-constexpr int kRenderThreads = DSX_RENDER_THREADS;
+constexpr int kRenderThreads = 512;
 // Launch shapes of the splat renderer: at most 80 VGPRs (six waves per SIMD),
 // so three 512-thread blocks fit a CU when each holds a 28 KB strip; with two
 // blocks per CU the strips are 48 KB (fewer strips, fewer barriers).  Three
@@ -1260,10 +1257,7 @@ __global__ __launch_bounds__(kInplaceThreads, 4) void render_inplace_kernel(Rend
 // ---------------------------------------------------------------------------
 
 constexpr int kLargeThreads = 1024;
-#ifndef DSX_LARGE_WINDOW_KB
-#define DSX_LARGE_WINDOW_KB 128
-#endif
-constexpr int kLargeWindowBytes = DSX_LARGE_WINDOW_KB * 1024;
+constexpr int kLargeWindowBytes = 128 * 1024;
 
 struct LargeArgs {
   RenderArgs r;

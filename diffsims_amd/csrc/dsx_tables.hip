@@ -27,7 +27,9 @@ constexpr int kEnumThreads = 256;
 constexpr int kEnumItems = 8;  // consecutive box entries per thread
 constexpr int kEnumPerBlock = kEnumThreads * kEnumItems;
 
-// Box entry i (row-major over descending h, k, l).
+// Box entry i (row-major over descending h, k, l; ascending with kAscend,
+// the order of itertools.product over ascending aranges).
+template <bool kAscend = false>
 __device__ __forceinline__ void hkl_index_to_hkl(long long i, const Box &b, int &h, int &k,
                                                  int &l) {
   const long long plane = static_cast<long long>(b.nk) * b.nl;
@@ -35,9 +37,9 @@ __device__ __forceinline__ void hkl_index_to_hkl(long long i, const Box &b, int 
   const int rem = static_cast<int>(i - static_cast<long long>(ih) * plane);
   const int ik = rem / b.nl;
   const int il = rem - ik * b.nl;
-  h = b.H - ih;
-  k = b.K - ik;
-  l = b.L - il;
+  h = kAscend ? ih - b.H : b.H - ih;
+  k = kAscend ? ik - b.K : b.K - ik;
+  l = kAscend ? il - b.L : b.L - il;
 }
 
 // d-spacing filter in IEEE FP64 with no contraction, in the same operation
@@ -57,8 +59,25 @@ __device__ __forceinline__ bool keep_hkl(int h, int k, int l, const Metric &m, d
   return d >= dmin;
 }
 
+// Sphere test of the legacy generator's get_points_in_sphere
+// (diffsims/utils/sim_utils.py:436-474): |g| < radius from the same FP64
+// quadratic form, (000) kept.
+__device__ __forceinline__ bool keep_sphere(int h, int k, int l, const Metric &m, double radius) {
+#pragma clang fp contract(off)
+  const double H = h, K = k, L = l;
+  double q = H * H * m.g00;
+  q = q + K * K * m.g11;
+  q = q + L * L * m.g22;
+  double cross = H * K * m.g01;
+  cross = cross + H * L * m.g02;
+  cross = cross + K * L * m.g12;
+  q = q + 2.0 * cross;
+  return sqrt(q) < radius;
+}
+
 // Pass 0 counts survivors per block; pass 1 writes them, order preserved.
-template <int kPass>
+// kSphere: ascending box order and keep_sphere (dmin is then the radius).
+template <int kPass, bool kSphere = false>
 __global__ __launch_bounds__(kEnumThreads) void hkl_enum_kernel(Box box, Metric m, double dmin,
                                                                 int *__restrict__ work,
                                                                 int *__restrict__ hkl_out,
@@ -73,8 +92,8 @@ __global__ __launch_bounds__(kEnumThreads) void hkl_enum_kernel(Box box, Metric 
     const long long i = first + j;
     if (i < box.total) {
       int h, k, l;
-      hkl_index_to_hkl(i, box, h, k, l);
-      if (keep_hkl(h, k, l, m, dmin)) {
+      hkl_index_to_hkl<kSphere>(i, box, h, k, l);
+      if (kSphere ? keep_sphere(h, k, l, m, dmin) : keep_hkl(h, k, l, m, dmin)) {
         keep_bits |= 1u << j;
         ++n;
       }
@@ -106,7 +125,7 @@ __global__ __launch_bounds__(kEnumThreads) void hkl_enum_kernel(Box box, Metric 
     if (keep_bits & (1u << j)) {
       if (pos < capacity) {
         int h, k, l;
-        hkl_index_to_hkl(first + j, box, h, k, l);
+        hkl_index_to_hkl<kSphere>(first + j, box, h, k, l);
         reinterpret_cast<int4 *>(hkl_out)[pos] = make_int4(h, k, l, 0);
       }
       ++pos;
@@ -291,17 +310,17 @@ extern "C" size_t dsx_hkl_workspace_ints(const int highest[3]) {
   return static_cast<size_t>(2 * nb);
 }
 
-extern "C" int dsx_hkl_enumerate(const int highest[3], const double metric[6], double d_min,
-                                 int *hkl_d, int capacity, int *count_d, int *work_d,
-                                 void *stream) {
+static int hkl_box_launch(bool sphere, const int highest[3], const double metric[6], double bound,
+                          int *hkl_d, int capacity, int *count_d, int *work_d, void *stream,
+                          const char *name) {
   clear_error();
   if (!highest || !metric || !count_d || !work_d || (capacity > 0 && !hkl_d)) {
-    set_error("dsx_hkl_enumerate: null argument");
+    set_error("%s: null argument", name);
     return DSX_EINVAL;
   }
-  if (highest[0] < 0 || highest[1] < 0 || highest[2] < 0 || !(d_min > 0.0)) {
-    set_error("dsx_hkl_enumerate: bad box {%d,%d,%d} or d_min %g", highest[0], highest[1],
-              highest[2], d_min);
+  if (highest[0] < 0 || highest[1] < 0 || highest[2] < 0 || !(bound > 0.0)) {
+    set_error("%s: bad box {%d,%d,%d} or bound %g", name, highest[0], highest[1], highest[2],
+              bound);
     return DSX_EINVAL;
   }
   Box b;
@@ -312,18 +331,40 @@ extern "C" int dsx_hkl_enumerate(const int highest[3], const double metric[6], d
   b.nl = 2 * b.L + 1;
   b.total = static_cast<long long>(2 * b.H + 1) * b.nk * b.nl;
   if (b.total > (1ll << 31)) {
-    set_error("dsx_hkl_enumerate: box of %lld entries exceeds 2^31", b.total);
+    set_error("%s: box of %lld entries exceeds 2^31", name, b.total);
     return DSX_EINVAL;
   }
   Metric m{metric[0], metric[1], metric[2], metric[3], metric[4], metric[5]};
   const int nb = static_cast<int>((b.total + kEnumPerBlock - 1) / kEnumPerBlock);
   hipStream_t s = as_stream(stream);
-  hipLaunchKernelGGL(hkl_enum_kernel<0>, dim3(nb), dim3(kEnumThreads), 0, s, b, m, d_min, work_d,
-                     hkl_d, capacity);
-  hipLaunchKernelGGL(hkl_scan_kernel, dim3(1), dim3(1024), 0, s, work_d, nb, count_d);
-  hipLaunchKernelGGL(hkl_enum_kernel<1>, dim3(nb), dim3(kEnumThreads), 0, s, b, m, d_min, work_d,
-                     hkl_d, capacity);
-  return check_launch("dsx_hkl_enumerate");
+  if (sphere) {
+    hipLaunchKernelGGL((hkl_enum_kernel<0, true>), dim3(nb), dim3(kEnumThreads), 0, s, b, m, bound,
+                       work_d, hkl_d, capacity);
+    hipLaunchKernelGGL(hkl_scan_kernel, dim3(1), dim3(1024), 0, s, work_d, nb, count_d);
+    hipLaunchKernelGGL((hkl_enum_kernel<1, true>), dim3(nb), dim3(kEnumThreads), 0, s, b, m, bound,
+                       work_d, hkl_d, capacity);
+  } else {
+    hipLaunchKernelGGL((hkl_enum_kernel<0, false>), dim3(nb), dim3(kEnumThreads), 0, s, b, m,
+                       bound, work_d, hkl_d, capacity);
+    hipLaunchKernelGGL(hkl_scan_kernel, dim3(1), dim3(1024), 0, s, work_d, nb, count_d);
+    hipLaunchKernelGGL((hkl_enum_kernel<1, false>), dim3(nb), dim3(kEnumThreads), 0, s, b, m,
+                       bound, work_d, hkl_d, capacity);
+  }
+  return check_launch(name);
+}
+
+extern "C" int dsx_hkl_enumerate(const int highest[3], const double metric[6], double d_min,
+                                 int *hkl_d, int capacity, int *count_d, int *work_d,
+                                 void *stream) {
+  return hkl_box_launch(false, highest, metric, d_min, hkl_d, capacity, count_d, work_d, stream,
+                        "dsx_hkl_enumerate");
+}
+
+extern "C" int dsx_hkl_points_in_sphere(const int highest[3], const double metric[6],
+                                        double radius, int *hkl_d, int capacity, int *count_d,
+                                        int *work_d, void *stream) {
+  return hkl_box_launch(true, highest, metric, radius, hkl_d, capacity, count_d, work_d, stream,
+                        "dsx_hkl_points_in_sphere");
 }
 
 // Atom staging for dsx_structure_factor from device-resident parameters:

@@ -113,7 +113,7 @@ def build_lines(hkl: torch.Tensor, g4: torch.Tensor, n_lines: Optional[int] = No
 
 def build_phase_table(pa: PhaseArrays, reciprocal_radius: float, with_direct_beam: bool,
                       scattering_params: Optional[str] = "lobato", device=None,
-                      warn_missing: bool = True) -> PhaseTable:
+                      warn_missing: bool = True, d_min: Optional[float] = None) -> PhaseTable:
     """hkl set + g vectors + |F|^2 of a phase on the GPU.
 
     Follows DiffractingVector.from_min_dspacing(phase, 1/reciprocal_radius,
@@ -121,12 +121,14 @@ def build_phase_table(pa: PhaseArrays, reciprocal_radius: float, with_direct_bea
     get_kinematical_intensities (reciprocal_lattice_vector.py:1077-1142,
     sim_utils.py:256-354); the extra (000) row the reference stacks on in
     get_intersecting_reflections (simulation_generator.py:351-353) is included.
+    ``d_min`` (default 1 / reciprocal_radius) is the box's d-spacing bound.
     """
     dev = _device(device)
     L = _lib.load()
     if not reciprocal_radius > 0:
         raise ValueError("reciprocal_radius must be positive")
-    d_min = 1.0 / reciprocal_radius
+    if d_min is None:
+        d_min = 1.0 / reciprocal_radius
     base, rec = lattice_setup(pa.abc, pa.angles)
     metric = reciprocal_metric(rec)
     H = highest_hkl(rec, d_min)
@@ -143,6 +145,18 @@ def build_phase_table(pa: PhaseArrays, reciprocal_radius: float, with_direct_bea
     _lib.check(L.dsx_hkl_enumerate(h3, m6, d_min, _lib.ptr(hkl), n_box, _lib.ptr(count),
                                    _lib.ptr(work), stream), "dsx_hkl_enumerate")
 
+    g4, F2, F = _structure_factor(pa, hkl, rec, scattering_params, dev, warn_missing, stream)
+    N = n_box + n_extra
+    g_max = float(torch.sqrt(g4[:, 3].max()).item()) if N else 0.0
+    return PhaseTable(hkl, g4, F2, F, n_box, with_direct_beam, g_max, rec, base, pa.name,
+                      build_lines(hkl, g4, stream=stream))
+
+
+def _structure_factor(pa: PhaseArrays, hkl: torch.Tensor, rec: np.ndarray,
+                      scattering_params: Optional[str], dev, warn_missing: bool, stream):
+    """g vectors and |F|^2, F of the rows of ``hkl`` (int32 [N, 4]) for the
+    phase's atoms (dsx_structure_factor); g = rec hkl."""
+    L = _lib.load()
     # atoms sorted by element so the kernel evaluates f once per element
     n_atoms = len(pa.symbols)
     elements = [element_of(s) for s in pa.symbols]
@@ -175,18 +189,86 @@ def build_phase_table(pa: PhaseArrays, reciprocal_radius: float, with_direct_bea
     t_occ = torch.from_numpy(np.ascontiguousarray(occ)).to(dev)
     t_B = torch.from_numpy(np.ascontiguousarray(B)).to(dev)
     t_coeff = torch.from_numpy(coeff).to(dev)
-    N = n_box + n_extra
+    N = int(hkl.shape[0])
     g4 = torch.empty((N, 4), dtype=torch.float32, device=dev)
     F2 = torch.empty(N, dtype=torch.float32, device=dev)
     F = torch.empty((N, 2), dtype=torch.float32, device=dev)
-    rec9 = (ctypes.c_double * 9)(*rec.reshape(-1).tolist())
+    rec9 = (ctypes.c_double * 9)(*np.asarray(rec, dtype=np.float64).reshape(-1).tolist())
     _lib.check(L.dsx_structure_factor(_lib.ptr(hkl), N, rec9, _lib.ptr(t_xyz), _lib.ptr(t_el),
                                       _lib.ptr(t_occ), _lib.ptr(t_B), n_atoms, _lib.ptr(t_coeff),
                                       len(uniq), SCATTER_IDS[scattering_params], _lib.ptr(g4),
                                       _lib.ptr(F2), _lib.ptr(F), stream), "dsx_structure_factor")
+    return g4, F2, F
+
+
+def diffpy_standard_base(abc, angles) -> np.ndarray:
+    """Rows a, b, c of diffpy.structure's standard Cartesian base (a* || x,
+    c || z, b in the yz-plane), the frame of the legacy DiffractionGenerator
+    (its structures are diffpy lattices with no base rotation):
+    a = (1/a*, -cos(gamma*)/(sin(gamma*) a*), a cos(beta)),
+    b = (0, b sin(alpha), b cos(alpha)), c = (0, 0, c)."""
+    from .crystallography import cosd, sind
+
+    a, b, c = (float(v) for v in abc)
+    al, be, ga = (float(v) for v in angles)
+    ca, cb, cg = cosd(al), cosd(be), cosd(ga)
+    sa, sb, sg = sind(al), sind(be), sind(ga)
+    vunit = np.sqrt(1.0 + 2.0 * ca * cb * cg - ca * ca - cb * cb - cg * cg)
+    ar = sa / (a * vunit)
+    cgr = (ca * cb - cg) / (sa * sb)
+    sgr = np.sqrt(1.0 - cgr * cgr)
+    return np.array([[1.0 / ar, -cgr / sgr / ar, cb * a],
+                     [0.0, b * sa, b * ca],
+                     [0.0, 0.0, c]], dtype=np.float64)
+
+
+def build_legacy_table(pa: PhaseArrays, reciprocal_radius: float,
+                       scattering_params: Optional[str] = "lobato", device=None,
+                       warn_missing: bool = True) -> PhaseTable:
+    """Reflection table of the legacy DiffractionGenerator
+    (diffsims/generators/diffraction_generator.py:233-242): the reciprocal
+    lattice points of get_points_in_sphere (sim_utils.py:436-474) -- box
+    floor(r / |a*|) etc., ascending (h, k, l), |g| < r, (000) included --
+    with g in diffpy's standard frame (diffpy_standard_base) and |F|^2."""
+    dev = _device(device)
+    L = _lib.load()
+    if not reciprocal_radius > 0:
+        raise ValueError("reciprocal_radius must be positive")
+    base = diffpy_standard_base(pa.abc, pa.angles)
+    rec = np.linalg.inv(base)                       # columns a*, b*, c*
+    metric = reciprocal_metric(rec)
+    lengths = np.sqrt(np.sum(rec * rec, axis=0))    # |a*|, |b*|, |c*|
+    H = [int(np.floor(reciprocal_radius / v)) for v in lengths]
+    h3 = (ctypes.c_int * 3)(*H)
+    m6 = (ctypes.c_double * 6)(*metric.tolist())
+    stream = _lib.stream_handle(dev)
+    work = torch.empty(max(1, int(L.dsx_hkl_workspace_ints(h3))), dtype=torch.int32, device=dev)
+    count = torch.zeros(1, dtype=torch.int32, device=dev)
+    _lib.check(L.dsx_hkl_points_in_sphere(h3, m6, float(reciprocal_radius), None, 0,
+                                          _lib.ptr(count), _lib.ptr(work), stream),
+               "dsx_hkl_points_in_sphere")
+    N = int(count.item())
+    hkl = torch.zeros((N, 4), dtype=torch.int32, device=dev)
+    _lib.check(L.dsx_hkl_points_in_sphere(h3, m6, float(reciprocal_radius), _lib.ptr(hkl), N,
+                                          _lib.ptr(count), _lib.ptr(work), stream),
+               "dsx_hkl_points_in_sphere")
+    g4, F2, F = _structure_factor(pa, hkl, rec, scattering_params, dev, warn_missing, stream)
     g_max = float(torch.sqrt(g4[:, 3].max()).item()) if N else 0.0
-    return PhaseTable(hkl, g4, F2, F, n_box, with_direct_beam, g_max, rec, base, pa.name,
-                      build_lines(hkl, g4, stream=stream))
+    return PhaseTable(hkl, g4, F2, F, N, False, g_max, rec, base, pa.name, None)
+
+
+def structure_factor_table(pa: PhaseArrays, hkl: np.ndarray, rec: np.ndarray,
+                           scattering_params: Optional[str] = "lobato", device=None,
+                           warn_missing: bool = True) -> PhaseTable:
+    """|F|^2 (and g) of given integer indices for a phase, g = rec hkl."""
+    dev = _device(device)
+    h = np.zeros((len(hkl), 4), dtype=np.int32)
+    h[:, :3] = np.rint(np.asarray(hkl, dtype=np.float64)).astype(np.int32).reshape(-1, 3)
+    t_hkl = torch.from_numpy(h).to(dev)
+    g4, F2, F = _structure_factor(pa, t_hkl, rec, scattering_params, dev, warn_missing,
+                                  _lib.stream_handle(dev))
+    g_max = float(torch.sqrt(g4[:, 3].max()).item()) if len(h) else 0.0
+    return PhaseTable(t_hkl, g4, F2, F, len(h), False, g_max, rec, None, pa.name, None)
 
 
 def table_from_hkl(hkl: np.ndarray, abc, angles, F2: Optional[np.ndarray] = None,

@@ -223,8 +223,46 @@ class SimulationGenerator:
         vecs._hkl = hkl[idx].astype(np.float64)
         return vecs, hkl[idx].astype(np.float64), sf
 
-    def calculate_diffraction1d(self, *args, **kwargs):
-        raise NotImplementedError("1-D powder profiles are outside the GPU 2-D simulation path")
+    def calculate_diffraction1d(self, phase, reciprocal_radius: float = 1.0,
+                                minimum_intensity: float = 1e-3, debye_waller_factors: dict = None):
+        """1-D profile of one phase (simulation_generator.py:261-317): the
+        reflections of from_min_dspacing(phase) (d >= 0.7 A, the reference's
+        default; ``reciprocal_radius`` only travels to the result) on the
+        GPU with their |F|^2, grouped into families of the phase's point
+        group (host), intensity |F|^2 x multiplicity scaled to a maximum of
+        100, families at or below ``minimum_intensity`` dropped, hkil
+        indices for a hexagonal lattice."""
+        from .simulation1d import Simulation1D
+        from .symmetry import point_group_of_space_group, symmetry_families
+
+        pg = getattr(phase, "point_group", None)
+        if pg is not None and not isinstance(pg, str):
+            pg = getattr(pg, "name", None)
+        if pg is None:
+            sg = getattr(phase, "space_group", None)
+            if sg is not None and not isinstance(sg, (int, np.integer)):
+                sg = getattr(sg, "number", None)
+            if sg is None:
+                raise ValueError(f"The phase {phase.name!r} must have a point group (or a space "
+                                 "group) to reduce its reflections by symmetry")
+            pg = point_group_of_space_group(int(sg))
+        pa = phase_arrays(phase, debye_waller_factors or {})
+        tab = engine.build_phase_table(pa, 1.0 / 0.7, False, self.scattering_params, d_min=0.7)
+        hkl = tab.hkl_host().astype(np.int64)
+        g = hkl.astype(np.float64) @ tab.rec.T
+        rep, mult = symmetry_families(g, pg)
+        F2 = tab.F2.double().cpu().numpy()[rep]
+        intensities = F2 * mult
+        gs = np.sqrt(np.sum(g[rep] ** 2, axis=1))
+        hkl_r = hkl[rep].astype(np.float64)
+        lat = phase.structure.lattice
+        if lat.a == lat.b and lat.alpha == 90 and lat.beta == 90 and lat.gamma == 120:
+            hkl_r = np.stack([hkl_r[:, 0], hkl_r[:, 1], -(hkl_r[:, 0] + hkl_r[:, 1]), hkl_r[:, 2]], 1)
+        intensities = np.asarray(intensities) / max(intensities) * 100
+        mask = intensities > minimum_intensity
+        return Simulation1D(phase=phase, reciprocal_spacing=gs[mask],
+                            intensities=intensities[mask], hkl=hkl_r[mask],
+                            reciprocal_radius=reciprocal_radius, wavelength=self.wavelength)
 
 
 def _voltage_of(wavelength: float) -> float:

@@ -76,6 +76,19 @@ int dsx_hkl_enumerate(const int highest[3], const double metric[6], double d_min
                       void *stream);
 
 /*
+ * Reciprocal lattice points in a sphere — replaces get_points_in_sphere
+ * (diffsims/utils/sim_utils.py:436-474) as the legacy DiffractionGenerator
+ * (diffsims/generators/diffraction_generator.py:192-331) and
+ * calculate_profile_data (:334-417) use it: the box [-H,H]x[-K,K]x[-L,L]
+ * (H = floor(radius / |a*|), etc., from the caller) in ascending
+ * lexicographic (h,k,l) order (itertools.product of ascending ranges), kept
+ * where sqrt(q) < radius with q the quadratic form of dsx_hkl_enumerate;
+ * (000) is kept.  Arguments as for dsx_hkl_enumerate (same workspace).
+ */
+int dsx_hkl_points_in_sphere(const int highest[3], const double metric[6], double radius,
+                             int *hkl_d, int capacity, int *count_d, int *work_d, void *stream);
+
+/*
  * Per-reflection tables — replaces the per-orientation structure-factor work of
  * get_kinematical_intensities / _get_kinematical_structure_factor
  * (diffsims/utils/sim_utils.py:256-354), computed once per phase.

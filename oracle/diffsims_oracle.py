@@ -1028,3 +1028,108 @@ def beam_directions_grid(crystal_system: str, resolution: float,
         n = np.cross(p, q)
         v = v[np.dot(n, r) * np.dot(n, v.T) >= eps]
     return directions_to_euler(v)
+
+
+# ---------------------------------------------------------------------------
+# legacy generator (DiffractionGenerator / get_points_in_sphere / profiles)
+# ---------------------------------------------------------------------------
+
+def diffpy_base_from_conditions(a, b, c, alpha, beta, gamma) -> np.ndarray:
+    """Rows a, b, c of the frame the legacy generator works in (diffpy's
+    standard base), built from its defining conditions: c along +z, b in the
+    yz-plane with b_y > 0, a with a.c = ac cos(beta), a.b = ab cos(gamma),
+    |a| = a and a_x > 0 (so a* || +x)."""
+    ca, cb, cg = cosd(alpha), cosd(beta), cosd(gamma)
+    sa = sind(alpha)
+    cvec = np.array([0.0, 0.0, c])
+    bvec = np.array([0.0, b * sa, b * ca])
+    az = a * cb
+    ay = a * (cg - cb * ca) / sa
+    ax = math.sqrt(a * a - ay * ay - az * az)
+    return np.array([[ax, ay, az], bvec, cvec])
+
+
+def legacy_points_in_sphere(abc, angles, radius):
+    """get_points_in_sphere (diffsims/utils/sim_utils.py:436-474) on the
+    reciprocal lattice of the legacy frame: H = floor(r / |a*|) etc., the
+    product of ascending ranges (h outer, l inner), kept where |g| < r
+    (|g| from the quadratic form of rnorm), (000) included.  Returns hkl
+    (n, 3) int, g (n, 3) Cartesian and |g|."""
+    base = diffpy_base_from_conditions(*abc, *angles)
+    rec = np.linalg.inv(base)
+    lengths = np.sqrt(np.sum(rec * rec, axis=0))
+    H = [int(np.floor(radius / v)) for v in lengths]
+    grid = np.stack(np.meshgrid(np.arange(-H[0], H[0] + 1), np.arange(-H[1], H[1] + 1),
+                                np.arange(-H[2], H[2] + 1), indexing="ij"), -1).reshape(-1, 3)
+    d = rnorm(grid, rec)
+    keep = d < radius
+    hkl = grid[keep]
+    return hkl, g_cartesian(hkl, rec), d[keep]
+
+
+def rzxz_matrix(ai, aj, ak) -> np.ndarray:
+    """transforms3d euler2mat(ai, aj, ak, 'rzxz') (radians): rotating axes
+    z, x', z'', written out as the entries of Rz(ai) Rx(aj) Rz(ak)."""
+    ca, sa, cb, sb, cc, sc = (math.cos(ai), math.sin(ai), math.cos(aj), math.sin(aj),
+                              math.cos(ak), math.sin(ak))
+    return np.array([[ca * cc - sa * cb * sc, -ca * sc - sa * cb * cc, sa * sb],
+                     [sa * cc + ca * cb * sc, -sa * sc + ca * cb * cc, -ca * sb],
+                     [sb * sc, sb * cc, cb]])
+
+
+def legacy_tables(ph: PhaseSpec, p: SimParams) -> PhaseTables:
+    """Legacy reflection set with g and |F|^2 (diffraction_generator.py:233-242)."""
+    hkl, g, gn = legacy_points_in_sphere(ph.abc, ph.angles, p.reciprocal_radius)
+    coeffs, _ = atom_coeffs(ph.symbols, p.scattering_params)
+    F = structure_factor(hkl, gn, np.asarray(ph.xyz, float), np.asarray(ph.occ, float),
+                         np.asarray(ph.B, float), coeffs, p.scattering_params, phase_mode="exact")
+    return PhaseTables(hkl, g, gn, np.abs(F) ** 2, hkl.shape[0])
+
+
+def legacy_simulate_one(tab: PhaseTables, euler_deg, k: float, p: SimParams) -> Spots:
+    """calculate_ed_data for one orientation (diffraction_generator.py:192-331):
+    g_lab = R g with R = rzxz_matrix of the angles in degrees, then the same
+    intersection, shape factor and relative cut as simulate_one."""
+    R = rzxz_matrix(*np.deg2rad(np.asarray(euler_deg, dtype=np.float64)))
+    return simulate_one(tab, R.T, k, p)
+
+
+def legacy_profile(ph: PhaseSpec, p: SimParams, minimum_intensity=1e-3, hexagonal=False):
+    """calculate_profile_data (diffraction_generator.py:334-417) with
+    get_intensities_params / get_unique_families (sim_utils.py:101-136,
+    513-556): families by permutation of |h|,|k|,|l| in first-appearance
+    order, representative = the largest member, intensity |F|^2 x
+    multiplicity; returns (magnitudes, intensities, labels)."""
+    hkl, g, gn = legacy_points_in_sphere(ph.abc, ph.angles, p.reciprocal_radius)
+    fam = {}
+    for row in hkl.tolist():
+        hit = None
+        for key in fam:
+            if sorted(abs(v) for v in key) == sorted(abs(v) for v in row):
+                hit = key
+                break
+        if hit is None:
+            fam[tuple(row)] = [row]
+        else:
+            fam[hit].append(row)
+    reps = np.array([sorted(v)[-1] for v in fam.values()])
+    mult = np.array([len(v) for v in fam.values()], dtype=float)
+    d_of = {tuple(h): d for h, d in zip(hkl.tolist(), gn)}
+    g_reps = np.array([d_of[tuple(r)] for r in reps.tolist()])
+    coeffs, _ = atom_coeffs(ph.symbols, p.scattering_params)
+    F = structure_factor(reps, g_reps, np.asarray(ph.xyz, float), np.asarray(ph.occ, float),
+                         np.asarray(ph.B, float), coeffs, p.scattering_params, phase_mode="exact")
+    I = np.abs(F) ** 2 * mult
+    rows = (reps[0], reps[1], -reps[0] - reps[1], reps[2]) if hexagonal else reps.tolist()
+    labels = ["".join(str(int(x)) for x in xs) for xs in rows]
+    peaks = {}
+    for lab, i, d in zip(labels, I, g_reps):
+        peaks[lab] = [i, d]
+    mx = max(v[0] for v in peaks.values())
+    x, y, keep = [], [], []
+    for lab, (i, d) in peaks.items():
+        if i / mx * 100 > minimum_intensity and lab != "000":
+            x.append(d)
+            y.append(i)
+            keep.append(lab)
+    return np.array(x), np.asarray(y) / max(y) * 100, keep
