@@ -198,7 +198,15 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
     int lb = -kWave, lo = kOT - 1, win = 0, total = 0;
     int first = 0, nrow = 0, lh = 0, lk = 0, l0 = 0, P = 0, rb = 0;
     double zlo = 0.0, zhi = -1.0;
-    float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
+    // along a line (hkl = (h, k, l0 - t)) the pretest value is a quadratic in
+    // t: e(t) = |g0 - t c*|^2 - 2k (zt - t ga) = A + B t + c2 t^2, so a window
+    // forms it from this lane's line coefficients (eA, eB at the interval's
+    // first row) instead of loading the row's g4; w0 = |g0|^2, cd = g0 . c*
+    // (FP32: their rounding, ~1e-6 of e, is far inside the pretest's slack)
+    float w0 = 0.0f, cd = 0.0f;
+    float eA = 0.0f, eB = 0.0f;
+    const double c2 = rec[2] * rec[2] + rec[5] * rec[5] + rec[8] * rec[8];
+    const float c2f = static_cast<float>(c2);
     // pretest pass => |w - 2k zf| < thr (1 + 2^-24); zf (FP32 R and g, two
     // fmaf) is within 1e-6 |g| of the FP64 beam coordinate, ten times inside
     // the margin zm = 1e-5 (1 + max |g| of the line) below
@@ -251,6 +259,12 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
               const double zm = 1e-5 * (1.0 + sqrt(wmax));
               zlo = (wmin - thr) / two_kd - zm;
               zhi = (wmax + thr) / two_kd + zm;
+              const double hx = lh, hy = lk, hz = l0;
+              const double gx = rec[0] * hx + rec[1] * hy + rec[2] * hz;
+              const double gy = rec[3] * hx + rec[4] * hy + rec[5] * hz;
+              const double gz = rec[6] * hx + rec[7] * hy + rec[8] * hz;
+              w0 = static_cast<float>(gx * gx + gy * gy + gz * gz);
+              cd = static_cast<float>(gx * rec[2] + gy * rec[5] + gz * rec[8]);
             }
           }
           // z of row first + t is zt - t ga (l = l0 - t): the rows with z in [zlo, zhi]
@@ -270,6 +284,12 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
               len = static_cast<int>(thi - tlo) + 1;
             }
           }
+          {
+            const double tt = static_cast<double>(t0);
+            const double eb = two_kd * ga - 2.0 * static_cast<double>(cd);
+            eA = static_cast<float>((static_cast<double>(w0) - two_kd * zt) + tt * (eb + c2 * tt));
+            eB = static_cast<float>(eb + 2.0 * c2 * tt);
+          }
           int incl = len;  // inclusive prefix over the lanes (lines in row order)
 #pragma unroll
           for (int off = 1; off < kWave; off <<= 1) {
@@ -280,17 +300,6 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
           P = incl - len;  // first candidate of this lane's line
           rb = first + t0;
           win = 0;
-          r0 = R[0][0];
-          r1 = R[0][1];
-          r2 = R[0][2];
-#pragma unroll
-          for (int o = 1; o < kOT; ++o) {
-            if (lo == o) {
-              r0 = R[o][0];
-              r1 = R[o][1];
-              r2 = R[o][2];
-            }
-          }
         }
         if (!done) {
           const int c = win + lane;
@@ -301,10 +310,17 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
             const int p = __shfl(P, src + step);
             if (p <= c) src += step;
           }
-          const int row = __shfl(rb, src) + (c - __shfl(P, src));
+          const int u = c - __shfl(P, src);
+          const int row = __shfl(rb, src) + u;
           const bool v = c < total && static_cast<unsigned>(row) < static_cast<unsigned>(n_refl);
-          const float4 g = v ? g4[row] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-          pretest(v, row, g, r0, r1, r2, lo);
+          const float uf = static_cast<float>(u);
+          const float e = fmaf(fmaf(c2f, uf, __shfl(eB, src)), uf, __shfl(eA, src));
+          const bool pre = v && fabsf(e) < pre_thr;
+          const uint64_t pm = __ballot(pre);
+          if (pm != 0) {
+            if (pre) queue[(head + qn + mask_rank(pm)) % kQueue] = (row << kSlotBits) | lo;
+            qn += __popcll(pm);
+          }
         }
       } else {
         done = base >= n_refl;
