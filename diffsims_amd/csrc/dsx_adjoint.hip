@@ -186,7 +186,7 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 // Fixed-point value of one Gaussian term amp exp(ep r^2) at offset (dx, dy),
 // written once so the splat and the maximum search round it identically.
 __device__ __forceinline__ int gauss_term(float amp, float dx, float dy, float ep, float scale) {
-  return __float2int_rn(amp * expf(ep * fmaf(dy, dy, dx * dx)) * scale);
+  return __float2int_rn(amp * fast_exp(ep * fmaf(dy, dy, dx * dx)) * scale);
 }
 
 // Splat (kGather = false) or gather (true) of every binned spot footprint

@@ -668,7 +668,7 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
             racc_t *col = s_acc + x;
             for (int y = ya; y <= yb; ++y) {
               const float dy = static_cast<float>(y) - cy;
-              racc_add(col + (y - y0) * W, amp * expf(a.ep * (dy * dy + dx * dx)), scale);
+              racc_add(col + (y - y0) * W, amp * fast_exp(a.ep * (dy * dy + dx * dx)), scale);
             }
           }
         }
