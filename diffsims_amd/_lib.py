@@ -49,6 +49,9 @@ MESH_NORMALIZED_CUBE = 1
 MESH_SPHERIFIED_EDGE = 2
 MESH_SPHERIFIED_CORNER = 3
 
+SO3_FUNDAMENTAL_ZONE = 0
+SO3_LOCAL = 1
+
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTED = (
     "dsx_version",
@@ -68,6 +71,9 @@ EXPORTED = (
     "dsx_icosahedral_nn_dots",
     "dsx_beam_directions_icosahedral",
     "dsx_beam_directions_from_vertices",
+    "dsx_so3_grid_candidates",
+    "dsx_so3_grid_workspace_ints",
+    "dsx_so3_grid",
     "dsx_simulate",
     "dsx_render_max_spots",
     "dsx_render_fast",
@@ -173,6 +179,11 @@ _SIGS = {
                                                ctypes.POINTER(ctypes.c_int), _P,
                                                ctypes.c_longlong, _P, _P, ctypes.c_longlong,
                                                _P]),
+    "dsx_so3_grid_candidates": (ctypes.c_longlong, [_I]),
+    "dsx_so3_grid_workspace_ints": (ctypes.c_longlong, [_I]),
+    "dsx_so3_grid": (_I, [_I, _I, _I, ctypes.POINTER(ctypes.c_double), ctypes.c_double,
+                          ctypes.POINTER(ctypes.c_double), _P, ctypes.c_longlong, _P, _P,
+                          ctypes.c_longlong, _P]),
     "dsx_hkl_lines_workspace_ints": (ctypes.c_longlong, [_I]),
     "dsx_hkl_lines": (_I, [_P, _P, _I, _P, _I, _P, _P, ctypes.c_longlong, _P]),
     "dsx_simulate": (_I, [_P, _P, _P, _I, _P, _I, _P, _I, ctypes.POINTER(SimParams), _I, _P, _P,

@@ -524,6 +524,254 @@ bool icosa_topology(const int *edge_order, IcosaTopo &t) {
 
 constexpr int kIcoMaxLevel = 1 << 14;
 
+// ---- SO(3) cubochoric grids ------------------------------------------------
+//
+// get_fundamental_zone_grid / get_local_grid (rotation_list_generators.py:
+// 85-134) sample SO(3) on the cubochoric grid (Rosca, Morawiec & De Graef,
+// Modelling Simul. Mater. Sci. Eng. 22 (2014) 075013): points i a/n,
+// i = -n+1 .. n, of the cube of semi-edge a = pi^(2/3)/2 (the -a faces are
+// antipodes of the +a faces), mapped volume-preservingly onto the homochoric
+// ball and from there to rotations.  One thread per grid point, FP64 with
+// contraction off; rows leave in grid order (x outermost, z innermost) through
+// the count / scan / write compaction of the sphere meshes.  Bunge angles
+// follow Rowenhorst et al., Modelling Simul. Mater. Sci. Eng. 23 (2015)
+// 083501, eq. A.14 with P = +1.
+
+constexpr int kSo3MaxSteps = 1024;
+constexpr int kSo3NewtonSteps = 6;
+constexpr double kFzEps = 1e-12;
+
+struct So3Args {
+  int n;             // semi-edge steps
+  long long side;    // 2 n points per axis
+  long long n_cand;  // side^3
+  double step;       // a / n
+  // map constants, formed on the host: (pi/6)^(1/6); (3pi/4)^(1/3) 2^(1/4) /
+  // (pi^(5/6) / (2 6^(1/6))); sqrt(6/pi); sqrt(2); pi/12; sqrt(pi); sqrt(24)
+  double sc, prek, pref, sqrt2, pi12, sqrtpi, sqrt24;
+  int mode;          // DSX_SO3_FUNDAMENTAL_ZONE or DSX_SO3_LOCAL
+  int n_ops;         // group rotations other than the identity
+  double ops[24][4];
+  double half;       // local: keep acos(w) < half
+  int has_center;
+  double center[4];
+  double *euler;
+  long long capacity;
+};
+
+// Cube -> homochoric ball.  The dominant axis (z, then x, then y on ties)
+// is rotated onto z; the square at height Z goes area-preservingly onto the
+// Lambert-azimuthal preimage of the curved square, then onto the sphere of
+// radius sqrt(6/pi) Z.
+__device__ __forceinline__ void cubo_to_homo(const So3Args &a, double x, double y, double z,
+                                             double h[3]) {
+#pragma clang fp contract(off)
+  const double ax = fabs(x), ay = fabs(y), az = fabs(z);
+  int pyr;  // 0: z, 1: x, 2: y
+  double s0, s1, s2;
+  if (fmax(ax, ay) <= az) {
+    pyr = 0; s0 = x; s1 = y; s2 = z;
+  } else if (fmax(az, ay) <= ax) {
+    pyr = 1; s0 = y; s1 = z; s2 = x;
+  } else {
+    pyr = 2; s0 = z; s1 = x; s2 = y;
+  }
+  const double X = a.sc * s0, Y = a.sc * s1, Z = a.sc * s2;
+  double L0 = 0.0, L1 = 0.0, L2;
+  if (X == 0.0 && Y == 0.0) {
+    L2 = a.pref * Z;
+  } else {
+    const bool ylex = fabs(Y) <= fabs(X);
+    const double num = ylex ? Y : X, den = ylex ? X : Y;
+    const double t = a.pi12 * num / den;
+    double sn, c;
+    sincos(t, &sn, &c);
+    const double q = a.prek * den / sqrt(a.sqrt2 - c);
+    const double u = (a.sqrt2 * c - 1.0) * q;
+    const double v = a.sqrt2 * sn * q;
+    const double T1 = ylex ? u : v, T2 = ylex ? v : u;
+    const double rho2 = T1 * T1 + T2 * T2;
+    const double lift = sqrt(1.0 - M_PI * rho2 / (24.0 * Z * Z));
+    L0 = T1 * lift;
+    L1 = T2 * lift;
+    L2 = a.pref * Z - a.sqrtpi * rho2 / a.sqrt24 / Z;
+  }
+  if (pyr == 0) {
+    h[0] = L0; h[1] = L1; h[2] = L2;
+  } else if (pyr == 1) {
+    h[0] = L2; h[1] = L0; h[2] = L1;
+  } else {
+    h[0] = L1; h[1] = L2; h[2] = L0;
+  }
+}
+
+// w - sin(w), by its Taylor series below 0.5 (no cancellation).
+__device__ __forceinline__ double omega_minus_sin(double w) {
+#pragma clang fp contract(off)
+  if (w < 0.5) {
+    const double w2 = w * w;
+    return w * w2 / 6.0 *
+           (1.0 - w2 / 20.0 * (1.0 - w2 / 42.0 * (1.0 - w2 / 72.0 * (1.0 - w2 / 110.0 *
+                                                                    (1.0 - w2 / 156.0)))));
+  }
+  return w - sin(w);
+}
+
+// Homochoric vector -> unit quaternion (w >= 0): |h|^3 = (3/4)(w - sin w)
+// by a fixed number of Newton steps from w0 = 2|h|, which lies below the root
+// of this convex function, so the steps converge from above; four reach the
+// last bit anywhere on [0, pi].
+__device__ __forceinline__ void homo_to_quat(const double h[3], double q[4]) {
+#pragma clang fp contract(off)
+  const double hn = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+  const double t = 4.0 / 3.0 * (hn * hn * hn);
+  double w = fmin(2.0 * hn, M_PI);
+  for (int it = 0; it < kSo3NewtonSteps; ++it) {
+    const double f = omega_minus_sin(w) - t;
+    const double sh = sin(0.5 * w);
+    const double d = 2.0 * (sh * sh);
+    const double st = d > 0.0 ? f / d : 0.0;
+    w = fmin(fmax(w - st, 0.0), M_PI);
+  }
+  double sw, cw;
+  sincos(0.5 * w, &sw, &cw);
+  q[0] = cw;
+  if (hn > 0.0) {
+    q[1] = sw * (h[0] / hn);
+    q[2] = sw * (h[1] / hn);
+    q[3] = sw * (h[2] / hn);
+  } else {
+    q[1] = 0.0;
+    q[2] = 0.0;
+    q[3] = sw;
+  }
+}
+
+__device__ __forceinline__ void quat_mul(const double p[4], const double q[4], double r[4]) {
+#pragma clang fp contract(off)
+  r[0] = p[0] * q[0] - p[1] * q[1] - p[2] * q[2] - p[3] * q[3];
+  r[1] = p[0] * q[1] + p[1] * q[0] + p[2] * q[3] - p[3] * q[2];
+  r[2] = p[0] * q[2] - p[1] * q[3] + p[2] * q[0] + p[3] * q[1];
+  r[3] = p[0] * q[3] + p[1] * q[2] - p[2] * q[1] + p[3] * q[0];
+}
+
+// Bunge angles (radians) of a quaternion: Rowenhorst et al. (2015) eq. A.14,
+// P = +1, with the Phi = 0 and Phi = pi branches; negative angles moved into
+// [0, 2 pi) (phi1, phi2) and [0, pi] (Phi).
+__device__ __forceinline__ void quat_to_bunge(const double qin[4], double e[3]) {
+#pragma clang fp contract(off)
+  const double nrm = sqrt(qin[0] * qin[0] + qin[1] * qin[1] + qin[2] * qin[2] + qin[3] * qin[3]);
+  const double w = qin[0] / nrm, x = qin[1] / nrm, y = qin[2] / nrm, z = qin[3] / nrm;
+  const double q03 = w * w + z * z, q12 = x * x + y * y;
+  const double chi = sqrt(q03 * q12);
+  if (chi == 0.0 && q12 == 0.0) {
+    e[0] = atan2(-2.0 * w * z, w * w - z * z);
+    e[1] = 0.0;
+    e[2] = 0.0;
+  } else if (chi == 0.0 && q03 == 0.0) {
+    e[0] = atan2(2.0 * x * y, x * x - y * y);
+    e[1] = M_PI;
+    e[2] = 0.0;
+  } else {
+    e[0] = atan2((x * z - w * y) / chi, (-w * x - y * z) / chi);
+    e[1] = atan2(2.0 * chi, q03 - q12);
+    e[2] = atan2((w * y + x * z) / chi, (y * z - w * x) / chi);
+  }
+  if (e[0] < 0.0) e[0] = fmod(e[0] + 100.0 * M_PI, 2.0 * M_PI);
+  if (e[1] < 0.0) e[1] = fmod(e[1] + 100.0 * M_PI, M_PI);
+  if (e[2] < 0.0) e[2] = fmod(e[2] + 100.0 * M_PI, 2.0 * M_PI);
+}
+
+__device__ __forceinline__ bool so3_keep(const So3Args &a, long long c, double q[4]) {
+#pragma clang fp contract(off)
+  if (c >= a.n_cand) return false;
+  const long long i = c / (a.side * a.side);
+  const long long j = (c / a.side) % a.side;
+  const long long k = c % a.side;
+  const double x = static_cast<double>(i - a.n + 1) * a.step;
+  const double y = static_cast<double>(j - a.n + 1) * a.step;
+  const double z = static_cast<double>(k - a.n + 1) * a.step;
+  double h[3];
+  cubo_to_homo(a, x, y, z, h);
+  homo_to_quat(h, q);
+  if (a.mode == DSX_SO3_FUNDAMENTAL_ZONE) {
+    double worst = 0.0;
+    for (int s = 0; s < a.n_ops; ++s) {
+      const double d = fabs(q[0] * a.ops[s][0] + q[1] * a.ops[s][1] + q[2] * a.ops[s][2] +
+                            q[3] * a.ops[s][3]);
+      worst = fmax(worst, d);
+    }
+    return a.n_ops == 0 || fabs(q[0]) - worst >= -kFzEps;
+  }
+  if (!(acos(q[0]) < a.half)) return false;
+  if (a.has_center) {
+    double r[4];
+    quat_mul(a.center, q, r);
+    q[0] = r[0]; q[1] = r[1]; q[2] = r[2]; q[3] = r[3];
+  }
+  return true;
+}
+
+template <int kPass>  // 0: count per block; 1: write rows
+__global__ __launch_bounds__(kGridThreads) void so3_grid_kernel(So3Args a, int *work) {
+  __shared__ int s_wave[kGridThreads / kWave];
+  const long long c = static_cast<long long>(blockIdx.x) * kGridThreads + threadIdx.x;
+  double q[4];
+  const bool keep = so3_keep(a, c, q);
+  const uint64_t m = __ballot(keep);
+  const int lane = lane_id(), wid = threadIdx.x / kWave;
+  if (lane == 0) s_wave[wid] = __popcll(m);
+  __syncthreads();
+  if (kPass == 0) {
+    if (threadIdx.x == 0) {
+      int t = 0;
+      for (int w = 0; w < kGridThreads / kWave; ++w) t += s_wave[w];
+      work[blockIdx.x] = t;
+    }
+    return;
+  }
+  if (!keep) return;
+  const long long nb = gridDim.x;
+  const long long *offs = reinterpret_cast<const long long *>(work + ((nb + 1) & ~1ll));
+  long long row = offs[blockIdx.x] + mask_rank(m);
+  for (int w = 0; w < wid; ++w) row += s_wave[w];
+  if (row >= a.capacity) return;
+  double e[3];
+  quat_to_bunge(q, e);
+  a.euler[3 * row + 0] = e[0];
+  a.euler[3 * row + 1] = e[1];
+  a.euler[3 * row + 2] = e[2];
+}
+
+// 64-bit exclusive scan of the block counts (the SO(3) grids can pass 2^31
+// rows): block offsets as long long in the workspace after the counts.
+__global__ __launch_bounds__(1024) void so3_scan_kernel(int *work, long long nblocks,
+                                                        long long *count_out) {
+  __shared__ long long s[1024];
+  __shared__ long long carry;
+  const int tid = threadIdx.x;
+  long long *offs = reinterpret_cast<long long *>(work + ((nblocks + 1) & ~1ll));
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (long long base = 0; base < nblocks; base += 1024) {
+    const long long i = base + tid;
+    const long long v = i < nblocks ? work[i] : 0;
+    s[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const long long t = tid >= off ? s[tid - off] : 0;
+      __syncthreads();
+      s[tid] += t;
+      __syncthreads();
+    }
+    if (i < nblocks) offs[i] = carry + s[tid] - v;
+    __syncthreads();
+    if (tid == 1023) carry += s[1023];
+    __syncthreads();
+  }
+  if (tid == 0) *count_out = carry;
+}
+
 }  // namespace dsx
 
 using namespace dsx;
@@ -625,4 +873,72 @@ extern "C" int dsx_beam_directions_from_vertices(const double *xyz_d, long long 
   a.xyz_out = 0;
   return launch_grid(a, n_planes, planes, euler_d, capacity, count_d, work_d, work_ints, stream,
                      "dsx_beam_directions_from_vertices");
+}
+
+extern "C" long long dsx_so3_grid_candidates(int semi_edge_steps) {
+  if (semi_edge_steps < 1 || semi_edge_steps > kSo3MaxSteps) return -1;
+  const long long side = 2ll * semi_edge_steps;
+  return side * side * side;
+}
+
+extern "C" long long dsx_so3_grid_workspace_ints(int semi_edge_steps) {
+  const long long n = dsx_so3_grid_candidates(semi_edge_steps);
+  if (n < 0) return -1;
+  const long long nb = (n + kGridThreads - 1) / kGridThreads;
+  return ((nb + 1) & ~1ll) + 2 * nb;
+}
+
+extern "C" int dsx_so3_grid(int semi_edge_steps, int mode, int n_ops, const double *ops,
+                            double half_angle, const double *center, double *euler_d,
+                            long long capacity, long long *count_d, int *work_d,
+                            long long work_ints, void *stream) {
+  clear_error();
+  const long long n_cand = dsx_so3_grid_candidates(semi_edge_steps);
+  if (n_cand < 0) {
+    set_error("dsx_so3_grid: semi-edge steps %d outside 1..%d", semi_edge_steps, kSo3MaxSteps);
+    return DSX_EINVAL;
+  }
+  if ((mode != DSX_SO3_FUNDAMENTAL_ZONE && mode != DSX_SO3_LOCAL) || n_ops < 0 || n_ops > 24 ||
+      (n_ops > 0 && !ops) || capacity < 0 || (euler_d == nullptr && capacity > 0)) {
+    set_error("dsx_so3_grid: bad mode %d, group of %d rotations or capacity %lld", mode, n_ops,
+              capacity);
+    return DSX_EINVAL;
+  }
+  const long long nb = (n_cand + kGridThreads - 1) / kGridThreads;
+  if (!count_d || !work_d || work_ints < dsx_so3_grid_workspace_ints(semi_edge_steps)) {
+    set_error("dsx_so3_grid: null pointer or workspace of %lld ints < %lld", work_ints,
+              dsx_so3_grid_workspace_ints(semi_edge_steps));
+    return DSX_EINVAL;
+  }
+  So3Args a;
+  a.n = semi_edge_steps;
+  a.side = 2ll * semi_edge_steps;
+  a.n_cand = n_cand;
+  const double semi = 0.5 * pow(M_PI, 2.0 / 3.0);
+  a.step = semi / semi_edge_steps;
+  a.sc = pow(M_PI / 6.0, 1.0 / 6.0);
+  const double beta = 0.5 * pow(M_PI, 5.0 / 6.0) / pow(6.0, 1.0 / 6.0);
+  a.prek = pow(0.75 * M_PI, 1.0 / 3.0) * pow(2.0, 0.25) / beta;
+  a.pref = sqrt(6.0 / M_PI);
+  a.sqrt2 = sqrt(2.0);
+  a.pi12 = M_PI / 12.0;
+  a.sqrtpi = sqrt(M_PI);
+  a.sqrt24 = sqrt(24.0);
+  a.mode = mode;
+  a.n_ops = n_ops;
+  for (int s = 0; s < n_ops; ++s)
+    for (int k = 0; k < 4; ++k) a.ops[s][k] = ops[4 * s + k];
+  a.half = half_angle;
+  a.has_center = center != nullptr;
+  for (int k = 0; k < 4; ++k) a.center[k] = center ? center[k] : 0.0;
+  a.euler = euler_d;
+  a.capacity = euler_d ? capacity : 0;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(so3_grid_kernel<0>, dim3(static_cast<unsigned>(nb)), dim3(kGridThreads), 0, s,
+                     a, work_d);
+  hipLaunchKernelGGL(so3_scan_kernel, dim3(1), dim3(1024), 0, s, work_d, nb, count_d);
+  if (euler_d)
+    hipLaunchKernelGGL(so3_grid_kernel<1>, dim3(static_cast<unsigned>(nb)), dim3(kGridThreads), 0,
+                       s, a, work_d);
+  return check_launch("dsx_so3_grid");
 }

@@ -14,6 +14,13 @@ refinement level comes from ``dsx_icosahedral_nn_dots``);
 mesh draws its normal deviates from numpy's generator on the host, as the
 reference does (the draws are the definition of that mesh), and is cropped
 and converted on the device (``dsx_beam_directions_from_vertices``).
+
+The SO(3) grids of the same module (rotation_list_generators.py:58-173),
+which the reference samples with orix, are built on the device too
+(``dsx_so3_grid``): the cubochoric grid (Rosca, Morawiec & De Graef 2014)
+cropped to a fundamental zone or to a ball about the identity, as Bunge
+angles.  ``get_grid_around_beam_direction`` composes a few hundred
+quaternions at most and stays on the host.
 """
 from __future__ import annotations
 
@@ -256,3 +263,131 @@ def get_beam_directions_grid(crystal_system: str, resolution: float,
     """Drop-in for the reference function: the device grid copied to a numpy
     array (N, 3) of Bunge angles in degrees."""
     return beam_directions_grid_device(crystal_system, resolution, mesh).cpu().numpy()
+
+
+# ---- SO(3) grids (rotation_list_generators.py:58-173) ------------------------
+
+_CUBO_SEMI_EDGE = 0.5 * np.pi ** (2.0 / 3.0)
+
+
+def cubochoric_steps(resolution: float) -> int:
+    """Semi-edge steps of the cubochoric grid for a neighbour distance in
+    degrees: a step a/n of the cube is 2 (6/pi)^(1/3) a / n radians of
+    rotation next to the identity, and n is the smallest count for which
+    that does not exceed the resolution."""
+    rad = np.deg2rad(float(resolution))
+    if not rad > 0:
+        raise ValueError(f"resolution must be positive, got {resolution}")
+    n = max(1, int(np.ceil(2.0 * (6.0 / np.pi) ** (1.0 / 3.0) * _CUBO_SEMI_EDGE / rad - 1e-9)))
+    if n > 1024:
+        raise ValueError(f"resolution {resolution} deg needs {n} > 1024 semi-edge steps")
+    return n
+
+
+def _quat_from_bunge_deg(euler_deg) -> np.ndarray:
+    """orix Rotation.from_euler of Bunge angles in degrees."""
+    from .rotation import Rotation
+
+    return Rotation.from_euler(np.deg2rad(np.asarray(euler_deg, dtype=np.float64).reshape(3))).data[0]
+
+
+def _center_quaternion(center):
+    """get_local_grid's centre: a Bunge tuple in degrees, or an object
+    holding unit quaternions in ``.data`` (an orix Rotation)."""
+    if center is None:
+        return None
+    if isinstance(center, tuple):
+        return _quat_from_bunge_deg(center)
+    data = getattr(center, "data", None)
+    if data is not None and np.asarray(data).reshape(-1, 4).shape[0] == 1:
+        return np.asarray(data, dtype=np.float64).reshape(4)
+    raise TypeError("center must be a tuple of Bunge angles in degrees or a single rotation "
+                    "with unit quaternions in .data")
+
+
+def _so3_grid_device(n: int, mode: int, ops, half: float, center, device):
+    import torch
+
+    from . import engine
+
+    L = _lib.load(require_gpu=True)
+    dev = engine._device(device)
+    ops = np.ascontiguousarray(np.asarray(ops, dtype=np.float64).reshape(-1, 4))
+    ops_arr = (ctypes.c_double * max(1, ops.size))(*ops.ravel().tolist())
+    c_arr = None if center is None else (ctypes.c_double * 4)(*np.asarray(center, float).tolist())
+    work = torch.empty(int(L.dsx_so3_grid_workspace_ints(n)), dtype=torch.int32, device=dev)
+    count = torch.zeros(1, dtype=torch.int64, device=dev)
+    stream = _lib.stream_handle(dev)
+    _lib.check(L.dsx_so3_grid(n, mode, ops.shape[0], ops_arr, half, c_arr, None, 0,
+                              _lib.ptr(count), _lib.ptr(work), work.numel(), stream),
+               "dsx_so3_grid")
+    m = int(count.item())
+    out = torch.empty((max(m, 1), 3), dtype=torch.float64, device=dev)
+    _lib.check(L.dsx_so3_grid(n, mode, ops.shape[0], ops_arr, half, c_arr, _lib.ptr(out), m,
+                              _lib.ptr(count), _lib.ptr(work), work.numel(), stream),
+               "dsx_so3_grid")
+    return out[:m]
+
+
+def fundamental_zone_grid_device(resolution: float = 2, space_group: int = None, device=None):
+    """Rotations of the cubochoric grid inside the fundamental zone of the
+    space group's proper point group (orix get_sample_fundamental) as Bunge
+    angles in radians, a float64 torch tensor [n, 3] on the device."""
+    from . import symmetry
+
+    if space_group is None:
+        raise ValueError("get_fundamental_zone_grid needs a space group number: the "
+                         "fundamental zone is taken from space_group alone")
+    ops = symmetry.rotation_group_quaternions(symmetry.rotation_group_of_space_group(space_group))
+    return _so3_grid_device(cubochoric_steps(resolution), _lib.SO3_FUNDAMENTAL_ZONE, ops[1:],
+                            0.0, None, device)
+
+
+def local_grid_device(resolution: float = 2, center=None, grid_width: float = 10, device=None):
+    """Rotations of the cubochoric grid by less than grid_width degrees,
+    premultiplied by the centre rotation (orix get_sample_local), as Bunge
+    angles in radians, a float64 torch tensor [n, 3] on the device."""
+    half = float(np.deg2rad(grid_width / 2))
+    return _so3_grid_device(cubochoric_steps(resolution), _lib.SO3_LOCAL, np.zeros((0, 4)), half,
+                            _center_quaternion(center), device)
+
+
+def get_list_from_orix(grid, rounding: int = 2):
+    """get_list_from_orix (rotation_list_generators.py:58-82): Bunge angles
+    in degrees, rounded to `rounding` decimals, one tuple per rotation.
+    ``grid`` is anything with ``to_euler()`` (radians) or an array / tensor
+    [n, 3] of Bunge angles in radians."""
+    if hasattr(grid, "to_euler"):
+        z = grid.to_euler()
+    elif hasattr(grid, "detach"):
+        z = grid.detach().cpu().numpy()
+    else:
+        z = grid
+    rows = np.asarray(z, dtype=np.float64).reshape(-1, 3).tolist()
+    return [tuple(np.round(np.rad2deg(r), decimals=rounding)) for r in rows]
+
+
+def get_fundamental_zone_grid(resolution=2, point_group=None, space_group=None):
+    """Drop-in for get_fundamental_zone_grid (rotation_list_generators.py:
+    85-106): a list of Bunge tuples (degrees, 2 decimals) inside the
+    fundamental zone.  As in the reference, only ``space_group`` selects the
+    zone (``point_group`` is accepted and not used)."""
+    return get_list_from_orix(fundamental_zone_grid_device(resolution, space_group), rounding=2)
+
+
+def get_local_grid(resolution=2, center=None, grid_width=10):
+    """Drop-in for get_local_grid (rotation_list_generators.py:109-134)."""
+    return get_list_from_orix(local_grid_device(resolution, center, grid_width), rounding=2)
+
+
+def get_grid_around_beam_direction(beam_rotation, resolution, angular_range=(0, 360)):
+    """Drop-in for get_grid_around_beam_direction (rotation_list_generators.py:
+    137-173): the beam rotation (Bunge degrees) times rotations about z by
+    arange(start, stop, resolution) degrees, as Bunge tuples.  A few hundred
+    rotations: host FP64."""
+    from .rotation import Rotation
+
+    beam = Rotation.from_euler(np.deg2rad(np.asarray(beam_rotation, dtype=np.float64)))
+    ang = np.deg2rad(np.arange(start=angular_range[0], stop=angular_range[1], step=resolution))
+    axes = np.repeat([[0, 0, 1]], ang.shape[0], axis=0)
+    return get_list_from_orix(beam * Rotation.from_axes_angles(axes, ang), rounding=2)

@@ -15,7 +15,7 @@ from __future__ import annotations
 
 import numpy as np
 
-__all__ = ["Rotation", "uniform_so3_grid"]
+__all__ = ["Rotation", "bunge_from_quaternions", "uniform_so3_grid"]
 
 
 def _qmul(p, q):
@@ -72,6 +72,35 @@ def _from_matrix(M):
         q = np.asarray(q)
         out[i] = -q if q[0] < 0 else q
     return out.reshape(M.shape[:-2] + (4,))
+
+
+def bunge_from_quaternions(q) -> np.ndarray:
+    """Bunge angles (radians) of quaternions [..., 4]: Rowenhorst et al.,
+    Modelling Simul. Mater. Sci. Eng. 23 (2015) 083501, eq. A.14 with P = +1,
+    with its Phi = 0 and Phi = pi branches; negative phi1 / phi2 are moved
+    into [0, 2 pi) and a negative Phi into [0, pi].  Returns [n, 3]."""
+    q = np.asarray(q, dtype=np.float64).reshape(-1, 4)
+    q = q / np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    q03 = w * w + z * z
+    q12 = x * x + y * y
+    chi = np.sqrt(q03 * q12)
+    out = np.zeros((q.shape[0], 3))
+    flat = (chi == 0) & (q12 == 0)
+    flip = (chi == 0) & (q03 == 0) & ~flat
+    gen = ~(flat | flip)
+    out[flat, 0] = np.arctan2(-2.0 * w[flat] * z[flat], w[flat] ** 2 - z[flat] ** 2)
+    out[flip, 0] = np.arctan2(2.0 * x[flip] * y[flip], x[flip] ** 2 - y[flip] ** 2)
+    out[flip, 1] = np.pi
+    c = chi[gen]
+    W, X, Y, Z = w[gen], x[gen], y[gen], z[gen]
+    out[gen, 0] = np.arctan2((X * Z - W * Y) / c, (-W * X - Y * Z) / c)
+    out[gen, 1] = np.arctan2(2.0 * c, q03[gen] - q12[gen])
+    out[gen, 2] = np.arctan2((W * Y + X * Z) / c, (Y * Z - W * X) / c)
+    for k, period in ((0, 2 * np.pi), (1, np.pi), (2, 2 * np.pi)):
+        neg = out[:, k] < 0
+        out[neg, k] = np.mod(out[neg, k] + 100 * np.pi, period)
+    return out
 
 
 class Rotation:
@@ -164,17 +193,9 @@ class Rotation:
         return _matrix(self.data)
 
     def to_euler(self, degrees: bool = False) -> np.ndarray:
-        """Bunge angles of the passive matrix (inverse of from_euler)."""
-        M = self.to_matrix().reshape(-1, 3, 3)
-        P = np.arccos(np.clip(M[:, 2, 2], -1, 1))
-        p1 = np.arctan2(M[:, 2, 0], -M[:, 2, 1])
-        p2 = np.arctan2(M[:, 0, 2], M[:, 1, 2])
-        small = np.abs(np.sin(P)) < 1e-12
-        p1 = np.where(small, np.arctan2(M[:, 0, 1], M[:, 0, 0]), p1)
-        p2 = np.where(small, 0.0, p2)
-        e = np.mod(np.stack([p1, P, p2], -1), 2 * np.pi)
-        e[:, 1] = P
-        e = e.reshape(self.shape + (3,))
+        """Bunge angles (inverse of from_euler): Rowenhorst et al. (2015)
+        eq. A.14, as orix computes them."""
+        e = bunge_from_quaternions(self.data).reshape(self.shape + (3,))
         return np.rad2deg(e) if degrees else e
 
     def __invert__(self):

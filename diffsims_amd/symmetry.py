@@ -120,3 +120,78 @@ def symmetry_families(xyz: np.ndarray, point_group: str):
     first = first[::-1]
     mult = np.array([np.unique(orb[i], axis=0).shape[0] for i in first], dtype=np.int64)
     return order[first], mult
+
+
+# ---- proper rotation groups for SO(3) fundamental zones ----------------------
+#
+# orix's fundamental zones (get_sample_fundamental, used by the reference's
+# get_fundamental_zone_grid) are those of the 11 proper point groups, with
+# n-fold axes along z, two-fold axes along x and three-folds along <111>.
+
+_ROTATION_GROUP_GENERATORS = {
+    "1": [], "2": [((0, 0, 1), 2)], "222": [((0, 0, 1), 2), ((1, 0, 0), 2)],
+    "4": [((0, 0, 1), 4)], "422": [((0, 0, 1), 4), ((1, 0, 0), 2)],
+    "3": [((0, 0, 1), 3)], "32": [((0, 0, 1), 3), ((1, 0, 0), 2)],
+    "6": [((0, 0, 1), 6)], "622": [((0, 0, 1), 6), ((1, 0, 0), 2)],
+    "23": [((0, 0, 1), 2), ((1, 0, 0), 2), ((1, 1, 1), 3)],
+    "432": [((0, 0, 1), 4), ((1, 1, 1), 3)],
+}
+
+# rotation group of each point group's Laue class
+_LAUE_ROTATIONS = {
+    "1": "1", "-1": "1", "2": "2", "m": "2", "2/m": "2", "222": "222", "mm2": "222",
+    "mmm": "222", "4": "4", "-4": "4", "4/m": "4", "422": "422", "4mm": "422", "-42m": "422",
+    "4/mmm": "422", "3": "3", "-3": "3", "32": "32", "3m": "32", "-3m": "32", "6": "6",
+    "-6": "6", "6/m": "6", "622": "622", "6mm": "622", "-6m2": "622", "6/mmm": "622",
+    "23": "23", "m-3": "23", "432": "432", "-43m": "432", "m-3m": "432",
+}
+
+
+def rotation_group_of_space_group(number: int) -> str:
+    """The proper point group whose fundamental zone a space group's
+    orientations are sampled in: the rotations of its Laue class."""
+    return _LAUE_ROTATIONS[point_group_of_space_group(number)]
+
+
+def _qmul(p, q):
+    a1, b1, c1, d1 = p
+    a2, b2, c2, d2 = q
+    return np.array([a1 * a2 - b1 * b2 - c1 * c2 - d1 * d2, a1 * b2 + b1 * a2 + c1 * d2 - d1 * c2,
+                     a1 * c2 - b1 * d2 + c1 * a2 + d1 * b2, a1 * d2 + b1 * c2 - c1 * b2 + d1 * a2])
+
+
+@lru_cache(maxsize=None)
+def _rotation_group(name: str) -> tuple:
+    if name not in _ROTATION_GROUP_GENERATORS:
+        raise ValueError(f"unknown proper point group {name!r}")
+    gens = []
+    for axis, n in _ROTATION_GROUP_GENERATORS[name]:
+        u = np.asarray(axis, dtype=np.float64)
+        u = u / np.linalg.norm(u)
+        gens.append(np.concatenate([[np.cos(np.pi / n)], np.sin(np.pi / n) * u]))
+
+    def canon(q):  # q and -q are one rotation
+        nz = np.flatnonzero(np.abs(q) > 1e-9)
+        return tuple(np.round(q if q[nz[0]] > 0 else -q, 9))
+
+    elems = [np.array([1.0, 0.0, 0.0, 0.0])]
+    seen = {canon(elems[0])}
+    frontier = list(elems)
+    while frontier:
+        nxt = []
+        for a in frontier:
+            for g in gens:
+                m = _qmul(g, a)
+                k = canon(m)
+                if k not in seen:
+                    seen.add(k)
+                    elems.append(m)
+                    nxt.append(m)
+        frontier = nxt
+    return tuple(elems)
+
+
+def rotation_group_quaternions(name: str) -> np.ndarray:
+    """The rotations of a proper point group as unit quaternions (order, 4),
+    the identity first."""
+    return np.array(_rotation_group(str(name)))

@@ -505,6 +505,31 @@ int dsx_beam_directions_from_vertices(const double *xyz_d, long long n, int n_pl
                                       void *stream);
 
 /*
+ * SO(3) rotation grids — replace get_fundamental_zone_grid and get_local_grid
+ * (diffsims/generators/rotation_list_generators.py:85-134), whose sampling the
+ * reference takes from orix (get_sample_fundamental / get_sample_local on the
+ * cubochoric grid of Rosca, Morawiec & De Graef 2014).  The grid has
+ * (2 semi_edge_steps)^3 points i a / n, i = -n+1 .. n, a = pi^(2/3) / 2,
+ * x outermost; each becomes a unit quaternion (w >= 0) and is kept
+ *   DSX_SO3_FUNDAMENTAL_ZONE: if |w| - max_s |q . s| >= -1e-12 over the n_ops
+ *     rotations s of ops (quaternions, [n_ops][4], identity excluded, <= 24);
+ *   DSX_SO3_LOCAL: if acos(w) < half_angle; the kept quaternion is then
+ *     premultiplied by *center ([4], or NULL for none).
+ * Kept rows are written in grid order as Bunge angles (radians, double
+ * [capacity][3]; Rowenhorst et al. 2015 eq. A.14 with P = +1).  *count_d gets
+ * the number of kept points; with euler_d NULL (capacity 0) only the count is
+ * formed.  work_d: dsx_so3_grid_workspace_ints(semi_edge_steps) ints.
+ * semi_edge_steps is 1..1024; dsx_so3_grid_candidates returns -1 outside.
+ */
+#define DSX_SO3_FUNDAMENTAL_ZONE 0
+#define DSX_SO3_LOCAL 1
+long long dsx_so3_grid_candidates(int semi_edge_steps);
+long long dsx_so3_grid_workspace_ints(int semi_edge_steps);
+int dsx_so3_grid(int semi_edge_steps, int mode, int n_ops, const double *ops, double half_angle,
+                 const double *center, double *euler_d, long long capacity, long long *count_d,
+                 int *work_d, long long work_ints, void *stream);
+
+/*
  * Polar templates from spot lists — replaces Simulation2D.polar_flatten_simulations
  * (simulation2d.py:313-355) with DiffractingVector.to_flat_polar
  * (_diffracting_vector.py:186-194) and get_closest (simulation2d.py:767-781).

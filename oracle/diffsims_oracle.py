@@ -48,12 +48,23 @@ Parity pins (what fixes each function):
                              icosahedron = arctan(2) (63.435 deg); level n+1 finer
   random mesh                10313 points at 1 deg, seeds 7/7/8
                              (test_sphere_mesh_generators.py:33-45)
+  cubochoric SO(3) grids     cube -> ball map: Jacobian 1, surface onto the ball,
+                             faces / edges / corners on their directions,
+                             continuous and odd (planted constant error fails);
+                             the homochoric angle's equation; Euler <-> quaternion
+                             round trip and the passive Bunge matrix; group
+                             closure, the zone tiling the grid, Mackenzie's
+                             maximal disorientations; beam-direction grid KAT
+                             (test_rotation_list_generator.py:47-54) and closed form
 Parity unpinned (no reference test and no closed form fixes it):
   * the order of the Cartesian sort for oblique cells when rounding could tie
     (exact arithmetic gives descending lexicographic (h,k,l); see hkl_box);
   * orix's Euler/quaternion branch handling beyond the golden case;
   * the d == d_min equality case of the hkl box;
-  * the custom-callable shape-model count (test_custom_shape_func expects 52).
+  * the custom-callable shape-model count (test_custom_shape_func expects 52);
+  * orix's resolution -> cubochoric steps rule, grid loop order, zone boundary
+    tolerance and space group -> proper group table (cubochoric_steps,
+    proper_group_of_space_group).
 """
 
 from __future__ import annotations
@@ -1133,3 +1144,290 @@ def legacy_profile(ph: PhaseSpec, p: SimParams, minimum_intensity=1e-3, hexagona
             y.append(i)
             keep.append(lab)
     return np.array(x), np.asarray(y) / max(y) * 100, keep
+
+
+# ---------------------------------------------------------------------------
+# SO(3) rotation grids: get_fundamental_zone_grid / get_local_grid /
+# get_grid_around_beam_direction / get_list_from_orix
+# (diffsims/generators/rotation_list_generators.py:58-173).  The reference
+# samples SO(3) with orix (absent here) on the cubochoric grid of Rosca,
+# Morawiec & De Graef, Modelling Simul. Mater. Sci. Eng. 22 (2014) 075013;
+# Euler <-> quaternion conversions follow Rowenhorst et al., Modelling Simul.
+# Mater. Sci. Eng. 23 (2015) 083501, with P = +1.
+# ---------------------------------------------------------------------------
+
+CUBO_SEMI_EDGE = 0.5 * np.pi ** (2.0 / 3.0)   # cube of volume pi^2
+HOMO_RADIUS = (0.75 * np.pi) ** (1.0 / 3.0)   # ball of volume pi^2
+
+
+def cubochoric_to_homochoric(cu) -> np.ndarray:
+    """Volume-preserving map of the cube [-a, a]^3, a = pi^(2/3) / 2, onto
+    the homochoric ball of radius (3 pi / 4)^(1/3) (Rosca et al. 2014): the
+    pyramid of the dominant axis (z, then x, then y on ties) is rotated onto
+    +-z; the square cross-section at height Z is mapped area-preservingly onto
+    the Lambert-azimuthal preimage of the curved square (angle pi/12 * Y/X),
+    then lifted onto the sphere of radius sqrt(6/pi) Z."""
+    cu = np.asarray(cu, dtype=np.float64).reshape(-1, 3)
+    a = np.abs(cu)
+    pz = np.maximum(a[:, 0], a[:, 1]) <= a[:, 2]
+    px = ~pz & (np.maximum(a[:, 2], a[:, 1]) <= a[:, 0])
+    py = ~pz & ~px
+    s = cu.copy()
+    s[px] = cu[px][:, [1, 2, 0]]
+    s[py] = cu[py][:, [2, 0, 1]]
+    sc = (np.pi / 6.0) ** (1.0 / 6.0)
+    beta = 0.5 * np.pi ** (5.0 / 6.0) / 6.0 ** (1.0 / 6.0)
+    prek = HOMO_RADIUS * 2.0 ** 0.25 / beta
+    pref = np.sqrt(6.0 / np.pi)
+    X, Y, Z = (sc * s).T
+    out = np.zeros_like(s)
+    on_axis = (X == 0) & (Y == 0)
+    out[on_axis, 2] = pref * Z[on_axis]
+    g = ~on_axis
+    Xg, Yg, Zg = X[g], Y[g], Z[g]
+    ylex = np.abs(Yg) <= np.abs(Xg)
+    num = np.where(ylex, Yg, Xg)
+    den = np.where(ylex, Xg, Yg)
+    t = np.pi / 12.0 * num / den
+    c, sn = np.cos(t), np.sin(t)
+    q = prek * den / np.sqrt(np.sqrt(2.0) - c)
+    u = (np.sqrt(2.0) * c - 1.0) * q   # along the dominant in-plane axis
+    v = np.sqrt(2.0) * sn * q
+    T1 = np.where(ylex, u, v)
+    T2 = np.where(ylex, v, u)
+    rho2 = T1 * T1 + T2 * T2
+    lift = np.sqrt(1.0 - np.pi * rho2 / (24.0 * Zg * Zg))
+    out[g, 0] = T1 * lift
+    out[g, 1] = T2 * lift
+    out[g, 2] = pref * Zg - np.sqrt(np.pi) * rho2 / np.sqrt(24.0) / Zg
+    res = out.copy()
+    res[px] = out[px][:, [2, 0, 1]]
+    res[py] = out[py][:, [1, 2, 0]]
+    return res
+
+
+def _omega_minus_sin(w):
+    """w - sin(w) without cancellation for small w (Taylor series below 0.5)."""
+    w = np.asarray(w, dtype=np.float64)
+    w2 = w * w
+    series = w * w2 / 6.0 * (1.0 - w2 / 20.0 * (1.0 - w2 / 42.0 * (1.0 - w2 / 72.0 * (
+        1.0 - w2 / 110.0 * (1.0 - w2 / 156.0)))))
+    return np.where(w < 0.5, series, w - np.sin(w))
+
+
+NEWTON_STEPS = 6
+
+
+def homochoric_to_axis_angle(ho):
+    """Unit axis and angle of homochoric vectors: |h|^3 = (3/4)(w - sin w),
+    solved by NEWTON_STEPS steps of Newton's method from w0 = 2|h| (below the
+    root; the function is convex on [0, pi], so the iteration then converges
+    from above: four steps reach the last bit from any start on [0, pi])."""
+    ho = np.asarray(ho, dtype=np.float64).reshape(-1, 3)
+    hn = np.linalg.norm(ho, axis=1)
+    t = (4.0 / 3.0) * (hn * hn * hn)
+    w = np.minimum(2.0 * hn, np.pi)
+    for _ in range(NEWTON_STEPS):
+        f = _omega_minus_sin(w) - t
+        d = 2.0 * np.sin(0.5 * w) ** 2
+        step = np.where(d > 0, f / np.where(d > 0, d, 1.0), 0.0)
+        w = np.clip(w - step, 0.0, np.pi)
+    axis = np.zeros_like(ho)
+    nz = hn > 0
+    axis[nz] = ho[nz] / hn[nz, None]
+    axis[~nz] = (0.0, 0.0, 1.0)
+    return axis, w
+
+
+def quat_from_axis_angle(axis, w) -> np.ndarray:
+    axis = np.asarray(axis, dtype=np.float64).reshape(-1, 3)
+    w = np.asarray(w, dtype=np.float64).reshape(-1)
+    return np.concatenate([np.cos(0.5 * w)[:, None], np.sin(0.5 * w)[:, None] * axis], axis=1)
+
+
+def quat_multiply(p, q) -> np.ndarray:
+    """Hamilton product p q (rotation q first, then p)."""
+    p = np.asarray(p, dtype=np.float64)
+    q = np.asarray(q, dtype=np.float64)
+    a1, b1, c1, d1 = np.moveaxis(p, -1, 0)
+    a2, b2, c2, d2 = np.moveaxis(q, -1, 0)
+    return np.stack([a1 * a2 - b1 * b2 - c1 * c2 - d1 * d2,
+                     a1 * b2 + b1 * a2 + c1 * d2 - d1 * c2,
+                     a1 * c2 - b1 * d2 + c1 * a2 + d1 * b2,
+                     a1 * d2 + b1 * c2 - c1 * b2 + d1 * a2], axis=-1)
+
+
+def bunge_from_quat(q) -> np.ndarray:
+    """Bunge angles (radians) of quaternions, Rowenhorst et al. (2015) eq.
+    A.14 with P = +1, including its two degenerate branches (Phi = 0 and
+    Phi = pi); negative phi1 / phi2 are moved into [0, 2 pi), Phi into
+    [0, pi]: what orix ``Rotation.to_euler`` returns."""
+    q = np.asarray(q, dtype=np.float64).reshape(-1, 4)
+    q = q / np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    q03 = w * w + z * z
+    q12 = x * x + y * y
+    chi = np.sqrt(q03 * q12)
+    out = np.zeros((q.shape[0], 3))
+    m1 = (chi == 0) & (q12 == 0)
+    m2 = (chi == 0) & (q03 == 0) & ~m1
+    m3 = ~(m1 | m2)
+    out[m1, 0] = np.arctan2(-2.0 * w[m1] * z[m1], w[m1] ** 2 - z[m1] ** 2)
+    out[m2, 0] = np.arctan2(2.0 * x[m2] * y[m2], x[m2] ** 2 - y[m2] ** 2)
+    out[m2, 1] = np.pi
+    cm = chi[m3]
+    W, X, Y, Z = w[m3], x[m3], y[m3], z[m3]
+    out[m3, 0] = np.arctan2((X * Z - W * Y) / cm, (-W * X - Y * Z) / cm)
+    out[m3, 1] = np.arctan2(2.0 * cm, q03[m3] - q12[m3])
+    out[m3, 2] = np.arctan2((W * Y + X * Z) / cm, (Y * Z - W * X) / cm)
+    for k, period in ((0, 2 * np.pi), (1, np.pi), (2, 2 * np.pi)):
+        neg = out[:, k] < 0
+        out[neg, k] = np.mod(out[neg, k] + 100 * np.pi, period)
+    return out
+
+
+def cubochoric_steps(resolution: float) -> int:
+    """Semi-edge steps N for a requested neighbour distance (degrees): the
+    grid spacing a/N maps to 2 (6/pi)^(1/3) a / N radians of rotation next
+    to the identity; N is the smallest count whose spacing there does not
+    exceed the resolution.  (orix's own resolution -> N rule is not in the
+    reference and is parity unpinned.)"""
+    rad = np.deg2rad(float(resolution))
+    if not rad > 0:
+        raise ValueError(f"resolution must be positive, got {resolution}")
+    return max(1, int(np.ceil(2.0 * (6.0 / np.pi) ** (1.0 / 3.0) * CUBO_SEMI_EDGE / rad - 1e-9)))
+
+
+def cubochoric_quaternions(n: int) -> np.ndarray:
+    """The cubochoric grid with n semi-edge steps: coordinates i a/n for
+    i = -n+1 .. n on each axis (the -a faces are the +a faces' antipodes),
+    x outermost and z innermost, as unit quaternions with w >= 0."""
+    ax = np.arange(-n + 1, n + 1, dtype=np.float64) * (CUBO_SEMI_EDGE / n)
+    X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
+    cu = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    axis, w = homochoric_to_axis_angle(cubochoric_to_homochoric(cu))
+    return quat_from_axis_angle(axis, w)
+
+
+def _axis_rot(axis, n):
+    axis = np.asarray(axis, dtype=np.float64) / np.linalg.norm(axis)
+    return quat_from_axis_angle(axis, 2 * np.pi / n)[0]
+
+
+# generators of the 11 proper point groups, axes as orix sets them: n-fold
+# axes along z, two-fold axes along x, three-folds along <111>
+_PROPER_GENERATORS = {
+    "1": [], "2": [(0, 0, 1, 2)], "222": [(0, 0, 1, 2), (1, 0, 0, 2)],
+    "4": [(0, 0, 1, 4)], "422": [(0, 0, 1, 4), (1, 0, 0, 2)],
+    "3": [(0, 0, 1, 3)], "32": [(0, 0, 1, 3), (1, 0, 0, 2)],
+    "6": [(0, 0, 1, 6)], "622": [(0, 0, 1, 6), (1, 0, 0, 2)],
+    "23": [(0, 0, 1, 2), (1, 0, 0, 2), (1, 1, 1, 3)], "432": [(0, 0, 1, 4), (1, 1, 1, 3)],
+}
+
+# point group -> rotation group of its Laue class
+_LAUE_PROPER = {"1": "1", "-1": "1", "2": "2", "m": "2", "2/m": "2", "222": "222",
+                "mm2": "222", "mmm": "222", "4": "4", "-4": "4", "4/m": "4", "422": "422",
+                "4mm": "422", "-42m": "422", "4/mmm": "422", "3": "3", "-3": "3", "32": "32",
+                "3m": "32", "-3m": "32", "6": "6", "-6": "6", "6/m": "6", "622": "622",
+                "6mm": "622", "-6m2": "622", "6/mmm": "622", "23": "23", "m-3": "23",
+                "432": "432", "-43m": "432", "m-3m": "432"}
+
+
+def proper_group_quaternions(name: str) -> np.ndarray:
+    """All rotations of a proper point group as quaternions (w >= 0), by
+    closure of the generators."""
+    gens = [_axis_rot(g[:3], g[3]) for g in _PROPER_GENERATORS[name]]
+    elems = [np.array([1.0, 0.0, 0.0, 0.0])]
+    frontier = list(elems)
+
+    def key(q):
+        q = q if (q[0] > 1e-9 or (abs(q[0]) <= 1e-9 and tuple(np.round(q[1:], 9)) > (0, 0, 0))) else -q
+        return tuple(np.round(q, 9))
+
+    seen = {key(elems[0])}
+    while frontier:
+        nxt = []
+        for a in frontier:
+            for g in gens:
+                m = quat_multiply(g, a)
+                k = key(m)
+                if k not in seen:
+                    seen.add(k)
+                    elems.append(m)
+                    nxt.append(m)
+        frontier = nxt
+    return np.array(elems)
+
+
+def proper_group_of_space_group(number: int) -> str:
+    from_sg = [(1, "1"), (2, "-1"), (5, "2"), (9, "m"), (15, "2/m"), (24, "222"), (46, "mm2"),
+               (74, "mmm"), (80, "4"), (82, "-4"), (88, "4/m"), (98, "422"), (110, "4mm"),
+               (122, "-42m"), (142, "4/mmm"), (146, "3"), (148, "-3"), (155, "32"),
+               (161, "3m"), (167, "-3m"), (173, "6"), (174, "-6"), (176, "6/m"), (182, "622"),
+               (186, "6mm"), (190, "-6m2"), (194, "6/mmm"), (199, "23"), (206, "m-3"),
+               (214, "432"), (220, "-43m"), (230, "m-3m")]
+    n = int(number)
+    if not 1 <= n <= 230:
+        raise ValueError(f"space group number {number} is not in 1..230")
+    for last, pg in from_sg:
+        if n <= last:
+            return _LAUE_PROPER[pg]
+    raise AssertionError
+
+
+FZ_EPS = 1e-12
+
+
+def fundamental_zone_margin(q, ops) -> np.ndarray:
+    """|w| - max_s |q . s| over the group's non-identity rotations: >= 0 when
+    q is the member of its class q s closest to the identity (the
+    fundamental zone of orix's OrientationRegion, whose faces are the
+    bisectors between the identity and each s)."""
+    q = np.asarray(q, dtype=np.float64).reshape(-1, 4)
+    others = [s for s in ops if not np.allclose(np.abs(s), [1, 0, 0, 0])]
+    if not others:
+        return np.full(q.shape[0], np.inf)
+    d = np.abs(q @ np.array(others).T).max(axis=1)
+    return np.abs(q[:, 0]) - d
+
+
+def fundamental_zone_grid(resolution: float, space_group: int) -> np.ndarray:
+    """get_fundamental_zone_grid (rotation_list_generators.py:85-106) before
+    the rounding of get_list_from_orix: Bunge angles (radians) of the
+    cubochoric grid rows inside the fundamental zone of the space group's
+    proper point group (faces inclusive by FZ_EPS)."""
+    q = cubochoric_quaternions(cubochoric_steps(resolution))
+    ops = proper_group_quaternions(proper_group_of_space_group(space_group))
+    keep = fundamental_zone_margin(q, ops) >= -FZ_EPS
+    return bunge_from_quat(q[keep])
+
+
+def local_grid(resolution: float, center=None, grid_width: float = 10.0) -> np.ndarray:
+    """get_local_grid (rotation_list_generators.py:109-134): cubochoric rows
+    whose rotation angle is below grid_width (arccos(w) < grid_width / 2),
+    premultiplied by the centre rotation (Bunge degrees) when one is given."""
+    q = cubochoric_quaternions(cubochoric_steps(resolution))
+    half = np.deg2rad(grid_width / 2.0)
+    q = q[np.arccos(q[:, 0]) < half]
+    if center is not None:
+        c = quat_from_bunge(np.deg2rad(np.asarray(center, dtype=np.float64)))
+        q = quat_multiply(c[None, :], q)
+    return bunge_from_quat(q)
+
+
+def grid_around_beam_direction(beam_rotation, resolution, angular_range=(0, 360)) -> np.ndarray:
+    """get_grid_around_beam_direction (rotation_list_generators.py:137-173):
+    the beam rotation (Bunge degrees) times rotations about z by
+    arange(start, stop, resolution) degrees."""
+    b = quat_from_bunge(np.deg2rad(np.asarray(beam_rotation, dtype=np.float64)))
+    ang = np.deg2rad(np.arange(start=angular_range[0], stop=angular_range[1], step=resolution))
+    axes = np.repeat([[0.0, 0.0, 1.0]], ang.shape[0], axis=0)
+    q = quat_multiply(b[None, :], quat_from_axis_angle(axes, ang))
+    return bunge_from_quat(q)
+
+
+def rotation_list(euler_rad, rounding: int = 2):
+    """get_list_from_orix (rotation_list_generators.py:58-82): rows in degrees
+    rounded to `rounding` decimals, as tuples."""
+    return [tuple(np.round(np.rad2deg(r), decimals=rounding))
+            for r in np.asarray(euler_rad, dtype=np.float64).reshape(-1, 3).tolist()]

@@ -134,3 +134,95 @@ def test_vectors_to_euler_kat_on_the_device():
     v = v / np.linalg.norm(v, axis=1)[:, None]
     want = np.array([[0, 90, 90], [0, 90, 0], [0, 45, 0], [0, 45, 90]], dtype=np.float64)
     assert np.allclose(generators.beam_directions_grid_to_euler(v), want)
+
+
+# -- SO(3) cubochoric grids (rotation_list_generators.py:85-134) ---------------
+
+def _walk_match(got, want, ambiguous):
+    """Device rows against oracle rows, both in grid order: every oracle row
+    is matched in order except rows within the fundamental-zone tolerance
+    band, which either side may keep.  Angles compared modulo 2 pi."""
+    i = 0
+    for k in range(want.shape[0]):
+        if i < got.shape[0]:
+            d = (got[i] - want[k] + np.pi) % (2 * np.pi) - np.pi
+            if np.abs(d).max() <= 1e-9:
+                i += 1
+                continue
+        assert ambiguous[k], (k, want[k], got[i] if i < got.shape[0] else None)
+    assert i == got.shape[0]
+
+
+@pytest.mark.parametrize("space_group", [1, 4, 20, 75, 89, 143, 150, 168, 177, 195, 207, 225])
+def test_fundamental_zone_grid_matches_oracle(space_group):
+    from diffsims_amd import generators, symmetry
+
+    for res in (15.0, 7.0):
+        n = O.cubochoric_steps(res)
+        q = O.cubochoric_quaternions(n)
+        ops = O.proper_group_quaternions(O.proper_group_of_space_group(space_group))
+        margin = O.fundamental_zone_margin(q, ops)
+        keep = margin >= -O.FZ_EPS
+        want = O.bunge_from_quat(q[keep])
+        amb = np.abs(margin[keep] + O.FZ_EPS) < 1e-13
+        got = generators.fundamental_zone_grid_device(res, space_group).cpu().numpy()
+        print(f"sg {space_group} res {res}: {got.shape[0]} rows, {int(amb.sum())} on the boundary",
+              flush=True)
+        assert symmetry.rotation_group_of_space_group(space_group) == \
+            O.proper_group_of_space_group(space_group)
+        _walk_match(got, want, amb)
+
+
+def test_local_grid_matches_oracle():
+    from diffsims_amd import generators
+
+    for res, center, width in ((5.0, None, 20.0), (3.0, (12.0, 34.0, 56.0), 15.0),
+                               (30.0, (0, 1, 0), 35.0)):
+        want = O.local_grid(res, center, width)
+        got = generators.local_grid_device(res, center, width).cpu().numpy()
+        assert got.shape == want.shape, (res, center, width)
+        d = (got - want + np.pi) % (2 * np.pi) - np.pi
+        assert np.abs(d).max() <= 1e-9
+
+
+def test_reference_grid_calls_on_the_device():
+    """test_rotation_list_generator.py:30-44 through the drop-in functions,
+    against the oracle's rounded lists (entries may differ by one unit of
+    the last decimal where the FP64 angle sits on a rounding tie)."""
+    from diffsims_amd import generators
+
+    for got, want in ((generators.get_local_grid(resolution=30, center=(0, 1, 0), grid_width=35),
+                       O.rotation_list(O.local_grid(30, (0, 1, 0), 35))),
+                      (generators.get_fundamental_zone_grid(space_group=20, resolution=20),
+                       O.rotation_list(O.fundamental_zone_grid(20, 20)))):
+        assert isinstance(got, list) and len(got) > 0 and isinstance(got[0], tuple)
+        assert len(got) == len(want)
+        d = (np.array(got) - np.array(want) + 180.0) % 360.0 - 180.0
+        assert np.abs(d).max() <= 0.01 + 1e-9
+    with pytest.raises(ValueError):
+        generators.get_fundamental_zone_grid(resolution=20)
+
+
+def test_fine_fundamental_zone_on_the_device():
+    """0.5 degree cubic zone: 610^3 grid points, 1/24 of them kept (up to the
+    boundary rows), every kept rotation within Mackenzie's 62.8 degrees."""
+    import time
+
+    from diffsims_amd import generators
+
+    n = generators.cubochoric_steps(0.5)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    e = generators.fundamental_zone_grid_device(0.5, 225)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    total = (2 * n) ** 3
+    print(f"cubic zone at 0.5 deg: {e.shape[0]} of {total} grid points in {dt * 1e3:.1f} ms",
+          flush=True)
+    assert abs(e.shape[0] * 24 / total - 1) < 0.01
+    # the angle of each kept rotation from its passive Bunge matrix
+    sub = e[:: max(1, e.shape[0] // 200000)]
+    M = torch.from_numpy(O.bunge_matrix(sub.cpu().numpy()))
+    cosw = (M.diagonal(dim1=-2, dim2=-1).sum(-1) - 1) / 2
+    ang = torch.rad2deg(torch.arccos(cosw.clamp(-1, 1)))
+    assert float(ang.max()) <= 62.7994 + 1e-6
