@@ -214,6 +214,7 @@ __device__ __forceinline__ void strip_pass(const AdjArgs &a, const SpotTab &t, i
       const float cy = t.Y[q];
       const float amp = t.I[q] * a.pref;
       aacc_t *col = acc + x;
+      lds_reads_done();
       for (int y = ya; y <= yb; ++y) {
         const float dy = static_cast<float>(y) - cy;
         atomicAdd(col + (y - y0) * W, gauss_term(amp, dx, dy, a.ep, scale));
@@ -248,15 +249,25 @@ __device__ __forceinline__ void strip_pass(const AdjArgs &a, const SpotTab &t, i
         float k = a.pref * expf(a.ep * (dy * dy + dx * dx));
         float r = expf(a.ep * (2.0f * dy + 1.0f));
         const float *g = G + (ya - y0) * W + x;
-        for (int y = ya; y <= yb; ++y) {
-          const float kg = k * *g;
+        auto row = [&](float gv) {
+          const float kg = k * gv;
           sa += kg;
           sy = fmaf(kg, dy, sy);
           k *= r;
           r *= a.e2ep;
           dy += 1.0f;
-          g += W;
+        };
+        // four rows' G words are read together, so one LDS round trip
+        // serves four rows
+        int y = ya;
+        for (; y + 3 <= yb; y += 4, g += 4 * W) {
+          const float g0 = g[0], g1 = g[W], g2 = g[2 * W], g3 = g[3 * W];
+          row(g0);
+          row(g1);
+          row(g2);
+          row(g3);
         }
+        for (; y <= yb; ++y, g += W) row(*g);
         sx = sa * dx;
       }
     }
@@ -505,11 +516,16 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
   if (kLarge && tid < 9) sh.drec[tid] = 0.0;
   // large-pattern workspace of this block: float4 records[entries] (gathered
   // sums of each (spot, strip) bin entry and its spot) | int bins[entries] |
-  // int2 boxes[cap] (clip boxes cached by the first scan)
+  // int2 boxes[cap] (clip boxes cached by the first scan) | int q[H W] (the
+  // fixed-point pattern between the splat and the read-out when there is no
+  // pattern buffer to hold it)
   int *wblock = kLarge && a.work ? a.work + (size_t)blockIdx.x * a.work_per_block : nullptr;
   float4 *wrec = reinterpret_cast<float4 *>(wblock);
   int *wbins = wblock ? wblock + 4 * static_cast<size_t>(a.work_entries) : nullptr;
   int2 *wbox = wblock ? reinterpret_cast<int2 *>(wbins + a.work_entries) : nullptr;
+  int *wq = wblock ? wblock + ((5 * static_cast<size_t>(a.work_entries) + 2 * static_cast<size_t>(a.cap) + 3) &
+                               ~static_cast<size_t>(3))
+                   : nullptr;
   for (int i = tid; i < a.acc_words; i += kAdjThreads) s_acc[i] = 0;
   lds_sync();
 
@@ -677,13 +693,10 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
         return;
       }
       const int e0 = sh.bin_start[s], e1 = sh.bin_start[s + 1];
-      const bool single = e1 - e0 <= a.slots;
       for (int c0 = e0; c0 < e1; c0 += a.slots) {
         const int m = min(a.slots, e1 - c0);
-        if (!(gather && single)) {
-          load_chunk(a, t, spots, wbins, wbox, c0, m, tid);
-          lds_sync();
-        }
+        load_chunk(a, t, spots, wbins, wbox, c0, m, tid);
+        lds_sync();
         if (gather) {
           strip_pass<true>(a, t, cols_max, s_acc, s_G, 0, m, false, y0, y1, scale, tid);
           lds_sync();
@@ -709,7 +722,60 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
       for (int ti = tid; ti < n_tiles; ti += kAdjThreads) tile_u[ti] = 0;
       lds_sync();
     }
-    if (a.normalize && ns_all > 0 && !have_key) {
+    // A large pattern is splatted once, into the strip buffer (zero again
+    // from here): each strip's fixed-point values go to q (the pattern buffer
+    // when there is one, else the block's workspace) while the maximum key is
+    // formed, and pass B reads them back.  (Small patterns find the maximum
+    // from the tile bounds and splat in pass B.)
+    int *q = large ? (a.pattern ? reinterpret_cast<int *>(a.pattern + (size_t)p * HW) : wq)
+                   : nullptr;
+    if (large) {
+      for (int s = 0; s < a.n_strips; ++s) {
+        const int y0 = s * a.strip_rows;
+        const int y1 = min(H, y0 + a.strip_rows);
+        if (sh.bin_start[s + 1] == sh.bin_start[s]) continue;
+        strip_spots(s, false, y0, y1);
+        lds_sync();
+        const int cnt = (y1 - y0) * W;
+        const int pix0 = y0 * W;
+        auto take = [&](int v, int pix) {
+          const long long kv = (static_cast<long long>(max(v, 0)) << kPixBits) |
+                               static_cast<long long>((1 << kPixBits) - 1 - pix);
+          key = kv > key ? kv : key;
+        };
+        if ((W & 3) == 0) {
+          for (int i = tid; i < cnt / 4; i += kAdjThreads) {
+            int4 *pv = reinterpret_cast<int4 *>(s_acc) + i;
+            const int4 u = *pv;
+            *pv = make_int4(0, 0, 0, 0);
+            reinterpret_cast<int4 *>(q + pix0)[i] = u;
+            if (a.normalize) {
+              take(u.x, pix0 + 4 * i);
+              take(u.y, pix0 + 4 * i + 1);
+              take(u.z, pix0 + 4 * i + 2);
+              take(u.w, pix0 + 4 * i + 3);
+            }
+          }
+        } else {
+          for (int i = tid; i < cnt; i += kAdjThreads) {
+            const aacc_t v = s_acc[i];
+            s_acc[i] = 0;
+            q[pix0 + i] = v;
+            if (a.normalize) take(v, pix0 + i);
+          }
+        }
+        lds_sync();
+      }
+      if (a.normalize) {
+        key = wave_max_acc(key);
+        if (lane == 0) sh.wkey[wid] = key;
+        lds_sync();
+        key = 0;
+        for (int w = 0; w < kAdjWaves; ++w) key = sh.wkey[w] > key ? sh.wkey[w] : key;
+      }
+      have_key = true;
+    }
+    if (a.normalize && ns_all > 0 && !have_key && !large) {
       for (int s = 0; s < a.n_strips; ++s) {
         const int y0 = s * a.strip_rows;
         const int y1 = min(H, y0 + a.strip_rows);
@@ -762,8 +828,61 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
         if (i < cnt4) tp[k] = __builtin_nontemporal_load(src + i);
       }
     };
+    if (large) {
+      __syncthreads();  // q complete (each thread reads back what it wrote)
+      for (int s = 0; s < a.n_strips; ++s) {
+        const int y0 = s * a.strip_rows;
+        const int y1 = min(H, y0 + a.strip_rows);
+        const int cnt = (y1 - y0) * W;
+        const bool any = sh.bin_start[s + 1] > sh.bin_start[s];
+        const size_t off = (size_t)y0 * W;
+        // one pixel: pattern, loss and G / m (q may be the pattern buffer: it
+        // is read before the same thread overwrites it)
+        auto pixel = [&](float P, float tv, float &g) {
+          if (l2) {
+            const float d = P - tv;
+            lsum += static_cast<double>(d * d);
+            g = 2.0f * d;
+          } else {
+            g = tv;
+          }
+          csum += static_cast<double>(g * P);
+        };
+        if ((W & 3) == 0) {
+          for (int i = tid; i < cnt / 4; i += kAdjThreads) {
+            const int4 u = any ? reinterpret_cast<const int4 *>(q + off)[i] : make_int4(0, 0, 0, 0);
+            const float4 P = make_float4(static_cast<float>(u.x) * out_scale, static_cast<float>(u.y) * out_scale,
+                                         static_cast<float>(u.z) * out_scale, static_cast<float>(u.w) * out_scale);
+            if (pat) store_nt(P, reinterpret_cast<float4 *>(pat + off) + i);
+            if (fwd_only) continue;
+            const native_float4 tv =
+                __builtin_nontemporal_load(reinterpret_cast<const native_float4 *>(tgt + off) + i);
+            float4 g;
+            pixel(P.x, tv.x, g.x);
+            pixel(P.y, tv.y, g.y);
+            pixel(P.z, tv.z, g.z);
+            pixel(P.w, tv.w, g.w);
+            reinterpret_cast<float4 *>(s_G)[i] =
+                make_float4(g.x * gscale, g.y * gscale, g.z * gscale, g.w * gscale);
+          }
+        } else {
+          for (int i = tid; i < cnt; i += kAdjThreads) {
+            const float P = static_cast<float>(any ? q[off + i] : 0) * out_scale;
+            if (pat) __builtin_nontemporal_store(P, pat + off + i);
+            if (fwd_only) continue;
+            float g;
+            pixel(P, tgt[off + i], g);
+            s_G[i] = g * gscale;
+          }
+        }
+        if (fwd_only) continue;
+        lds_sync();
+        if (a.want_grad && any) strip_spots(s, true, y0, y1);
+        lds_sync();
+      }
+    }
     prefetch(0);
-    for (int s = 0; s < a.n_strips; ++s) {
+    for (int s = 0; s < a.n_strips && !large; ++s) {
       const int y0 = s * a.strip_rows;
       const int y1 = min(H, y0 + a.strip_rows);
       const int cnt = (y1 - y0) * W;
@@ -1185,9 +1304,10 @@ __global__ __launch_bounds__(kSfaThreads) void structure_factor_adjoint_kernel(S
 using namespace dsx;
 
 // Workspace of the large-pattern kernel, per block (E = kWorkEntriesPerSpot *
-// cap bin entries): float4 records[E] | int bins[E] | int2 boxes[cap], padded
-// to 16 bytes.  Of the small-pattern kernel with gradients: float4 gsum[n *
-// cap] | int gcount[n] | (8-byte aligned) double dpart[chain blocks][9].
+// cap bin entries): float4 records[E] | int bins[E] | int2 boxes[cap] | int
+// q[H W], each part padded to 16 bytes.  Of the small-pattern kernel with
+// gradients: float4 gsum[n * cap] | int gcount[n] | (8-byte aligned) double
+// dpart[chain blocks][9].
 constexpr int kWorkEntriesPerSpot = 4;
 constexpr int kAdjLdsBytes = 160 * 1024;
 
@@ -1196,14 +1316,17 @@ static int chain_blocks_max() { return cu_count() * 8; }
 static long long small_work_ints(int cap, int n) {
   return 4ll * n * cap + n + 1 + 18ll * chain_blocks_max();
 }
-static long long large_block_ints(int cap) {
+static long long large_block_ints(int cap, long long hw) {
   const long long e = kWorkEntriesPerSpot * static_cast<long long>(cap);
-  return (5 * e + 2ll * cap + 3) & ~3ll;
+  return ((5 * e + 2ll * cap + 3) & ~3ll) + ((hw + 3) & ~3ll);
 }
+static int large_blocks(int n) { return std::min(n, cu_count()); }
 
-extern "C" long long dsx_render_adjoint_workspace_ints(int cap, int n) {
-  if (cap < 1 || n < 0) return 0;
-  if (adjoint_large(cap)) return static_cast<long long>(cu_count()) * large_block_ints(cap);
+extern "C" long long dsx_render_adjoint_workspace_ints(int cap, int n, int height, int width) {
+  if (cap < 1 || n < 0 || height < 1 || width < 1) return 0;
+  if (adjoint_large(cap))
+    return static_cast<long long>(large_blocks(n)) *
+           large_block_ints(cap, static_cast<long long>(height) * width);
   return small_work_ints(cap, n);
 }
 
@@ -1358,9 +1481,9 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   a.strip_rows = rows;
   a.n_strips = n_strips;
   a.acc_words = rows * a.W;
-  const int blocks = std::min(n, cu_count() * bpc);
+  const int blocks = large ? large_blocks(n) : std::min(n, cu_count() * bpc);
   if (large) {
-    const long long per_block = large_block_ints(cap);
+    const long long per_block = large_block_ints(cap, static_cast<long long>(a.H) * a.W);
     if (work_d && reinterpret_cast<uintptr_t>(work_d) % 16 != 0) {
       set_error("dsx_render_adjoint: workspace must be 16-byte aligned");
       return DSX_EINVAL;
@@ -1457,11 +1580,11 @@ extern "C" int dsx_render_adjoint_noalloc(const int *idx_d, const float *spot_d,
                                           float *dF2_d, double *drec_d, int *status_d,
                                           void *stream) {
   clear_error();
-  if (cap < 1 || n < 0) {
+  if (cap < 1 || n < 0 || !rp) {
     set_error("dsx_render_adjoint_noalloc: bad arguments (n=%d cap=%d)", n, cap);
     return DSX_EINVAL;
   }
-  const long long need = dsx_render_adjoint_workspace_ints(cap, n);
+  const long long need = dsx_render_adjoint_workspace_ints(cap, n, rp->height, rp->width);
   if (need > kNoallocScratchInts) {
     set_error("dsx_render_adjoint_noalloc: %d patterns with capacity %d need %lld ints of "
               "scratch; the static scratch holds %lld (use dsx_render_adjoint with a workspace, "

@@ -38,6 +38,16 @@ __device__ __forceinline__ void lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Wait until every LDS read the wave has issued has returned (s_waitcnt
+// lgkmcnt(0); the empty asm keeps the reads above it).  Called before a loop
+// of LDS atomics whose operands were read from LDS ahead of the loop: the
+// compiler otherwise places that wait inside the loop, where lgkmcnt(0) also
+// waits for each previous iteration's atomic, one LDS round trip per term.
+__device__ __forceinline__ void lds_reads_done() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // vmcnt 63, expcnt 7, lgkmcnt 0 (no wait on the others)
+}
+
 // exp(x) through the hardware base-2 exponential: one multiply and
 // v_exp_f32 (the library expf adds range reduction, ~13 instructions).  The
 // result is within a few 1e-7 of exp(x) for the Gaussian arguments here

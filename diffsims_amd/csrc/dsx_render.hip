@@ -666,6 +666,7 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
             const float amp = I * a.pref;
             const float cy = sh.fy[q];
             racc_t *col = s_acc + x;
+            lds_reads_done();
             for (int y = ya; y <= yb; ++y) {
               const float dy = static_cast<float>(y) - cy;
               racc_add(col + (y - y0) * W, amp * fast_exp(a.ep * (dy * dy + dx * dx)), scale);

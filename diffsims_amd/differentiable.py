@@ -344,8 +344,9 @@ class RefinementModel:
     def workspace(self, cap: int, n: int) -> torch.Tensor:
         """Scratch of dsx_render_adjoint: per-spot sums between its render and
         chain-rule passes, or, for spot lists longer than the render LDS
-        holds, per-block strip bins."""
-        need = max(1, int(self.L.dsx_render_adjoint_workspace_ints(int(cap), int(n))))
+        holds, per-block strip bins and fixed-point pixels."""
+        H, W = self.shape
+        need = max(1, int(self.L.dsx_render_adjoint_workspace_ints(int(cap), int(n), H, W)))
         if getattr(self, "_work", None) is None or self._work.numel() < need:
             self._work = torch.empty(need, dtype=torch.int32, device=self.dev)
         return self._work

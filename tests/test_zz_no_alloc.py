@@ -45,7 +45,7 @@ def _call(model, t, Rt, b, n, target, noalloc):
         _lib.check(L.dsx_render_adjoint_noalloc(*common, _lib.ptr(status), stream),
                    "dsx_render_adjoint_noalloc")
     else:
-        work = torch.empty(max(1, int(L.dsx_render_adjoint_workspace_ints(b.cap, n))),
+        work = torch.empty(max(1, int(L.dsx_render_adjoint_workspace_ints(b.cap, n, H, W))),
                            dtype=torch.int32, device="cuda")
         _lib.check(L.dsx_render_adjoint(*common, _lib.ptr(work), work.numel(), _lib.ptr(status),
                                         stream), "dsx_render_adjoint")
@@ -78,7 +78,8 @@ def test_noalloc_equals_workspace_entry(case):
     b = model.spots(t, Rt)
     from diffsims_amd import _lib
     L = _lib.load()
-    assert int(L.dsx_render_adjoint_workspace_ints(b.cap, sizes[0])) <= \
+    H, W = model.shape
+    assert int(L.dsx_render_adjoint_workspace_ints(b.cap, sizes[0], H, W)) <= \
         int(L.dsx_render_adjoint_noalloc_max_ints())
     if case == "long_lists":
         assert b.cap > int(L.dsx_render_max_spots())
@@ -99,9 +100,10 @@ def test_noalloc_rejects_sizes_beyond_the_scratch():
     assert max_ints == 16 << 20
     cap = 1000
     n = max_ints // (4 * cap) + 16
-    assert int(L.dsx_render_adjoint_workspace_ints(cap, n)) > max_ints
+    assert int(L.dsx_render_adjoint_workspace_ints(cap, n, 64, 64)) > max_ints
     sim = _lib.SimParams()
     rp = _lib.RenderParams()
+    rp.height, rp.width = 64, 64
     with pytest.raises(ValueError, match="static scratch"):
         _lib.check(L.dsx_render_adjoint_noalloc(
             None, None, None, cap, n, None, None, 0, None, ctypes.byref(sim), ctypes.byref(rp),
