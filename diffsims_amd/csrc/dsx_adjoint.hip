@@ -41,6 +41,11 @@ constexpr int kAdjThreads = 512;
 constexpr int kAdjWaves = kAdjThreads / kWave;
 constexpr int kAdjMaxSpots = 1024;   // longest spot list of the two-blocks-per-CU kernel
 constexpr int kAdjMaxStrips = 256;
+// The large-pattern kernel splits its strip bins by clip-box width (<= 4,
+// <= 8, wider columns), so the (spot, column) items of a class run with that
+// class's column count instead of the pattern's widest box.  (Small patterns
+// hold few spots per strip; one pass per strip fills the waves better.)
+constexpr int kWidthClasses = 3;
 constexpr int kAdjBinCap = 4096;
 // large-pattern kernel: 2,048 LDS spot slots and 64 KB strips, so a 512-wide
 // strip's bin usually fits one chunk (loaded once for the splat and reused
@@ -107,9 +112,10 @@ struct AdjArgs {
   int *status;
 };
 
+template <int kCls>  // width classes per strip
 struct AdjShared {
-  int bin_start[kAdjMaxStrips + 1];
-  int bin_fill[kAdjMaxStrips];
+  int bin_start[kCls * kAdjMaxStrips + 1];  // bin b = kCls * strip + width class
+  int bin_fill[kCls * kAdjMaxStrips];
   int ns, cols_max;
   float wmaxf[kAdjWaves];
   long long wkey[kAdjWaves];
@@ -189,6 +195,16 @@ __device__ __forceinline__ int gauss_term(float amp, float dx, float dy, float e
   return __float2int_rn(amp * fast_exp(ep * fmaf(dy, dy, dx * dx)) * scale);
 }
 
+__device__ __forceinline__ int width_class(int w) { return w <= 4 ? 0 : (w <= 8 ? 1 : 2); }
+__device__ __forceinline__ int class_cols(int c, int cols_max) {
+  return c == 0 ? min(4, cols_max) : (c == 1 ? min(8, cols_max) : cols_max);
+}
+__device__ __forceinline__ int box_width(int xb) { return (xb & 0xffff) - (xb >> 16) + 1; }
+// it / ncols, with shifts for the class widths 4 and 8 (ncols is uniform)
+__device__ __forceinline__ int item_spot(int it, int ncols) {
+  return ncols == 4 ? it >> 2 : (ncols == 8 ? it >> 3 : it / ncols);
+}
+
 // Splat (kGather = false) or gather (true) of every binned spot footprint
 // inside strip rows [y0, y1).  Work item = (spot, footprint column) so a wave
 // touches consecutive LDS columns.
@@ -196,14 +212,15 @@ template <bool kGather>
 __device__ __forceinline__ void strip_pass(const AdjArgs &a, const SpotTab &t, int cols_max,
                                            aacc_t *acc, const float *G, int b0, int nb,
                                            bool binned, int y0, int y1, float scale, int tid) {
+  // unbinned: LDS slots b0 .. b0 + nb - 1
   const int W = a.W;
   const int ncols = cols_max;
   const int n_items = nb * ncols;
   if (!kGather) {
     for (int it = tid; it < n_items; it += kAdjThreads) {
-      const int bi = it / ncols;
+      const int bi = item_spot(it, ncols);
       const int c = it - bi * ncols;
-      const int q = binned ? t.bin[b0 + bi] : bi;
+      const int q = binned ? t.bin[b0 + bi] : b0 + bi;
       const int ya = max(static_cast<int>(t.ylo[q]), y0);
       const int yb = min(static_cast<int>(t.yhi[q]), y1 - 1);
       if (ya > yb) continue;
@@ -230,12 +247,12 @@ __device__ __forceinline__ void strip_pass(const AdjArgs &a, const SpotTab &t, i
   const int lane = lane_id();
   for (int base = tid - lane; base < n_items; base += kAdjThreads) {
     const int it = base + lane;
-    const int bi = it / ncols;
+    const int bi = item_spot(it, ncols);
     const int c = it - bi * ncols;
     int q = -1;
     float sa = 0.0f, sx = 0.0f, sy = 0.0f;
     if (it < n_items) {
-      q = binned ? t.bin[b0 + bi] : bi;
+      q = binned ? t.bin[b0 + bi] : b0 + bi;
       const int ya = max(static_cast<int>(t.ylo[q]), y0);
       const int yb = min(static_cast<int>(t.yhi[q]), y1 - 1);
       const int xlo = t.xb[q] >> 16, xhi = t.xb[q] & 0xffff;
@@ -409,7 +426,8 @@ __device__ __forceinline__ int spot_at(const AdjArgs &a, const SpotTab &t, int q
 // remaining tile, so the first of them is found.  Returns false (uniformly)
 // when too many tiles or spots remain; the caller then runs the full pass.
 // tile_u is overwritten (candidate slots); cand_q borrows the gradient buffer.
-__device__ bool bounded_max_key(const AdjArgs &a, AdjShared &sh, const SpotTab &t, int *tile_u,
+template <class Shared>
+__device__ bool bounded_max_key(const AdjArgs &a, Shared &sh, const SpotTab &t, int *tile_u,
                                 short (*cand_q)[kAdjCandSpots], int ntx, int n_tiles, int ns,
                                 float scale, float unit, int tid, int lane, int wid,
                                 long long &key_out) {
@@ -503,8 +521,10 @@ __device__ bool bounded_max_key(const AdjArgs &a, AdjShared &sh, const SpotTab &
 // in-frame spots than the slots (their bins and sums live in the per-block
 // workspace) and the chain rule runs here, per pattern.
 template <bool kLarge>
-__device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, AdjShared &sh,
+__device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
+                                             AdjShared<kLarge ? kWidthClasses : 1> &sh,
                                              ChainConsts &s_cc) {
+  constexpr int kCls = kLarge ? kWidthClasses : 1;
   float *s_G = reinterpret_cast<float *>(s_acc + a.acc_words);
   const SpotTab t = spot_tab(s_acc, a.acc_words, a.slots);
   const int tid = threadIdx.x;
@@ -538,7 +558,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
       sh.qstar = 0;
     }
     if (kLarge && tid < 9 && a.want_grad) sh.R[tid] = a.Rt[(size_t)p * 9 + tid];
-    for (int s = tid; s <= kAdjMaxStrips; s += kAdjThreads) sh.bin_start[s] = 0;
+    for (int b = tid; b <= kCls * kAdjMaxStrips; b += kAdjThreads) sh.bin_start[b] = 0;
     lds_sync();
 
     // ---- 1. spots in frame with their clip boxes --------------------------
@@ -556,7 +576,9 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
       if (!in) continue;
       tmax = fmaxf(tmax, fabsf(sp.w) * a.pref);
       atomicMax(&sh.cols_max, xhi - xlo + 1);
-      for (int s = lo / a.strip_rows; s <= hi / a.strip_rows; ++s) atomicAdd(&sh.bin_start[s + 1], 1);
+      const int wc = kCls == 1 ? 0 : width_class(xhi - xlo + 1);
+      for (int s = lo / a.strip_rows; s <= hi / a.strip_rows; ++s)
+        atomicAdd(&sh.bin_start[kCls * s + wc + 1], 1);
       const int q = atomicAdd(&sh.ns, 1);
       if (kLarge && q >= a.slots) continue;
       t.I[q] = sp.w;
@@ -582,15 +604,16 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
     float scale = 1.0f;  // set from the tile bounds below
 
     // ---- 2. strip bins ----------------------------------------------------
+    const int n_bins = kCls * a.n_strips;
     if (tid == 0) {
       int acc = 0;
-      for (int s = 0; s < a.n_strips; ++s) {
-        acc += sh.bin_start[s + 1];
-        sh.bin_start[s + 1] = acc;
+      for (int b = 0; b < n_bins; ++b) {
+        acc += sh.bin_start[b + 1];
+        sh.bin_start[b + 1] = acc;
       }
     }
     lds_sync();
-    const int entries = sh.bin_start[a.n_strips];
+    const int entries = sh.bin_start[n_bins];
     // large: more in-frame spots than LDS slots; their bins live in the
     // caller's workspace and the spots pass through LDS in chunks
     const bool large = kLarge && ns_all > a.slots;
@@ -598,7 +621,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
     if (tid == 0 && a.status) a.status[p] = fits ? 0 : 1;
     const int ns = large ? 0 : ns_all;  // spots held in LDS for the whole pattern
     const bool binned = !large && entries <= a.bin_cap;
-    for (int s = tid; s < a.n_strips; s += kAdjThreads) sh.bin_fill[s] = sh.bin_start[s];
+    for (int b = tid; b < n_bins; b += kAdjThreads) sh.bin_fill[b] = sh.bin_start[b];
     lds_sync();
     if (kLarge && !fits) {
       // no room: the pattern is reported in status and left as zeros
@@ -611,17 +634,19 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
     }
     if (large) {
       for (int j = tid; j < n; j += kAdjThreads) {
-        const int yb = wbox[j].y;
-        const int lo = yb >> 16, hi = yb & 0xffff;
+        const int2 bx = wbox[j];
+        const int lo = bx.y >> 16, hi = bx.y & 0xffff;
         if (lo > hi) continue;  // not in frame
+        const int wc = kCls == 1 ? 0 : width_class(box_width(bx.x));
         for (int s = lo / a.strip_rows; s <= hi / a.strip_rows; ++s)
-          wbins[atomicAdd(&sh.bin_fill[s], 1)] = j;
+          wbins[atomicAdd(&sh.bin_fill[kCls * s + wc], 1)] = j;
       }
       __syncthreads();  // bins in global memory, block-visible
     } else if (binned) {
       for (int q = tid; q < ns; q += kAdjThreads) {
+        const int wc = kCls == 1 ? 0 : width_class(box_width(t.xb[q]));
         for (int s = t.ylo[q] / a.strip_rows; s <= t.yhi[q] / a.strip_rows; ++s) {
-          const int pos = atomicAdd(&sh.bin_fill[s], 1);
+          const int pos = atomicAdd(&sh.bin_fill[kCls * s + wc], 1);
           t.bin[pos] = static_cast<short>(q);
         }
       }
@@ -681,30 +706,47 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
     // one chunk is gathered from the slots its splat just loaded.  Large
     // patterns store each entry's gathered sums in the workspace record of
     // the entry (no read-modify-write; every entry belongs to one strip).
+    auto pass_range = [&](bool gather, int ncols, int b0, int nb, bool bin, int y0, int y1) {
+      if (nb <= 0) return;
+      if (gather)
+        strip_pass<true>(a, t, ncols, s_acc, s_G, b0, nb, bin, y0, y1, scale, tid);
+      else
+        strip_pass<false>(a, t, ncols, s_acc, nullptr, b0, nb, bin, y0, y1, scale, tid);
+    };
     auto strip_spots = [&](int s, bool gather, int y0, int y1) {
+      const int bs = kCls * s;
       if (!large) {
-        const int b0 = binned ? sh.bin_start[s] : 0;
-        const int nb = binned ? sh.bin_start[s + 1] - b0 : ns;
-        if (nb == 0) return;
-        if (gather)
-          strip_pass<true>(a, t, cols_max, s_acc, s_G, b0, nb, binned, y0, y1, scale, tid);
+        // few spots per strip: one pass over the strip's classes together
+        // fills the waves better than a pass per class
+        if (binned)
+          pass_range(gather, cols_max, sh.bin_start[bs],
+                     sh.bin_start[bs + kCls] - sh.bin_start[bs], true, y0, y1);
         else
-          strip_pass<false>(a, t, cols_max, s_acc, nullptr, b0, nb, binned, y0, y1, scale, tid);
+          pass_range(gather, cols_max, 0, ns, false, y0, y1);
         return;
       }
-      const int e0 = sh.bin_start[s], e1 = sh.bin_start[s + 1];
+      const int e0 = sh.bin_start[bs], e1 = sh.bin_start[bs + kCls];
       for (int c0 = e0; c0 < e1; c0 += a.slots) {
         const int m = min(a.slots, e1 - c0);
         load_chunk(a, t, spots, wbins, wbox, c0, m, tid);
         lds_sync();
+        // the chunk's entries in slots 0..m-1, ordered by width class
+        auto classes = [&](bool g) {
+          for (int c = 0; c < kCls; ++c) {
+            const int lo = max(c0, sh.bin_start[bs + c]);
+            const int hi = min(c0 + m, sh.bin_start[bs + c + 1]);
+            pass_range(g, kCls == 1 ? cols_max : class_cols(c, cols_max), lo - c0, hi - lo, false,
+                       y0, y1);
+          }
+        };
         if (gather) {
-          strip_pass<true>(a, t, cols_max, s_acc, s_G, 0, m, false, y0, y1, scale, tid);
+          classes(true);
           lds_sync();
           for (int i = tid; i < m; i += kAdjThreads)
             wrec[c0 + i] = make_float4(t.A[i], t.BX[i], t.BY[i], __int_as_float(t.jpos[i]));
           lds_sync();  // the next chunk overwrites the slots
         } else {
-          strip_pass<false>(a, t, cols_max, s_acc, nullptr, 0, m, false, y0, y1, scale, tid);
+          classes(false);
           lds_sync();  // the next chunk overwrites the slots
         }
       }
@@ -733,7 +775,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
       for (int s = 0; s < a.n_strips; ++s) {
         const int y0 = s * a.strip_rows;
         const int y1 = min(H, y0 + a.strip_rows);
-        if (sh.bin_start[s + 1] == sh.bin_start[s]) continue;
+        if (sh.bin_start[kCls * (s + 1)] == sh.bin_start[kCls * s]) continue;
         strip_spots(s, false, y0, y1);
         lds_sync();
         const int cnt = (y1 - y0) * W;
@@ -779,7 +821,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
       for (int s = 0; s < a.n_strips; ++s) {
         const int y0 = s * a.strip_rows;
         const int y1 = min(H, y0 + a.strip_rows);
-        if (sh.bin_start[s + 1] == sh.bin_start[s]) continue;
+        if (sh.bin_start[kCls * (s + 1)] == sh.bin_start[kCls * s]) continue;
         strip_spots(s, false, y0, y1);
         lds_sync();
         const int cnt = (y1 - y0) * W;
@@ -834,7 +876,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
         const int y0 = s * a.strip_rows;
         const int y1 = min(H, y0 + a.strip_rows);
         const int cnt = (y1 - y0) * W;
-        const bool any = sh.bin_start[s + 1] > sh.bin_start[s];
+        const bool any = sh.bin_start[kCls * (s + 1)] > sh.bin_start[kCls * s];
         const size_t off = (size_t)y0 * W;
         // one pixel: pattern, loss and G / m (q may be the pattern buffer: it
         // is read before the same thread overwrites it)
@@ -886,7 +928,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
       const int y0 = s * a.strip_rows;
       const int y1 = min(H, y0 + a.strip_rows);
       const int cnt = (y1 - y0) * W;
-      const bool any = sh.bin_start[s + 1] > sh.bin_start[s];
+      const bool any = sh.bin_start[kCls * (s + 1)] > sh.bin_start[kCls * s];
       if (any) {
         strip_spots(s, false, y0, y1);
         lds_sync();
@@ -1058,14 +1100,14 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc, Ad
 // Two blocks per CU: at most 128 VGPRs (four waves per SIMD).
 __global__ __launch_bounds__(kAdjThreads, 4) void render_adjoint_kernel(AdjArgs a) {
   extern __shared__ __align__(16) aacc_t s_acc[];  // 16-byte aligned for the 16-byte read-out
-  __shared__ AdjShared sh;
+  __shared__ AdjShared<1> sh;
   __shared__ ChainConsts s_cc;  // unused here
   adjoint_body<false>(a, s_acc, sh, s_cc);
 }
 
 __global__ __launch_bounds__(kAdjThreads) void render_adjoint_large_kernel(AdjArgs a) {
   extern __shared__ __align__(16) aacc_t s_acc[];
-  __shared__ AdjShared sh;
+  __shared__ AdjShared<kWidthClasses> sh;
   __shared__ ChainConsts s_cc;
   adjoint_body<true>(a, s_acc, sh, s_cc);
 }
