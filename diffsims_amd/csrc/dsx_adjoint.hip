@@ -47,11 +47,10 @@ constexpr int kAdjMaxStrips = 256;
 // hold few spots per strip; one pass per strip fills the waves better.)
 constexpr int kWidthClasses = 3;
 constexpr int kAdjBinCap = 4096;
-// large-pattern kernel: 2,048 LDS spot slots and 64 KB strips, so a 512-wide
-// strip's bin usually fits one chunk (loaded once for the splat and reused
-// by the gather)
-constexpr int kAdjLargeSlots = 2048;
+// large-pattern kernel: one 64 KB strip buffer (the splat's fixed-point words
+// in pass 1, G in pass 2) and as many LDS spot slots as the rest holds
 constexpr int kAdjStripBytes = 64 * 1024;
+constexpr int kAdjLargeMaxSlots = 4096;
 constexpr int kPixBits = 22;  // argmax key: value << 22 | (2^22 - 1 - pixel)
 constexpr int kNRed = 20;     // block sums: loss, sum G P, dRt[9], dL/drec[9]
 constexpr int kAdjTile = 16;      // tile edge of the fixed-point bound
@@ -140,8 +139,8 @@ struct SpotTab {
 
 constexpr int kSpotTabBytes = 6 * 4 + 2 * 4 + 2 * 2;  // per slot
 
-__device__ __forceinline__ SpotTab spot_tab(aacc_t *s_acc, int acc_words, int slots) {
-  float *f = reinterpret_cast<float *>(s_acc + 2 * static_cast<size_t>(acc_words));
+__device__ __forceinline__ SpotTab spot_tab(aacc_t *s_acc, size_t strip_words, int slots) {
+  float *f = reinterpret_cast<float *>(s_acc + strip_words);
   SpotTab t;
   t.I = f;
   t.X = f + slots;
@@ -525,8 +524,12 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
                                              AdjShared<kLarge ? kWidthClasses : 1> &sh,
                                              ChainConsts &s_cc) {
   constexpr int kCls = kLarge ? kWidthClasses : 1;
-  float *s_G = reinterpret_cast<float *>(s_acc + a.acc_words);
-  const SpotTab t = spot_tab(s_acc, a.acc_words, a.slots);
+  // The large-pattern kernel has one strip buffer: pass 1 splats into it, and
+  // pass 2 keeps G there while the fixed-point pixels wait in q.  The small
+  // one splats and reads out into G in the same step, so it has two.
+  float *s_G = kLarge ? reinterpret_cast<float *>(s_acc)
+                      : reinterpret_cast<float *>(s_acc + a.acc_words);
+  const SpotTab t = spot_tab(s_acc, (kLarge ? 1 : 2) * static_cast<size_t>(a.acc_words), a.slots);
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int wid = tid / kWave;
@@ -580,7 +583,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
       for (int s = lo / a.strip_rows; s <= hi / a.strip_rows; ++s)
         atomicAdd(&sh.bin_start[kCls * s + wc + 1], 1);
       const int q = atomicAdd(&sh.ns, 1);
-      if (kLarge && q >= a.slots) continue;
+      if (kLarge) continue;  // its chunks are loaded per strip
       t.I[q] = sp.w;
       t.X[q] = X;
       t.Y[q] = Y;
@@ -614,9 +617,9 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
     }
     lds_sync();
     const int entries = sh.bin_start[n_bins];
-    // large: more in-frame spots than LDS slots; their bins live in the
-    // caller's workspace and the spots pass through LDS in chunks
-    const bool large = kLarge && ns_all > a.slots;
+    // large-pattern kernel: the bins live in the caller's workspace and the
+    // spots pass through LDS in chunks
+    const bool large = kLarge;
     const bool fits = !large || (wbins != nullptr && entries <= a.work_entries);
     if (tid == 0 && a.status) a.status[p] = fits ? 0 : 1;
     const int ns = large ? 0 : ns_all;  // spots held in LDS for the whole pattern
@@ -922,6 +925,10 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
         if (a.want_grad && any) strip_spots(s, true, y0, y1);
         lds_sync();
       }
+      // G leaves the shared strip buffer; the next pattern's tile bounds and
+      // splat start from zero (ordered by the barrier that ends the pattern)
+      if (!fwd_only)
+        for (int i = tid; i < a.acc_words; i += kAdjThreads) s_acc[i] = 0;
     }
     prefetch(0);
     for (int s = 0; s < a.n_strips && !large; ++s) {
@@ -1496,19 +1503,31 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
     set_error("dsx_render_adjoint: cannot query or reserve the kernel's LDS");
     return DSX_ELAUNCH;
   }
-  a.slots = large ? kAdjLargeSlots : cap;
-  a.bin_cap = large ? kAdjBinCap : std::min(kAdjBinCap, 4 * cap + 256);
-  const long long table = static_cast<long long>(kSpotTabBytes) * a.slots + 2ll * a.bin_cap + 16;
-  auto rows_for = [&](int per_block) {
-    const long long strip =
-        std::min<long long>(large ? kAdjStripBytes : kAdjSmallStripBytes, per_block - st - table);
-    return strip < 8ll * a.W ? 0 : static_cast<int>(std::min<long long>(a.H, strip / (8ll * a.W)));
-  };
   int bpc = large ? 1 : 2;
-  int rows = rows_for(kAdjLdsBytes / bpc);
-  if (bpc == 2 && rows < std::min(a.H, 8)) {
-    bpc = 1;
-    rows = rows_for(kAdjLdsBytes);
+  int rows = 0;
+  long long table = 0;
+  if (large) {
+    // one strip buffer of up to 64 KB, at least 512 spot slots, the rest slots
+    a.bin_cap = 0;
+    const long long strip =
+        std::min<long long>(kAdjStripBytes, kAdjLdsBytes - st - 16 - 512ll * kSpotTabBytes);
+    rows = strip < 4ll * a.W ? 0 : static_cast<int>(std::min<long long>(a.H, strip / (4ll * a.W)));
+    a.slots = static_cast<int>(std::min<long long>(
+        kAdjLargeMaxSlots, (kAdjLdsBytes - st - 16 - 4ll * rows * a.W) / kSpotTabBytes));
+    table = static_cast<long long>(kSpotTabBytes) * a.slots + 16;
+  } else {
+    a.slots = cap;
+    a.bin_cap = std::min(kAdjBinCap, 4 * cap + 256);
+    table = static_cast<long long>(kSpotTabBytes) * a.slots + 2ll * a.bin_cap + 16;
+    auto rows_for = [&](int per_block) {
+      const long long strip = std::min<long long>(kAdjSmallStripBytes, per_block - st - table);
+      return strip < 8ll * a.W ? 0 : static_cast<int>(std::min<long long>(a.H, strip / (8ll * a.W)));
+    };
+    rows = rows_for(kAdjLdsBytes / bpc);
+    if (rows < std::min(a.H, 8)) {
+      bpc = 1;
+      rows = rows_for(kAdjLdsBytes);
+    }
   }
   if (rows < 1) {
     set_error("dsx_render_adjoint: pattern width %d exceeds the LDS strip budget", a.W);
@@ -1556,7 +1575,8 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
     const size_t off = (static_cast<size_t>(n) * (4ull * cap + 1) + 1) & ~static_cast<size_t>(1);
     a.dpart = reinterpret_cast<double *>(work_d + off);
   }
-  const size_t lds = ((2ull * a.acc_words * 4 + static_cast<size_t>(table)) + 15) & ~static_cast<size_t>(15);
+  const size_t lds = (((large ? 1ull : 2ull) * a.acc_words * 4 + static_cast<size_t>(table)) + 15) &
+                     ~static_cast<size_t>(15);
   hipStream_t s = as_stream(stream);
   if (large) {
     hipLaunchKernelGGL(render_adjoint_large_kernel, dim3(blocks), dim3(kAdjThreads), lds, s, a);
