@@ -1119,6 +1119,25 @@ __global__ __launch_bounds__(kAdjThreads) void render_adjoint_large_kernel(AdjAr
   adjoint_body<true>(a, s_acc, sh, s_cc);
 }
 
+// Forward mode of the large-pattern kernel (the sub-pixel renderer of long
+// spot lists): with the mode and the missing gradient outputs known at
+// compile time the gather and the FP64 chain rule drop out, and the kernel
+// needs about 100 VGPRs instead of 256 and spills, so two blocks share a CU
+// (each with half the LDS): one block's barrier waits overlap the other's work.
+__global__ __launch_bounds__(kAdjThreads, 4) void render_adjoint_large_fwd_kernel(AdjArgs a) {
+  extern __shared__ __align__(16) aacc_t s_acc[];
+  __shared__ AdjShared<kWidthClasses> sh;
+  __shared__ ChainConsts s_cc;
+  a.mode = DSX_ADJ_FORWARD;
+  a.want_grad = 0;
+  a.target = nullptr;
+  a.dRt = nullptr;
+  a.dF2 = nullptr;
+  a.drec = nullptr;
+  a.loss = nullptr;
+  adjoint_body<true>(a, s_acc, sh, s_cc);
+}
+
 // Chain rule of every spot gathered by render_adjoint_kernel: one wave per
 // pattern; dL/dRt of the pattern (a wave sum) and dL/dg, dL/d|F|^2 of the
 // reflections (global atomics).
@@ -1369,7 +1388,8 @@ static long long large_block_ints(int cap, long long hw) {
   const long long e = kWorkEntriesPerSpot * static_cast<long long>(cap);
   return ((5 * e + 2ll * cap + 3) & ~3ll) + ((hw + 3) & ~3ll);
 }
-static int large_blocks(int n) { return std::min(n, cu_count()); }
+// blocks per CU: two for the forward-only kernel, one with gradients
+static int large_blocks(int n, int per_cu = 2) { return std::min(n, per_cu * cu_count()); }
 
 extern "C" long long dsx_render_adjoint_workspace_ints(int cap, int n, int height, int width) {
   if (cap < 1 || n < 0 || height < 1 || width < 1) return 0;
@@ -1494,26 +1514,32 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   // use two blocks per CU when their strips hold at least 8 rows (or the whole
   // pattern); larger lists use the large-pattern kernel, one block per CU.
   const bool large = adjoint_large(cap);
-  static int static_small = -1, static_large = -1, raised_small = 0, raised_large = 0;
-  const void *kern = large ? reinterpret_cast<const void *>(&render_adjoint_large_kernel)
-                           : reinterpret_cast<const void *>(&render_adjoint_kernel);
-  const int st = large ? adjoint_static_lds(kern, &static_large, &raised_large)
-                       : adjoint_static_lds(kern, &static_small, &raised_small);
+  static int static_small = -1, static_large = -1, static_fwd = -1;
+  static int raised_small = 0, raised_large = 0, raised_fwd = 0;
+  const bool fwd_large = large && mode == DSX_ADJ_FORWARD;
+  const void *kern = fwd_large ? reinterpret_cast<const void *>(&render_adjoint_large_fwd_kernel)
+                     : large   ? reinterpret_cast<const void *>(&render_adjoint_large_kernel)
+                               : reinterpret_cast<const void *>(&render_adjoint_kernel);
+  const int st = fwd_large ? adjoint_static_lds(kern, &static_fwd, &raised_fwd)
+                 : large   ? adjoint_static_lds(kern, &static_large, &raised_large)
+                           : adjoint_static_lds(kern, &static_small, &raised_small);
   if (st < 0) {
     set_error("dsx_render_adjoint: cannot query or reserve the kernel's LDS");
     return DSX_ELAUNCH;
   }
-  int bpc = large ? 1 : 2;
+  int bpc = large ? (fwd_large ? 2 : 1) : 2;
   int rows = 0;
   long long table = 0;
   if (large) {
-    // one strip buffer of up to 64 KB, at least 512 spot slots, the rest slots
+    // one strip buffer of up to 64 KB (32 KB with two blocks per CU), at least
+    // 512 spot slots, the rest slots
     a.bin_cap = 0;
-    const long long strip =
-        std::min<long long>(kAdjStripBytes, kAdjLdsBytes - st - 16 - 512ll * kSpotTabBytes);
+    const long long per_block = kAdjLdsBytes / bpc;
+    const long long strip = std::min<long long>(kAdjStripBytes / bpc,
+                                                per_block - st - 16 - 512ll * kSpotTabBytes);
     rows = strip < 4ll * a.W ? 0 : static_cast<int>(std::min<long long>(a.H, strip / (4ll * a.W)));
     a.slots = static_cast<int>(std::min<long long>(
-        kAdjLargeMaxSlots, (kAdjLdsBytes - st - 16 - 4ll * rows * a.W) / kSpotTabBytes));
+        kAdjLargeMaxSlots, (per_block - st - 16 - 4ll * rows * a.W) / kSpotTabBytes));
     table = static_cast<long long>(kSpotTabBytes) * a.slots + 16;
   } else {
     a.slots = cap;
@@ -1542,7 +1568,7 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   a.strip_rows = rows;
   a.n_strips = n_strips;
   a.acc_words = rows * a.W;
-  const int blocks = large ? large_blocks(n) : std::min(n, cu_count() * bpc);
+  const int blocks = large ? large_blocks(n, bpc) : std::min(n, cu_count() * bpc);
   if (large) {
     const long long per_block = large_block_ints(cap, static_cast<long long>(a.H) * a.W);
     if (work_d && reinterpret_cast<uintptr_t>(work_d) % 16 != 0) {
@@ -1579,7 +1605,10 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
                      ~static_cast<size_t>(15);
   hipStream_t s = as_stream(stream);
   if (large) {
-    hipLaunchKernelGGL(render_adjoint_large_kernel, dim3(blocks), dim3(kAdjThreads), lds, s, a);
+    if (fwd_large)
+      hipLaunchKernelGGL(render_adjoint_large_fwd_kernel, dim3(blocks), dim3(kAdjThreads), lds, s, a);
+    else
+      hipLaunchKernelGGL(render_adjoint_large_kernel, dim3(blocks), dim3(kAdjThreads), lds, s, a);
     return check_launch("dsx_render_adjoint");
   }
   hipLaunchKernelGGL(render_adjoint_kernel, dim3(blocks), dim3(kAdjThreads), lds, s, a);
