@@ -194,7 +194,7 @@ _SIGS = {
     "dsx_render_adjoint": (_I, [_P, _P, _P, _I, _I, _P, _P, _I, _P, ctypes.POINTER(SimParams),
                                 ctypes.POINTER(RenderParams), _I, _P, ctypes.c_longlong, _P, _P,
                                 _P, _P, _P, _P, ctypes.c_longlong, _P, _P]),
-    "dsx_render_adjoint_workspace_ints": (ctypes.c_longlong, [_I, _I, _I, _I]),
+    "dsx_render_adjoint_workspace_ints": (ctypes.c_longlong, [_I, _I, _I, _I, _I]),
     "dsx_render_adjoint_noalloc_max_ints": (ctypes.c_longlong, []),
     "dsx_render_adjoint_noalloc": (_I, [_P, _P, _P, _I, _I, _P, _P, _I, _P,
                                         ctypes.POINTER(SimParams), ctypes.POINTER(RenderParams),

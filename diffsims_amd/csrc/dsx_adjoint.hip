@@ -23,8 +23,10 @@
 //   per reflection (atomics) and dL/drec through g (block partials, summed by
 //   sum_partials_kernel in a fixed order).
 // render_adjoint_large_kernel: the same for spot lists longer than the LDS
-//   spot table (their bins live in a per-block workspace and the spots pass
-//   through LDS in chunks), with the chain rule in the kernel.
+//   spot table: their bins live in a per-block workspace and the spots pass
+//   through LDS in chunks; each spot's gathered sums are added into its
+//   position of the pattern's gsum row (float atomics over the strips it
+//   spans), which spot_chain_kernel then reads like the small kernel's.
 // structure_factor_adjoint_kernel (one thread per reflection)
 //   recomputes F(hkl) and reduces dL/d|F|^2 into dL/dxyz and dL/dB per atom
 //   and into dL/d(rec) through |g|.
@@ -123,8 +125,6 @@ struct AdjShared {
   long long lkey;
   int cand_tile[kAdjMaxCand], cand_n[kAdjMaxCand];
   double red[kAdjWaves][kNRed];
-  double R[9];
-  double drec[9];  // large-pattern kernel: this block's dL/drec over its patterns
 };
 
 // The spot table lives in dynamic LDS after the two strip buffers (acc_words
@@ -521,8 +521,7 @@ __device__ bool bounded_max_key(const AdjArgs &a, Shared &sh, const SpotTab &t, 
 // workspace) and the chain rule runs here, per pattern.
 template <bool kLarge>
 __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
-                                             AdjShared<kLarge ? kWidthClasses : 1> &sh,
-                                             ChainConsts &s_cc) {
+                                             AdjShared<kLarge ? kWidthClasses : 1> &sh) {
   constexpr int kCls = kLarge ? kWidthClasses : 1;
   // The large-pattern kernel has one strip buffer: pass 1 splats into it, and
   // pass 2 keeps G there while the fixed-point pixels wait in q.  The small
@@ -535,19 +534,17 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
   const int wid = tid / kWave;
   const int H = a.H, W = a.W;
   const size_t HW = static_cast<size_t>(H) * W;
-  if (kLarge && tid == 0) stage_chain_consts(a, s_cc);
-  if (kLarge && tid < 9) sh.drec[tid] = 0.0;
-  // large-pattern workspace of this block: float4 records[entries] (gathered
-  // sums of each (spot, strip) bin entry and its spot) | int bins[entries] |
-  // int2 boxes[cap] (clip boxes cached by the first scan) | int q[H W] (the
+  // large-pattern workspace of this block: int bins[entries] | int2
+  // boxes[cap] (clip boxes cached by the first scan) | int q[H W] (the
   // fixed-point pattern between the splat and the read-out when there is no
-  // pattern buffer to hold it)
+  // pattern buffer to hold it), each part 16-byte aligned
   int *wblock = kLarge && a.work ? a.work + (size_t)blockIdx.x * a.work_per_block : nullptr;
-  float4 *wrec = reinterpret_cast<float4 *>(wblock);
-  int *wbins = wblock ? wblock + 4 * static_cast<size_t>(a.work_entries) : nullptr;
-  int2 *wbox = wblock ? reinterpret_cast<int2 *>(wbins + a.work_entries) : nullptr;
-  int *wq = wblock ? wblock + ((5 * static_cast<size_t>(a.work_entries) + 2 * static_cast<size_t>(a.cap) + 3) &
-                               ~static_cast<size_t>(3))
+  int *wbins = wblock;
+  int2 *wbox = wblock ? reinterpret_cast<int2 *>(wbins + ((static_cast<size_t>(a.work_entries) + 3) &
+                                                          ~static_cast<size_t>(3)))
+                      : nullptr;
+  int *wq = wblock ? reinterpret_cast<int *>(wbox) + ((2 * static_cast<size_t>(a.cap) + 3) &
+                                                       ~static_cast<size_t>(3))
                    : nullptr;
   for (int i = tid; i < a.acc_words; i += kAdjThreads) s_acc[i] = 0;
   lds_sync();
@@ -560,7 +557,6 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
       sh.cols_max = 1;
       sh.qstar = 0;
     }
-    if (kLarge && tid < 9 && a.want_grad) sh.R[tid] = a.Rt[(size_t)p * 9 + tid];
     for (int b = tid; b <= kCls * kAdjMaxStrips; b += kAdjThreads) sh.bin_start[b] = 0;
     lds_sync();
 
@@ -568,7 +564,10 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
     // every such spot counts in the strip bins and the maximum; the first
     // `slots` also go to LDS (all of them unless the pattern is large)
     float tmax = 0.0f;
+    float4 *gsp = kLarge && a.gsum ? a.gsum + (size_t)p * a.cap : nullptr;
     for (int j = tid; j < n; j += kAdjThreads) {
+      // large kernel with gradients: the spot's gathered sums start at zero
+      if (gsp) gsp[j] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(j));
       const float4 sp = spots[j];
       float X = 0.0f, Y = 0.0f;
       int lo = 1, hi = 0, xlo = 0, xhi = 0;
@@ -745,8 +744,17 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
         if (gather) {
           classes(true);
           lds_sync();
-          for (int i = tid; i < m; i += kAdjThreads)
-            wrec[c0 + i] = make_float4(t.A[i], t.BX[i], t.BY[i], __int_as_float(t.jpos[i]));
+          // the sums of this strip into the spot's gsum entry (a spot's
+          // strips add in turn; the first scan zeroed it behind a barrier)
+          for (int i = tid; i < m; i += kAdjThreads) {
+            const float A = t.A[i], BX = t.BX[i], BY = t.BY[i];
+            if (A != 0.0f || BX != 0.0f || BY != 0.0f) {
+              float *g = reinterpret_cast<float *>(gsp + t.jpos[i]);
+              unsafeAtomicAdd(g, A);
+              unsafeAtomicAdd(g + 1, BX);
+              unsafeAtomicAdd(g + 2, BY);
+            }
+          }
           lds_sync();  // the next chunk overwrites the slots
         } else {
           classes(false);
@@ -1031,68 +1039,25 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
         }
         if (tid == 0) a.gcount[p] = ns;
       } else {
-        double dR[9], dG[9];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-          dR[i] = 0.0;
-          dG[i] = 0.0;
-        }
-        const int *idx = a.idx + (size_t)p * a.cap;
-        if (!large) {
-          for (int q = tid; q < ns; q += kAdjThreads) {
-            float A = t.A[q], BX = t.BX[q], BY = t.BY[q];
-            if (norm)
-              max_term(a, corr, px, py, t.X[q], t.Y[q], t.xb[q] >> 16, t.xb[q] & 0xffff, t.ylo[q],
-                       t.yhi[q], A, BX, BY);
-            spot_chain(a, s_cc, sh.R, idx[t.jpos[q]], t.I[q], A, BX, BY, dR, dG);
-          }
-        } else {
-          // the chain rule is linear in the gathered sums: apply it per bin
-          // entry (a spot's strips in turn), then once more for the max term
-          // of each spot whose box holds p*
-          __syncthreads();  // the entry records in global memory, block-visible
-          for (int e = tid; e < entries; e += kAdjThreads) {
-            const float4 r = wrec[e];
-            const int j = __float_as_int(r.w);
-            spot_chain(a, s_cc, sh.R, idx[j], spots[j].w, r.x, r.y, r.z, dR, dG);
-          }
-          if (norm) {
-            for (int j = tid; j < n; j += kAdjThreads) {
-              const int2 bx = wbox[j];
-              const int lo = bx.y >> 16, hi = bx.y & 0xffff, xlo = bx.x >> 16, xhi = bx.x & 0xffff;
-              if (py < lo || py > hi || px < xlo || px > xhi) continue;
-              const float4 sp = spots[j];
-              float X, Y;
-              to_pixel(sp, a, X, Y);
-              float A = 0.0f, BX = 0.0f, BY = 0.0f;
-              max_term(a, corr, px, py, X, Y, xlo, xhi, lo, hi, A, BX, BY);
-              spot_chain(a, s_cc, sh.R, idx[j], sp.w, A, BX, BY, dR, dG);
-            }
+        // the max term of each spot whose box holds p*, into its gsum entry;
+        // spot_chain_kernel applies the chain rule to the whole row
+        if (norm) {
+          for (int j = tid; j < n; j += kAdjThreads) {
+            const int2 bx = wbox[j];
+            const int lo = bx.y >> 16, hi = bx.y & 0xffff, xlo = bx.x >> 16, xhi = bx.x & 0xffff;
+            if (py < lo || py > hi || px < xlo || px > xhi) continue;
+            const float4 sp = spots[j];
+            float X, Y;
+            to_pixel(sp, a, X, Y);
+            float A = 0.0f, BX = 0.0f, BY = 0.0f;
+            max_term(a, corr, px, py, X, Y, xlo, xhi, lo, hi, A, BX, BY);
+            float *g = reinterpret_cast<float *>(gsp + j);
+            unsafeAtomicAdd(g, A);
+            unsafeAtomicAdd(g + 1, BX);
+            unsafeAtomicAdd(g + 2, BY);
           }
         }
-#pragma unroll
-        for (int i = 0; i < 9; ++i) dR[i] = wave_sum_d(dR[i]);
-        if (a.drec) {
-#pragma unroll
-          for (int i = 0; i < 9; ++i) dG[i] = wave_sum_d(dG[i]);
-        }
-        if (lane == 0) {
-#pragma unroll
-          for (int i = 0; i < 9; ++i) {
-            sh.red[wid][2 + i] = dR[i];
-            sh.red[wid][11 + i] = dG[i];
-          }
-        }
-        lds_sync();
-        if (tid < 9) {
-          double v = 0.0, u = 0.0;
-          for (int w = 0; w < kAdjWaves; ++w) {
-            v += sh.red[w][2 + tid];
-            u += sh.red[w][11 + tid];
-          }
-          a.dRt[(size_t)p * 9 + tid] = v;
-          sh.drec[tid] += u;
-        }
+        if (tid == 0) a.gcount[p] = n;
       }
     }
     if (tid == 0 && a.loss) a.loss[p] = static_cast<float>(ltot);
@@ -1101,33 +1066,29 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
     else
       lds_sync();  // LDS state of this pattern is dead before the next one
   }
-  if (kLarge && a.drec && tid < 9 && sh.drec[tid] != 0.0) unsafeAtomicAdd(a.drec + tid, sh.drec[tid]);
 }
 
 // Two blocks per CU: at most 128 VGPRs (four waves per SIMD).
 __global__ __launch_bounds__(kAdjThreads, 4) void render_adjoint_kernel(AdjArgs a) {
   extern __shared__ __align__(16) aacc_t s_acc[];  // 16-byte aligned for the 16-byte read-out
   __shared__ AdjShared<1> sh;
-  __shared__ ChainConsts s_cc;  // unused here
-  adjoint_body<false>(a, s_acc, sh, s_cc);
+  adjoint_body<false>(a, s_acc, sh);
 }
 
-__global__ __launch_bounds__(kAdjThreads) void render_adjoint_large_kernel(AdjArgs a) {
+// Two blocks per CU, each with half the LDS (at most 128 VGPRs).
+__global__ __launch_bounds__(kAdjThreads, 4) void render_adjoint_large_kernel(AdjArgs a) {
   extern __shared__ __align__(16) aacc_t s_acc[];
   __shared__ AdjShared<kWidthClasses> sh;
-  __shared__ ChainConsts s_cc;
-  adjoint_body<true>(a, s_acc, sh, s_cc);
+  adjoint_body<true>(a, s_acc, sh);
 }
 
 // Forward mode of the large-pattern kernel (the sub-pixel renderer of long
 // spot lists): with the mode and the missing gradient outputs known at
 // compile time the gather and the FP64 chain rule drop out, and the kernel
-// needs about 100 VGPRs instead of 256 and spills, so two blocks share a CU
-// (each with half the LDS): one block's barrier waits overlap the other's work.
+// needs about 100 VGPRs.
 __global__ __launch_bounds__(kAdjThreads, 4) void render_adjoint_large_fwd_kernel(AdjArgs a) {
   extern __shared__ __align__(16) aacc_t s_acc[];
   __shared__ AdjShared<kWidthClasses> sh;
-  __shared__ ChainConsts s_cc;
   a.mode = DSX_ADJ_FORWARD;
   a.want_grad = 0;
   a.target = nullptr;
@@ -1135,7 +1096,8 @@ __global__ __launch_bounds__(kAdjThreads, 4) void render_adjoint_large_fwd_kerne
   a.dF2 = nullptr;
   a.drec = nullptr;
   a.loss = nullptr;
-  adjoint_body<true>(a, s_acc, sh, s_cc);
+  a.gsum = nullptr;
+  adjoint_body<true>(a, s_acc, sh);
 }
 
 // Chain rule of every spot gathered by render_adjoint_kernel: one wave per
@@ -1372,9 +1334,10 @@ __global__ __launch_bounds__(kSfaThreads) void structure_factor_adjoint_kernel(S
 using namespace dsx;
 
 // Workspace of the large-pattern kernel, per block (E = kWorkEntriesPerSpot *
-// cap bin entries): float4 records[E] | int bins[E] | int2 boxes[cap] | int
-// q[H W], each part padded to 16 bytes.  Of the small-pattern kernel with
-// gradients: float4 gsum[n * cap] | int gcount[n] | (8-byte aligned) double
+// cap bin entries): int bins[E] | int2 boxes[cap] | int q[H W], each part
+// padded to 16 bytes; with gradients the chain-rule hand-over follows the
+// blocks.  Chain-rule hand-over (the small-pattern kernel's whole workspace):
+// float4 gsum[n * cap] | int gcount[n] | (8-byte aligned) double
 // dpart[chain blocks][9].
 constexpr int kWorkEntriesPerSpot = 4;
 constexpr int kAdjLdsBytes = 160 * 1024;
@@ -1386,16 +1349,18 @@ static long long small_work_ints(int cap, int n) {
 }
 static long long large_block_ints(int cap, long long hw) {
   const long long e = kWorkEntriesPerSpot * static_cast<long long>(cap);
-  return ((5 * e + 2ll * cap + 3) & ~3ll) + ((hw + 3) & ~3ll);
+  return ((e + 3) & ~3ll) + ((2ll * cap + 3) & ~3ll) + ((hw + 3) & ~3ll);
 }
-// blocks per CU: two for the forward-only kernel, one with gradients
+// blocks of the large-pattern kernels: two per CU
 static int large_blocks(int n, int per_cu = 2) { return std::min(n, per_cu * cu_count()); }
 
-extern "C" long long dsx_render_adjoint_workspace_ints(int cap, int n, int height, int width) {
+extern "C" long long dsx_render_adjoint_workspace_ints(int cap, int n, int height, int width,
+                                                       int gradients) {
   if (cap < 1 || n < 0 || height < 1 || width < 1) return 0;
   if (adjoint_large(cap))
     return static_cast<long long>(large_blocks(n)) *
-           large_block_ints(cap, static_cast<long long>(height) * width);
+               large_block_ints(cap, static_cast<long long>(height) * width) +
+           (gradients ? small_work_ints(cap, n) : 0);
   return small_work_ints(cap, n);
 }
 
@@ -1527,20 +1492,28 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
     set_error("dsx_render_adjoint: cannot query or reserve the kernel's LDS");
     return DSX_ELAUNCH;
   }
-  int bpc = large ? (fwd_large ? 2 : 1) : 2;
+  int bpc = 2;  // large kernels: two blocks per CU, one block's barrier waits overlap the other's work
   int rows = 0;
   long long table = 0;
   if (large) {
     // one strip buffer of up to 64 KB (32 KB with two blocks per CU), at least
-    // 512 spot slots, the rest slots
+    // 512 spot slots, the rest slots; one block per CU when a row of the
+    // pattern does not fit half the budget
     a.bin_cap = 0;
-    const long long per_block = kAdjLdsBytes / bpc;
-    const long long strip = std::min<long long>(kAdjStripBytes / bpc,
-                                                per_block - st - 16 - 512ll * kSpotTabBytes);
-    rows = strip < 4ll * a.W ? 0 : static_cast<int>(std::min<long long>(a.H, strip / (4ll * a.W)));
-    a.slots = static_cast<int>(std::min<long long>(
-        kAdjLargeMaxSlots, (per_block - st - 16 - 4ll * rows * a.W) / kSpotTabBytes));
-    table = static_cast<long long>(kSpotTabBytes) * a.slots + 16;
+    auto size_large = [&](int b) {
+      const long long per_block = kAdjLdsBytes / b;
+      const long long strip = std::min<long long>(kAdjStripBytes / b,
+                                                  per_block - st - 16 - 512ll * kSpotTabBytes);
+      rows = strip < 4ll * a.W ? 0 : static_cast<int>(std::min<long long>(a.H, strip / (4ll * a.W)));
+      a.slots = static_cast<int>(std::min<long long>(
+          kAdjLargeMaxSlots, (per_block - st - 16 - 4ll * rows * a.W) / kSpotTabBytes));
+      table = static_cast<long long>(kSpotTabBytes) * a.slots + 16;
+    };
+    size_large(bpc);
+    if (rows < 1) {
+      bpc = 1;
+      size_large(bpc);
+    }
   } else {
     a.slots = cap;
     a.bin_cap = std::min(kAdjBinCap, 4 * cap + 256);
@@ -1571,18 +1544,31 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   const int blocks = large ? large_blocks(n, bpc) : std::min(n, cu_count() * bpc);
   if (large) {
     const long long per_block = large_block_ints(cap, static_cast<long long>(a.H) * a.W);
+    const long long need = per_block * blocks + (want_grad ? small_work_ints(cap, n) : 0);
     if (work_d && reinterpret_cast<uintptr_t>(work_d) % 16 != 0) {
       set_error("dsx_render_adjoint: workspace must be 16-byte aligned");
       return DSX_EINVAL;
     }
-    if (work_d && work_ints >= per_block * blocks &&
+    if (want_grad && !work_d) {
+      set_error("dsx_render_adjoint: gradients need a workspace of %lld ints "
+                "(dsx_render_adjoint_workspace_ints)", need);
+      return DSX_EINVAL;
+    }
+    if (work_d && work_ints >= need &&
         kWorkEntriesPerSpot * static_cast<long long>(cap) < (1ll << 31)) {
       a.work = work_d;
       a.work_per_block = per_block;
       a.work_entries = kWorkEntriesPerSpot * cap;
+      if (want_grad) {
+        int *hand = work_d + per_block * blocks;  // a multiple of 4 ints: 16-byte aligned
+        a.gsum = reinterpret_cast<float4 *>(hand);
+        a.gcount = hand + static_cast<size_t>(n) * 4 * cap;
+        const size_t off = (static_cast<size_t>(n) * (4ull * cap + 1) + 1) & ~static_cast<size_t>(1);
+        a.dpart = reinterpret_cast<double *>(hand + off);
+      }
     } else if (work_d) {
       set_error("dsx_render_adjoint: workspace of %lld ints is smaller than %lld "
-                "(dsx_render_adjoint_workspace_ints)", work_ints, per_block * blocks);
+                "(dsx_render_adjoint_workspace_ints)", work_ints, need);
       return DSX_EINVAL;
     }
   } else if (want_grad) {
@@ -1609,11 +1595,14 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
       hipLaunchKernelGGL(render_adjoint_large_fwd_kernel, dim3(blocks), dim3(kAdjThreads), lds, s, a);
     else
       hipLaunchKernelGGL(render_adjoint_large_kernel, dim3(blocks), dim3(kAdjThreads), lds, s, a);
-    return check_launch("dsx_render_adjoint");
+  } else {
+    hipLaunchKernelGGL(render_adjoint_kernel, dim3(blocks), dim3(kAdjThreads), lds, s, a);
   }
-  hipLaunchKernelGGL(render_adjoint_kernel, dim3(blocks), dim3(kAdjThreads), lds, s, a);
   int rc = check_launch("dsx_render_adjoint");
   if (rc != DSX_OK || !want_grad) return rc;
+  // the chain kernel reads gsum rows of `slots` entries: the large kernel's
+  // rows are indexed by spot position (cap entries)
+  if (large) a.slots = cap;
   const int cblocks =
       std::min((n + kChainThreads / kWave - 1) / (kChainThreads / kWave), chain_blocks_max());
   hipLaunchKernelGGL(spot_chain_kernel, dim3(cblocks), dim3(kChainThreads), 0, s, a);
@@ -1675,7 +1664,8 @@ extern "C" int dsx_render_adjoint_noalloc(const int *idx_d, const float *spot_d,
     set_error("dsx_render_adjoint_noalloc: bad arguments (n=%d cap=%d)", n, cap);
     return DSX_EINVAL;
   }
-  const long long need = dsx_render_adjoint_workspace_ints(cap, n, rp->height, rp->width);
+  const long long need =
+      dsx_render_adjoint_workspace_ints(cap, n, rp->height, rp->width, dRt_d != nullptr);
   if (need > kNoallocScratchInts) {
     set_error("dsx_render_adjoint_noalloc: %d patterns with capacity %d need %lld ints of "
               "scratch; the static scratch holds %lld (use dsx_render_adjoint with a workspace, "

@@ -48,6 +48,10 @@ from .crystallography import (PhaseArrays, element_of, highest_hkl, lattice_setu
                               reciprocal_metric)
 from .scattering import table
 
+# Long spot lists hand their per-spot gathered sums (n x cap x 16 B) from the
+# render adjoint to its chain rule; calls are split to keep that bounded.
+GRAD_SUMS_BYTES = 8 << 30
+
 __all__ = ["bunge_matrices", "rt_from_matrices", "euler_to_rt", "reciprocal_basis",
            "RefinementModel", "leaf_parameters", "forward_args"]
 
@@ -341,12 +345,13 @@ class RefinementModel:
     def _sim_params(self, t: _Tables) -> _lib.SimParams:
         return self.settings.params(t.g_max, t.rec)
 
-    def workspace(self, cap: int, n: int) -> torch.Tensor:
+    def workspace(self, cap: int, n: int, gradients: bool = True) -> torch.Tensor:
         """Scratch of dsx_render_adjoint: per-spot sums between its render and
-        chain-rule passes, or, for spot lists longer than the render LDS
+        chain-rule passes and, for spot lists longer than the render LDS
         holds, per-block strip bins and fixed-point pixels."""
         H, W = self.shape
-        need = max(1, int(self.L.dsx_render_adjoint_workspace_ints(int(cap), int(n), H, W)))
+        need = max(1, int(self.L.dsx_render_adjoint_workspace_ints(int(cap), int(n), H, W,
+                                                                   int(gradients))))
         if getattr(self, "_work", None) is None or self._work.numel() < need:
             self._work = torch.empty(need, dtype=torch.int32, device=self.dev)
         return self._work
@@ -386,15 +391,27 @@ class RefinementModel:
         sync_status = status is None
         if status is None:
             status = torch.zeros(n, dtype=torch.int32, device=self.dev)
-        work = self.workspace(batch.cap, n)
         sp = self._sim_params(t)
         stream = _lib.stream_handle(self.dev)
-        _lib.check(self.L.dsx_render_adjoint(
-            _lib.ptr(batch.idx), _lib.ptr(batch.spots), _lib.ptr(batch.count), batch.cap, n,
-            _lib.ptr(t.hkl), _lib.ptr(t.F2), self.n_refl, _lib.ptr(Rt), ctypes.byref(sp),
-            ctypes.byref(self.render), mode, _lib.ptr(target), stride, _lib.ptr(pattern),
-            _lib.ptr(loss), _lib.ptr(dRt), _lib.ptr(dF2), _lib.ptr(drec if lattice else None),
-            _lib.ptr(work), work.numel(), _lib.ptr(status), stream), "dsx_render_adjoint")
+        # long spot lists hand n * cap per-spot sums (16 B each) from the
+        # render to the chain rule: keep that under GRAD_SUMS_BYTES per call
+        step = n
+        if want_grad and batch.cap > int(self.L.dsx_render_max_spots()):
+            step = max(1, min(n, GRAD_SUMS_BYTES // (16 * batch.cap)))
+        work = self.workspace(batch.cap, step, want_grad)
+        for o in range(0, n, step):
+            e = min(n, o + step)
+
+            def sl(x):
+                return None if x is None else x[o:e]
+            _lib.check(self.L.dsx_render_adjoint(
+                _lib.ptr(batch.idx[o:e]), _lib.ptr(batch.spots[o:e]), _lib.ptr(batch.count[o:e]),
+                batch.cap, e - o, _lib.ptr(t.hkl), _lib.ptr(t.F2), self.n_refl,
+                _lib.ptr(Rt[o:e]), ctypes.byref(sp), ctypes.byref(self.render), mode,
+                _lib.ptr(target[o:e] if stride else target), stride, _lib.ptr(sl(pattern)),
+                _lib.ptr(loss[o:e]), _lib.ptr(sl(dRt)), _lib.ptr(dF2),
+                _lib.ptr(drec if lattice else None), _lib.ptr(work), work.numel(),
+                _lib.ptr(status[o:e]), stream), "dsx_render_adjoint")
         if sync_status:
             engine.check_render_status(status)
         dxyz = dB = None

@@ -478,7 +478,7 @@ def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigm
     if not fast and cap > int(L.dsx_render_max_spots()):
         # more spots than the render LDS holds: the adjoint kernel's forward
         # mode renders them chunk by chunk (same pixels as dsx_render_subpixel)
-        work = _workspace(dev, int(L.dsx_render_adjoint_workspace_ints(int(cap), int(n), H, W)),
+        work = _workspace(dev, int(L.dsx_render_adjoint_workspace_ints(int(cap), int(n), H, W, 0)),
                           stream)
         sim = _lib.SimParams()
         sim.k, sim.shape_width, sim.shape_model = 1.0, 1.0, _lib.SF_LINEAR

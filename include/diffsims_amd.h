@@ -357,20 +357,23 @@ int dsx_render_fast_ragged(const float *spot_d, const long long *spot_off_d, con
  * dF2_d and drec_d need dRt_d.  pattern_d
  * (float [n][H][W], may be NULL) receives P; loss_d (float [n], may be NULL).
  * work_d is a 16-byte aligned workspace of dsx_render_adjoint_workspace_ints(
- * cap, n, H, W) ints.  With cap <= dsx_render_max_spots() it is needed only for
- * gradients (per-spot sums handed from the render pass to the chain-rule pass;
- * EINVAL without it) and every status_d entry is 0.  With a larger cap it
- * holds the strip bins of patterns with more in-frame spots than the LDS
- * holds, and their fixed-point pixels between the splat and the read-out when
- * pattern_d is NULL; without it, or when it is too small for a pattern, that
- * pattern's status_d entry is 1.  At most 2^22 pixels per pattern.
+ * cap, n, H, W, gradients) ints (gradients: nonzero when dRt_d is passed).
+ * With cap <= dsx_render_max_spots() it is needed only for gradients
+ * (per-spot sums handed from the render pass to the chain-rule pass; EINVAL
+ * without it) and every status_d entry is 0.  With a larger cap it holds
+ * the strip bins and clip boxes of the patterns, their fixed-point pixels
+ * between the splat and the read-out when pattern_d is NULL, and with
+ * gradients n * cap per-spot sums (EINVAL without it); when it is missing
+ * without gradients, or too small for a pattern's bins, that pattern's
+ * status_d entry is 1.  At most 2^22 pixels per pattern.
  * This entry point also serves as the sub-pixel renderer for such large
  * patterns (mode DSX_ADJ_FORWARD).
  */
 #define DSX_ADJ_L2 0
 #define DSX_ADJ_PATTERN_GRAD 1
 #define DSX_ADJ_FORWARD 2
-long long dsx_render_adjoint_workspace_ints(int cap, int n, int height, int width);
+long long dsx_render_adjoint_workspace_ints(int cap, int n, int height, int width,
+                                            int gradients);
 int dsx_render_adjoint(const int *idx_d, const float *spot_d, const int *count_d, int cap,
                        int n, const int *hkl_d, const float *F2_d, int n_refl,
                        const double *Rt_d, const dsx_sim_params *sim,
@@ -388,7 +391,8 @@ int dsx_render_adjoint(const int *idx_d, const float *spot_d, const int *count_d
  * dsx_render_adjoint_noalloc_max_ints() ints (64 MiB), one per GPU, held until
  * the process ends; its address is looked up on the first call on each
  * device.  The call allocates and frees nothing.  Sizes whose
- * dsx_render_adjoint_workspace_ints(cap, n, H, W) exceeds the maximum are rejected
+ * dsx_render_adjoint_workspace_ints(cap, n, H, W, dRt_d != NULL) exceeds the maximum
+ * are rejected
  * with DSX_EINVAL.  Every call on one GPU shares that GPU's array, which
  * keeps the last call's contents (each call overwrites what it reads, so no
  * clearing is needed): calls that overlap in time on one GPU (other streams

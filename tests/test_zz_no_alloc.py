@@ -45,7 +45,7 @@ def _call(model, t, Rt, b, n, target, noalloc):
         _lib.check(L.dsx_render_adjoint_noalloc(*common, _lib.ptr(status), stream),
                    "dsx_render_adjoint_noalloc")
     else:
-        work = torch.empty(max(1, int(L.dsx_render_adjoint_workspace_ints(b.cap, n, H, W))),
+        work = torch.empty(max(1, int(L.dsx_render_adjoint_workspace_ints(b.cap, n, H, W, 1))),
                            dtype=torch.int32, device="cuda")
         _lib.check(L.dsx_render_adjoint(*common, _lib.ptr(work), work.numel(), _lib.ptr(status),
                                         stream), "dsx_render_adjoint")
@@ -79,7 +79,7 @@ def test_noalloc_equals_workspace_entry(case):
     from diffsims_amd import _lib
     L = _lib.load()
     H, W = model.shape
-    assert int(L.dsx_render_adjoint_workspace_ints(b.cap, sizes[0], H, W)) <= \
+    assert int(L.dsx_render_adjoint_workspace_ints(b.cap, sizes[0], H, W, 1)) <= \
         int(L.dsx_render_adjoint_noalloc_max_ints())
     if case == "long_lists":
         assert b.cap > int(L.dsx_render_max_spots())
@@ -100,7 +100,7 @@ def test_noalloc_rejects_sizes_beyond_the_scratch():
     assert max_ints == 16 << 20
     cap = 1000
     n = max_ints // (4 * cap) + 16
-    assert int(L.dsx_render_adjoint_workspace_ints(cap, n, 64, 64)) > max_ints
+    assert int(L.dsx_render_adjoint_workspace_ints(cap, n, 64, 64, 1)) > max_ints
     sim = _lib.SimParams()
     rp = _lib.RenderParams()
     rp.height, rp.width = 64, 64
