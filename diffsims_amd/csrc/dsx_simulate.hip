@@ -87,6 +87,10 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
   __shared__ int s_queue[kWavesPerBlock][kQueue];
   __shared__ double s_R[kWavesPerBlock][kOT][9];
   __shared__ double s_abc[kWavesPerBlock][kOT][4];  // beam . (a*, b*, c*), 1 / (beam . c*)
+  // FP32 beam rows of the full-sweep pretest: in the line-sweep kernels, which
+  // run the full sweep only for short-line phases, they wait in LDS instead
+  // of holding 12 VGPRs through the line machinery
+  __shared__ float s_Rf[kWavesPerBlock][kOT][3];
   const int lane = lane_id();
   const int wib = threadIdx.x / kWave;
   int *queue = s_queue[wib];
@@ -116,7 +120,7 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
     int *idx_out = d->idx;
     float4 *spot_out = reinterpret_cast<float4 *>(d->spots);
     const float pre_thr = pretest_threshold(a.me, a.k, d->g_max, a.sin_phi);
-    float R[kOT][3];  // third rows (beam direction) for the FP32 pretest
+    float R[kLines ? 1 : kOT][3];  // third rows (beam direction) for the FP32 pretest
     int cnt[kOT];
     float lmax[kOT], lmin[kOT];
     for (int e = lane; e < kOT * 9; e += kWave) {
@@ -127,7 +131,13 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
     for (int o = 0; o < kOT; ++o) {
       const bool ok = o < n_here;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) R[o][j] = ok ? static_cast<float>(Rt[(size_t)(o0 + o) * 9 + 6 + j]) : 0.0f;
+      for (int j = 0; j < 3; ++j) {
+        const float v = ok ? static_cast<float>(Rt[(size_t)(o0 + o) * 9 + 6 + j]) : 0.0f;
+        if (kLines)
+          s_Rf[wib][o][j] = v;
+        else
+          R[o][j] = v;
+      }
       cnt[o] = 0;
       lmax[o] = 0.0f;
       lmin[o] = __builtin_huge_valf();
@@ -331,7 +341,10 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
 #pragma unroll
           for (int o = 0; o < kOT; ++o) {
             if (o >= n_here) break;  // uniform
-            pretest(v, i, g, R[o][0], R[o][1], R[o][2], o);
+            if (kLines)
+              pretest(v, i, g, s_Rf[wib][o][0], s_Rf[wib][o][1], s_Rf[wib][o][2], o);
+            else
+              pretest(v, i, g, R[o][0], R[o][1], R[o][2], o);
           }
         }
       }
