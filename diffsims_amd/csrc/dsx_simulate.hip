@@ -91,6 +91,13 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
   // run the full sweep only for short-line phases, they wait in LDS instead
   // of holding 12 VGPRs through the line machinery
   __shared__ float s_Rf[kWavesPerBlock][kOT][3];
+  // line sweep: the current group's intervals per orientation and lane
+  constexpr int kLO = kLines ? kOT : 1, kLW = kLines ? kWave : 1;
+  __shared__ int s_P[kWavesPerBlock][kLO][kLW];
+  __shared__ int s_rb[kWavesPerBlock][kLO][kLW];
+  __shared__ float s_eA[kWavesPerBlock][kLO][kLW];
+  __shared__ float s_eB[kWavesPerBlock][kLO][kLW];
+  __shared__ int s_tot[kWavesPerBlock][kLO];
   const int lane = lane_id();
   const int wib = threadIdx.x / kWave;
   int *queue = s_queue[wib];
@@ -200,21 +207,23 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
       qn += __popcll(pm);
     };
 
-    // Line sweep state: current group of 64 lines (lb), orientation (lo),
-    // window (win) of the group's total candidates for lo; this lane's line.
+    // Line sweep state: the current group of 64 lines (lb), orientation (lo)
+    // and window (win).  When a group loads, the row interval of every line
+    // (one per lane) is formed for all kOT orientations at once -- four
+    // independent chains of FP64 arithmetic and prefix sums instead of one
+    // chain per orientation -- and parked in LDS (per orientation and lane:
+    // first candidate P, first row rb, pretest coefficients eA, eB; the
+    // orientation's candidate total in s_tot), so the line data is dead while
+    // the windows and drains run.
     const bool use_lines = kLines && d->lines != nullptr && lines_pay(n_refl, d->n_lines);
     const int4 *lines = reinterpret_cast<const int4 *>(d->lines);
     const int n_lines = use_lines ? d->n_lines : 0;
     int lb = -kWave, lo = kOT - 1, win = 0, total = 0;
-    int first = 0, nrow = 0, lh = 0, lk = 0, l0 = 0, P = 0, rb = 0;
-    double zlo = 0.0, zhi = -1.0;
     // along a line (hkl = (h, k, l0 - t)) the pretest value is a quadratic in
     // t: e(t) = |g0 - t c*|^2 - 2k (zt - t ga) = A + B t + c2 t^2, so a window
-    // forms it from this lane's line coefficients (eA, eB at the interval's
-    // first row) instead of loading the row's g4; w0 = |g0|^2, cd = g0 . c*
-    // (FP32: their rounding, ~1e-6 of e, is far inside the pretest's slack)
-    float w0 = 0.0f, cd = 0.0f;
-    float eA = 0.0f, eB = 0.0f;
+    // forms it from its line's coefficients (eA, eB at the interval's first
+    // row) instead of loading the row's g4; w0 = |g0|^2, cd = g0 . c* (FP32:
+    // their rounding, ~1e-6 of e, is far inside the pretest's slack)
     const double c2 = rec[2] * rec[2] + rec[5] * rec[5] + rec[8] * rec[8];
     const float c2f = static_cast<float>(c2);
     // pretest pass => |w - 2k zf| < thr (1 + 2^-24); zf (FP32 R and g, two
@@ -239,6 +248,81 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
     }
+    // Next group of lines: false once the lines are exhausted.
+    auto load_group = [&]() -> bool {
+      lb += kWave;
+      if (lb >= n_lines) return false;
+      const int j = lb + lane;
+      int first = 0, nrow = 0, lh = 0, lk = 0, l0 = 0;
+      double zlo = 0.0, zhi = -1.0;
+      float w0 = 0.0f, cd = 0.0f;
+      if (j < n_lines) {
+        const int4 e0 = lines[2 * (size_t)j], e1 = lines[2 * (size_t)j + 1];
+        first = e0.x;
+        nrow = e0.y;
+        lh = e0.z;
+        lk = e0.w;
+        l0 = e1.x;
+        const double wmin = __int_as_float(e1.y), wmax = __int_as_float(e1.z);
+        const double zm = 1e-5 * (1.0 + sqrt(wmax));
+        zlo = (wmin - thr) / two_kd - zm;
+        zhi = (wmax + thr) / two_kd + zm;
+        const double hx = lh, hy = lk, hz = l0;
+        const double gx = rec[0] * hx + rec[1] * hy + rec[2] * hz;
+        const double gy = rec[3] * hx + rec[4] * hy + rec[5] * hz;
+        const double gz = rec[6] * hx + rec[7] * hy + rec[8] * hz;
+        w0 = static_cast<float>(gx * gx + gy * gy + gz * gz);
+        cd = static_cast<float>(gx * rec[2] + gy * rec[5] + gz * rec[8]);
+      }
+      int t0[kOT], len[kOT], incl[kOT];
+#pragma unroll
+      for (int o = 0; o < kOT; ++o) {
+        // z of row first + t is zt - t ga (l = l0 - t): the rows with z in [zlo, zhi]
+        const double al = s_abc[wib][o][0], be = s_abc[wib][o][1], ga = s_abc[wib][o][2];
+        const double inv = s_abc[wib][o][3];
+        const double zt = static_cast<double>(lh) * al + static_cast<double>(lk) * be +
+                          static_cast<double>(l0) * ga;
+        t0[o] = 0;
+        len[o] = 0;
+        if (o >= n_here || nrow <= 0) {
+          // no rows (an orientation past the tile's end, or no line)
+        } else if (inv == 0.0) {  // line perpendicular to the beam: all rows or none
+          if (zt >= zlo && zt <= zhi) len[o] = nrow;
+        } else {
+          const double q1 = (zt - zhi) * inv, q2 = (zt - zlo) * inv;
+          const double tlo = fmax(ceil(fmin(q1, q2)), 0.0);
+          const double thi = fmin(floor(fmax(q1, q2)), static_cast<double>(nrow - 1));
+          if (thi >= tlo) {
+            t0[o] = static_cast<int>(tlo);
+            len[o] = static_cast<int>(thi - tlo) + 1;
+          }
+        }
+        const double tt = static_cast<double>(t0[o]);
+        const double eb = two_kd * ga - 2.0 * static_cast<double>(cd);
+        s_eA[wib][o][lane] =
+            static_cast<float>((static_cast<double>(w0) - two_kd * zt) + tt * (eb + c2 * tt));
+        s_eB[wib][o][lane] = static_cast<float>(eb + 2.0 * c2 * tt);
+        s_rb[wib][o][lane] = first + t0[o];
+        incl[o] = len[o];  // inclusive prefix over the lanes (lines in row order)
+      }
+#pragma unroll
+      for (int off = 1; off < kWave; off <<= 1) {
+#pragma unroll
+        for (int o = 0; o < kOT; ++o) {
+          const int v = __shfl_up(incl[o], off);
+          if (lane >= off) incl[o] += v;
+        }
+      }
+#pragma unroll
+      for (int o = 0; o < kOT; ++o) {
+        s_P[wib][o][lane] = incl[o] - len[o];  // first candidate of this lane's line
+        if (lane == kWave - 1) s_tot[wib][o] = incl[o];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+      return true;
+    };
 
     // One step per iteration: a window of 64 line candidates of one
     // orientation, or 64 table rows for every orientation; full batches are
@@ -251,80 +335,27 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
         while (win >= total) {
           if (++lo >= n_here) {  // next group of lines
             lo = 0;
-            lb += kWave;
-            if (lb >= n_lines) {
+            if (!load_group()) {
               done = true;
               break;
             }
-            const int j = lb + lane;
-            nrow = 0;
-            if (j < n_lines) {
-              const int4 e0 = lines[2 * (size_t)j], e1 = lines[2 * (size_t)j + 1];
-              first = e0.x;
-              nrow = e0.y;
-              lh = e0.z;
-              lk = e0.w;
-              l0 = e1.x;
-              const double wmin = __int_as_float(e1.y), wmax = __int_as_float(e1.z);
-              const double zm = 1e-5 * (1.0 + sqrt(wmax));
-              zlo = (wmin - thr) / two_kd - zm;
-              zhi = (wmax + thr) / two_kd + zm;
-              const double hx = lh, hy = lk, hz = l0;
-              const double gx = rec[0] * hx + rec[1] * hy + rec[2] * hz;
-              const double gy = rec[3] * hx + rec[4] * hy + rec[5] * hz;
-              const double gz = rec[6] * hx + rec[7] * hy + rec[8] * hz;
-              w0 = static_cast<float>(gx * gx + gy * gy + gz * gz);
-              cd = static_cast<float>(gx * rec[2] + gy * rec[5] + gz * rec[8]);
-            }
           }
-          // z of row first + t is zt - t ga (l = l0 - t): the rows with z in [zlo, zhi]
-          const double al = s_abc[wib][lo][0], be = s_abc[wib][lo][1], ga = s_abc[wib][lo][2];
-          const double inv = s_abc[wib][lo][3];
-          const double zt = static_cast<double>(lh) * al + static_cast<double>(lk) * be +
-                            static_cast<double>(l0) * ga;
-          int t0 = 0, len = 0;
-          if (inv == 0.0) {  // line perpendicular to the beam: all rows or none
-            if (nrow > 0 && zt >= zlo && zt <= zhi) len = nrow;
-          } else if (nrow > 0) {
-            const double q1 = (zt - zhi) * inv, q2 = (zt - zlo) * inv;
-            const double tlo = fmax(ceil(fmin(q1, q2)), 0.0);
-            const double thi = fmin(floor(fmax(q1, q2)), static_cast<double>(nrow - 1));
-            if (thi >= tlo) {
-              t0 = static_cast<int>(tlo);
-              len = static_cast<int>(thi - tlo) + 1;
-            }
-          }
-          {
-            const double tt = static_cast<double>(t0);
-            const double eb = two_kd * ga - 2.0 * static_cast<double>(cd);
-            eA = static_cast<float>((static_cast<double>(w0) - two_kd * zt) + tt * (eb + c2 * tt));
-            eB = static_cast<float>(eb + 2.0 * c2 * tt);
-          }
-          int incl = len;  // inclusive prefix over the lanes (lines in row order)
-#pragma unroll
-          for (int off = 1; off < kWave; off <<= 1) {
-            const int v = __shfl_up(incl, off);
-            if (lane >= off) incl += v;
-          }
-          total = __builtin_amdgcn_readfirstlane(__shfl(incl, kWave - 1));
-          P = incl - len;  // first candidate of this lane's line
-          rb = first + t0;
+          total = s_tot[wib][lo];
           win = 0;
         }
         if (!done) {
           const int c = win + lane;
           // the last lane whose line starts at or before candidate c holds it
+          const int *Pw = s_P[wib][lo];
           int src = 0;
 #pragma unroll
-          for (int step = kWave / 2; step > 0; step >>= 1) {
-            const int p = __shfl(P, src + step);
-            if (p <= c) src += step;
-          }
-          const int u = c - __shfl(P, src);
-          const int row = __shfl(rb, src) + u;
+          for (int step = kWave / 2; step > 0; step >>= 1)
+            if (Pw[src + step] <= c) src += step;
+          const int u = c - Pw[src];
+          const int row = s_rb[wib][lo][src] + u;
           const bool v = c < total && static_cast<unsigned>(row) < static_cast<unsigned>(n_refl);
           const float uf = static_cast<float>(u);
-          const float e = fmaf(fmaf(c2f, uf, __shfl(eB, src)), uf, __shfl(eA, src));
+          const float e = fmaf(fmaf(c2f, uf, s_eB[wib][lo][src]), uf, s_eA[wib][lo][src]);
           const bool pre = v && fabsf(e) < pre_thr;
           const uint64_t pm = __ballot(pre);
           if (pm != 0) {
