@@ -1100,26 +1100,29 @@ __global__ __launch_bounds__(kAdjThreads, 4) void render_adjoint_large_fwd_kerne
   adjoint_body<true>(a, s_acc, sh);
 }
 
-// Chain rule of every spot gathered by render_adjoint_kernel: one wave per
-// pattern; dL/dRt of the pattern (a wave sum) and dL/dg, dL/d|F|^2 of the
-// reflections (global atomics).
+// Chain rule of every spot gathered by the render adjoint: one block per
+// pattern (grid-stride), its spots spread over the block's threads; dL/dRt of
+// the pattern is a block sum, dL/d|F|^2 of the reflections global atomics,
+// dL/drec a per-block partial summed later in a fixed order.
 constexpr int kChainThreads = 256;
+constexpr int kChainWaves = kChainThreads / kWave;
 
 __global__ __launch_bounds__(kChainThreads) void spot_chain_kernel(AdjArgs a) {
   __shared__ ChainConsts s_cc;
-  __shared__ double s_part[kChainThreads / kWave][9];
+  __shared__ double s_R[9];
+  __shared__ double s_part[kChainWaves][9];
   if (threadIdx.x == 0) stage_chain_consts(a, s_cc);
-  __syncthreads();
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
-  const int waves = gridDim.x * (kChainThreads / kWave);
-  double dG[9];  // dL/drec over this wave's patterns
+  double dG[9];  // dL/drec over this thread's spots
 #pragma unroll
   for (int i = 0; i < 9; ++i) dG[i] = 0.0;
-  for (int p = blockIdx.x * (kChainThreads / kWave) + wid; p < a.n; p += waves) {
+  for (int p = blockIdx.x; p < a.n; p += gridDim.x) {
+    if (threadIdx.x < 9) s_R[threadIdx.x] = a.Rt[(size_t)p * 9 + threadIdx.x];
+    __syncthreads();
     double R[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) R[i] = a.Rt[(size_t)p * 9 + i];
+    for (int i = 0; i < 9; ++i) R[i] = s_R[i];
     double dR[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) dR[i] = 0.0;
@@ -1127,7 +1130,7 @@ __global__ __launch_bounds__(kChainThreads) void spot_chain_kernel(AdjArgs a) {
     const float4 *gs = a.gsum + (size_t)p * a.slots;
     const int *idx = a.idx + (size_t)p * a.cap;
     const float4 *spots = a.spots + (size_t)p * a.cap;
-    for (int q = lane; q < ns; q += kWave) {
+    for (int q = threadIdx.x; q < ns; q += kChainThreads) {
       const float4 v = gs[q];
       const int j = __float_as_int(v.w);
       spot_chain(a, s_cc, R, idx[j], spots[j].w, v.x, v.y, v.z, dR, dG);
@@ -1136,8 +1139,15 @@ __global__ __launch_bounds__(kChainThreads) void spot_chain_kernel(AdjArgs a) {
     for (int i = 0; i < 9; ++i) dR[i] = wave_sum_d(dR[i]);
     if (lane == 0) {
 #pragma unroll
-      for (int i = 0; i < 9; ++i) a.dRt[(size_t)p * 9 + i] = dR[i];
+      for (int i = 0; i < 9; ++i) s_part[wid][i] = dR[i];
     }
+    __syncthreads();
+    if (threadIdx.x < 9) {
+      double v = 0.0;
+      for (int w = 0; w < kChainWaves; ++w) v += s_part[w][threadIdx.x];
+      a.dRt[(size_t)p * 9 + threadIdx.x] = v;
+    }
+    __syncthreads();  // s_R and s_part are rewritten for the next pattern
   }
   if (!a.drec) return;  // uniform
   // this block's partial dL/drec (summed in a fixed order by sum_partials_kernel)
@@ -1150,7 +1160,7 @@ __global__ __launch_bounds__(kChainThreads) void spot_chain_kernel(AdjArgs a) {
   __syncthreads();
   if (threadIdx.x < 9) {
     double v = 0.0;
-    for (int w = 0; w < kChainThreads / kWave; ++w) v += s_part[w][threadIdx.x];
+    for (int w = 0; w < kChainWaves; ++w) v += s_part[w][threadIdx.x];
     a.dpart[(size_t)blockIdx.x * 9 + threadIdx.x] = v;
   }
 }
@@ -1603,8 +1613,7 @@ extern "C" int dsx_render_adjoint(const int *idx_d, const float *spot_d, const i
   // the chain kernel reads gsum rows of `slots` entries: the large kernel's
   // rows are indexed by spot position (cap entries)
   if (large) a.slots = cap;
-  const int cblocks =
-      std::min((n + kChainThreads / kWave - 1) / (kChainThreads / kWave), chain_blocks_max());
+  const int cblocks = std::min(n, chain_blocks_max());
   hipLaunchKernelGGL(spot_chain_kernel, dim3(cblocks), dim3(kChainThreads), 0, s, a);
   if (drec_d)
     hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(9 * kWave), 0, s, a.dpart, cblocks, drec_d);
