@@ -310,10 +310,11 @@ __device__ __forceinline__ void strip_pass(const AdjArgs &a, const SpotTab &t, i
 // Load entries [base, base + m) of a strip's global bin into LDS slots 0..m-1
 // (large patterns: more in-frame spots than the LDS holds).  The clip boxes
 // come from the per-spot cache the first scan wrote (boxes[j] = {(xlo << 16) |
-// xhi, (lo << 16) | hi}); only the pixel position is recomputed.
+// xhi, (lo << 16) | hi}); only the pixel position is recomputed.  jpos holds
+// the spot's row of the gradient hand-over (row_of[j]) when there is one.
 __device__ __forceinline__ void load_chunk(const AdjArgs &a, const SpotTab &t, const float4 *spots,
-                                           const int *bins, const int2 *boxes, int base, int m,
-                                           int tid) {
+                                           const int *bins, const int2 *boxes, const int *row_of,
+                                           int base, int m, int tid) {
   for (int i = tid; i < m; i += kAdjThreads) {
     const int j = bins[base + i];
     const float4 sp = spots[j];
@@ -327,7 +328,7 @@ __device__ __forceinline__ void load_chunk(const AdjArgs &a, const SpotTab &t, c
     t.BX[i] = 0.0f;
     t.BY[i] = 0.0f;
     t.xb[i] = bx.x;
-    t.jpos[i] = j;
+    t.jpos[i] = row_of ? row_of[j] : j;
     t.ylo[i] = static_cast<short>(bx.y >> 16);
     t.yhi[i] = static_cast<short>(bx.y & 0xffff);
   }
@@ -535,7 +536,8 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
   const int H = a.H, W = a.W;
   const size_t HW = static_cast<size_t>(H) * W;
   // large-pattern workspace of this block: int bins[entries] | int2
-  // boxes[cap] (clip boxes cached by the first scan) | int q[H W] (the
+  // boxes[cap] (clip boxes cached by the first scan) | int rows[cap] (each
+  // in-frame spot's row of the gradient hand-over) | int q[H W] (the
   // fixed-point pattern between the splat and the read-out when there is no
   // pattern buffer to hold it), each part 16-byte aligned
   int *wblock = kLarge && a.work ? a.work + (size_t)blockIdx.x * a.work_per_block : nullptr;
@@ -543,9 +545,10 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
   int2 *wbox = wblock ? reinterpret_cast<int2 *>(wbins + ((static_cast<size_t>(a.work_entries) + 3) &
                                                           ~static_cast<size_t>(3)))
                       : nullptr;
-  int *wq = wblock ? reinterpret_cast<int *>(wbox) + ((2 * static_cast<size_t>(a.cap) + 3) &
-                                                       ~static_cast<size_t>(3))
-                   : nullptr;
+  int *wrow = wblock ? reinterpret_cast<int *>(wbox) + ((2 * static_cast<size_t>(a.cap) + 3) &
+                                                         ~static_cast<size_t>(3))
+                     : nullptr;
+  int *wq = wblock ? wrow + ((static_cast<size_t>(a.cap) + 3) & ~static_cast<size_t>(3)) : nullptr;
   for (int i = tid; i < a.acc_words; i += kAdjThreads) s_acc[i] = 0;
   lds_sync();
 
@@ -566,8 +569,6 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
     float tmax = 0.0f;
     float4 *gsp = kLarge && a.gsum ? a.gsum + (size_t)p * a.cap : nullptr;
     for (int j = tid; j < n; j += kAdjThreads) {
-      // large kernel with gradients: the spot's gathered sums start at zero
-      if (gsp) gsp[j] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(j));
       const float4 sp = spots[j];
       float X = 0.0f, Y = 0.0f;
       int lo = 1, hi = 0, xlo = 0, xhi = 0;
@@ -582,7 +583,15 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
       for (int s = lo / a.strip_rows; s <= hi / a.strip_rows; ++s)
         atomicAdd(&sh.bin_start[kCls * s + wc + 1], 1);
       const int q = atomicAdd(&sh.ns, 1);
-      if (kLarge) continue;  // its chunks are loaded per strip
+      if (kLarge) {
+        // gradients: in-frame spot q's gathered sums start at zero in row q
+        // of the hand-over (which names its spot j)
+        if (gsp) {
+          gsp[q] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(j));
+          wrow[j] = q;
+        }
+        continue;  // its chunks are loaded per strip
+      }
       t.I[q] = sp.w;
       t.X[q] = X;
       t.Y[q] = Y;
@@ -730,7 +739,7 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
       const int e0 = sh.bin_start[bs], e1 = sh.bin_start[bs + kCls];
       for (int c0 = e0; c0 < e1; c0 += a.slots) {
         const int m = min(a.slots, e1 - c0);
-        load_chunk(a, t, spots, wbins, wbox, c0, m, tid);
+        load_chunk(a, t, spots, wbins, wbox, gsp ? wrow : nullptr, c0, m, tid);
         lds_sync();
         // the chunk's entries in slots 0..m-1, ordered by width class
         auto classes = [&](bool g) {
@@ -1051,13 +1060,13 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
             to_pixel(sp, a, X, Y);
             float A = 0.0f, BX = 0.0f, BY = 0.0f;
             max_term(a, corr, px, py, X, Y, xlo, xhi, lo, hi, A, BX, BY);
-            float *g = reinterpret_cast<float *>(gsp + j);
+            float *g = reinterpret_cast<float *>(gsp + wrow[j]);
             unsafeAtomicAdd(g, A);
             unsafeAtomicAdd(g + 1, BX);
             unsafeAtomicAdd(g + 2, BY);
           }
         }
-        if (tid == 0) a.gcount[p] = n;
+        if (tid == 0) a.gcount[p] = ns_all;
       }
     }
     if (tid == 0 && a.loss) a.loss[p] = static_cast<float>(ltot);
@@ -1344,8 +1353,8 @@ __global__ __launch_bounds__(kSfaThreads) void structure_factor_adjoint_kernel(S
 using namespace dsx;
 
 // Workspace of the large-pattern kernel, per block (E = kWorkEntriesPerSpot *
-// cap bin entries): int bins[E] | int2 boxes[cap] | int q[H W], each part
-// padded to 16 bytes; with gradients the chain-rule hand-over follows the
+// cap bin entries): int bins[E] | int2 boxes[cap] | int rows[cap] | int
+// q[H W], each part padded to 16 bytes; with gradients the chain-rule hand-over follows the
 // blocks.  Chain-rule hand-over (the small-pattern kernel's whole workspace):
 // float4 gsum[n * cap] | int gcount[n] | (8-byte aligned) double
 // dpart[chain blocks][9].
@@ -1359,7 +1368,7 @@ static long long small_work_ints(int cap, int n) {
 }
 static long long large_block_ints(int cap, long long hw) {
   const long long e = kWorkEntriesPerSpot * static_cast<long long>(cap);
-  return ((e + 3) & ~3ll) + ((2ll * cap + 3) & ~3ll) + ((hw + 3) & ~3ll);
+  return ((e + 3) & ~3ll) + ((2ll * cap + 3) & ~3ll) + ((cap + 3ll) & ~3ll) + ((hw + 3) & ~3ll);
 }
 // blocks of the large-pattern kernels: two per CU
 static int large_blocks(int n, int per_cu = 2) { return std::min(n, per_cu * cu_count()); }
