@@ -199,9 +199,16 @@ __device__ __forceinline__ int class_cols(int c, int cols_max) {
   return c == 0 ? min(4, cols_max) : (c == 1 ? min(8, cols_max) : cols_max);
 }
 __device__ __forceinline__ int box_width(int xb) { return (xb & 0xffff) - (xb >> 16) + 1; }
-// it / ncols, with shifts for the class widths 4 and 8 (ncols is uniform)
+// it / ncols, a shift for powers of two (ncols is uniform)
 __device__ __forceinline__ int item_spot(int it, int ncols) {
-  return ncols == 4 ? it >> 2 : (ncols == 8 ? it >> 3 : it / ncols);
+  return (ncols & (ncols - 1)) == 0 ? it >> __builtin_ctz(ncols) : it / ncols;
+}
+// Column count of a gather pass: the next power of two up to a wave, so a
+// spot's run of items never straddles two waves and its first lane ends up
+// with the whole sum (no LDS float atomics, which run far slower than
+// integer ones)
+__device__ __forceinline__ int gather_cols(int w) {
+  return w <= 1 ? 1 : (w <= kWave ? 1 << (32 - __builtin_clz(w - 1)) : w);
 }
 
 // Splat (kGather = false) or gather (true) of every binned spot footprint
@@ -210,8 +217,10 @@ __device__ __forceinline__ int item_spot(int it, int ncols) {
 template <bool kGather>
 __device__ __forceinline__ void strip_pass(const AdjArgs &a, const SpotTab &t, int cols_max,
                                            aacc_t *acc, const float *G, int b0, int nb,
-                                           bool binned, int y0, int y1, float scale, int tid) {
-  // unbinned: LDS slots b0 .. b0 + nb - 1
+                                           bool binned, int y0, int y1, float scale, int tid,
+                                           float4 *hand = nullptr) {
+  // unbinned: LDS slots b0 .. b0 + nb - 1; hand: the gather adds each spot's
+  // sums into row jpos of the gradient hand-over instead of its LDS slot
   const int W = a.W;
   const int ncols = cols_max;
   const int n_items = nb * ncols;
@@ -300,9 +309,24 @@ __device__ __forceinline__ void strip_pass(const AdjArgs &a, const SpotTab &t, i
       }
     }
     if (q >= 0 && (c == 0 || lane == 0) && (sa != 0.0f || sx != 0.0f || sy != 0.0f)) {
-      atomicAdd(&t.A[q], sa);
-      atomicAdd(&t.BX[q], sx);
-      atomicAdd(&t.BY[q], sy);
+      if ((ncols & (ncols - 1)) == 0 && ncols <= kWave) {
+        // the run lies in this wave and this lane (c == 0) holds its sum;
+        // a spot has one run per pass
+        if (hand) {
+          float *h = reinterpret_cast<float *>(hand + t.jpos[q]);
+          unsafeAtomicAdd(h, sa);
+          unsafeAtomicAdd(h + 1, sx);
+          unsafeAtomicAdd(h + 2, sy);
+        } else {
+          t.A[q] += sa;
+          t.BX[q] += sx;
+          t.BY[q] += sy;
+        }
+      } else {
+        atomicAdd(&t.A[q], sa);
+        atomicAdd(&t.BX[q], sx);
+        atomicAdd(&t.BY[q], sy);
+      }
     }
   }
 }
@@ -720,7 +744,8 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
     auto pass_range = [&](bool gather, int ncols, int b0, int nb, bool bin, int y0, int y1) {
       if (nb <= 0) return;
       if (gather)
-        strip_pass<true>(a, t, ncols, s_acc, s_G, b0, nb, bin, y0, y1, scale, tid);
+        strip_pass<true>(a, t, gather_cols(ncols), s_acc, s_G, b0, nb, bin, y0, y1, scale, tid,
+                         large ? gsp : nullptr);
       else
         strip_pass<false>(a, t, ncols, s_acc, nullptr, b0, nb, bin, y0, y1, scale, tid);
     };
