@@ -271,8 +271,8 @@ __device__ __forceinline__ void strip_pass(const AdjArgs &a, const SpotTab &t, i
         // (a few ulps over a box; the splat keeps its exact per-pixel terms)
         const float dx = static_cast<float>(x) - t.X[q];
         float dy = static_cast<float>(ya) - t.Y[q];
-        float k = a.pref * expf(a.ep * (dy * dy + dx * dx));
-        float r = expf(a.ep * (2.0f * dy + 1.0f));
+        float k = a.pref * fast_exp(a.ep * (dy * dy + dx * dx));
+        float r = fast_exp(a.ep * (2.0f * dy + 1.0f));
         const float *g = G + (ya - y0) * W + x;
         auto row = [&](float gv) {
           const float kg = k * gv;
