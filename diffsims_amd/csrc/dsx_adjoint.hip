@@ -776,20 +776,24 @@ __device__ __forceinline__ void adjoint_body(const AdjArgs &a, aacc_t *s_acc,
           }
         };
         if (gather) {
+          // with boxes up to a wave wide every run lies in one wave and the
+          // gather itself adds the sums into the hand-over rows; wider boxes
+          // collect theirs in the LDS slots, added here (a spot's strips add
+          // in turn; the first scan zeroed its row behind a barrier)
           classes(true);
           lds_sync();
-          // the sums of this strip into the spot's gsum entry (a spot's
-          // strips add in turn; the first scan zeroed it behind a barrier)
-          for (int i = tid; i < m; i += kAdjThreads) {
-            const float A = t.A[i], BX = t.BX[i], BY = t.BY[i];
-            if (A != 0.0f || BX != 0.0f || BY != 0.0f) {
-              float *g = reinterpret_cast<float *>(gsp + t.jpos[i]);
-              unsafeAtomicAdd(g, A);
-              unsafeAtomicAdd(g + 1, BX);
-              unsafeAtomicAdd(g + 2, BY);
+          if (cols_max > kWave) {
+            for (int i = tid; i < m; i += kAdjThreads) {
+              const float A = t.A[i], BX = t.BX[i], BY = t.BY[i];
+              if (A != 0.0f || BX != 0.0f || BY != 0.0f) {
+                float *g = reinterpret_cast<float *>(gsp + t.jpos[i]);
+                unsafeAtomicAdd(g, A);
+                unsafeAtomicAdd(g + 1, BX);
+                unsafeAtomicAdd(g + 2, BY);
+              }
             }
+            lds_sync();  // the next chunk overwrites the slots
           }
-          lds_sync();  // the next chunk overwrites the slots
         } else {
           classes(false);
           lds_sync();  // the next chunk overwrites the slots
