@@ -91,6 +91,8 @@ EXPORTED = (
     "dsx_polar_templates",
     "dsx_render_fast_large_workspace_ints",
     "dsx_render_fast_large",
+    "dsx_render_fast_bands_workspace_ints",
+    "dsx_render_fast_bands",
     "dsx_euler_to_rt",
     "dsx_euler_to_rt_backward",
 )
@@ -206,6 +208,9 @@ _SIGS = {
     "dsx_render_dense_max_pixels": (_I, []),
     "dsx_render_dense_inplace_fits": (_I, [_I, _I, _I]),
     "dsx_render_fast_large_workspace_ints": (ctypes.c_longlong, [_I]),
+    "dsx_render_fast_bands_workspace_ints": (ctypes.c_longlong, [_I, _I, _I]),
+    "dsx_render_fast_bands": (_I, [_P, _P, _I, _I, ctypes.POINTER(RenderParams), _P,
+                                   ctypes.c_longlong, _P, _P, _P]),
     "dsx_euler_to_rt": (_I, [_P, _I, _P, _P]),
     "dsx_euler_to_rt_backward": (_I, [_P, _P, _I, _P, _P]),
     "dsx_render_fast_large": (_I, [_P, _P, _I, _I, ctypes.POINTER(RenderParams), _P,

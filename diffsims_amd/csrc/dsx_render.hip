@@ -85,6 +85,7 @@ struct RenderArgs {
   const long long *spot_off; // ragged: first spot of pattern p (NULL: p * cap)
   const int *capv;           // ragged: capacity of pattern p (NULL: cap)
   const int *list;           // ragged: pattern ids to render (NULL: 0..n-1)
+  const int *n_dev;          // splat kernel: pattern count in device memory (NULL: n)
   int H, W;
   float cal, cx, cy, ca, sa;
   int mirrored, normalize;
@@ -379,6 +380,8 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
   // fraction of their rate on an 8-byte-aligned base
   extern __shared__ __align__(16) racc_t s_acc[];  // strip accumulator; hash table before the passes
   __shared__ RenderShared<kMode> sh;
+  // a pattern list in device memory may hold fewer patterns than the grid has blocks
+  if (a.n_dev && static_cast<int>(blockIdx.x) >= *a.n_dev) return;
   int *h_key = reinterpret_cast<int *>(s_acc);
   int *h_val = h_key + kHash;
   const int tid = threadIdx.x;
@@ -395,8 +398,9 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
   // current pattern renders so the scans below start without a memory wait
   float4 next_head = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   int next_count = 0;
+  const int n_pat = a.n_dev ? *a.n_dev : a.n;
   auto prefetch = [&](int pi) {
-    if (pi >= a.n) return;
+    if (pi >= n_pat) return;
     const int p = a.list ? a.list[pi] : pi;
     const int c = a.capv ? a.capv[p] : a.cap;
     const float4 *sp = a.spots + (a.spot_off ? a.spot_off[p] : (long long)p * a.cap);
@@ -406,7 +410,7 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
   prefetch(blockIdx.x);
   lds_sync();
 
-  for (int pi = blockIdx.x; pi < a.n; pi += gridDim.x) {
+  for (int pi = blockIdx.x; pi < n_pat; pi += gridDim.x) {
     const int p = a.list ? a.list[pi] : pi;
     const int n = next_count;
     const float4 head = next_head;  // spots[tid] when tid < n
@@ -725,6 +729,421 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
     }
     // every thread is done with this pattern's LDS state before the next one
     lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Band renderer (fast mode): the per-pattern spot work and the pattern writes
+// run as two kernels, so that the writes come from a one-shot grid of short
+// blocks (the store pattern that reaches 6-7 TB/s here, against ~5.9 TB/s
+// for blocks streaming whole patterns).
+//
+//   render_plan_kernel (one wave per pattern): pixel coordinates, last write
+//   wins, the 16x16 tile bounds, the fixed-point scale and the exact maximum
+//   -- the same integer and float operations as render_kernel, so the output
+//   is the same bit for bit -- then the pattern's spots sorted by row, with
+//   the first spot of every row, into the workspace;
+//
+//   render_band_kernel (one block per band of rows, at most 2048 pixels):
+//   the band's window of the row-sorted list (one scalar round for its
+//   bounds, one vector round for the spots; neighbouring bands read
+//   overlapping windows, which stay in cache), the splat into LDS (32-bit
+//   fixed point), two scaled non-temporal 16-byte stores per thread.
+//
+// Patterns the plan cannot hold (more than kPlanMax listed spots, too many
+// candidate tiles for the maximum) are put on a list that render_kernel
+// renders afterwards.
+// ---------------------------------------------------------------------------
+
+constexpr int kPlanWaves = 4;  // patterns per plan block
+constexpr int kPlanThreads = kPlanWaves * kWave;
+constexpr int kPlanPerLane = 8;
+constexpr int kPlanMax = kPlanPerLane * kWave;  // listed spots per pattern
+constexpr int kPlanMaxSide = 1024;              // 10-bit pixel coordinates in the sort key
+constexpr int kBandThreads = 256;
+constexpr int kBandVec = 2;                              // float4 stores per thread
+constexpr int kBandPix = 4 * kBandVec * kBandThreads;    // pixels per band
+constexpr int kBandChunk = 32768;                        // patterns per plan/writer round
+
+struct BandArgs {
+  RenderArgs r;     // spot lists, detector mapping, taps, output
+  int *rec;         // per pattern: {listed, spots, scale, out_scale}, first spot of rows 0..H
+  int2 *ent;        // per pattern: spots sorted by row, {(y << 16) | x, I}
+  int *fallback;    // [0]: patterns listed, [1 + i]: their ids
+  int rec_ints;     // stride of rec
+  int p0, m;        // first pattern of the round, patterns in it
+  int band_rows, nb;
+  int ntx, n_tiles;
+  int wave_ints;    // LDS words of one plan wave
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+__device__ __forceinline__ long long wave_sum_ll(long long v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// fast_weight for short taps: an interior source (the footprint clear of both
+// edges) has only its own image in reach, and reflect_weight3 then adds two
+// zeros to that tap -- the same value.
+__device__ __forceinline__ float band_weight(int y, int c, int n, int R, const float *taps) {
+  return (c >= R && c + R < n) ? tap_at(c - y, R, taps) : reflect_weight3(y, c, n, R, taps);
+}
+
+// Row-sorted key of the plan: (y << 19) | (x << 9) | list position (y, x < 1024, position < 512).
+__device__ __forceinline__ int key_y(int k) { return k >> 19; }
+__device__ __forceinline__ int key_x(int k) { return (k >> 9) & 1023; }
+
+// Per-wave LDS of the plan: row starts [H + 1] | tile bounds [n_tiles] |
+// row-sorted keys and intensities [kPlanMax each] | candidate spots
+// [kCandSpots] | candidate tiles [kMaxCand] | taps [kMaxR + 1].
+__host__ __device__ __forceinline__ int plan_wave_ints(int H, int n_tiles) {
+  return (H + 1) + n_tiles + 2 * kPlanMax + kCandSpots + kMaxCand + (kMaxR + 1);
+}
+
+template <int kR>
+__global__ __launch_bounds__(kPlanThreads) void render_plan_kernel(BandArgs b) {
+  extern __shared__ int s_plan[];
+  const RenderArgs &a = b.r;
+  constexpr int R = kR;
+  const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
+  // pattern pl is planned on XCD pl mod 8 (blocks are dispatched round-robin
+  // over the 8 XCDs), where its bands are rendered
+  const int pl = (blockIdx.x >> 3) * (8 * kPlanWaves) + wid * 8 + (blockIdx.x & 7);
+  if (pl >= b.m) return;  // waves are independent: no block barriers below
+  const int p = b.p0 + pl;
+  const int H = a.H, W = a.W, ntx = b.ntx, n_tiles = b.n_tiles;
+  int *rows = s_plan + wid * b.wave_ints;
+  int *tile_u = rows + (H + 1);
+  int *s_key = tile_u + n_tiles;
+  float *s_I = reinterpret_cast<float *>(s_key + kPlanMax);
+  int *s_cq = s_key + 2 * kPlanMax;
+  int *s_ct = s_cq + kCandSpots;
+  float *s_tp = reinterpret_cast<float *>(s_ct + kMaxCand);
+  int *rec = b.rec + static_cast<size_t>(pl) * b.rec_ints;
+  const int n = min(a.count[p], a.cap);
+  bool ok = n <= kPlanMax;
+  int ns = 0;
+  float scale = 1.0f, out_scale = 1.0f;
+  if (ok) {
+    for (int t = lane; t <= H; t += kWave) rows[t] = 0;
+    for (int t = lane; t < n_tiles; t += kWave) tile_u[t] = 0;
+    if (lane <= R) s_tp[lane] = a.taps[lane];
+    wave_lds_sync();
+    // ---- pixel of every listed spot; count per row ------------------------
+    const float4 *sp = a.spots + static_cast<size_t>(p) * a.cap;
+    int key[kPlanPerLane], pos[kPlanPerLane];
+    float Iv[kPlanPerLane];
+#pragma unroll
+    for (int k = 0; k < kPlanPerLane; ++k) {
+      const int j = lane + k * kWave;
+      key[k] = -1;
+      Iv[k] = 0.0f;
+      pos[k] = 0;
+      if (j < n) {
+        const float4 s = sp[j];
+        float X, Y;
+        if (to_pixel(s, a, X, Y)) {
+          const int yi = min(static_cast<int>(Y), H - 1), xi = min(static_cast<int>(X), W - 1);
+          key[k] = (yi << 19) | (xi << 9) | j;
+          Iv[k] = s.w;
+          pos[k] = atomicAdd(&rows[yi + 1], 1);
+        }
+      }
+    }
+    wave_lds_sync();
+    // ---- row starts (exclusive scan), spots sorted by row -----------------
+    {
+      const int per = (H + kWave - 1) / kWave;
+      const int i0 = min(H, lane * per), i1 = min(H, i0 + per);
+      int sum = 0;
+      for (int t = i0; t < i1; ++t) sum += rows[1 + t];
+      int run = wave_inclusive_scan(sum) - sum;
+      for (int t = i0; t < i1; ++t) {
+        run += rows[1 + t];
+        rows[1 + t] = run;
+      }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int k = 0; k < kPlanPerLane; ++k) {
+      if (key[k] >= 0) {
+        const int q = rows[key_y(key[k])] + pos[k];
+        pos[k] = q;
+        s_key[q] = key[k];
+        s_I[q] = Iv[k];
+      }
+    }
+    wave_lds_sync();
+    // ---- last write wins: a spot yields to a later one on its pixel -------
+#pragma unroll
+    for (int k = 0; k < kPlanPerLane; ++k) {
+      if (key[k] >= 0) {
+        const int yi = key_y(key[k]), px = key[k] >> 9, j = key[k] & 511;
+        bool lose = false;
+        for (int e = rows[yi]; e < rows[yi + 1]; ++e) {
+          const int o = s_key[e];
+          lose |= (o >> 9) == px && (o & 511) > j;
+        }
+        if (lose) {
+          key[k] = -1;
+          s_I[pos[k]] = 0.0f;  // contributes nothing where the bands read it
+        }
+      }
+    }
+    wave_lds_sync();
+    float pm = 0.0f;
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < kPlanPerLane; ++k) {
+      if (key[k] >= 0) {
+        pm = fmaxf(pm, fabsf(Iv[k]));
+        ++mine;
+      }
+    }
+    const float pmax = wave_max(pm);
+    ns = wave_sum(mine);
+    if (ns > 0) {
+      // ---- tile bounds and the fixed-point scale (as tile_bounds) ---------
+      const float w0 = s_tp[0];
+      const float inv_p = pmax > 0.0f ? kBoundUnits / pmax : 0.0f;
+#pragma unroll
+      for (int k = 0; k < kPlanPerLane; ++k) {
+        if (key[k] < 0) continue;
+        const int yi = key_y(key[k]), xi = key_x(key[k]);
+        const float I = Iv[k];
+        const bool inx = xi >= R && xi + R < W, iny = yi >= R && yi + R < H;
+        const float bx = inx ? w0 : fminf(1.0f, 3.0f * w0);
+        const float by = iny ? w0 : fminf(1.0f, 3.0f * w0);
+        const float uf = fabsf(I) * bx * by;
+        const int u = min(static_cast<int>(ceilf(uf * inv_p)) + 1, static_cast<int>(kBoundUnits) + 1);
+        const int tx0 = max(0, xi - R) / kTile, tx1 = min(W - 1, xi + R) / kTile;
+        const int ty0 = max(0, yi - R) / kTile, ty1 = min(H - 1, yi + R) / kTile;
+        for (int ty = ty0; ty <= ty1; ++ty)
+          for (int tx = tx0; tx <= tx1; ++tx) {
+            int ut = u;
+            if (inx && iny) {
+              const int dx = max(0, max(tx * kTile - xi, xi - (tx * kTile + kTile - 1)));
+              const int dy = max(0, max(ty * kTile - yi, yi - (ty * kTile + kTile - 1)));
+              const float uft = fabsf(I) * s_tp[dx] * s_tp[dy];
+              ut = min(static_cast<int>(ceilf(uft * inv_p)) + 1, static_cast<int>(kBoundUnits) + 1);
+            }
+            atomicAdd(&tile_u[ty * ntx + tx], ut);
+          }
+      }
+      wave_lds_sync();
+      int lu = 0;
+      for (int t = lane; t < n_tiles; t += kWave) lu = max(lu, tile_u[t]);
+      const int Ui = wave_max_i(lu);
+      const float unit = pmax / kBoundUnits;
+      const float U = static_cast<float>(Ui) * unit;
+      scale = U > 0.0f ? kScaleRange / (U * 1.00001f) : 1.0f;
+      out_scale = 1.0f / scale;
+      if (a.normalize) {
+        // ---- exact maximum (as bounded_max) ------------------------------
+        int qc = 0x7fffffff;
+#pragma unroll
+        for (int k = 0; k < kPlanPerLane; ++k)
+          if (key[k] >= 0 && fabsf(Iv[k]) == pmax) qc = min(qc, pos[k]);
+        const int qs = wave_min_i(qc);
+        const int ys = key_y(s_key[qs]), xs = key_x(s_key[qs]);
+        long long acc = 0;
+#pragma unroll
+        for (int k = 0; k < kPlanPerLane; ++k) {
+          if (key[k] < 0) continue;
+          const int yi = key_y(key[k]), xi = key_x(key[k]);
+          acc += fast_term(Iv[k], xi, yi, max(0, yi - R), min(H - 1, yi + R), xs, ys, H, W, R, true,
+                           s_tp, scale);
+        }
+        const long long lfix = wave_sum_ll(acc);
+        int n_cand = 0;
+        for (int t0 = 0; t0 < n_tiles; t0 += kWave) {
+          const int t = t0 + lane;
+          bool c = false;
+          if (t < n_tiles) {
+            const float bound = static_cast<float>(tile_u[t]) * unit * scale * 1.00001f + 1.0f;
+            c = bound >= static_cast<float>(lfix);
+          }
+          const uint64_t mk = __ballot(c);
+          if (c) {
+            const int sl = n_cand + mask_rank(mk);
+            if (sl < kMaxCand) s_ct[sl] = t;
+          }
+          n_cand += __popcll(mk);
+        }
+        ok = n_cand <= kMaxCand;
+        wave_lds_sync();
+        long long lmax = lfix;
+        // a lane's four pixels of a tile share one column
+        const int cx = lane & (kTile - 1), cy = lane / kTile;
+        for (int c = 0; ok && c < n_cand; ++c) {
+          const int t = s_ct[c];
+          const int ty = t / ntx, tx = t - ty * ntx;
+          const int e0 = rows[max(0, ty * kTile - R)];
+          const int e1 = rows[min(H - 1, ty * kTile + kTile - 1 + R) + 1];
+          int ncq = 0;
+          for (int eb = e0; eb < e1; eb += kWave) {
+            const int e = eb + lane;
+            bool keep = false;
+            if (e < e1) {
+              const int xi = key_x(s_key[e]);
+              keep = s_I[e] != 0.0f && xi + R >= tx * kTile && xi - R <= tx * kTile + kTile - 1;
+            }
+            const uint64_t mk = __ballot(keep);
+            if (keep) {
+              const int sl = ncq + mask_rank(mk);
+              if (sl < kCandSpots) s_cq[sl] = e;
+            }
+            ncq += __popcll(mk);
+          }
+          if (ncq > kCandSpots) {
+            ok = false;
+            break;
+          }
+          wave_lds_sync();
+          const int x = tx * kTile + cx;
+          long long v[kTile * kTile / kWave] = {};
+          if (x < W) {
+            for (int k2 = 0; k2 < ncq; ++k2) {
+              const int e = s_cq[k2];
+              const int qk = s_key[e];
+              const int qy = key_y(qk), qx = key_x(qk);
+              if (x < qx - R || x > qx + R) continue;
+              // fast_term's arithmetic: ((I wx) scale) wy, truncated
+              const float iws = s_I[e] * band_weight(x, qx, W, R, s_tp) * scale;
+              const int lo = max(0, qy - R), hi = min(H - 1, qy + R);
+#pragma unroll
+              for (int u = 0; u < kTile * kTile / kWave; ++u) {
+                const int y = ty * kTile + cy + u * (kWave / kTile);
+                if (y >= lo && y <= hi) v[u] += fast_fixed(iws, band_weight(y, qy, H, R, s_tp));
+              }
+            }
+          }
+          if (x < W) {
+#pragma unroll
+            for (int u = 0; u < kTile * kTile / kWave; ++u)
+              if (ty * kTile + cy + u * (kWave / kTile) < H) lmax = v[u] > lmax ? v[u] : lmax;
+          }
+          wave_lds_sync();  // the candidate list is rebuilt for the next tile
+        }
+        if (ok) {
+          // the bound keeps every pixel sum below 2^31
+          const racc_t m = wave_max_i(static_cast<racc_t>(lmax));
+          out_scale = m > 0 ? 1.0f / static_cast<float>(m) : 1.0f / scale;
+        }
+      }
+    }
+  }
+  if (!ok) {
+    if (lane == 0) {
+      const int q = atomicAdd(b.fallback, 1);
+      b.fallback[1 + q] = p;
+      rec[0] = 1;  // the band writer leaves this pattern to render_kernel
+    }
+    return;
+  }
+  if (lane == 0) {
+    *reinterpret_cast<int4 *>(rec) =
+        make_int4(0, ns, __float_as_int(scale), __float_as_int(out_scale));
+    if (a.status) a.status[p] = 0;
+  }
+  for (int t = lane; t <= H; t += kWave) rec[4 + t] = rows[t];
+  int2 *ent = b.ent + static_cast<size_t>(pl) * kPlanMax;
+  const int total = rows[H];
+  for (int e = lane; e < total; e += kWave) {
+    const int k = s_key[e];
+    ent[e] = make_int2((key_y(k) << 16) | key_x(k), __float_as_int(s_I[e]));
+  }
+}
+
+template <int kR>
+__global__ __launch_bounds__(kBandThreads) void render_band_kernel(BandArgs b) {
+  __shared__ __align__(16) racc_t s_acc[kBandPix];
+  __shared__ int2 s_ent[kPlanMax];
+  __shared__ float s_tp[kR + 1];
+  const RenderArgs &a = b.r;
+  constexpr int R = kR;
+  constexpr int ncols = 2 * kR + 1;
+  // every band of pattern pl runs on XCD pl mod 8 (blocks are dispatched
+  // round-robin over the 8 XCDs), in order: the pattern's header and
+  // row-sorted spots come from memory once and from that XCD's L2 after
+  const int xcd = blockIdx.x & 7, seq = blockIdx.x >> 3;
+  const int grp = seq / b.nb, bb = seq - grp * b.nb;
+  const int pl = grp * 8 + xcd;
+  if (pl >= b.m) return;
+  const int p = b.p0 + pl;
+  const int tid = threadIdx.x;
+  const int H = a.H, W = a.W;
+  const int y0 = bb * b.band_rows;
+  const int y1 = min(H, y0 + b.band_rows) - 1;  // last row of the band
+  const int nvec = (y1 - y0 + 1) * W / 4;
+  // one scalar round: the pattern's header and the band's window of the
+  // row-sorted list (rows y0 - R .. y1 + R)
+  const int *rec = b.rec + static_cast<size_t>(pl) * b.rec_ints;
+  const int4 h = *reinterpret_cast<const int4 *>(rec);
+  const int e0 = rec[4 + max(0, y0 - R)], e1 = rec[4 + min(H - 1, y1 + R) + 1];
+#pragma unroll
+  for (int i = 0; i <= R; ++i)
+    if (tid == i) s_tp[i] = a.taps[i];
+  for (int v = tid; v < nvec; v += kBandThreads) reinterpret_cast<int4 *>(s_acc)[v] = make_int4(0, 0, 0, 0);
+  if (h.x != 0) return;  // listed for render_kernel (uniform across the block)
+  const int cnt = e1 - e0;
+  const int2 *ent = b.ent + static_cast<size_t>(pl) * kPlanMax + e0;
+  for (int q = tid; q < cnt; q += kBandThreads) s_ent[q] = ent[q];
+  lds_sync();
+  const float scale = __int_as_float(h.z), out_scale = __int_as_float(h.w);
+  for (int it = tid; it < cnt * ncols; it += kBandThreads) {
+    const int q = it / ncols, c = it - q * ncols;
+    const int2 en = s_ent[q];
+    const int yi = en.x >> 16, xi = en.x & 0xffff;
+    const int x = xi - R + c;
+    if (x < 0 || x >= W) continue;
+    // render_kernel's splat terms: ((I wx) scale) wy, truncated
+    const float wx = (xi >= R && xi + R < W) ? s_tp[c > R ? c - R : R - c]
+                                             : reflect_weight3(x, xi, W, R, s_tp);
+    const float iws = __int_as_float(en.y) * wx * scale;
+    const int ya = max(yi - R, y0), yb = min(yi + R, y1);
+    racc_t *col = s_acc + (x - y0 * W);
+    if (yi >= R && yi + R < H) {
+#pragma unroll
+      for (int dd = -R; dd <= R; ++dd) {
+        const int y = yi + dd;
+        if (y >= ya && y <= yb) atomicAdd(col + y * W, fast_fixed(iws, a.taps[dd < 0 ? -dd : dd]));
+      }
+    } else {
+      for (int y = ya; y <= yb; ++y) atomicAdd(col + y * W, fast_fixed(iws, reflect_weight3(y, yi, H, R, s_tp)));
+    }
+  }
+  lds_sync();
+  float4 *dst = reinterpret_cast<float4 *>(a.out + static_cast<size_t>(p) * H * W + static_cast<size_t>(y0) * W);
+#pragma unroll
+  for (int k = 0; k < kBandVec; ++k) {
+    const int v = tid + k * kBandThreads;
+    if (v < nvec) {
+      const int4 u = reinterpret_cast<const int4 *>(s_acc)[v];
+      store_nt(make_float4(static_cast<float>(u.x) * out_scale, static_cast<float>(u.y) * out_scale,
+                           static_cast<float>(u.z) * out_scale, static_cast<float>(u.w) * out_scale),
+               dst + v);
+    }
   }
 }
 
@@ -1413,6 +1832,7 @@ static int fill_render_args(int mode, const dsx_render_params *p, RenderArgs &a,
   a.spot_off = nullptr;
   a.capv = nullptr;
   a.list = nullptr;
+  a.n_dev = nullptr;
   a.H = p->height;
   a.W = p->width;
   a.cal = p->calibration;
@@ -1586,6 +2006,44 @@ static int render_common(int mode, const float *spot_d, const int *count_d, int 
   return launch_splat(mode, a, bpc, as_stream(stream), name);
 }
 
+// ---- band renderer: host side ----------------------------------------------
+
+static void band_shape(int H, int W, int &band_rows, int &nb) {
+  band_rows = std::max(1, std::min(H, kBandPix / std::max(1, W)));
+  nb = (H + band_rows - 1) / band_rows;
+}
+
+// Shapes the plan and the band writer take: tap radius 1..8 below both
+// pattern sides, rows of whole float4 vectors, sides of at most 1024 pixels
+// (10-bit coordinates in the plan's sort key), tile bounds in the plan's LDS.
+static bool band_fits(const RenderArgs &a) {
+  const int n_tiles = ((a.W + kTile - 1) / kTile) * ((a.H + kTile - 1) / kTile);
+  return a.R >= 1 && a.R <= 8 && a.R < std::min(a.H, a.W) && (a.W & 3) == 0 &&
+         a.W <= kPlanMaxSide && a.H <= kPlanMaxSide && n_tiles <= kMaxTiles;
+}
+
+static long long band_list_ints(int n) { return (1LL + n + 3) & ~3LL; }
+static int band_rec_ints(int H) { return (4 + H + 1 + 3) & ~3; }
+
+// Workspace: the list of patterns left to render_kernel (count + ids, all
+// rounds) | per pattern of a round: header and row starts | row-sorted spots.
+static long long band_workspace_ints(int n, int H) {
+  const long long chunk = std::min(std::max(n, 1), kBandChunk);
+  return band_list_ints(std::max(n, 1)) + chunk * band_rec_ints(H) + chunk * kPlanMax * 2;
+}
+
+__global__ void band_list_reset_kernel(int *count) {
+  if (threadIdx.x == 0) *count = 0;
+}
+
+template <int kR>
+static void launch_bands_r(const BandArgs &b, size_t plan_lds, hipStream_t st) {
+  const int plan_blocks = (b.m + 8 * kPlanWaves - 1) / (8 * kPlanWaves) * 8;
+  hipLaunchKernelGGL(render_plan_kernel<kR>, dim3(plan_blocks), dim3(kPlanThreads), plan_lds, st, b);
+  const unsigned band_blocks = static_cast<unsigned>((b.m + 7) / 8) * 8u * static_cast<unsigned>(b.nb);
+  hipLaunchKernelGGL(render_band_kernel<kR>, dim3(band_blocks), dim3(kBandThreads), 0, st, b);
+}
+
 }  // namespace dsx
 
 using namespace dsx;
@@ -1634,6 +2092,78 @@ extern "C" int dsx_render_fast_ragged(const float *spot_d, const long long *spot
     return launch_inplace(a, blocks_per_cu, as_stream(stream), name);
   return method == DSX_RENDER_DENSE ? launch_dense(a, blocks_per_cu, as_stream(stream), name)
                                     : launch_splat(kModeFast, a, bpc, as_stream(stream), name);
+}
+
+extern "C" long long dsx_render_fast_bands_workspace_ints(int n, int height, int width) {
+  if (n < 0 || height < 1 || width < 1) return -1;
+  return band_workspace_ints(n, height);
+}
+
+extern "C" int dsx_render_fast_bands(const float *spot_d, const int *count_d, int cap, int n,
+                                     const dsx_render_params *params, int *work_d,
+                                     long long work_ints, float *out_d, int *status_d,
+                                     void *stream) {
+  const char *name = "dsx_render_fast_bands";
+  clear_error();
+  if (!params || n < 0 || cap < 1 || work_ints < 0) {
+    set_error("%s: bad arguments (n=%d cap=%d)", name, n, cap);
+    return DSX_EINVAL;
+  }
+  if (n == 0) return DSX_OK;
+  if (!spot_d || !count_d || !out_d) {
+    set_error("%s: null device pointer", name);
+    return DSX_EINVAL;
+  }
+  RenderArgs a;
+  const int bpc = splat_blocks_per_cu(kModeFast, params->normalize != 0, 0);
+  const int st = fill_render_args(kModeFast, params, a, name, strip_bytes_for(bpc));
+  if (st != DSX_OK) return st;
+  a.spots = reinterpret_cast<const float4 *>(spot_d);
+  a.count = count_d;
+  a.cap = cap;
+  a.n = n;
+  a.out = out_d;
+  a.status = status_d;
+  hipStream_t s = as_stream(stream);
+  if (!band_fits(a)) return launch_splat(kModeFast, a, bpc, s, name);
+  const long long need = band_workspace_ints(n, a.H);
+  if (!work_d || work_ints < need) {
+    set_error("%s: workspace of %lld ints is smaller than %lld", name, work_ints, need);
+    return DSX_EINVAL;
+  }
+  BandArgs b;
+  b.r = a;
+  band_shape(a.H, a.W, b.band_rows, b.nb);
+  b.ntx = (a.W + kTile - 1) / kTile;
+  b.n_tiles = b.ntx * ((a.H + kTile - 1) / kTile);
+  b.wave_ints = plan_wave_ints(a.H, b.n_tiles);
+  const int chunk = std::min(n, kBandChunk);
+  b.fallback = work_d;
+  b.rec_ints = band_rec_ints(a.H);
+  b.rec = work_d + band_list_ints(n);
+  b.ent = reinterpret_cast<int2 *>(b.rec + static_cast<long long>(chunk) * b.rec_ints);
+  const size_t plan_lds = static_cast<size_t>(kPlanWaves) * b.wave_ints * sizeof(int);
+  hipLaunchKernelGGL(band_list_reset_kernel, dim3(1), dim3(64), 0, s, b.fallback);
+  for (int c0 = 0; c0 < n; c0 += chunk) {
+    b.p0 = c0;
+    b.m = std::min(chunk, n - c0);
+    switch (a.R) {
+      case 1: launch_bands_r<1>(b, plan_lds, s); break;
+      case 2: launch_bands_r<2>(b, plan_lds, s); break;
+      case 3: launch_bands_r<3>(b, plan_lds, s); break;
+      case 4: launch_bands_r<4>(b, plan_lds, s); break;
+      case 5: launch_bands_r<5>(b, plan_lds, s); break;
+      case 6: launch_bands_r<6>(b, plan_lds, s); break;
+      case 7: launch_bands_r<7>(b, plan_lds, s); break;
+      default: launch_bands_r<8>(b, plan_lds, s);
+    }
+  }
+  // the patterns the plans listed, by the splat renderer (blocks without a
+  // listed pattern exit at once)
+  RenderArgs f = a;
+  f.list = b.fallback + 1;
+  f.n_dev = b.fallback;
+  return launch_splat(kModeFast, f, bpc, s, name);
 }
 
 extern "C" int dsx_render_dense_max_pixels(void) { return kDenseMaxPix; }

@@ -504,10 +504,16 @@ def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigm
             _lib.check(L.dsx_render_fast_large(
                 _lib.ptr(spots), _lib.ptr(count), cap, n, ctypes.byref(p), _lib.ptr(work),
                 work.numel(), _lib.ptr(out), _lib.ptr(status), stream), "dsx_render_fast_large")
+    elif fast:
+        # plan kernel + one-shot band writer (dsx_render_fast's output, written
+        # at the store rate of a one-shot grid)
+        work = _workspace(dev, int(L.dsx_render_fast_bands_workspace_ints(n, H, W)), stream)
+        _lib.check(L.dsx_render_fast_bands(
+            _lib.ptr(spots), _lib.ptr(count), cap, n, ctypes.byref(p), _lib.ptr(work), work.numel(),
+            _lib.ptr(out), _lib.ptr(status), stream), "dsx_render_fast_bands")
     else:
-        fn = L.dsx_render_fast if fast else L.dsx_render_subpixel
-        _lib.check(fn(_lib.ptr(spots), _lib.ptr(count), cap, n, ctypes.byref(p), _lib.ptr(out),
-                      _lib.ptr(status), stream), "dsx_render")
+        _lib.check(L.dsx_render_subpixel(_lib.ptr(spots), _lib.ptr(count), cap, n, ctypes.byref(p),
+                                         _lib.ptr(out), _lib.ptr(status), stream), "dsx_render")
     if sync_check:
         check_render_status(status)
     return out

@@ -278,6 +278,27 @@ int dsx_render_fast(const float *spot_d, const int *count_d, int cap, int n,
                     void *stream);
 
 /*
+ * fast=True rendering of n patterns, the same output as dsx_render_fast bit
+ * for bit — replaces the same reference path (simulation2d.py:405-442,
+ * detector_functions.py:293-300) — in two kernels: a plan kernel (one wave per
+ * pattern: pixels, last write wins, the exact maximum) hands each band of
+ * rows its spots through work_d, then a one-shot grid writes every band
+ * (one 16-byte store per thread).  Patterns the plan cannot hold are rendered
+ * by dsx_render_fast's kernel in the same call.  Shapes outside the band
+ * path (tap radius above 8 or not below both sides, width not a multiple of 4
+ * or above 1024, height above 1024) run dsx_render_fast's kernel alone and
+ * leave work_d unused.
+ *   work_d : dsx_render_fast_bands_workspace_ints(n, height, width) ints
+ *            (patterns are taken in rounds of at most 32,768); overlapping
+ *            calls must not share it
+ * Other arguments as for dsx_render_fast.
+ */
+long long dsx_render_fast_bands_workspace_ints(int n, int height, int width);
+int dsx_render_fast_bands(const float *spot_d, const int *count_d, int cap, int n,
+                          const dsx_render_params *params, int *work_d, long long work_ints,
+                          float *out_d, int *status_d, void *stream);
+
+/*
  * fast=True rendering of large patterns with many spots (the same result as
  * dsx_render_fast, no limit on spots per pattern): each output strip rebuilds
  * the spot image of its window in LDS and filters it; in-frame spots are
