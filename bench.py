@@ -32,13 +32,15 @@ METRIC = "patterns/s: batched fwd & fwd+bwd vs plain fwd, 1 MI355X, rel err ≤1
 HBM_PEAK_GBS = 8000.0
 
 
-PROFILE = os.path.join(ROOT, "profiles", "render_fast_ortho40_library_kernel_trace.md")
+PROFILE = os.path.join(ROOT, "profiles", "render_bands_ortho40_library.md")
 
 
 def measured_traffic(render_bytes):
-    """HBM bytes of one render launch: the FETCH_SIZE/WRITE_SIZE measurement
-    committed in profiles/ (taken on the 200,000-orientation library), scaled
-    to this run's algorithmic bytes; None when the summary is absent."""
+    """HBM bytes of one render stage (plan kernel, band writer and the splat
+    kernel for the patterns the plan lists): the FETCH_SIZE/WRITE_SIZE
+    measurement committed in profiles/ (taken on the 200,000-orientation
+    library), scaled to this run's algorithmic bytes; None when the summary
+    is absent."""
     import re
     try:
         text = open(PROFILE).read()
@@ -701,8 +703,10 @@ def main():
     ren_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
     counts = batch.count.float()
     mean_spots = float(counts.mean())
-    # algorithmic bytes of one render launch: every pattern written once, its
-    # spot list and count read once
+    # algorithmic bytes of one render stage: every pattern written once, its
+    # spot list and count read once.  The stage is the plan kernel, the band
+    # writer and the splat kernel for listed patterns (dsx_render_fast_bands),
+    # timed together by the events around engine.render_spots
     render_bytes = n * (H * W * 4) + float(counts.sum()) * 16 + n * 4
     achieved = render_bytes / (ren_ms * 1e-3) / 1e9
     value = args.steps * n / dt
@@ -727,6 +731,8 @@ def main():
             "n_orientations": n, "n_hkl": tab.n, "mean_spots": round(mean_spots, 2),
             "mode": "batched forward (simulate + render)",
             "kernel_ms": {"simulate": round(sim_ms, 3), "render": round(ren_ms, 3)},
+            "render_kernels": "render_plan_kernel + render_band_kernel (+ render_kernel for "
+                              "patterns the plan lists)",
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

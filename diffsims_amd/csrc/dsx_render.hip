@@ -808,15 +808,25 @@ __device__ __forceinline__ float band_weight(int y, int c, int n, int R, const f
   return (c >= R && c + R < n) ? tap_at(c - y, R, taps) : reflect_weight3(y, c, n, R, taps);
 }
 
+// Distance from pixel c to the nearest pixel of tile t along one axis.
+__device__ __forceinline__ int tile_gap(int t, int c) {
+  return max(0, max(t * kTile - c, c - (t * kTile + kTile - 1)));
+}
+
+// tile_bounds' integer units of a term |I| wx wy (rounded up, plus one).
+__device__ __forceinline__ int bound_units(float aI, float wx, float wy, float inv_p) {
+  return min(static_cast<int>(ceilf(aI * wx * wy * inv_p)) + 1, static_cast<int>(kBoundUnits) + 1);
+}
+
 // Row-sorted key of the plan: (y << 19) | (x << 9) | list position (y, x < 1024, position < 512).
 __device__ __forceinline__ int key_y(int k) { return k >> 19; }
 __device__ __forceinline__ int key_x(int k) { return (k >> 9) & 1023; }
 
 // Per-wave LDS of the plan: row starts [H + 1] | tile bounds [n_tiles] |
 // row-sorted keys and intensities [kPlanMax each] | candidate spots
-// [kCandSpots] | candidate tiles [kMaxCand] | taps [kMaxR + 1].
+// [kCandSpots] | candidate tiles and their bounds [kMaxCand each] | taps [kMaxR + 1].
 __host__ __device__ __forceinline__ int plan_wave_ints(int H, int n_tiles) {
-  return (H + 1) + n_tiles + 2 * kPlanMax + kCandSpots + kMaxCand + (kMaxR + 1);
+  return (H + 1) + n_tiles + 2 * kPlanMax + kCandSpots + 2 * kMaxCand + (kMaxR + 1);
 }
 
 template <int kR>
@@ -838,7 +848,8 @@ __global__ __launch_bounds__(kPlanThreads) void render_plan_kernel(BandArgs b) {
   float *s_I = reinterpret_cast<float *>(s_key + kPlanMax);
   int *s_cq = s_key + 2 * kPlanMax;
   int *s_ct = s_cq + kCandSpots;
-  float *s_tp = reinterpret_cast<float *>(s_ct + kMaxCand);
+  float *s_cb = reinterpret_cast<float *>(s_ct + kMaxCand);
+  float *s_tp = s_cb + kMaxCand;
   int *rec = b.rec + static_cast<size_t>(pl) * b.rec_ints;
   const int n = min(a.count[p], a.cap);
   bool ok = n <= kPlanMax;
@@ -936,19 +947,27 @@ __global__ __launch_bounds__(kPlanThreads) void render_plan_kernel(BandArgs b) {
         const float by = iny ? w0 : fminf(1.0f, 3.0f * w0);
         const float uf = fabsf(I) * bx * by;
         const int u = min(static_cast<int>(ceilf(uf * inv_p)) + 1, static_cast<int>(kBoundUnits) + 1);
+        // a footprint of at most 17 pixels reaches at most two tiles per axis
         const int tx0 = max(0, xi - R) / kTile, tx1 = min(W - 1, xi + R) / kTile;
         const int ty0 = max(0, yi - R) / kTile, ty1 = min(H - 1, yi + R) / kTile;
-        for (int ty = ty0; ty <= ty1; ++ty)
-          for (int tx = tx0; tx <= tx1; ++tx) {
-            int ut = u;
-            if (inx && iny) {
-              const int dx = max(0, max(tx * kTile - xi, xi - (tx * kTile + kTile - 1)));
-              const int dy = max(0, max(ty * kTile - yi, yi - (ty * kTile + kTile - 1)));
-              const float uft = fabsf(I) * s_tp[dx] * s_tp[dy];
-              ut = min(static_cast<int>(ceilf(uft * inv_p)) + 1, static_cast<int>(kBoundUnits) + 1);
-            }
-            atomicAdd(&tile_u[ty * ntx + tx], ut);
-          }
+        int u00 = u, u01 = u, u10 = u, u11 = u;
+        if (inx && iny) {
+          // the taps fall off from the centre: a tile's largest weight is at
+          // its pixel nearest (xi, yi)
+          const float aI = fabsf(I);
+          const float wx0 = s_tp[tile_gap(tx0, xi)], wx1 = s_tp[tile_gap(tx1, xi)];
+          const float wy0 = s_tp[tile_gap(ty0, yi)], wy1 = s_tp[tile_gap(ty1, yi)];
+          u00 = bound_units(aI, wx0, wy0, inv_p);
+          u01 = bound_units(aI, wx1, wy0, inv_p);
+          u10 = bound_units(aI, wx0, wy1, inv_p);
+          u11 = bound_units(aI, wx1, wy1, inv_p);
+        }
+        atomicAdd(&tile_u[ty0 * ntx + tx0], u00);
+        if (tx1 != tx0) atomicAdd(&tile_u[ty0 * ntx + tx1], u01);
+        if (ty1 != ty0) {
+          atomicAdd(&tile_u[ty1 * ntx + tx0], u10);
+          if (tx1 != tx0) atomicAdd(&tile_u[ty1 * ntx + tx1], u11);
+        }
       }
       wave_lds_sync();
       int lu = 0;
@@ -992,10 +1011,31 @@ __global__ __launch_bounds__(kPlanThreads) void render_plan_kernel(BandArgs b) {
         }
         ok = n_cand <= kMaxCand;
         wave_lds_sync();
+        // the candidates in descending order of their bounds: once a bound
+        // is below the largest exact value found, no later tile can hold a
+        // larger one (exact comparison in double)
+        float cb = 0.0f;
+        int ct = 0, rank = 0;
+        if (ok && lane < n_cand) {
+          ct = s_ct[lane];
+          cb = static_cast<float>(tile_u[ct]) * unit * scale * 1.00001f + 1.0f;
+        }
+        for (int o = 0; ok && o < n_cand; ++o) {
+          const float ob = __shfl(cb, o, kWave);
+          rank += (ob > cb || (ob == cb && o < lane)) ? 1 : 0;
+        }
+        wave_lds_sync();
+        if (ok && lane < n_cand) {
+          s_ct[rank] = ct;
+          s_cb[rank] = cb;
+        }
+        wave_lds_sync();
         long long lmax = lfix;
+        int best = static_cast<int>(lfix);  // the bound keeps every pixel sum below 2^31
         // a lane's four pixels of a tile share one column
         const int cx = lane & (kTile - 1), cy = lane / kTile;
         for (int c = 0; ok && c < n_cand; ++c) {
+          if (static_cast<double>(s_cb[c]) < static_cast<double>(best)) break;
           const int t = s_ct[c];
           const int ty = t / ntx, tx = t - ty * ntx;
           const int e0 = rows[max(0, ty * kTile - R)];
@@ -1043,6 +1083,7 @@ __global__ __launch_bounds__(kPlanThreads) void render_plan_kernel(BandArgs b) {
             for (int u = 0; u < kTile * kTile / kWave; ++u)
               if (ty * kTile + cy + u * (kWave / kTile) < H) lmax = v[u] > lmax ? v[u] : lmax;
           }
+          best = wave_max_i(static_cast<int>(lmax));
           wave_lds_sync();  // the candidate list is rebuilt for the next tile
         }
         if (ok) {
@@ -1066,7 +1107,11 @@ __global__ __launch_bounds__(kPlanThreads) void render_plan_kernel(BandArgs b) {
         make_int4(0, ns, __float_as_int(scale), __float_as_int(out_scale));
     if (a.status) a.status[p] = 0;
   }
-  for (int t = lane; t <= H; t += kWave) rec[4 + t] = rows[t];
+  // each band's window of the row-sorted list: rows y0 - R .. y1 + R
+  for (int bb = lane; bb < b.nb; bb += kWave) {
+    const int y0 = bb * b.band_rows, y1 = min(H, y0 + b.band_rows) - 1;
+    reinterpret_cast<int2 *>(rec + 4)[bb] = make_int2(rows[max(0, y0 - R)], rows[min(H - 1, y1 + R) + 1]);
+  }
   int2 *ent = b.ent + static_cast<size_t>(pl) * kPlanMax;
   const int total = rows[H];
   for (int e = lane; e < total; e += kWave) {
@@ -1100,7 +1145,8 @@ __global__ __launch_bounds__(kBandThreads) void render_band_kernel(BandArgs b) {
   // row-sorted list (rows y0 - R .. y1 + R)
   const int *rec = b.rec + static_cast<size_t>(pl) * b.rec_ints;
   const int4 h = *reinterpret_cast<const int4 *>(rec);
-  const int e0 = rec[4 + max(0, y0 - R)], e1 = rec[4 + min(H - 1, y1 + R) + 1];
+  const int2 win = reinterpret_cast<const int2 *>(rec + 4)[bb];
+  const int e0 = win.x, e1 = win.y;
 #pragma unroll
   for (int i = 0; i <= R; ++i)
     if (tid == i) s_tp[i] = a.taps[i];
@@ -2023,13 +2069,15 @@ static bool band_fits(const RenderArgs &a) {
 }
 
 static long long band_list_ints(int n) { return (1LL + n + 3) & ~3LL; }
-static int band_rec_ints(int H) { return (4 + H + 1 + 3) & ~3; }
+static int band_rec_ints(int nb) { return (4 + 2 * nb + 3) & ~3; }
 
 // Workspace: the list of patterns left to render_kernel (count + ids, all
-// rounds) | per pattern of a round: header and row starts | row-sorted spots.
-static long long band_workspace_ints(int n, int H) {
+// rounds) | per pattern of a round: header and each band's window | row-sorted spots.
+static long long band_workspace_ints(int n, int H, int W) {
+  int br, nb;
+  band_shape(H, W, br, nb);
   const long long chunk = std::min(std::max(n, 1), kBandChunk);
-  return band_list_ints(std::max(n, 1)) + chunk * band_rec_ints(H) + chunk * kPlanMax * 2;
+  return band_list_ints(std::max(n, 1)) + chunk * band_rec_ints(nb) + chunk * kPlanMax * 2;
 }
 
 __global__ void band_list_reset_kernel(int *count) {
@@ -2096,7 +2144,7 @@ extern "C" int dsx_render_fast_ragged(const float *spot_d, const long long *spot
 
 extern "C" long long dsx_render_fast_bands_workspace_ints(int n, int height, int width) {
   if (n < 0 || height < 1 || width < 1) return -1;
-  return band_workspace_ints(n, height);
+  return band_workspace_ints(n, height, width);
 }
 
 extern "C" int dsx_render_fast_bands(const float *spot_d, const int *count_d, int cap, int n,
@@ -2126,7 +2174,7 @@ extern "C" int dsx_render_fast_bands(const float *spot_d, const int *count_d, in
   a.status = status_d;
   hipStream_t s = as_stream(stream);
   if (!band_fits(a)) return launch_splat(kModeFast, a, bpc, s, name);
-  const long long need = band_workspace_ints(n, a.H);
+  const long long need = band_workspace_ints(n, a.H, a.W);
   if (!work_d || work_ints < need) {
     set_error("%s: workspace of %lld ints is smaller than %lld", name, work_ints, need);
     return DSX_EINVAL;
@@ -2139,7 +2187,7 @@ extern "C" int dsx_render_fast_bands(const float *spot_d, const int *count_d, in
   b.wave_ints = plan_wave_ints(a.H, b.n_tiles);
   const int chunk = std::min(n, kBandChunk);
   b.fallback = work_d;
-  b.rec_ints = band_rec_ints(a.H);
+  b.rec_ints = band_rec_ints(b.nb);
   b.rec = work_d + band_list_ints(n);
   b.ent = reinterpret_cast<int2 *>(b.rec + static_cast<long long>(chunk) * b.rec_ints);
   const size_t plan_lds = static_cast<size_t>(kPlanWaves) * b.wave_ints * sizeof(int);

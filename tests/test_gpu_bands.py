@@ -157,9 +157,10 @@ def test_dense_rows_read_the_row_sorted_list():
 
 
 def test_rounds_of_patterns():
-    # more patterns than one plan/writer round holds
-    print("bands: 33000 x 32x32", flush=True)
-    spots, cnt = _spot_lists(33000, 8, 32, 32, 4, seed=6, dup=0.05)
+    # three plan/writer rounds: the third reuses the first round's buffers
+    # while the second round's bands are written
+    print("bands: 70000 x 32x32", flush=True)
+    spots, cnt = _spot_lists(70000, 8, 32, 32, 4, seed=6, dup=0.05)
     ref, out, st_ref, st, _ = _render_both(spots, cnt, 8, (32, 32), 1.0)
     _assert_same(ref, out, st_ref, st)
 
