@@ -823,10 +823,11 @@ __device__ __forceinline__ int key_y(int k) { return k >> 19; }
 __device__ __forceinline__ int key_x(int k) { return (k >> 9) & 1023; }
 
 // Per-wave LDS of the plan: row starts [H + 1] | tile bounds [n_tiles] |
-// row-sorted keys and intensities [kPlanMax each] | candidate spots
-// [kCandSpots] | candidate tiles and their bounds [kMaxCand each] | taps [kMaxR + 1].
+// row-sorted keys and intensities [kPlanMax each] | one candidate tile's
+// pixels [kTile^2] | candidate spots [kCandSpots] | candidate tiles and their
+// bounds [kMaxCand each] | taps [kMaxR + 1].
 __host__ __device__ __forceinline__ int plan_wave_ints(int H, int n_tiles) {
-  return (H + 1) + n_tiles + 2 * kPlanMax + kCandSpots + 2 * kMaxCand + (kMaxR + 1);
+  return (H + 1) + n_tiles + 2 * kPlanMax + kTile * kTile + kCandSpots + 2 * kMaxCand + (kMaxR + 1);
 }
 
 template <int kR>
@@ -846,7 +847,8 @@ __global__ __launch_bounds__(kPlanThreads) void render_plan_kernel(BandArgs b) {
   int *tile_u = rows + (H + 1);
   int *s_key = tile_u + n_tiles;
   float *s_I = reinterpret_cast<float *>(s_key + kPlanMax);
-  int *s_cq = s_key + 2 * kPlanMax;
+  int *s_tile = s_key + 2 * kPlanMax;
+  int *s_cq = s_tile + kTile * kTile;
   int *s_ct = s_cq + kCandSpots;
   float *s_cb = reinterpret_cast<float *>(s_ct + kMaxCand);
   float *s_tp = s_cb + kMaxCand;
@@ -1030,10 +1032,8 @@ __global__ __launch_bounds__(kPlanThreads) void render_plan_kernel(BandArgs b) {
           s_cb[rank] = cb;
         }
         wave_lds_sync();
-        long long lmax = lfix;
-        int best = static_cast<int>(lfix);  // the bound keeps every pixel sum below 2^31
-        // a lane's four pixels of a tile share one column
-        const int cx = lane & (kTile - 1), cy = lane / kTile;
+        // the bound keeps every pixel sum below 2^31
+        int best = static_cast<int>(lfix);
         for (int c = 0; ok && c < n_cand; ++c) {
           if (static_cast<double>(s_cb[c]) < static_cast<double>(best)) break;
           const int t = s_ct[c];
@@ -1059,38 +1059,42 @@ __global__ __launch_bounds__(kPlanThreads) void render_plan_kernel(BandArgs b) {
             ok = false;
             break;
           }
-          wave_lds_sync();
-          const int x = tx * kTile + cx;
-          long long v[kTile * kTile / kWave] = {};
-          if (x < W) {
-            for (int k2 = 0; k2 < ncq; ++k2) {
-              const int e = s_cq[k2];
-              const int qk = s_key[e];
-              const int qy = key_y(qk), qx = key_x(qk);
-              if (x < qx - R || x > qx + R) continue;
-              // fast_term's arithmetic: ((I wx) scale) wy, truncated
-              const float iws = s_I[e] * band_weight(x, qx, W, R, s_tp) * scale;
-              const int lo = max(0, qy - R), hi = min(H - 1, qy + R);
 #pragma unroll
-              for (int u = 0; u < kTile * kTile / kWave; ++u) {
-                const int y = ty * kTile + cy + u * (kWave / kTile);
-                if (y >= lo && y <= hi) v[u] += fast_fixed(iws, band_weight(y, qy, H, R, s_tp));
+          for (int u = 0; u < kTile * kTile / kWave; ++u) s_tile[lane + u * kWave] = 0;
+          wave_lds_sync();
+          // the tile's pixels from a splat of its spots, item = (spot, column):
+          // fast_term's terms ((I wx) scale) wy, truncated, summed in 32 bits
+          const int ty0 = ty * kTile, ty1 = min(H, ty0 + kTile) - 1;
+          for (int it = lane; it < ncq * kTile; it += kWave) {
+            const int e = s_cq[it / kTile], cx = it & (kTile - 1);
+            const int x = tx * kTile + cx;
+            const int qk = s_key[e];
+            const int qy = key_y(qk), qx = key_x(qk);
+            if (x >= W || x < qx - R || x > qx + R) continue;
+            const float iws = s_I[e] * band_weight(x, qx, W, R, s_tp) * scale;
+            int *col = s_tile + cx - ty0 * kTile;
+            if (qy >= R && qy + R < H) {
+#pragma unroll
+              for (int dd = -R; dd <= R; ++dd) {
+                const int y = qy + dd;
+                if (y >= ty0 && y <= ty1) atomicAdd(col + y * kTile, fast_fixed(iws, a.taps[dd < 0 ? -dd : dd]));
               }
+            } else {
+              for (int y = max(ty0, qy - R); y <= min(ty1, qy + R); ++y)
+                atomicAdd(col + y * kTile, fast_fixed(iws, reflect_weight3(y, qy, H, R, s_tp)));
             }
           }
-          if (x < W) {
+          wave_lds_sync();
+          int lm = best;
 #pragma unroll
-            for (int u = 0; u < kTile * kTile / kWave; ++u)
-              if (ty * kTile + cy + u * (kWave / kTile) < H) lmax = v[u] > lmax ? v[u] : lmax;
+          for (int u = 0; u < kTile * kTile / kWave; ++u) {
+            const int pix = lane + u * kWave;
+            if (tx * kTile + (pix & (kTile - 1)) < W && ty0 + pix / kTile <= ty1) lm = max(lm, s_tile[pix]);
           }
-          best = wave_max_i(static_cast<int>(lmax));
-          wave_lds_sync();  // the candidate list is rebuilt for the next tile
+          best = wave_max_i(lm);
+          wave_lds_sync();  // the candidate list and the tile are rebuilt for the next tile
         }
-        if (ok) {
-          // the bound keeps every pixel sum below 2^31
-          const racc_t m = wave_max_i(static_cast<racc_t>(lmax));
-          out_scale = m > 0 ? 1.0f / static_cast<float>(m) : 1.0f / scale;
-        }
+        if (ok) out_scale = best > 0 ? 1.0f / static_cast<float>(best) : 1.0f / scale;
       }
     }
   }
