@@ -73,7 +73,14 @@ __host__ __device__ __forceinline__ float pretest_threshold(double me, double k,
 
 // Candidate queue per wave: (reflection row << kSlotBits) | orientation slot, in the
 // order the sweep produced them (row-major, so per orientation ascending rows).
-constexpr int kQueue = 64 + kOT * kWave;  // one batch plus one worst-case chunk
+// Ring of at least one batch plus one worst-case step (64 + kOT * 64
+// entries), a power of two so that positions wrap with a mask: candidates
+// are spread thinly over the rows, so nearly every step queues a few for
+// every orientation, and a modulo by a non-power of two there cost more
+// than the pretest itself.
+constexpr int kQueue = 512;
+static_assert(kQueue >= 64 + kOT * kWave && (kQueue & (kQueue - 1)) == 0, "queue ring size");
+__device__ __forceinline__ int qpos(int i) { return i & (kQueue - 1); }
 constexpr int kWavesPerBlock = kSimThreads / kWave;
 
 // Register budgets (waves per SIMD), measured: the full sweep at 5 (96 VGPRs)
@@ -158,7 +165,7 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
       double x = 0.0, y = 0.0, z = 0.0;
       float I = 0.0f;
       if (lane < nb) {
-        const int e = queue[(head + lane) % kQueue];
+        const int e = queue[qpos(head + lane)];
         row = e >> kSlotBits;
         slot = e & (kOT - 1);
         const int4 h = hkl[row];
@@ -192,7 +199,7 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
         }
         cnt[o] += __popcll(m);
       }
-      head = (head + nb) % kQueue;
+      head = qpos(head + nb);
       qn -= nb;
     };
 
@@ -203,7 +210,7 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
       const bool pre = v && fabsf(fmaf(-two_k, zf, g.w)) < pre_thr;
       const uint64_t pm = __ballot(pre);
       if (pm == 0) return;
-      if (pre) queue[(head + qn + mask_rank(pm)) % kQueue] = (row << kSlotBits) | o;
+      if (pre) queue[qpos(head + qn + mask_rank(pm))] = (row << kSlotBits) | o;
       qn += __popcll(pm);
     };
 
@@ -359,7 +366,7 @@ __global__ __launch_bounds__(kSimThreads, sim_min_waves(kAvg, kLines)) void simu
           const bool pre = v && fabsf(e) < pre_thr;
           const uint64_t pm = __ballot(pre);
           if (pm != 0) {
-            if (pre) queue[(head + qn + mask_rank(pm)) % kQueue] = (row << kSlotBits) | lo;
+            if (pre) queue[qpos(head + qn + mask_rank(pm))] = (row << kSlotBits) | lo;
             qn += __popcll(pm);
           }
         }
