@@ -44,6 +44,7 @@ RENDER_SPLAT = 0
 RENDER_DENSE = 1
 RENDER_DENSE_INPLACE = 2
 
+LINE_SWEEP_MIN_ROWS = 10  # DSX_LINE_SWEEP_MIN_ROWS
 MESH_UV_SPHERE = 0
 MESH_NORMALIZED_CUBE = 1
 MESH_SPHERIFIED_EDGE = 2

@@ -36,9 +36,12 @@ constexpr int kOT = 4;
 constexpr int kSlotBits = 2;      // bits of an orientation slot in a queue entry
 static_assert((1 << kSlotBits) == kOT, "orientation slot bits");
 constexpr int kSimThreads = 256;  // 4 waves per block
-// The line sweep pays once lines average this many rows (measured break-even
-// about 15: a line costs about as much as a 64-row step of the full sweep).
-constexpr int kMinLineRows = 16;
+// The line sweep pays once lines average this many rows.  Measured with the
+// intervals of a group formed for all orientations at once: the 40-atom
+// library table (14.6 rows per line) 2.19 ms full sweep against 1.55 ms line
+// sweep for 200,000 orientations; thresholds of 6 or 3 gain nothing more on
+// the 64-phase ragged batch.
+constexpr int kMinLineRows = DSX_LINE_SWEEP_MIN_ROWS;
 
 __host__ __device__ __forceinline__ bool lines_pay(int n_refl, int n_lines) {
   return n_lines > 0 && static_cast<long long>(n_refl) >= static_cast<long long>(kMinLineRows) * n_lines;

@@ -218,6 +218,18 @@ def forward_args(leaves: dict):
 # ---------------------------------------------------------------------------
 
 
+# The line index follows the lattice, so a refinement step rebuilds it (three
+# launches) before the spot kernel can use it.  That pays on tables whose lines
+# average at least LINE_INDEX_LONG_ROWS rows whatever the batch (the 500-atom
+# cell, 109 rows per line, in slices of 8,192 orientations), and on lines of
+# DSX_LINE_SWEEP_MIN_ROWS or more only for large batches: the 24-atom
+# refinement (14.1 rows per line, 4,096 orientations) measured 0.92 ms per
+# fused step without the index and 1.01 ms with it, while the 40-atom library
+# (14.6 rows per line, 200,000 orientations) gains 0.64 ms from it.
+LINE_INDEX_LONG_ROWS = 16
+LINE_INDEX_MIN_ORIENTATIONS = 16384
+
+
 @dataclass
 class _Tables:
     hkl: torch.Tensor   # int32 [N, 4]
@@ -266,7 +278,7 @@ class RefinementModel:
         # the line index's rows are fixed by the hkl set, its |g|^2 bounds follow
         # the lattice; built per step only where the spot kernel uses it
         self.n_lines = init.n_lines
-        self.use_lines = init.n_lines > 0 and init.n >= 16 * init.n_lines
+        self.use_lines = init.n_lines > 0 and init.n >= _lib.LINE_SWEEP_MIN_ROWS * init.n_lines
         # atoms sorted by element (the structure-factor kernel's order)
         elements = [element_of(s) for s in phase.symbols]
         uniq = sorted(set(elements))
@@ -319,8 +331,7 @@ class RefinementModel:
             engine.SCATTER_IDS[self.scatter], _lib.ptr(g4), _lib.ptr(F2), _lib.ptr(F),
             stream), "dsx_structure_factor")
         g_max = float(np.sqrt(np.max(np.sum((self.hkl_np @ rec_np.T) ** 2, axis=1)))) if N else 0.0
-        lines = engine.build_lines(self.hkl, g4, self.n_lines, stream) if self.use_lines else None
-        return _Tables(self.hkl, g4, F2, F, fx, Bs, rec_np, g_max, lines)
+        return _Tables(self.hkl, g4, F2, F, fx, Bs, rec_np, g_max)
 
     def _phase_table(self, t: _Tables) -> engine.PhaseTable:
         return engine.PhaseTable(t.hkl, t.g4, t.F2, t.F, self.n_refl, False, t.g_max, t.rec,
@@ -329,8 +340,12 @@ class RefinementModel:
     def spots(self, t: _Tables, Rt: torch.Tensor, batch: Optional[engine.SpotBatch] = None,
               check: bool = True) -> engine.SpotBatch:
         """Spot lists of every orientation (dsx_simulate)."""
-        tab = self._phase_table(t)
         Rt = Rt.detach()
+        long_lines = self.n_refl >= LINE_INDEX_LONG_ROWS * self.n_lines
+        if t.lines is None and self.use_lines and (
+                long_lines or int(Rt.shape[0]) >= LINE_INDEX_MIN_ORIENTATIONS):
+            t.lines = engine.build_lines(self.hkl, t.g4, self.n_lines, _lib.stream_handle(self.dev))
+        tab = self._phase_table(t)
         if batch is None:
             batch = engine.empty_batch(tab, int(Rt.shape[0]), self.cap, self.dev)
         engine.simulate_into(tab, Rt.contiguous(), self.settings, batch)

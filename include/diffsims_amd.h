@@ -131,6 +131,10 @@ int dsx_structure_factor(const int *hkl_d, int n, const double rec[9],
  *   work_d      : device int32 scratch of dsx_hkl_lines_workspace_ints ints
  */
 long long dsx_hkl_lines_workspace_ints(int n_refl);
+/* dsx_simulate / dsx_simulate_ragged take the line sweep for a table whose
+ * lines average at least this many rows (n_refl >= DSX_LINE_SWEEP_MIN_ROWS *
+ * n_lines); shorter lines take the full sweep.  The output is the same. */
+#define DSX_LINE_SWEEP_MIN_ROWS 10
 int dsx_hkl_lines(const int *hkl_d, const float *g4_d, int n_refl, int *lines_d,
                   int capacity, int *count_d, int *work_d, long long work_ints,
                   void *stream);

@@ -77,8 +77,10 @@ def assert_same_batches(a, b):
 
 
 def uses_lines(tab):
-    """The kernel takes the line sweep when lines average >= 16 rows."""
-    return tab.lines is not None and tab.n >= 16 * tab.n_lines
+    """The kernel takes the line sweep when lines average at least
+    DSX_LINE_SWEEP_MIN_ROWS rows."""
+    from diffsims_amd import _lib
+    return tab.lines is not None and tab.n >= _lib.LINE_SWEEP_MIN_ROWS * tab.n_lines
 
 
 def sweep_and_lines(tab, M, st, rr):
@@ -180,8 +182,8 @@ def test_line_sweep_equals_full_sweep(model, kw):
 
 
 def test_line_sweep_at_library_size():
-    """The large template library's phase at reciprocal radius 2.5 (its lines
-    at 2.0 are too short for the line sweep): 8,192 orientations, both sweeps."""
+    """The large template library's phase at reciprocal radius 2.5: 8,192
+    orientations, both sweeps."""
     from diffsims_amd import engine
 
     print("line sweep library size", flush=True)
@@ -200,7 +202,8 @@ def test_ragged_line_sweep_equals_full_sweep():
 
     print("ragged line sweep", flush=True)
     specs = [P.al_c0(), hexagonal(), P.ortho40(), P.triclinic(9, 4, edges=(8, 16))]
-    radii = [1.2, 2.5, 1.7, 1.4]  # phases 1 and 3 take the line sweep, 0 and 2 the full one
+    # rows per line 6.4, 19.1, 8.8, 29.0: phases 1 and 3 take the line sweep, 0 and 2 the full one
+    radii = [1.2, 2.5, 1.2, 1.4]
     rots = [rotations(n, 40 + i) for i, n in enumerate((33, 70, 129, 12))]
     st = engine.SimSettings(accelerating_voltage=200, max_excitation_error=0.02)
     lib = RaggedLibrary([P.to_arrays(s, f"p{i}") for i, s in enumerate(specs)], rots, radii, st,

@@ -126,6 +126,8 @@ def test_beam_direction_mesh_sizes_on_the_host():
     for name in ("UV_SPHERE", "NORMALIZED_CUBE", "SPHERIFIED_EDGE", "SPHERIFIED_CORNER"):
         v = int(re.search(rf"#define DSX_MESH_{name} (\d+)", text).group(1))
         assert v == getattr(_lib, f"MESH_{name}")
+    v = int(re.search(r"#define DSX_LINE_SWEEP_MIN_ROWS (\d+)", text).group(1))
+    assert v == _lib.LINE_SWEEP_MIN_ROWS
 
 
 def test_crop_planes_match_the_oracle_triangles():
