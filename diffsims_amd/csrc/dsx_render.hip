@@ -822,12 +822,13 @@ __device__ __forceinline__ int bound_units(float aI, float wx, float wy, float i
 __device__ __forceinline__ int key_y(int k) { return k >> 19; }
 __device__ __forceinline__ int key_x(int k) { return (k >> 9) & 1023; }
 
-// Per-wave LDS of the plan: row starts [H + 1] | tile bounds [n_tiles] |
-// row-sorted keys and intensities [kPlanMax each] | one candidate tile's
-// pixels [kTile^2] | candidate spots [kCandSpots] | candidate tiles and their
-// bounds [kMaxCand each] | taps [kMaxR + 1].
+// Per-wave LDS of the plan: row starts [H + 1] | tile bounds, later one
+// candidate tile's pixels [max(n_tiles, kTile^2)] | row-sorted keys and
+// intensities [kPlanMax each] | candidate spots [kCandSpots] | candidate
+// tiles and their bounds [kMaxCand each] | taps [kMaxR + 1].
 __host__ __device__ __forceinline__ int plan_wave_ints(int H, int n_tiles) {
-  return (H + 1) + n_tiles + 2 * kPlanMax + kTile * kTile + kCandSpots + 2 * kMaxCand + (kMaxR + 1);
+  return (H + 1) + (n_tiles > kTile * kTile ? n_tiles : kTile * kTile) + 2 * kPlanMax + kCandSpots +
+         2 * kMaxCand + (kMaxR + 1);
 }
 
 template <int kR>
@@ -845,10 +846,12 @@ __global__ __launch_bounds__(kPlanThreads) void render_plan_kernel(BandArgs b) {
   const int H = a.H, W = a.W, ntx = b.ntx, n_tiles = b.n_tiles;
   int *rows = s_plan + wid * b.wave_ints;
   int *tile_u = rows + (H + 1);
-  int *s_key = tile_u + n_tiles;
+  // the candidate tiles' pixels take the bounds' place once the candidates
+  // and their bounds have been read out
+  int *s_tile = tile_u;
+  int *s_key = tile_u + max(n_tiles, kTile * kTile);
   float *s_I = reinterpret_cast<float *>(s_key + kPlanMax);
-  int *s_tile = s_key + 2 * kPlanMax;
-  int *s_cq = s_tile + kTile * kTile;
+  int *s_cq = s_key + 2 * kPlanMax;
   int *s_ct = s_cq + kCandSpots;
   float *s_cb = reinterpret_cast<float *>(s_ct + kMaxCand);
   float *s_tp = s_cb + kMaxCand;
