@@ -140,6 +140,21 @@ def test_degenerate_lists():
     assert torch.equal(out[0:16], torch.zeros_like(out[0:16]))
 
 
+@pytest.mark.parametrize("shape", [(100, 132), (250, 36), (17, 1020)])
+def test_partial_tiles_and_counts_above_capacity(shape):
+    # sides that are not multiples of the 16-pixel tile (partial tiles in the
+    # bounds and in the candidate evaluation), and counts above the list
+    # capacity (only the first cap spots exist)
+    import torch
+    H, W = shape
+    print(f"bands: 300 x {H}x{W}, partial tiles", flush=True)
+    spots, cnt = _spot_lists(300, 96, H, W, 60, seed=20 + W, edge=0.2, dup=0.05)
+    cnt[::7] = 96 + 40
+    ref, out, st_ref, st, listed = _render_both(spots, cnt, 96, shape, 1.0)
+    _assert_same(ref, out, st_ref, st)
+    assert listed < 300
+
+
 def test_dense_rows_read_the_row_sorted_list():
     # most spots in a 12-row stripe: bands with more spots than their slot
     # holds take the band's window of the row-sorted list instead
