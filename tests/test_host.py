@@ -279,6 +279,29 @@ def test_host_shape_models_match_oracle():
                                O.sf_lorentzian_precession(s, 0.02, 1.3, 0.01), rtol=1e-12)
 
 
+def test_extreme_rows_hold_the_largest_g():
+    """The refinement's largest |g| over the convex hull's rows equals the
+    maximum over the whole hkl set for arbitrary bases (ball-shaped sets as
+    the box enumeration makes them, a planar set, a line and tiny sets, which
+    keep every row)."""
+    from diffsims_amd.differentiable import _extreme_rows
+
+    rng = np.random.default_rng(5)
+    _, rec = O.lattice_basis(7.3, 9.1, 11.7, 83.0, 101.0, 95.0)
+    ball = O.hkl_box(rec, 1.0 / 1.6).astype(np.float64)
+    plane = np.array([[h, k, 0] for h in range(-6, 7) for k in range(-5, 6)], dtype=np.float64)
+    line = np.array([[0, 0, l] for l in range(-9, 10)], dtype=np.float64)
+    tiny = np.array([[1, 0, 0], [0, 1, 0], [1, 1, 1]], dtype=np.float64)
+    for rows in (ball, plane, line, tiny):
+        ext = _extreme_rows(rows)
+        assert ext.shape[0] <= rows.shape[0]
+        for _ in range(20):
+            r = rec @ (np.eye(3) + 0.3 * rng.normal(size=(3, 3)))
+            full = np.max(np.sum((rows @ r.T) ** 2, axis=1))
+            assert np.max(np.sum((ext @ r.T) ** 2, axis=1)) == full
+    assert _extreme_rows(ball).shape[0] < ball.shape[0] // 5
+
+
 # -- bench contract ---------------------------------------------------------------
 
 def test_bench_refuses_more_than_one_gpu():
