@@ -742,13 +742,14 @@ __global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel
 //   wins, the 16x16 tile bounds, the fixed-point scale and the exact maximum
 //   -- the same integer and float operations as render_kernel, so the output
 //   is the same bit for bit -- then the pattern's spots sorted by row, with
-//   the first spot of every row, into the workspace;
+//   each band's window of that list, into the workspace;
 //
-//   render_band_kernel (one block per band of rows, at most 2048 pixels):
-//   the band's window of the row-sorted list (one scalar round for its
-//   bounds, one vector round for the spots; neighbouring bands read
-//   overlapping windows, which stay in cache), the splat into LDS (32-bit
-//   fixed point), two scaled non-temporal 16-byte stores per thread.
+//   render_band_kernel (one block per band of rows, at most 2048 pixels,
+//   every band of a pattern on one XCD): the band's window of the row-sorted
+//   list (one scalar round for its bounds, one vector round for the spots;
+//   neighbouring bands read overlapping windows, which stay in that XCD's
+//   L2), the splat into LDS (32-bit fixed point), two scaled non-temporal
+//   16-byte stores per thread.
 //
 // Patterns the plan cannot hold (more than kPlanMax listed spots, too many
 // candidate tiles for the maximum) are put on a list that render_kernel

@@ -287,7 +287,7 @@ int dsx_render_fast(const float *spot_d, const int *count_d, int cap, int n,
  * detector_functions.py:293-300) — in two kernels: a plan kernel (one wave per
  * pattern: pixels, last write wins, the exact maximum) hands each band of
  * rows its spots through work_d, then a one-shot grid writes every band
- * (one 16-byte store per thread).  Patterns the plan cannot hold are rendered
+ * (two 16-byte stores per thread).  Patterns the plan cannot hold are rendered
  * by dsx_render_fast's kernel in the same call.  Shapes outside the band
  * path (tap radius above 8 or not below both sides, width not a multiple of 4
  * or above 1024, height above 1024) run dsx_render_fast's kernel alone and
