@@ -234,7 +234,7 @@ def plain_leg(dev, steps, warmup):
         "n_patterns": n, "n_hkl": tab.n,
         "patterns_per_s": round(n / dt, 1), "ms_per_step": round(1e3 * dt, 4),
         "kernel_ms": {"simulate": round(sim_ms, 4), "render": round(ren_ms, 4)},
-        "roofline": {"bound": "hbm", "kernel": "render stage (dsx_render_fast_bands)",
+        "roofline": {"bound": "hbm", "kernel": f"render stage ({engine.fast_renderer(n, H, W)})",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu_baseline_entry(n / cdt, "all 500 orientations, FP64 NumPy "

@@ -44,6 +44,28 @@ SHAPE_MODEL_IDS = {
 SCATTER_IDS = {None: _lib.SCATTER_NONE, "lobato": _lib.SCATTER_LOBATO,
                "xtables": _lib.SCATTER_XTABLES}
 
+# Fast rendering of small batches: below SPLAT_MAX_PATTERNS patterns of at
+# most SPLAT_MAX_PIXELS pixels the persistent splat kernel (dsx_render_fast,
+# one launch) finishes before the band renderer (dsx_render_fast_bands: four
+# launches, and a plan wave per pattern whose latency is not hidden by a
+# small grid).  Measured with HIP events, both giving the same bits:
+# 500 x 144x144 with ~40 spots 26 us (splat) against 38 us (bands), 500 x
+# 256x256 with ~260 spots 47 us against 58 us; at 1,000 patterns the two are
+# equal, and from 2,000 on the band writer is faster (32,768 x 256x256:
+# 1.63 ms against 1.33 ms).  Larger patterns always take the band renderer,
+# since a few hundred persistent blocks cannot keep the stores in flight.
+SPLAT_MAX_PATTERNS = 768
+SPLAT_MAX_PIXELS = 256 * 256
+
+
+def fast_renderer(n: int, height: int, width: int) -> str:
+    """Which entry point ``render_spots`` uses for ``n`` fast patterns of at
+    most dsx_render_max_spots() spots: "dsx_render_fast" or
+    "dsx_render_fast_bands" (the same output bit for bit)."""
+    if n < SPLAT_MAX_PATTERNS and height * width <= SPLAT_MAX_PIXELS:
+        return "dsx_render_fast"
+    return "dsx_render_fast_bands"
+
 
 def _device(device=None) -> torch.device:
     _lib.load(require_gpu=True)
@@ -504,6 +526,9 @@ def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigm
             _lib.check(L.dsx_render_fast_large(
                 _lib.ptr(spots), _lib.ptr(count), cap, n, ctypes.byref(p), _lib.ptr(work),
                 work.numel(), _lib.ptr(out), _lib.ptr(status), stream), "dsx_render_fast_large")
+    elif fast and fast_renderer(n, H, W) == "dsx_render_fast":
+        _lib.check(L.dsx_render_fast(_lib.ptr(spots), _lib.ptr(count), cap, n, ctypes.byref(p),
+                                     _lib.ptr(out), _lib.ptr(status), stream), "dsx_render_fast")
     elif fast:
         # plan kernel + one-shot band writer (dsx_render_fast's output, written
         # at the store rate of a one-shot grid)

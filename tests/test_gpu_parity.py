@@ -190,6 +190,13 @@ def test_render_fast_small_case():
     _render_parity(P.al_c0(), st, 1.0, 500, 20261202, (144, 144), 1.0, 0.01)
 
 
+def test_render_fast_band_path():
+    # above the small-batch threshold engine.render takes the band renderer
+    assert eng().fast_renderer(1200, 144, 144) == "dsx_render_fast_bands"
+    st = eng().SimSettings(max_excitation_error=0.01, shape_model="linear")
+    _render_parity(P.al_c0(), st, 1.0, 1200, 20261203, (144, 144), 1.0, 0.01)
+
+
 def test_render_fast_large_cell():
     st = eng().SimSettings(max_excitation_error=0.02, shape_model="lorentzian")
     _render_parity(P.ortho40(), st, 2.0, 64, 3, (256, 256), 1.0, 0.015)

@@ -302,6 +302,19 @@ def test_extreme_rows_hold_the_largest_g():
     assert _extreme_rows(ball).shape[0] < ball.shape[0] // 5
 
 
+def test_fast_renderer_choice():
+    """Small batches of patterns up to 256x256 take the splat kernel alone,
+    everything else the band renderer (the same output either way)."""
+    from diffsims_amd.engine import fast_renderer
+
+    assert fast_renderer(500, 144, 144) == "dsx_render_fast"
+    assert fast_renderer(767, 256, 256) == "dsx_render_fast"
+    assert fast_renderer(768, 256, 256) == "dsx_render_fast_bands"
+    assert fast_renderer(200000, 256, 256) == "dsx_render_fast_bands"
+    assert fast_renderer(300, 512, 512) == "dsx_render_fast_bands"
+    assert fast_renderer(10, 256, 257) == "dsx_render_fast_bands"
+
+
 # -- bench contract ---------------------------------------------------------------
 
 def test_bench_refuses_more_than_one_gpu():
