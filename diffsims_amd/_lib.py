@@ -222,11 +222,12 @@ _SIGS = {
 }
 
 _lib = None
+_gpu_seen = False  # a GPU was visible once (it stays visible for the process)
 
 
 def load(require_gpu: bool = False):
     """Load the native library (once).  Raises RuntimeError when it is absent."""
-    global _lib
+    global _lib, _gpu_seen
     if _lib is None:
         import torch  # noqa: F401  (binds the library to torch's HIP runtime)
 
@@ -240,11 +241,12 @@ def load(require_gpu: bool = False):
             fn.restype = res
             fn.argtypes = args
         _lib = lib
-    if require_gpu:
+    if require_gpu and not _gpu_seen:
         import torch
 
         if not torch.cuda.is_available():
             raise RuntimeError("diffsims_amd needs a ROCm GPU; none is visible")
+        _gpu_seen = True
     return _lib
 
 
@@ -268,6 +270,18 @@ def ptr(t) -> int:
 
 
 def stream_handle(device=None) -> int:
+    """hipStream_t of torch's current stream on ``device`` (a torch.device,
+    device string or index; the current device when None), as an int.  Read
+    with torch's raw-stream accessor, which returns the same handle as
+    ``torch.cuda.current_stream(device).cuda_stream`` without building a
+    Stream object (a few microseconds per launch on the refinement path)."""
     import torch
 
-    return torch.cuda.current_stream(device).cuda_stream
+    if device is None:
+        idx = torch.cuda.current_device()
+    elif isinstance(device, int):
+        idx = device
+    else:
+        d = device if isinstance(device, torch.device) else torch.device(device)
+        idx = d.index if d.index is not None else torch.cuda.current_device()
+    return torch._C._cuda_getCurrentRawStream(idx)

@@ -315,7 +315,13 @@ class RefinementModel:
             np.asarray(phase.occ, np.float32)[self.order.numpy()])).to(self.dev)
         self.coeff = torch.from_numpy(coeff).to(self.dev)
         self.order_dev = self.order.to(self.dev)
-        self.inv_order_dev = torch.argsort(self.order).to(self.dev)
+        inv = torch.argsort(self.order)
+        self.inv_order_dev = inv.to(self.dev)
+        # the atoms' gradient words in the caller's order (dxyz, 3 per atom,
+        # then dB), read from the sorted accumulator by one gather
+        na = self.n_atoms
+        self.grad_perm_dev = torch.cat([(inv[:, None] * 3 + torch.arange(3)).reshape(-1),
+                                        3 * na + inv]).to(self.dev)
         if cap is None:
             cap = engine.default_cap(init, settings, reciprocal_radius)
         self.cap = int(cap)
@@ -402,6 +408,9 @@ class RefinementModel:
         computed."""
         n = batch.n
         H, W = self.shape
+        Rt = Rt.detach().contiguous()
+        if pattern is not None and not pattern.is_contiguous():
+            raise ValueError("pattern must be a contiguous float32 [n, H, W] tensor")
         stride = 0
         if target is not None:
             if target.dtype != torch.float32 or not target.is_contiguous():
@@ -417,11 +426,15 @@ class RefinementModel:
         if want_grad:
             dRt = torch.empty((n, 9), dtype=torch.float64, device=self.dev)
         if table:
-            # the accumulated gradients, zeroed by one fill each:
-            # dF2 [n_refl] | dxyz [na, 3] | dB [na] (float32) ; drec [9] (float64)
-            acc = torch.zeros(self.n_refl + 4 * na, dtype=torch.float32, device=self.dev)
+            # the accumulated gradients, zeroed by one fill: dF2 [n_refl] |
+            # dxyz [na, 3] | dB [na] (float32), then drec [9] (float64) at an
+            # 8-byte aligned offset
+            nf = self.n_refl + 4 * na
+            nf += nf & 1
+            words = torch.zeros(nf + 18, dtype=torch.int32, device=self.dev)
+            acc = words[:nf].view(torch.float32)
             dF2 = acc[:self.n_refl]
-            drec = torch.zeros(9, dtype=torch.float64, device=self.dev)
+            drec = words[nf:].view(torch.float64)
         sync_status = status is None
         if status is None:
             status = torch.zeros(n, dtype=torch.int32, device=self.dev)
@@ -433,19 +446,20 @@ class RefinementModel:
         if want_grad and batch.cap > int(self.L.dsx_render_max_spots()):
             step = max(1, min(n, GRAD_SUMS_BYTES // (16 * batch.cap)))
         work = self.workspace(batch.cap, step, want_grad)
+        # per-pattern buffers are contiguous rows: a call's first pattern is
+        # addressed by pointer offsets (no tensor views per call)
+        cap = batch.cap
+        rows = ((batch.idx, 4 * cap), (batch.spots, 16 * cap), (batch.count, 4), (Rt, 72),
+                (target, 4 * stride), (pattern, 4 * H * W), (loss, 4), (dRt, 72), (status, 4))
+        base = [(_lib.ptr(x), b if x is not None else 0) for x, b in rows]
         for o in range(0, n, step):
             e = min(n, o + step)
-
-            def sl(x):
-                return None if x is None else x[o:e]
+            a = [p + o * b if p else 0 for p, b in base]
             _lib.check(self.L.dsx_render_adjoint(
-                _lib.ptr(batch.idx[o:e]), _lib.ptr(batch.spots[o:e]), _lib.ptr(batch.count[o:e]),
-                batch.cap, e - o, _lib.ptr(t.hkl), _lib.ptr(t.F2), self.n_refl,
-                _lib.ptr(Rt[o:e]), ctypes.byref(sp), ctypes.byref(self.render), mode,
-                _lib.ptr(target[o:e] if stride else target), stride, _lib.ptr(sl(pattern)),
-                _lib.ptr(loss[o:e]), _lib.ptr(sl(dRt)), _lib.ptr(dF2),
-                _lib.ptr(drec if lattice else None), _lib.ptr(work), work.numel(),
-                _lib.ptr(status[o:e]), stream), "dsx_render_adjoint")
+                a[0], a[1], a[2], cap, e - o, _lib.ptr(t.hkl), _lib.ptr(t.F2), self.n_refl,
+                a[3], ctypes.byref(sp), ctypes.byref(self.render), mode, a[4], stride, a[5],
+                a[6], a[7], _lib.ptr(dF2), _lib.ptr(drec if lattice else None), _lib.ptr(work),
+                work.numel(), a[8], stream), "dsx_render_adjoint")
         if sync_status:
             engine.check_render_status(status)
         dxyz = dB = None
@@ -460,8 +474,10 @@ class RefinementModel:
                 engine.SCATTER_IDS[self.scatter], _lib.ptr(dF2), _lib.ptr(dxyz_s), _lib.ptr(dB_s),
                 _lib.ptr(drec), stream), "dsx_structure_factor_adjoint")
             if atoms:
-                dxyz = dxyz_s[:self.n_atoms][self.inv_order_dev]
-                dB = dB_s[:self.n_atoms][self.inv_order_dev]
+                # float64 in the caller's atom order, one gather and one conversion
+                g = acc[o:].index_select(0, self.grad_perm_dev).to(torch.float64)
+                dxyz = g[:3 * self.n_atoms].view(self.n_atoms, 3)
+                dB = g[3 * self.n_atoms:]
             if not lattice:
                 drec = None
         return loss, dRt, dF2, dxyz, dB, drec
@@ -505,13 +521,18 @@ def _grow_if_needed(model: "RefinementModel", batch: engine.SpotBatch, status: t
 
 def _fetch_step(batch: engine.SpotBatch, status: torch.Tensor, loss: torch.Tensor,
                 drec: Optional[torch.Tensor]) -> np.ndarray:
-    """The step's one device->host copy: [sum of the losses, max ncand, max
-    status, dL/drec (9, when given)] as float64."""
-    parts = [loss.sum(dtype=torch.float64).view(1),
-             torch.stack([batch.ncand.max(), status.max()]).to(torch.float64)]
+    """The step's one device->host copy (the per-pattern losses, candidate
+    counts and statuses and dL/drec, gathered as 32-bit words by one
+    concatenation), reduced on the host to [sum of the losses (float64),
+    max ncand, max status, dL/drec (9, when given)] as float64."""
+    n = batch.n
+    parts = [loss.view(torch.int32), batch.ncand, status]
     if drec is not None:
-        parts.append(drec)
-    return torch.cat(parts).cpu().numpy()
+        parts.append(drec.view(torch.int32))
+    w = torch.cat(parts).cpu().numpy()
+    head = [w[:n].view(np.float32).sum(dtype=np.float64), w[n:2 * n].max(), w[2 * n:3 * n].max()]
+    tail = w[3 * n:].view(np.float64) if drec is not None else np.zeros(0)
+    return np.concatenate([np.array(head, dtype=np.float64), tail])
 
 
 class _L2Loss(torch.autograd.Function):

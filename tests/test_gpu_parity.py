@@ -388,3 +388,23 @@ def test_render_workspace_on_two_streams():
         for out, status in zip(outs, stats):
             e.check_render_status(status)
             assert float((out - ref).abs().max()) <= tol
+
+
+def test_stream_handle_is_the_current_stream():
+    """The handle every entry point receives is torch's current stream, on
+    the default stream and inside a stream context, however the device is
+    named."""
+    import torch
+
+    from diffsims_amd import _lib
+
+    idx = torch.cuda.current_device()
+    names = (None, idx, f"cuda:{idx}", torch.device("cuda", idx), torch.device("cuda"), "cuda")
+    side = torch.cuda.Stream()
+    for s in (torch.cuda.current_stream(), side):
+        with torch.cuda.stream(s):
+            want = torch.cuda.current_stream(idx).cuda_stream
+            assert want == s.cuda_stream
+            for d in names:
+                assert _lib.stream_handle(d) == want, d
+    assert side.cuda_stream != 0
