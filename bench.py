@@ -33,17 +33,18 @@ HBM_PEAK_GBS = 8000.0
 
 
 PROFILE = os.path.join(ROOT, "profiles", "render_bands_ortho40_library.md")
+REFINE_PROFILE = os.path.join(ROOT, "profiles", "render_adjoint_mono24_refinement_counters.md")
 
 
-def measured_traffic(render_bytes):
-    """HBM bytes of one render stage (plan kernel, band writer and the splat
-    kernel for the patterns the plan lists): the FETCH_SIZE/WRITE_SIZE
-    measurement committed in profiles/ (taken on the 200,000-orientation
-    library), scaled to this run's algorithmic bytes; None when the summary
-    is absent."""
+def measured_traffic(render_bytes, profile=PROFILE):
+    """HBM bytes per launch from a FETCH_SIZE/WRITE_SIZE measurement committed
+    in profiles/ (by default the library's render stage: plan kernel, band
+    writer and the splat kernel for the patterns the plan lists, taken on the
+    200,000-orientation library), scaled to this run's algorithmic bytes;
+    None when the summary is absent."""
     import re
     try:
-        text = open(PROFILE).read()
+        text = open(profile).read()
         t = float(re.search(r"traffic_bytes_per_launch: (\d+)", text).group(1))
         a = float(re.search(r"algorithmic_bytes_per_launch: (\d+)", text).group(1))
     except (OSError, AttributeError):
@@ -360,7 +361,8 @@ def refinement_leg(dev, steps, warmup):
         "dtype": "f32 (FP64 tables and excitation errors)",
         "roofline": {"bound": "hbm", "kernel": "dsx_render_adjoint + dsx_structure_factor_adjoint "
                      "(timed together)", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(adj_bytes, REFINE_PROFILE)},
         "cpu_baseline": cpu_baseline_entry(
             n_cpu / cdt, f"first {n_cpu} of {n} patterns, FP64 torch restatement of the fused "
             "forward+backward (oracle forward, frozen structure, autograd)", cdt),
