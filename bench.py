@@ -34,6 +34,7 @@ HBM_PEAK_GBS = 8000.0
 
 PROFILE = os.path.join(ROOT, "profiles", "render_bands_ortho40_library.md")
 REFINE_PROFILE = os.path.join(ROOT, "profiles", "render_adjoint_mono24_refinement_counters.md")
+RAGGED_PROFILE = os.path.join(ROOT, "profiles", "render_dense_inplace_ragged_64phase.md")
 
 
 def measured_traffic(render_bytes, profile=PROFILE):
@@ -474,7 +475,9 @@ def ragged_leg(dev, steps, scale):
                   "render_blocks_per_cu": {str(k_): v for k_, v in lib.render_bpc.items()}},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     # the counters measured the in-place dense renderer
+                     "traffic": (measured_traffic(dom_bytes, RAGGED_PROFILE)
+                                 if dom == _lib.RENDER_DENSE_INPLACE else None)},
         "cpu_baseline": cpu_baseline_entry(
             cpu_n / cpu_s, f"first 8 orientations of each of the 64 phases ({cpu_n} patterns), "
             "FP64 NumPy restatement of the reference loop incl. render", cpu_s),
