@@ -68,7 +68,17 @@ constexpr int kStripBytes3 = 28 * 1024;      // three blocks per CU
 
 enum { kModeFast = 0, kModeSubpixel = 1 };
 
+// Waves per SIMD the splat kernel is compiled for.  The sub-pixel mode always
+// runs two blocks per CU (four waves per SIMD), so it may use 128 VGPRs: held
+// to 80 it spilled 36 bytes per lane to scratch, written and read back for
+// every pattern.
+template <int kMode>
+constexpr int render_min_waves() {
+  return kMode == kModeFast ? kRenderMinWaves : 4;
+}
+
 static int splat_blocks_per_cu(int mode, bool normalize, int requested) {
+  if (mode == kModeSubpixel) return 2;  // its kernel is compiled for four waves per SIMD
   if (requested > 0) return std::min(requested, kMaxRenderBlocksPerCU);
   return mode == kModeFast && normalize ? 3 : 2;
 }
@@ -375,7 +385,7 @@ __device__ bool bounded_max(RenderShared<kMode> &sh, int *tile_u, short (*cand_q
 // kR > 0: the fast-mode tap radius as a compile-time constant (the footprint
 // loops unroll and the taps stay in scalar registers); 0: taken from the args.
 template <int kMode, int kR>
-__global__ __launch_bounds__(kRenderThreads, kRenderMinWaves) void render_kernel(RenderArgs a) {
+__global__ __launch_bounds__(kRenderThreads, render_min_waves<kMode>()) void render_kernel(RenderArgs a) {
   // 16-byte aligned: the read-outs use 16-byte LDS accesses, which run at a
   // fraction of their rate on an 8-byte-aligned base
   extern __shared__ __align__(16) racc_t s_acc[];  // strip accumulator; hash table before the passes
