@@ -35,6 +35,7 @@ HBM_PEAK_GBS = 8000.0
 PROFILE = os.path.join(ROOT, "profiles", "render_bands_ortho40_library.md")
 REFINE_PROFILE = os.path.join(ROOT, "profiles", "render_adjoint_mono24_refinement_counters.md")
 RAGGED_PROFILE = os.path.join(ROOT, "profiles", "render_dense_inplace_ragged_64phase.md")
+SMALL_PROFILE = os.path.join(ROOT, "profiles", "render_fast_fcc_al_small_kernel_trace.md")
 
 
 def measured_traffic(render_bytes, profile=PROFILE):
@@ -238,7 +239,10 @@ def plain_leg(dev, steps, warmup):
         "kernel_ms": {"simulate": round(sim_ms, 4), "render": round(ren_ms, 4)},
         "roofline": {"bound": "hbm", "kernel": f"render stage ({engine.fast_renderer(n, H, W)})",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     # the counters measured the splat kernel, which renders this batch
+                     "traffic": (measured_traffic(render_bytes, SMALL_PROFILE)
+                                 if engine.fast_renderer(n, H, W) == "dsx_render_fast" else None)},
         "cpu_baseline": cpu_baseline_entry(n / cdt, "all 500 orientations, FP64 NumPy "
                                            "restatement of the reference loop incl. render", cdt),
         "accuracy": acc,
