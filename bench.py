@@ -36,6 +36,7 @@ PROFILE = os.path.join(ROOT, "profiles", "render_bands_ortho40_library.md")
 REFINE_PROFILE = os.path.join(ROOT, "profiles", "render_adjoint_mono24_refinement_counters.md")
 RAGGED_PROFILE = os.path.join(ROOT, "profiles", "render_dense_inplace_ragged_64phase.md")
 SMALL_PROFILE = os.path.join(ROOT, "profiles", "render_fast_fcc_al_small_kernel_trace.md")
+LARGE_PROFILE = os.path.join(ROOT, "profiles", "render_adjoint_large_tri500_counters.md")
 
 
 def measured_traffic(render_bytes, profile=PROFILE):
@@ -621,9 +622,12 @@ def large_precession_leg(dev, n_chunks):
         "setup_s": round(setup, 2), "loss_first_chunk": losses[0],
         "roofline": {"bound": "hbm", "kernel": "fused chunk (render_adjoint_large_kernel dominant)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "note": "spot lists read once and the shared target: the kernel is bound by "
-                             "LDS and VALU work per spot, not by HBM"},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": measured_traffic(chunk_bytes, LARGE_PROFILE),
+                     "note": "algorithmic bytes: the spot lists read once and the shared target; "
+                             "the measured traffic adds the spot lists' writing and the adjoint's "
+                             "intermediate pixels and per-spot sums. The kernel is bound by LDS "
+                             "and VALU work per spot, not by HBM"},
         "cpu_baseline": cpu_baseline_entry(
             1 / cdt, "1 orientation, FP64 NumPy restatement of the reference loop (F of the "
             "intersecting rows, precession shape factor) incl. sub-pixel render, forward only",
