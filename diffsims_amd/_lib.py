@@ -284,4 +284,7 @@ def stream_handle(device=None) -> int:
     else:
         d = device if isinstance(device, torch.device) else torch.device(device)
         idx = d.index if d.index is not None else torch.cuda.current_device()
-    return torch._C._cuda_getCurrentRawStream(idx)
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if raw is None:  # a torch build without the accessor
+        return torch.cuda.current_stream(idx).cuda_stream
+    return raw(idx)
