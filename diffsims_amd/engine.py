@@ -360,8 +360,7 @@ class SimSettings:
         p.precession_angle = math.radians(phi)
         p.minimum_intensity = self.minimum_intensity
         p.g_max = g_max
-        for i, v in enumerate(np.asarray(rec, dtype=np.float64).reshape(-1)):
-            p.rec[i] = float(v)
+        p.rec[:] = np.asarray(rec, dtype=np.float64).reshape(-1).tolist()
         return p
 
 
@@ -472,6 +471,28 @@ def render_params(shape, sigma, calibration, direct_beam_position=None, in_plane
     return p
 
 
+_RENDER_PARAMS_CACHE: dict = {}
+
+
+def _cached_render_params(shape, sigma, calibration, direct_beam_position, in_plane_angle,
+                          mirrored, normalize, fast, clip_threshold) -> _lib.RenderParams:
+    """render_params for render_spots, built once per distinct argument set
+    (the Gaussian taps cost several microseconds per call; the structs stay
+    private to this module and the library only reads them)."""
+    key = (int(shape[0]), int(shape[1]), float(sigma), float(calibration),
+           None if direct_beam_position is None else tuple(float(v) for v in direct_beam_position),
+           float(in_plane_angle), bool(mirrored), bool(normalize), bool(fast),
+           float(clip_threshold))
+    p = _RENDER_PARAMS_CACHE.get(key)
+    if p is None:
+        p = render_params(shape, sigma, calibration, direct_beam_position, in_plane_angle,
+                          mirrored, normalize, fast, clip_threshold)
+        if len(_RENDER_PARAMS_CACHE) >= 256:
+            _RENDER_PARAMS_CACHE.clear()
+        _RENDER_PARAMS_CACHE[key] = p
+    return p
+
+
 def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigma, calibration,
                  direct_beam_position=None, in_plane_angle=0.0, mirrored=False, normalize=True,
                  fast=True, clip_threshold=1.0, out: Optional[torch.Tensor] = None,
@@ -494,8 +515,8 @@ def render_spots(spots: torch.Tensor, count: torch.Tensor, cap: int, shape, sigm
     sync_check = status is None
     if status is None:
         status = torch.zeros(n, dtype=torch.int32, device=dev)
-    p = render_params(shape, sigma, calibration, direct_beam_position, in_plane_angle, mirrored,
-                      normalize, fast, clip_threshold)
+    p = _cached_render_params(shape, sigma, calibration, direct_beam_position, in_plane_angle,
+                              mirrored, normalize, fast, clip_threshold)
     stream = _lib.stream_handle(dev) if stream is None else stream
     if not fast and cap > int(L.dsx_render_max_spots()):
         # more spots than the render LDS holds: the adjoint kernel's forward
