@@ -293,7 +293,8 @@ int dsx_render_fast(const float *spot_d, const int *count_d, int cap, int n,
  * or above 1024, height above 1024) run dsx_render_fast's kernel alone and
  * leave work_d unused.
  *   work_d : dsx_render_fast_bands_workspace_ints(n, height, width) ints
- *            (patterns are taken in rounds of at most 32,768); overlapping
+ *            (patterns are taken in rounds of at most 16,384, which reuse
+ *            it in stream order); overlapping
  *            calls must not share it
  * Other arguments as for dsx_render_fast.
  */

@@ -172,8 +172,8 @@ def test_dense_rows_read_the_row_sorted_list():
 
 
 def test_rounds_of_patterns():
-    # three plan/writer rounds: the third reuses the first round's buffers
-    # while the second round's bands are written
+    # five plan/writer rounds of at most 16,384 patterns (the last partial),
+    # each reusing the workspace after the previous round's bands are written
     print("bands: 70000 x 32x32", flush=True)
     spots, cnt = _spot_lists(70000, 8, 32, 32, 4, seed=6, dup=0.05)
     ref, out, st_ref, st, _ = _render_both(spots, cnt, 8, (32, 32), 1.0)
