@@ -47,7 +47,8 @@ class Autotuner:
         whether its result is usable."""
         k = self._key(key)
         with self._lock:
-            if k in self._cache:
+            # a winner cached from another candidate list is timed again
+            if k in self._cache and self._cache[k] in list(candidates):
                 return self._cache[k]
         stream = torch.cuda.current_stream()
         best_c, best_ms = None, float("inf")

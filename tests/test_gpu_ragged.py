@@ -199,3 +199,6 @@ def test_autotuner_cache_roundtrip(tmp_path):
     t2 = Autotuner(reps=1, cache_file=f)
     assert t2.best(("k", 1), [8, 1, 4], run) == 1
     assert len(calls) == n                            # served from the cache file
+    # a cached winner that is not among the candidates is timed again
+    assert t2.best(("k", 1), [8, 4], run) == 4
+    assert len(calls) > n
