@@ -203,3 +203,19 @@ def test_workspace_query_and_rejection():
                                  16, lib.ptr(out), None, lib.stream_handle(spots.device))
     assert rc == lib.DSX_EINVAL
     assert b"workspace" in L.dsx_last_error()
+
+
+def test_render_spots_same_bits_across_the_small_batch_threshold():
+    # render_spots takes the splat kernel alone below 768 patterns and the
+    # band renderer above: a batch and its leading part give the same pixels
+    import torch
+    from diffsims_amd import engine
+    print("render_spots: 800 and 700 x 256x256 across the dispatch threshold", flush=True)
+    spots, cnt = _spot_lists(800, 400, 256, 256, 260, seed=30, edge=0.05, dup=0.02)
+    assert engine.fast_renderer(800, 256, 256) == "dsx_render_fast_bands"
+    assert engine.fast_renderer(700, 256, 256) == "dsx_render_fast"
+    big = engine.render_spots(spots, cnt, 400, (256, 256), 1.0, 1.0)
+    small = engine.render_spots(spots[:700].contiguous(), cnt[:700].contiguous(), 400, (256, 256),
+                                1.0, 1.0)
+    torch.cuda.synchronize()
+    assert torch.equal(big[:700], small)
