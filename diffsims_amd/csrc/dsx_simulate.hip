@@ -35,7 +35,10 @@ namespace dsx {
 constexpr int kOT = 4;
 constexpr int kSlotBits = 2;      // bits of an orientation slot in a queue entry
 static_assert((1 << kSlotBits) == kOT, "orientation slot bits");
-constexpr int kSimThreads = 256;  // 4 waves per block
+// one wave per block: the waves share nothing, and single-wave blocks let the
+// dispatcher refill a finished wave's slot at once (the library's spot kernel
+// 1.46 -> 1.43 ms, the 64-phase batch's 18.5 -> 18.2 ms against four waves)
+constexpr int kSimThreads = 64;
 // The line sweep pays once lines average this many rows.  Measured with the
 // intervals of a group formed for all orientations at once: the 40-atom
 // library table (14.6 rows per line) 2.19 ms full sweep against 1.55 ms line
@@ -482,16 +485,16 @@ static void fill_args(SimArgs &a, const dsx_sim_params *p) {
 }
 
 // Blocks of the spot kernel's launch: each wave takes every n_waves-th tile.
-// Far more blocks than stay resident (default 32 per CU, one or two tiles
-// per wave on the library) let the dispatcher hand the slots of blocks that
-// finish early to the next ones, which balances tiles of unequal cost; a
-// launch of only the resident blocks leaves each wave a fixed share.  The
-// library's 50,000 tiles: 1.60 ms with 4 blocks per CU, 1.55 with 8, 1.52
-// with 12, 1.50 with 16, 1.45-1.46 with 32 or 64.
+// Far more waves than stay resident (default 128 one-wave blocks per CU, one
+// or two tiles per wave on the library) let the dispatcher hand the slots of
+// waves that finish early to the next ones, which balances tiles of unequal
+// cost; a launch of only the resident waves leaves each a fixed share.  The
+// library's 50,000 tiles, in waves per CU (four-wave blocks): 1.60 ms with 16,
+// 1.55 with 32, 1.52 with 48, 1.50 with 64, 1.45-1.46 with 128 or 256.
 static int launch_blocks(int n_items, int blocks_per_cu) {
   const int waves_per_block = kSimThreads / kWave;
   const int blocks = (n_items + waves_per_block - 1) / waves_per_block;
-  return std::max(1, std::min(blocks, cu_count() * (blocks_per_cu > 0 ? blocks_per_cu : 32)));
+  return std::max(1, std::min(blocks, cu_count() * (blocks_per_cu > 0 ? blocks_per_cu : 128)));
 }
 
 }  // namespace dsx

@@ -243,14 +243,14 @@ class RaggedLibrary:
     def tune(self):
         """Pick the simulate launch width, and per phase bucket the render
         method and width, by timing candidates on this device."""
-        # blocks per CU of the spot kernel's launch: the work items are in
-        # decreasing cost, so a launch far wider than what stays resident (at
-        # most one item per wave) lets the dispatcher hand the costly items out
-        # first and fill in behind them (64-phase batch: 21.6 ms at 16 blocks
-        # per CU, 18.5 ms at one item per wave)
+        # one-wave blocks per CU of the spot kernel's launch: the work items
+        # are in decreasing cost, so a launch far wider than what stays
+        # resident (at most one item per wave) lets the dispatcher hand the
+        # costly items out first and fill in behind them (64-phase batch:
+        # 21.6 ms at 64 waves per CU, 18.2 ms at one item per wave)
         self.sim_bpc = self.tuner.best(
             ("simulate_ragged", len(self.slots), int(self.items.shape[0])),
-            [8, 16, 32, 64, 128, 256, 1024], lambda bpc: self._simulate(bpc))
+            [32, 64, 128, 256, 512, 1024, 4096], lambda bpc: self._simulate(bpc))
         torch.cuda.synchronize(self.dev)
         self.check_spots_only()
         max_pix = int(self.L.dsx_render_dense_max_pixels())

@@ -239,7 +239,7 @@ int dsx_simulate_tile_size(void);
  *             the given order (put the most expensive first)
  *   params    physics shared by all phases (rec and g_max are ignored: each
  *             phase carries its own)
- *   blocks_per_cu  launch width (0 = default)
+ *   blocks_per_cu  launch width in blocks of one wave per CU (0 = default)
  * Outputs per phase as for dsx_simulate.
  */
 int dsx_simulate_ragged(const dsx_phase_desc *phases_d, int n_phases, const int *items_d,
