@@ -521,7 +521,7 @@ def large_precession_leg(dev, n_chunks):
     """500-atom triclinic cell, r = 3.0 (~1.6e6 reflections), precession 1 deg,
     512x512 sub-pixel patterns; each 65,536-orientation chunk computes the summed
     L2 loss against one target and its orientation and B gradients.  Also the
-    forward alone (the chunk's patterns rendered, in slices of 8,192), the
+    forward alone (the chunk's patterns rendered, in slices of 16,384), the
     FP64 error of one pattern and the oracle's loop on one orientation."""
     import torch
     from threadpoolctl import threadpool_limits
@@ -568,7 +568,9 @@ def large_precession_leg(dev, n_chunks):
     steady = times[1:] if len(times) > 1 else times
     per_chunk = float(np.mean(steady))
     # the forward alone: the first chunk's patterns, 8,192 at a time
-    sl = 8192
+    # 16,384 orientations are 4,096 spot-kernel tiles, one per resident wave
+    # (a slice of 8,192 left half the GPU idle in the spot kernel)
+    sl = 16384
     with torch.no_grad():
         args = D.forward_args(D.leaf_parameters(pa, d["euler"][:sl], dev, requires_grad=False))
         model.patterns(*args)                                   # warm (capacity, allocator)
